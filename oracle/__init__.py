@@ -1,0 +1,128 @@
+"""ORACLE TEST INFRASTRUCTURE -- not part of the product.
+
+CPU checkers for the aerosol-optics + discrete-ordinate radiance path:
+
+* ``oracle.chem_mie``   numpy float32 restatement of the Chem_Mie query and the
+  per-tracer optics summation;
+* ``oracle.ref``        the reference VLIDORT 2.8.3 core compiled from its
+  unmodified sources by ``oracle/Makefile.ref`` (into ``oracle/_ref/``) and
+  driven through ``oracle/ref_driver.f90``;
+* ``oracle.port``       the C++ restatement of the same path
+  (``oracle/vlidort_oracle.cpp``, built by ``oracle/Makefile``).
+
+Only tests/, ``__graft_entry__.smoke()`` and the bench's ``cpu_baseline`` leg may
+import this package, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_SO = os.path.join(HERE, "_ref", "libvlidort_ref.so")
+PORT_SO = os.path.join(HERE, "liboracle_vlidort.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _p(a: np.ndarray, dt=np.float64):
+    assert a.dtype == dt and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data_as(_dp if dt == np.float64 else _ip)
+
+
+def _big_stack_call(fn, *args):
+    """Run fn(*args) on a thread with a 1 GiB stack: the flang-built reference
+    keeps its large work arrays (e.g. 9 MB Greek-matrix inputs) on the stack."""
+    import threading
+    box = {}
+
+    def run():
+        try:
+            box["r"] = fn(*args)
+        except BaseException as e:  # re-raised in the caller
+            box["e"] = e
+
+    old = threading.stack_size()
+    threading.stack_size(1 << 30)
+    try:
+        t = threading.Thread(target=run)
+        t.start()
+    finally:
+        threading.stack_size(old)
+    t.join()
+    if "e" in box:
+        raise box["e"]
+    return box.get("r")
+
+
+def _load(path: str) -> Optional[ctypes.CDLL]:
+    if not os.path.exists(path):
+        return None
+    return ctypes.CDLL(path)
+
+
+class LambertDriver:
+    """ctypes binding of a library exporting the Fortran-layout drivers
+    ``*_scalar_lambert`` / ``*_vector_lambert`` (see ref_driver.f90)."""
+
+    def __init__(self, lib: ctypes.CDLL, prefix: str):
+        self.lib = lib
+        self.fs = getattr(lib, prefix + "_scalar_lambert")
+        self.fv = getattr(lib, prefix + "_vector_lambert")
+        ci = ctypes.c_int
+        self.fs.restype = None
+        self.fs.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                            _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp, _ip, _ip]
+        self.fv.restype = None
+        self.fv.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                            _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp, _ip, _ip]
+
+    def scalar_lambert(self, km, channels, nstreams, nmom, rot, depol, tau, ssa, g, pmom, pe, he, te,
+                       albedo, sza, raa, vza, missing=1e15, plane_parallel=False):
+        """Arrays in reversed-Fortran C order: rot[nch,nobs,km], tau/ssa/g[nobs,nch,km],
+        pmom[npol,nmom,nobs,nch,km], pe/he/te[nobs,km+1], albedo[nch,nobs]."""
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        channels, rot, depol, tau, ssa, g, pmom, pe, he, te, albedo, sza, raa, vza = map(
+            f, (channels, rot, depol, tau, ssa, g, pmom, pe, he, te, albedo, sza, raa, vza))
+        nch = len(channels)
+        nobs = len(sza)
+        npol = pmom.shape[0]
+        rad = np.zeros((nch, nobs))
+        refl = np.zeros((nch, nobs))
+        nf = np.zeros((nch, nobs), dtype=np.int32)
+        rc = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fs, km, nch, nobs, _p(channels), nstreams, int(plane_parallel), nmom, npol, _p(rot), _p(depol),
+                _p(tau), _p(ssa), _p(g), _p(pmom), _p(pe), _p(he), _p(te), _p(albedo), _p(sza), _p(raa),
+                _p(vza), missing, _p(rad), _p(refl), _p(nf, np.int32), _p(rc, np.int32))
+        return rad, refl, nf, int(rc[0])
+
+    def vector_lambert(self, km, channels, nstreams, nmom, rot, depol, alpha, tau, ssa, pmom, pe, he, te,
+                       albedo, sza, raa, vza, flux_factor, missing=1e15, plane_parallel=False):
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        (channels, rot, depol, alpha, tau, ssa, pmom, pe, he, te, albedo, sza, raa, vza,
+         flux_factor) = map(f, (channels, rot, depol, alpha, tau, ssa, pmom, pe, he, te, albedo, sza, raa,
+                                vza, flux_factor))
+        nch = len(channels)
+        nobs = len(sza)
+        npol = pmom.shape[0]
+        out = [np.zeros((nch, nobs)) for _ in range(4)]
+        nf = np.zeros((nch, nobs), dtype=np.int32)
+        rc = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fv, km, nch, nobs, _p(channels), nstreams, int(plane_parallel), nmom, npol, _p(rot), _p(depol),
+                _p(alpha), _p(tau), _p(ssa), _p(pmom), _p(pe), _p(he), _p(te), _p(albedo), _p(sza), _p(raa),
+                _p(vza), _p(flux_factor), missing, *[_p(o) for o in out], _p(nf, np.int32), _p(rc, np.int32))
+        return out[0], out[1], out[2], out[3], nf, int(rc[0])
+
+
+def ref() -> Optional[LambertDriver]:
+    lib = _load(REF_SO)
+    return None if lib is None else LambertDriver(lib, "ref")
+
+
+def port() -> Optional[LambertDriver]:
+    lib = _load(PORT_SO)
+    return None if lib is None else LambertDriver(lib, "oracle")

@@ -1,0 +1,300 @@
+! ORACLE TEST INFRASTRUCTURE -- not part of the product.
+!
+! C-callable harness around the reference VLIDORT 2.8.3 core, built from the
+! unmodified reference sources by oracle/Makefile.ref into oracle/_ref/.
+! It restates the per-column setup that the AeroApps drivers perform before
+! calling VLIDORT_MASTER, because VLIDORT_ScatMod.F90 (which holds that setup)
+! does not compile with flang (LOGICAL == comparison in an unused legacy
+! routine at VLIDORT_ScatMod.F90:1352).  Everything numerical below the
+! setup is the reference's own code; VLIDORT_Init (VLIDORT_Mod.F90:65-383)
+! is the reference's own routine as well.
+!
+! Restated (reference lines followed statement for statement, including the
+! single-precision literals the reference writes in double-precision
+! context):
+!   ref_scalar_lambert  <- GEO/VLIDORT_LAMBERT.F90:34-168   (VLIDORT_Scalar_Lambert)
+!                          V283/VLIDORT_ScatMod.F90:975-1247 (VLIDORT_Run_Scalar)
+!   ref_vector_lambert  <- GEO/VLIDORT_LAMBERT.F90:172-342  (VLIDORT_Vector_Lambert_SingleGeom)
+!                          V283/VLIDORT_ScatMod.F90:284-617  (VLIDORT_Run_Vector_SingleGeom)
+! (GEO = src/Components/rtms/geo_vlidort, V283 = src/Components/rtms/RTSI/VLIDORT2p8p3)
+!
+! Extra output: nfourier(nobs,nch) = TS_FOURIER_SAVED(1) + 1, the number of
+! Fourier terms the reference executed (0 where MISSING).
+module ref_driver_m
+  use iso_c_binding
+  use VLIDORT_PARS_m
+  use VLIDORT_IO_DEFS_m
+  use VLIDORT_MASTERS_m
+  use VLIDORT_Mod
+  implicit none
+
+contains
+
+  logical function is_missing(x, missing)
+    real*8, intent(in) :: x, missing
+    is_missing = abs(x/missing-1) < 0.001
+  end function is_missing
+
+  ! Per-column input setup shared by the scalar and vector paths
+  ! (VLIDORT_ScatMod.F90:1045-1137 / 362-441).
+  subroutine set_column(V, nstokes, nmom, albedo, sza, vza, raa, pe, ze, te, km)
+    type(VLIDORT), intent(inout) :: V
+    integer, intent(in) :: nstokes, nmom, km
+    real*8, intent(in) :: albedo, sza, vza, raa
+    real*8, intent(in) :: pe(km+1), ze(km+1), te(km+1)
+    V%VIO%VLIDORT_FixIn%Cont%TS_NSTOKES = nstokes
+    V%VIO%VLIDORT_ModIn%MCont%TS_NGREEK_MOMENTS_INPUT = nmom
+    V%VIO%VLIDORT_FixIn%Bool%TS_DO_LAMBERTIAN_SURFACE = .true.
+    V%VIO%VLIDORT_FixIn%Optical%TS_LAMBERTIAN_ALBEDO = albedo
+    V%VIO%VLIDORT_ModIn%MSunRays%TS_SZANGLES(1) = sza
+    V%VIO%VLIDORT_ModIn%MUserVal%TS_USER_RELAZMS(1) = raa
+    V%VIO%VLIDORT_ModIn%MUserVal%TS_USER_VZANGLES_INPUT(1) = vza
+    V%VIO%VLIDORT_ModIn%MUserVal%TS_N_USER_OBSGEOMS = 1
+    V%VIO%VLIDORT_ModIn%MUserVal%TS_USER_OBSGEOMS_INPUT(1,1) = sza
+    V%VIO%VLIDORT_ModIn%MUserVal%TS_USER_OBSGEOMS_INPUT(1,2) = vza
+    V%VIO%VLIDORT_ModIn%MUserVal%TS_USER_OBSGEOMS_INPUT(1,3) = raa
+    V%VIO%VLIDORT_ModIn%MUserVal%TS_USER_LEVELS(1) = 0.0
+    ! SZA == 0 latches nadir FO for the rest of the driver call
+    ! (VLIDORT_ScatMod.F90:1123-1126)
+    if (sza == 0.0) then
+      V%VIO%VLIDORT_ModIn%MBool%TS_DO_FOCORR_NADIR = .true.
+      V%VIO%VLIDORT_ModIn%MBool%TS_DO_FOCORR_OUTGOING = .false.
+    end if
+    V%VIO%VLIDORT_FixIn%Chapman%TS_height_grid(0:km) = ze * 1.E-3
+    V%VIO%VLIDORT_FixIn%Chapman%TS_pressure_grid(0:km) = pe * 1.E-2
+    V%VIO%VLIDORT_FixIn%Chapman%TS_temperature_grid(0:km) = te
+  end subroutine set_column
+
+  subroutine ref_scalar_lambert(km, nch, nobs, channels, nstreams, plane_parallel, nMom, &
+       nPol, ROT, depol, tau, ssa, g, pmom, pe, he, te, albedo, &
+       solar_zenith, relat_azymuth, sensor_zenith, MISSING, &
+       radiance_VL, reflectance_VL, nfourier, rc) bind(C, name="ref_scalar_lambert")
+    integer(c_int), value, intent(in) :: km, nch, nobs, nstreams, plane_parallel, nMom, nPol
+    real(c_double), intent(in) :: channels(nch), ROT(km,nobs,nch), depol(nch)
+    real(c_double), intent(in) :: tau(km,nch,nobs), ssa(km,nch,nobs), g(km,nch,nobs)
+    real(c_double), intent(in) :: pmom(km,nch,nobs,nMom,nPol)
+    real(c_double), intent(in) :: pe(km+1,nobs), he(km+1,nobs), te(km+1,nobs)
+    real(c_double), intent(in) :: albedo(nobs,nch)
+    real(c_double), intent(in) :: solar_zenith(nobs), relat_azymuth(nobs), sensor_zenith(nobs)
+    real(c_double), value, intent(in) :: MISSING
+    real(c_double), intent(out) :: radiance_VL(nobs,nch), reflectance_VL(nobs,nch)
+    integer(c_int), intent(out) :: nfourier(nobs,nch), rc
+
+    type(VLIDORT), allocatable :: V
+    integer :: i, j, l, ier, irc
+    real*8 :: ray_l, alpha_l, tau_l, ssa_l, tau_ext, tau_scat, ssa_tot
+    real*8 :: raysmom2, aerswt, rayswt, depol_ratio
+    real*8, allocatable :: greekmat_total_input(:,:,:)
+    real*8 :: aersmom(0:MAXMOMENTS_INPUT)
+    real*8 :: deltau_vert_input(MAXLAYERS), omega_total_input(MAXLAYERS)
+    real*8, parameter :: pi = 4.*atan(1.0)
+
+    allocate(V)
+    allocate(greekmat_total_input(0:MAXMOMENTS_INPUT,MAXLAYERS,16))
+    greekmat_total_input = 0.0d0
+    rc = 0
+    nfourier = 0
+    V%NSTREAMS = nstreams
+    V%DO_PLANE_PARALLEL = (plane_parallel /= 0)
+    call VLIDORT_Init(V, km, irc)
+    rc = irc
+    if (rc /= 0) return
+
+    do j = 1, nobs
+      if (is_missing(solar_zenith(j),MISSING) .or. is_missing(sensor_zenith(j),MISSING) .or. &
+          is_missing(relat_azymuth(j),MISSING)) then
+        radiance_VL(j,:) = MISSING
+        reflectance_VL(j,:) = MISSING
+        cycle
+      end if
+      do i = 1, nch
+        if (is_missing(albedo(j,i),MISSING)) then
+          radiance_VL(j,i) = MISSING
+          reflectance_VL(j,i) = MISSING
+          cycle
+        end if
+        ier = 0
+        if (nMom > MAXMOMENTS_INPUT) ier = 5
+        if (ier == 0) then
+          call set_column(V, 1, nMom, albedo(j,i), solar_zenith(j), sensor_zenith(j), &
+                          relat_azymuth(j), pe(:,j), he(:,j), te(:,j), km)
+          depol_ratio = depol(i)
+          aersmom = 0.0
+          raysmom2 = (1.0 - depol_ratio)/(2.0 + depol_ratio)
+          do l = 1, km
+            ray_l = ROT(l,j,i)
+            alpha_l = 0.0
+            tau_l = tau(l,i,j)
+            ssa_l = ssa(l,i,j)
+            tau_ext = alpha_l + ray_l + tau_l
+            tau_scat = ray_l + ssa_l * tau_l
+            ssa_tot = tau_scat / tau_ext
+            if (ssa_tot > 0.99999) then
+              ssa_tot = 0.99999
+            end if
+            deltau_vert_input(l) = tau_ext
+            omega_total_input(l) = ssa_tot
+            aerswt = ssa_l * tau_l / tau_scat
+            rayswt = ray_l / tau_scat
+            aersmom(0:nMom-1) = pmom(l,i,j,1:nMom,1)
+            greekmat_total_input(0,l,1) = 1.0
+            greekmat_total_input(1,l,1) = aersmom(1) * aerswt
+            greekmat_total_input(2,l,1) = raysmom2 * rayswt + aersmom(2) * aerswt
+            greekmat_total_input(3:nMom,l,1) = aersmom(3:nMom) * aerswt
+          end do
+          V%VIO%VLIDORT_FixIn%Optical%TS_DELTAU_VERT_INPUT = deltau_vert_input
+          V%VIO%VLIDORT_ModIn%MOptical%TS_OMEGA_TOTAL_INPUT = omega_total_input
+          V%VIO%VLIDORT_FixIn%Optical%TS_GREEKMAT_TOTAL_INPUT = greekmat_total_input
+          call VLIDORT_MASTER(V%DO_DEBUG_INPUT, V%VIO%VLIDORT_FixIn, V%VIO%VLIDORT_ModIn, &
+                              V%VIO%VLIDORT_Sup, V%VIO%VLIDORT_Out)
+          if (V%VIO%VLIDORT_Out%Status%TS_STATUS_INPUTCHECK /= 0) ier = 3
+          if (V%VIO%VLIDORT_Out%Status%TS_STATUS_CALCULATION /= 0) ier = 4
+        end if
+        if (ier /= 0) then
+          radiance_VL(j,i) = MISSING
+          reflectance_VL(j,i) = MISSING
+          cycle
+        end if
+        radiance_VL(j,i) = V%VIO%VLIDORT_Out%Main%TS_STOKES(1,1,1,1)
+        reflectance_VL(j,i) = (pi * radiance_VL(j,i)) / &
+             (cos(V%VIO%VLIDORT_ModIn%MSunRays%TS_SZANGLES(1)*pi/180.0) * &
+              V%VIO%VLIDORT_FixIn%SunRays%TS_FLUX_FACTOR)
+        nfourier(j,i) = V%VIO%VLIDORT_Out%Main%TS_FOURIER_SAVED(1) + 1
+      end do
+    end do
+    deallocate(greekmat_total_input)
+    deallocate(V)
+  end subroutine ref_scalar_lambert
+
+  subroutine ref_vector_lambert(km, nch, nobs, channels, nstreams, plane_parallel, nMom, &
+       nPol, ROT, depol, alpha, tau, ssa, pmom, pe, he, te, albedo, &
+       solar_zenith, relat_azymuth, sensor_zenith, flux_factor, MISSING, &
+       radiance_VL, reflectance_VL, Q, U, nfourier, rc) bind(C, name="ref_vector_lambert")
+    integer(c_int), value, intent(in) :: km, nch, nobs, nstreams, plane_parallel, nMom, nPol
+    real(c_double), intent(in) :: channels(nch), ROT(km,nobs,nch), depol(nch), alpha(km,nobs,nch)
+    real(c_double), intent(in) :: tau(km,nch,nobs), ssa(km,nch,nobs)
+    real(c_double), intent(in) :: pmom(km,nch,nobs,nMom,nPol)
+    real(c_double), intent(in) :: pe(km+1,nobs), he(km+1,nobs), te(km+1,nobs)
+    real(c_double), intent(in) :: albedo(nobs,nch), flux_factor(nch,nobs)
+    real(c_double), intent(in) :: solar_zenith(nobs), relat_azymuth(nobs), sensor_zenith(nobs)
+    real(c_double), value, intent(in) :: MISSING
+    real(c_double), intent(out) :: radiance_VL(nobs,nch), reflectance_VL(nobs,nch)
+    real(c_double), intent(out) :: Q(nobs,nch), U(nobs,nch)
+    integer(c_int), intent(out) :: nfourier(nobs,nch), rc
+
+    type(VLIDORT), allocatable :: V
+    integer :: i, j, k, l, ier, irc
+    real*8 :: ray_l, alpha_l, tau_l, ssa_l, tau_ext, tau_scat, ssa_tot
+    real*8 :: raysmom2, gammamom2, alphamom2, deltamom1, aerswt, rayswt, depol_ratio
+    real*8, allocatable :: greekmat_total_input(:,:,:), aervmoms(:,:,:)
+    real*8 :: rayvmoms(0:2,16)
+    real*8 :: deltau_vert_input(MAXLAYERS), omega_total_input(MAXLAYERS)
+    real*8, parameter :: pi = 4.*atan(1.0)
+
+    allocate(V)
+    allocate(greekmat_total_input(0:MAXMOMENTS_INPUT,MAXLAYERS,16))
+    allocate(aervmoms(0:MAXMOMENTS_INPUT,MAXLAYERS,16))
+    greekmat_total_input = 0.0d0
+    rc = 0
+    nfourier = 0
+    V%NSTREAMS = nstreams
+    V%DO_PLANE_PARALLEL = (plane_parallel /= 0)
+    V%NGREEK_MOMENTS_INPUT = nMom
+    call VLIDORT_Init(V, km, irc)
+    rc = irc
+    if (rc /= 0) return
+
+    do j = 1, nobs
+      if (is_missing(solar_zenith(j),MISSING) .or. is_missing(sensor_zenith(j),MISSING) .or. &
+          is_missing(relat_azymuth(j),MISSING)) then
+        radiance_VL(j,:) = MISSING
+        reflectance_VL(j,:) = MISSING
+        Q(j,:) = MISSING
+        U(j,:) = MISSING
+        cycle
+      end if
+      do i = 1, nch
+        V%VIO%VLIDORT_FixIn%SunRays%TS_FLUX_FACTOR = flux_factor(i,j)
+        if (is_missing(albedo(j,i),MISSING)) then
+          radiance_VL(j,i) = MISSING
+          reflectance_VL(j,i) = MISSING
+          Q(j,i) = MISSING
+          U(j,i) = MISSING
+          cycle
+        end if
+        ier = 0
+        if (nMom > MAXMOMENTS_INPUT) ier = 5
+        if (ier == 0) then
+          call set_column(V, 3, nMom, albedo(j,i), solar_zenith(j), sensor_zenith(j), &
+                          relat_azymuth(j), pe(:,j), he(:,j), te(:,j), km)
+          depol_ratio = depol(i)
+          rayvmoms = 0.0
+          aervmoms = 0.0
+          gammamom2 = -SQRT(6.) * (1 - depol_ratio) / (2 + depol_ratio)
+          alphamom2 = 6 * (1 - depol_ratio) / (2 + depol_ratio)
+          deltamom1 = 3 * (1 - 2 * depol_ratio) / (2 + depol_ratio)
+          raysmom2 = (1.0 - depol_ratio)/(2.0 + depol_ratio)
+          rayvmoms(0,1) = 1.0
+          rayvmoms(2,1) = raysmom2
+          rayvmoms(2,2) = gammamom2
+          rayvmoms(2,5) = gammamom2
+          rayvmoms(2,6) = alphamom2
+          rayvmoms(1,16) = deltamom1
+          do l = 1, km
+            ray_l = ROT(l,j,i)
+            alpha_l = alpha(l,j,i)
+            tau_l = tau(l,i,j)
+            ssa_l = ssa(l,i,j)
+            tau_ext = alpha_l + ray_l + tau_l
+            tau_scat = ray_l + ssa_l * tau_l
+            ssa_tot = tau_scat / tau_ext
+            if (ssa_tot > 0.99999) then
+              ssa_tot = 0.99999
+            end if
+            deltau_vert_input(l) = tau_ext
+            omega_total_input(l) = ssa_tot
+            aervmoms(0:nMom-1,l,1)  = pmom(l,i,j,1:nMom,1)
+            aervmoms(0:nMom-1,l,2)  = pmom(l,i,j,1:nMom,2)
+            aervmoms(0:nMom-1,l,5)  = pmom(l,i,j,1:nMom,2)
+            aervmoms(0:nMom-1,l,11) = pmom(l,i,j,1:nMom,3)
+            aervmoms(0:nMom-1,l,12) = pmom(l,i,j,1:nMom,4)
+            aervmoms(0:nMom-1,l,15) = -pmom(l,i,j,1:nMom,4)
+            aervmoms(0:nMom-1,l,6)  = pmom(l,i,j,1:nMom,5)
+            aervmoms(0:nMom-1,l,16) = pmom(l,i,j,1:nMom,6)
+            aerswt = ssa_l * tau_l / tau_scat
+            rayswt = ray_l / tau_scat
+            do k = 1, 16
+              greekmat_total_input(0:2,l,k) = rayvmoms(0:2,k) * rayswt + aervmoms(0:2,l,k) * aerswt
+              greekmat_total_input(3:nMom,l,k) = aervmoms(3:nMom,l,k) * aerswt
+            end do
+            greekmat_total_input(0,l,1) = 1.0
+          end do
+          V%VIO%VLIDORT_FixIn%Optical%TS_DELTAU_VERT_INPUT = deltau_vert_input
+          V%VIO%VLIDORT_ModIn%MOptical%TS_OMEGA_TOTAL_INPUT = omega_total_input
+          V%VIO%VLIDORT_FixIn%Optical%TS_GREEKMAT_TOTAL_INPUT = greekmat_total_input
+          call VLIDORT_MASTER(V%DO_DEBUG_INPUT, V%VIO%VLIDORT_FixIn, V%VIO%VLIDORT_ModIn, &
+                              V%VIO%VLIDORT_Sup, V%VIO%VLIDORT_Out)
+          if (V%VIO%VLIDORT_Out%Status%TS_STATUS_INPUTCHECK /= 0) ier = 3
+          if (V%VIO%VLIDORT_Out%Status%TS_STATUS_CALCULATION /= 0) ier = 4
+        end if
+        if (ier /= 0) then
+          radiance_VL(j,i) = MISSING
+          reflectance_VL(j,i) = MISSING
+          Q(j,i) = MISSING
+          U(j,i) = MISSING
+          cycle
+        end if
+        radiance_VL(j,i) = V%VIO%VLIDORT_Out%Main%TS_STOKES(1,1,1,1)
+        Q(j,i) = V%VIO%VLIDORT_Out%Main%TS_STOKES(1,1,2,1)
+        U(j,i) = V%VIO%VLIDORT_Out%Main%TS_STOKES(1,1,3,1)
+        reflectance_VL(j,i) = (pi * radiance_VL(j,i)) / &
+             (cos(V%VIO%VLIDORT_ModIn%MSunRays%TS_SZANGLES(1)*pi/180.0) * &
+              V%VIO%VLIDORT_FixIn%SunRays%TS_FLUX_FACTOR)
+        nfourier(j,i) = V%VIO%VLIDORT_Out%Main%TS_FOURIER_SAVED(1) + 1
+      end do
+    end do
+    deallocate(greekmat_total_input, aervmoms)
+    deallocate(V)
+  end subroutine ref_vector_lambert
+
+end module ref_driver_m

@@ -1,0 +1,72 @@
+"""Loader of the in-tree HIP library ``libaeroapps_amd.so`` (ctypes).
+
+The product has no CPU fallback: when the library is missing or cannot be
+loaded, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libaeroapps_amd.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_ci = ctypes.c_int
+_cd = ctypes.c_double
+_cf = ctypes.c_float
+_cp = ctypes.c_char_p
+
+# (name, restype, argtypes) of every symbol include/aeroapps_amd.h declares
+SIGNATURES = {
+    "aa_mie_create": (_ci, [ctypes.POINTER(_vp), _ci, _fp]),
+    "aa_mie_add_table": (_ci, [_vp, _ci, _ci, _ci, _ci, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _cp]),
+    "aa_mie_query_idx": (_ci, [_vp, _cp]),
+    "aa_mie_query": (_ci, [_vp, _ci, _cf, _ci, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _ci, _ci, _fp]),
+    "aa_mie_dims": (_ci, [_vp, _ip, _ip, _ip, _ip]),
+    "aa_mie_destroy": (_ci, [_vp]),
+    "aa_get_aop_vector": (_ci, [_vp, _ci, _ci, _ci, _ci, _fp, _cp, _ci, _fp, _fp, _ci, _ci, _fp, _fp, _fp, _fp, _ip]),
+    "aa_get_aop_scalar": (_ci, [_vp, _ci, _ci, _ci, _ci, _fp, _cp, _ci, _fp, _fp, _fp, _fp, _fp, _ip]),
+    "aa_get_aop_vector_dev": (_ci, [_vp, _ci, _ci, _ci, _ci, _vp, _vp, _vp, _vp, _ci, _ci, _vp, _vp, _vp, _vp, _vp]),
+    "aa_vlidort_scalar_lambert": (_ci, [_ci, _ci, _ci, _dp, _ci, _ci, _ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                        _dp, _dp, _dp, _dp, _dp, _dp, _cd, _ci, _ci, _dp, _dp, _ip, _ip]),
+    "aa_vlidort_vector_lambert": (_ci, [_ci, _ci, _ci, _dp, _ci, _ci, _ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                        _dp, _dp, _dp, _dp, _dp, _dp, _dp, _cd, _ci, _ci, _dp, _dp, _dp, _dp, _ip,
+                                        _ip]),
+    "aa_column_radiance_dev": (_ci, [_vp, _ci, _ci, _ci, _ci, _vp, _vp, _vp, _vp, _ci, _ci, _ci, _ci, _ci, _vp, _vp,
+                                     _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _cd, _vp, _vp, _vp, _vp, _vp,
+                                     _vp]),
+    "aa_version": (ctypes.c_char_p, []),
+}
+
+ERRORS = {-1: "AA_EARG (bad argument)", -2: "AA_EHIP (HIP runtime error)", -3: "AA_ENOTABLE (no table)",
+          -4: "AA_EDIMS (beyond compiled maxima)"}
+
+_LIB = None
+
+
+class AAError(RuntimeError):
+    pass
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library; raises when it is not built or does not load."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise AAError(f"{LIB_PATH} is not built (run python -m aeroapps_amd.build_lib)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(code: int, what: str) -> None:
+    if code != 0:
+        raise AAError(f"{what} failed: {ERRORS.get(code, code)}")

@@ -1,0 +1,79 @@
+"""Builds the in-tree HIP library aeroapps_amd/libaeroapps_amd.so for gfx950.
+
+Every translation unit under csrc/ is compiled with hipcc in parallel; the
+Fourier-term kernels are compiled once per (streams, Stokes) instantiation.
+Run as ``python -m aeroapps_amd.build_lib`` or through ``__graft_entry__.build``.
+"""
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libaeroapps_amd.so")
+BUILD = os.path.join(HERE, "_build")
+ARCH = os.environ.get("AA_OFFLOAD_ARCH", "gfx950")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+COMMON = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-pass-failed",
+]
+# Chem_Mie arithmetic must not contract (float32 bit-exactness); the radiance
+# kernels keep the compiler default.
+NO_CONTRACT = {"mie.hip"}
+
+FOURIER_PARTS = [(n, 1) for n in range(1, 9)] + [(n, 3) for n in range(1, 5)]
+
+
+def _jobs():
+    jobs = []
+    for f in sorted(os.listdir(CSRC)):
+        if not f.endswith(".hip") or f == "rt_fourier_inst.hip":
+            continue
+        flags = list(COMMON)
+        if f in NO_CONTRACT:
+            flags.append("-ffp-contract=off")
+        jobs.append((os.path.join(CSRC, f), os.path.join(BUILD, f[:-4] + ".o"), flags))
+    for n, s in FOURIER_PARTS:
+        flags = list(COMMON) + [f"-DAA_PART_N={n}", f"-DAA_PART_S={s}"]
+        jobs.append((os.path.join(CSRC, "rt_fourier_inst.hip"), os.path.join(BUILD, f"rt_fourier_{n}_{s}.o"), flags))
+    return jobs
+
+
+def _compile(job):
+    src, obj, flags = job
+    cmd = [HIPCC] + flags + ["-c", src, "-o", obj]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    err = "\n".join(l for l in p.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    return p.returncode, " ".join(cmd), err
+
+
+def build(verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = _jobs()
+    nproc = int(os.environ.get("MAX_JOBS", "0")) or min(16, os.cpu_count() or 4)
+    with cf.ThreadPoolExecutor(max_workers=nproc) as ex:
+        results = list(ex.map(_compile, jobs))
+    bad = [r for r in results if r[0] != 0]
+    for rc, cmd, err in results:
+        if err and (verbose or rc):
+            print(err, file=sys.stderr)
+    if bad:
+        raise RuntimeError("hipcc failed:\n" + "\n".join(c for _, c, _ in bad))
+    objs = [o for _, o, _ in jobs]
+    tmp = OUT + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("link failed: " + p.stderr)
+    os.replace(tmp, OUT)
+    if verbose:
+        print(f"built {OUT}")
+    return OUT
+
+
+if __name__ == "__main__":
+    build()

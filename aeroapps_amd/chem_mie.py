@@ -1,0 +1,133 @@
+"""Python binding of the device-resident Chem_Mie optics tables and the batch
+optics query ``VLIDORT_getAOPvector`` / ``VLIDORT_getAOPscalar``
+(src/Components/rtms/geo_vlidort/VLIDORT_MieMod.F90:10-336; f2py contract of
+``mieobs.getAOPvector``, src/Shared/GMAO_Shared/Chem_Base/MieObs_py.F90:218-456).
+
+Arrays are numpy arrays in their Fortran shapes (qm[km, nq, nobs], ...), float32.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Sequence
+
+import numpy as np
+
+from ._lib import check, lib, _fp, _ip
+
+
+def _f32(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float32))
+
+
+def _p(a, kind=_fp):
+    return a.ctypes.data_as(kind)
+
+
+class MieTables:
+    """Chem_Mie object (src/Shared/GMAO_Shared/Chem_Base/Chem_MieMod.F90:65-99):
+    species tables already interpolated to the channels, uploaded once and kept
+    on the GPU until ``close``."""
+
+    def __init__(self, channels_m: Sequence[float]):
+        self.channels_m = np.asarray(channels_m, dtype=np.float32)
+        h = ctypes.c_void_p()
+        check(lib().aa_mie_create(ctypes.byref(h), len(self.channels_m), _p(self.channels_m)), "aa_mie_create")
+        self.handle = h
+
+    def add_table(self, rh, bext, bsca, bbck, g, refr, refi, pmom, bin_names: Sequence[str]):
+        """One species table; arrays in Fortran shape (nrh, nch, nbin) and
+        pmom (nrh, nch, nbin, nmom, npol)."""
+        rh = _f32(rh)
+        bext, bsca, bbck, g, refr, refi, pmom = map(_f32, (bext, bsca, bbck, g, refr, refi, pmom))
+        nrh, _, nbin, nmom, npol = pmom.shape
+        names = b"".join(n.encode().ljust(16)[:16] for n in bin_names)
+        assert len(bin_names) == nbin
+        check(lib().aa_mie_add_table(self.handle, nrh, nbin, nmom, npol, _p(rh), _p(bext), _p(bsca), _p(bbck),
+                                     _p(g), _p(refr), _p(refi), _p(pmom), names), "aa_mie_add_table")
+
+    @classmethod
+    def from_synthetic(cls, tables) -> "MieTables":
+        """Tables built by aeroapps_amd.synthetic.make_mie_tables."""
+        t = cls(tables[0].lambda_m)
+        for tb in tables:
+            t.add_table(tb.rh, tb.bext.T, tb.bsca.T, tb.bbck.T, tb.g.T, tb.refr.T, tb.refi.T, tb.pmom.T, tb.tracers)
+        return t
+
+    def query_idx(self, qname: str) -> int:
+        return lib().aa_mie_query_idx(self.handle, qname.encode())
+
+    def dims(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        check(lib().aa_mie_dims(self.handle, *[ctypes.byref(x) for x in v]), "aa_mie_dims")
+        return tuple(x.value for x in v)
+
+    def query(self, idx: int, channel: float, q_mass, rh, nmom: int = 0, npol: int = 1):
+        """Chem_MieQuery over arrays of (q_mass, rh) for one tracer index and
+        1-based channel; returns dict of tau, ssa, gasym, bext, bsca, bbck,
+        refr, refi and pmom (n, nmom, npol) when nmom > 0."""
+        q = np.ascontiguousarray(q_mass, dtype=np.float32).ravel()
+        r = np.ascontiguousarray(rh, dtype=np.float32).ravel()
+        n = len(q)
+        outs = {k: np.zeros(n, dtype=np.float32) for k in ("tau", "ssa", "gasym", "bext", "bsca", "bbck", "refr",
+                                                             "refi")}
+        pm = np.zeros((n, nmom, npol), dtype=np.float32, order="F") if nmom > 0 else None
+        check(lib().aa_mie_query(self.handle, idx, float(channel), n, _p(q), _p(r),
+                                 *[_p(outs[k]) for k in ("tau", "ssa", "gasym", "bext", "bsca", "bbck", "refr",
+                                                         "refi")], nmom, npol,
+                                 _p(pm) if pm is not None else None), "aa_mie_query")
+        if pm is not None:
+            outs["pmom"] = pm
+        return outs
+
+    def close(self):
+        if self.handle:
+            lib().aa_mie_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _vname(vnames: Sequence[str]) -> bytes:
+    nq = len(vnames)
+    buf = bytearray(16 * nq)
+    for iq, nm in enumerate(vnames):
+        s = nm.encode().ljust(16)[:16]
+        for c in range(16):
+            buf[c * nq + iq] = s[c]   # Fortran character array vname(nq,16)
+    return bytes(buf)
+
+
+def get_aop_vector(mie: MieTables, channels, vname: Sequence[str], verbose, qm, rh, nMom: int, nPol: int):
+    """VLIDORT_getAOPvector: qm(km,nq,nobs), rh(km,nobs) ->
+    tau, ssa, g (km,nch,nobs), pmom (km,nch,nobs,nMom,nPol), rc."""
+    ch = _f32(channels)
+    qm, rh = _f32(qm), _f32(rh)
+    km, nq, nobs = qm.shape
+    nch = len(ch)
+    tau = np.zeros((km, nch, nobs), dtype=np.float32, order="F")
+    ssa = np.zeros_like(tau)
+    g = np.zeros_like(tau)
+    pm = np.zeros((km, nch, nobs, nMom, nPol), dtype=np.float32, order="F")
+    rc = np.zeros(1, dtype=np.int32)
+    check(lib().aa_get_aop_vector(mie.handle, km, nobs, nch, nq, _p(ch), _vname(vname), int(verbose), _p(qm), _p(rh),
+                                  nMom, nPol, _p(tau), _p(ssa), _p(g), _p(pm), _p(rc, _ip)), "aa_get_aop_vector")
+    return tau, ssa, g, pm, int(rc[0])
+
+
+def get_aop_scalar(mie: MieTables, channels, vname: Sequence[str], verbose, qm, rh):
+    """VLIDORT_getAOPscalar: -> tau, ssa, g (km,nch,nobs), rc."""
+    ch = _f32(channels)
+    qm, rh = _f32(qm), _f32(rh)
+    km, nq, nobs = qm.shape
+    nch = len(ch)
+    tau = np.zeros((km, nch, nobs), dtype=np.float32, order="F")
+    ssa = np.zeros_like(tau)
+    g = np.zeros_like(tau)
+    rc = np.zeros(1, dtype=np.int32)
+    check(lib().aa_get_aop_scalar(mie.handle, km, nobs, nch, nq, _p(ch), _vname(vname), int(verbose), _p(qm), _p(rh),
+                                  _p(tau), _p(ssa), _p(g), _p(rc, _ip)), "aa_get_aop_scalar")
+    return tau, ssa, g, int(rc[0])

@@ -1,0 +1,509 @@
+// Chem_Mie optics query and per-tracer optics summation on MI355X (kernel a).
+//
+// Reference: CB/Chem_MieMod.F90:789-965 (Chem_MieQueryByInt),
+//            CB/Chem_MieTableMod.F90:612-627 (RH hash),
+//            GEO/VLIDORT_MieMod.F90:124-336 (VLIDORT_getAOPvector)
+// (CB = src/Shared/GMAO_Shared/Chem_Base, GEO = src/Components/rtms/geo_vlidort).
+//
+// The reference computes in IEEE float32 with a fixed operation order; the
+// kernels below use explicitly rounded float32 intrinsics (no contraction into
+// fused multiply-adds) in that same order, so outputs are bit-identical.
+//
+// Layout: one thread per (column, layer) element; the Fortran arrays
+// qm(km,nq,nobs), rh(km,nobs), tau(km,nch,nobs), pmom(km,nch,nobs,nmom,npol)
+// have the layer index fastest, so a wavefront reads and writes 64 consecutive
+// layers of the same column: every access is coalesced.  The tables (< 1 MB)
+// stay in L2.
+#include "aa_common.h"
+
+#include <algorithm>
+#include <cctype>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace aa {
+
+struct SnapF {
+  int irh, irhp1;   // 0-based table nodes
+  float arh;
+};
+
+// RH snap onto the 991-slot hash (CB/Chem_MieMod.F90:860-867).  The reference
+// evaluates rhUse = max(rh,0.) and then overwrites it with min(rh,0.99);
+// negative RH is caught by the isnap < 1 clamp.
+__device__ __forceinline__ SnapF rh_snap(const MieDev& T, int tr, float rh) {
+  float rhuse = fminf(rh, 0.99f);
+  float x = __fmul_rn(__fadd_rn(rhuse, 0.001f), 1000.0f);
+  int isnap = (int)x;   // Fortran INT(): truncation toward zero
+  if (isnap < 1) isnap = 1;
+  SnapF s;
+  s.arh = T.rha[tr * NRH_MAP + isnap - 1];
+  s.irh = T.rhi[tr * NRH_MAP + isnap - 1] - 1;
+  int nrh = T.nrh_t[tr];
+  s.irhp1 = (s.irh + 1 > nrh - 1) ? nrh - 1 : s.irh + 1;
+  return s;
+}
+
+// table(irh)*(1.-arh) + table(irhp1)*arh in float32, no contraction
+__device__ __forceinline__ float lin(float lo, float hi, float arh) {
+  return __fadd_rn(__fmul_rn(lo, __fsub_rn(1.0f, arh)), __fmul_rn(hi, arh));
+}
+
+constexpr int MCHUNK = 8;   // moments per accumulation pass
+constexpr int MAXPOL = 6;
+constexpr int MAXTR = 32;
+
+// VLIDORT_getAOPvector for a range of tracers [0, ntr_eff) (ntr_eff < nq only
+// when the reference would have returned early with rc 666/777, in which case
+// the partial, un-normalised sums are what the reference leaves behind).
+__global__ void __launch_bounds__(256) get_aop_kernel(MieDev T, int km, int nobs, int nch, int nq,
+                                                      const int* __restrict__ tracer_idx,
+                                                      const int* __restrict__ chan_idx,
+                                                      const float* __restrict__ qm, const float* __restrict__ rh,
+                                                      int nmom, int npol, int ntr_eff, int normalize,
+                                                      float* __restrict__ tau_o, float* __restrict__ ssa_o,
+                                                      float* __restrict__ g_o, float* __restrict__ pmom_o) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)km * nobs) return;
+  const int k = (int)(e % km);
+  const int i = (int)(e / km);
+  const float rhv = rh[(long)i * km + k];
+  for (int n = 0; n < nch; ++n) {
+    const int ich = chan_idx[n];
+    float tau = 0.0f, ssa = 0.0f, g = 0.0f;
+    // pass 1: tau, ssa*tau, g*ssa*tau in tracer order
+    for (int iq = 0; iq < ntr_eff; ++iq) {
+      const int tr = tracer_idx[iq];
+      if (tr < 0) continue;
+      SnapF s = rh_snap(T, tr, rhv);
+      const long tb = ((long)tr * T.nch + ich) * T.nrh;
+      float bext_in = lin(T.bext[tb + s.irh], T.bext[tb + s.irhp1], s.arh);
+      float bsca_in = lin(T.bsca[tb + s.irh], T.bsca[tb + s.irhp1], s.arh);
+      float g_in = lin(T.g[tb + s.irh], T.g[tb + s.irhp1], s.arh);
+      float q = qm[((long)i * nq + iq) * km + k];
+      float tau_ = __fmul_rn(bext_in, q);
+      float ssa_ = __fdiv_rn(bsca_in, bext_in);
+      tau = __fadd_rn(tau, tau_);
+      ssa = __fadd_rn(ssa, __fmul_rn(ssa_, tau_));
+      g = __fadd_rn(g, __fmul_rn(__fmul_rn(g_in, ssa_), tau_));
+    }
+    // normalisation (GEO/VLIDORT_MieMod.F90:320-334); note g and pmom use the
+    // already-normalised ssa
+    float st = 0.0f;
+    if (normalize) {
+      if (tau > 0.0f) ssa = __fdiv_rn(ssa, tau);
+      st = __fmul_rn(ssa, tau);
+      if (st > 0.0f) g = __fdiv_rn(g, st);
+    }
+    const long o = ((long)i * nch + n) * km + k;
+    tau_o[o] = tau; ssa_o[o] = ssa; g_o[o] = g;
+    if (pmom_o == nullptr) continue;
+    // pass 2: moments in chunks, all polarizations of a chunk together so the
+    // spherical P22 = P11 / P44 = P33 extension sees the running sums
+    const long ostride = (long)nobs * nch * km;   // stride of one moment
+    for (int m0 = 0; m0 < nmom; m0 += MCHUNK) {
+      float acc[MAXPOL][MCHUNK];
+#pragma unroll
+      for (int p = 0; p < MAXPOL; ++p)
+#pragma unroll
+        for (int j = 0; j < MCHUNK; ++j) acc[p][j] = 0.0f;
+      for (int iq = 0; iq < ntr_eff; ++iq) {
+        const int tr = tracer_idx[iq];
+        if (tr < 0) continue;
+        const int tpol = T.npol_t[tr];
+        const bool sph = (npol == 6 && tpol == 4);
+        const int npol_ = sph ? 4 : npol;
+        SnapF s = rh_snap(T, tr, rhv);
+        const long pb = ((long)tr * T.nch + ich) * T.nrh;
+        // tau_ and ssa_ of this tracer, recomputed exactly as in pass 1
+        const float bext_in = lin(T.bext[pb + s.irh], T.bext[pb + s.irhp1], s.arh);
+        const float bsca_in = lin(T.bsca[pb + s.irh], T.bsca[pb + s.irhp1], s.arh);
+        const float wt = __fmul_rn(bext_in, qm[((long)i * nq + iq) * km + k]);
+        const float ws = __fdiv_rn(bsca_in, bext_in);
+        const float* plo = T.pmom + (pb + s.irh) * (long)T.npol * T.nmom;
+        const float* phi = T.pmom + (pb + s.irhp1) * (long)T.npol * T.nmom;
+#pragma unroll
+        for (int p = 0; p < MAXPOL; ++p) {
+          if (p >= npol_) continue;
+#pragma unroll
+          for (int j = 0; j < MCHUNK; ++j) {
+            int m = m0 + j;
+            if (m >= nmom) continue;
+            float pm = lin(plo[p * T.nmom + m], phi[p * T.nmom + m], s.arh);
+            acc[p][j] = __fadd_rn(acc[p][j], __fmul_rn(__fmul_rn(pm, ws), wt));
+          }
+        }
+        if (sph) {
+#pragma unroll
+          for (int j = 0; j < MCHUNK; ++j) { acc[4][j] = acc[0][j]; acc[5][j] = acc[2][j]; }
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < MAXPOL; ++p) {
+        if (p >= npol) continue;
+#pragma unroll
+        for (int j = 0; j < MCHUNK; ++j) {
+          int m = m0 + j;
+          if (m >= nmom) continue;
+          float v = acc[p][j];
+          if (normalize && st > 0.0f) v = __fdiv_rn(v, st);
+          pmom_o[((long)p * nmom + m) * ostride + o] = v;
+        }
+      }
+    }
+  }
+}
+
+// Chem_MieQuery over n elements of one tracer / table channel.
+__global__ void mie_query_kernel(MieDev T, int tr, int ich, int n, const float* __restrict__ q,
+                                 const float* __restrict__ rh, float* tau, float* ssa, float* gasym, float* bext,
+                                 float* bsca, float* bbck, float* refr, float* refi, int nmom, int npol,
+                                 float* pmom) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  SnapF s = rh_snap(T, tr, rh[e]);
+  const long tb = ((long)tr * T.nch + ich) * T.nrh;
+  float bext_in = lin(T.bext[tb + s.irh], T.bext[tb + s.irhp1], s.arh);
+  float bsca_in = lin(T.bsca[tb + s.irh], T.bsca[tb + s.irhp1], s.arh);
+  if (tau) tau[e] = __fmul_rn(bext_in, q[e]);
+  if (ssa) ssa[e] = __fdiv_rn(bsca_in, bext_in);
+  if (bext) bext[e] = bext_in;
+  if (bsca) bsca[e] = bsca_in;
+  if (gasym) gasym[e] = lin(T.g[tb + s.irh], T.g[tb + s.irhp1], s.arh);
+  if (bbck) bbck[e] = lin(T.bbck[tb + s.irh], T.bbck[tb + s.irhp1], s.arh);
+  if (refr) refr[e] = lin(T.refr[tb + s.irh], T.refr[tb + s.irhp1], s.arh);
+  if (refi) refi[e] = lin(T.refi[tb + s.irh], T.refi[tb + s.irhp1], s.arh);
+  if (pmom) {
+    const float* plo = T.pmom + (tb + s.irh) * (long)T.npol * T.nmom;
+    const float* phi = T.pmom + (tb + s.irhp1) * (long)T.npol * T.nmom;
+    for (int p = 0; p < npol; ++p)
+      for (int m = 0; m < nmom; ++m)
+        pmom[((long)p * nmom + m) * n + e] = lin(plo[p * T.nmom + m], phi[p * T.nmom + m], s.arh);
+  }
+}
+
+}  // namespace aa
+
+// ====================================================================== host side
+struct aa_mie {
+  int nch = 0;
+  std::vector<float> channels_m;
+  // host staging (tracer-major), rebuilt on each add
+  int nrh = 0, nmom = 0, npol = 0;
+  std::vector<std::string> names;
+  std::vector<int> nrh_t, nmom_t, npol_t;
+  std::vector<float> bext, bsca, g, bbck, refr, refi, pmom;   // [tr][ch][rh], pmom [tr][ch][rh][pol][mom]
+  std::vector<int> rhi;
+  std::vector<float> rha;
+  // device copies
+  void* dmem[12] = {nullptr};
+  aa::MieDev dev;
+  bool dirty = true;
+};
+
+namespace {
+
+std::string norm_name(const char* s, size_t len) {
+  std::string n(s, len);
+  size_t p = n.find("::");
+  if (p != std::string::npos) n = n.substr(p + 2);
+  while (!n.empty() && (n.back() == ' ' || n.back() == '\0')) n.pop_back();
+  size_t b = 0;
+  while (b < n.size() && n[b] == ' ') ++b;
+  n = n.substr(b);
+  for (auto& c : n) c = (char)std::toupper((unsigned char)c);
+  return n;
+}
+
+// RH hash (CB/Chem_MieTableMod.F90:612-627), float32 arithmetic as the reference.
+void build_rh_map(const float* rh, int nrh, int* rhi, float* rha) {
+  for (int j = 0; j < aa::NRH_MAP; ++j) {
+    rhi[j] = 0; rha[j] = 0.0f;
+    for (int i = nrh - 1; i >= 0; --i) {
+      volatile float prod = rh[i] * 1000.0f;
+      int th = (int)prod;
+      if (j >= th) {
+        rhi[j] = i + 1;
+        if (i + 1 >= nrh) rha[j] = 0.0f;
+        else {
+          volatile float a = (float)j / 1000.0f;
+          volatile float num = a - rh[i];
+          volatile float den = rh[i + 1] - rh[i];
+          rha[j] = num / den;
+        }
+        break;
+      }
+    }
+  }
+}
+
+void free_dev(aa_mie* t) {
+  for (auto& p : t->dmem) { if (p) (void)hipFree(p); p = nullptr; }
+}
+
+int upload(aa_mie* t) {
+  if (!t->dirty) return 0;
+  free_dev(t);
+  auto put = [&](int slot, const void* src, size_t bytes) -> int {
+    if (bytes == 0) bytes = 4;
+    AA_CHECK(hipMalloc(&t->dmem[slot], bytes));
+    AA_CHECK(hipMemcpy(t->dmem[slot], src, bytes, hipMemcpyHostToDevice));
+    return 0;
+  };
+  int rc = 0;
+  rc |= put(0, t->bext.data(), t->bext.size() * 4);
+  rc |= put(1, t->bsca.data(), t->bsca.size() * 4);
+  rc |= put(2, t->g.data(), t->g.size() * 4);
+  rc |= put(3, t->bbck.data(), t->bbck.size() * 4);
+  rc |= put(4, t->refr.data(), t->refr.size() * 4);
+  rc |= put(5, t->refi.data(), t->refi.size() * 4);
+  rc |= put(6, t->pmom.data(), t->pmom.size() * 4);
+  rc |= put(7, t->rhi.data(), t->rhi.size() * 4);
+  rc |= put(8, t->rha.data(), t->rha.size() * 4);
+  rc |= put(9, t->nrh_t.data(), t->nrh_t.size() * 4);
+  rc |= put(10, t->nmom_t.data(), t->nmom_t.size() * 4);
+  rc |= put(11, t->npol_t.data(), t->npol_t.size() * 4);
+  if (rc) return -2;
+  aa::MieDev& d = t->dev;
+  d.ntracer = (int)t->names.size(); d.nch = t->nch; d.nrh = t->nrh; d.nmom = t->nmom; d.npol = t->npol;
+  d.bext = (const float*)t->dmem[0]; d.bsca = (const float*)t->dmem[1]; d.g = (const float*)t->dmem[2];
+  d.bbck = (const float*)t->dmem[3]; d.refr = (const float*)t->dmem[4]; d.refi = (const float*)t->dmem[5];
+  d.pmom = (const float*)t->dmem[6]; d.rhi = (const int*)t->dmem[7]; d.rha = (const float*)t->dmem[8];
+  d.nrh_t = (const int*)t->dmem[9]; d.nmom_t = (const int*)t->dmem[10]; d.npol_t = (const int*)t->dmem[11];
+  t->dirty = false;
+  return 0;
+}
+
+}  // namespace
+
+namespace aa {
+// Accessors used by the fused radiance path.
+int mie_upload(aa_mie* t) { return upload(t); }
+const MieDev& mie_dev(const aa_mie* t) { return t->dev; }
+
+int launch_get_aop(const aa_mie* t, int km, int nobs, int nch, int nq, const int* d_tracer_idx,
+                   const int* d_chan_idx, const float* d_qm, const float* d_rh, int nmom, int npol, int ntr_eff,
+                   int normalize, float* d_tau, float* d_ssa, float* d_g, float* d_pmom, hipStream_t st) {
+  long ne = (long)km * nobs;
+  if (ne == 0) return 0;
+  int threads = 256;
+  long blocks = (ne + threads - 1) / threads;
+  hipLaunchKernelGGL(get_aop_kernel, dim3((unsigned)blocks), dim3(threads), 0, st, t->dev, km, nobs, nch, nq,
+                     d_tracer_idx, d_chan_idx, d_qm, d_rh, nmom, npol, ntr_eff, normalize, d_tau, d_ssa, d_g,
+                     d_pmom);
+  AA_CHECK(hipGetLastError());
+  return 0;
+}
+}  // namespace aa
+
+extern "C" {
+
+int aa_mie_create(aa_mie** out, int nch, const float* channels_m) {
+  if (!out || nch <= 0 || !channels_m) return -1;
+  aa_mie* t = new aa_mie();
+  t->nch = nch;
+  t->channels_m.assign(channels_m, channels_m + nch);
+  *out = t;
+  return 0;
+}
+
+int aa_mie_add_table(aa_mie* t, int nrh, int nbin, int nmom, int npol, const float* rh, const float* bext,
+                     const float* bsca, const float* bbck, const float* g, const float* refr, const float* refi,
+                     const float* pmom, const char* bin_names) {
+  if (!t || nrh <= 0 || nbin <= 0 || nmom < 0 || npol <= 0 || npol > 6 || !rh || !bext || !bsca || !g ||
+      !bin_names)
+    return -1;
+  const int nch = t->nch;
+  const int old_tr = (int)t->names.size();
+  // common padded dimensions across tables
+  int new_nrh = std::max(t->nrh, nrh), new_nmom = std::max(t->nmom, nmom), new_npol = std::max(t->npol, npol);
+  auto regrid3 = [&](std::vector<float>& v) {   // [tr][ch][rh] with new nrh
+    if (new_nrh == t->nrh) return;
+    std::vector<float> nv((size_t)old_tr * nch * new_nrh, 0.0f);
+    for (int tr = 0; tr < old_tr; ++tr) for (int c = 0; c < nch; ++c) for (int r = 0; r < t->nrh; ++r)
+      nv[((size_t)tr * nch + c) * new_nrh + r] = v[((size_t)tr * nch + c) * t->nrh + r];
+    v.swap(nv);
+  };
+  regrid3(t->bext); regrid3(t->bsca); regrid3(t->g); regrid3(t->bbck); regrid3(t->refr); regrid3(t->refi);
+  if (new_nrh != t->nrh || new_nmom != t->nmom || new_npol != t->npol) {
+    std::vector<float> nv((size_t)old_tr * nch * new_nrh * new_npol * new_nmom, 0.0f);
+    for (int tr = 0; tr < old_tr; ++tr) for (int c = 0; c < nch; ++c) for (int r = 0; r < t->nrh; ++r)
+      for (int p = 0; p < t->npol; ++p) for (int m = 0; m < t->nmom; ++m)
+        nv[((((size_t)tr * nch + c) * new_nrh + r) * new_npol + p) * new_nmom + m] =
+            t->pmom[((((size_t)tr * nch + c) * t->nrh + r) * t->npol + p) * t->nmom + m];
+    t->pmom.swap(nv);
+  }
+  t->nrh = new_nrh; t->nmom = new_nmom; t->npol = new_npol;
+  for (int b = 0; b < nbin; ++b) {
+    t->names.push_back(norm_name(bin_names + 16 * b, 16));
+    t->nrh_t.push_back(nrh); t->nmom_t.push_back(nmom); t->npol_t.push_back(npol);
+    for (int c = 0; c < nch; ++c) for (int r = 0; r < new_nrh; ++r) {
+      size_t src = ((size_t)b * nch + c) * nrh + r;   // Fortran (nrh,nch,nbin)
+      bool in = r < nrh;
+      t->bext.push_back(in ? bext[src] : 0.0f);
+      t->bsca.push_back(in ? bsca[src] : 0.0f);
+      t->g.push_back(in ? g[src] : 0.0f);
+      t->bbck.push_back(in && bbck ? bbck[src] : 0.0f);
+      t->refr.push_back(in && refr ? refr[src] : 0.0f);
+      t->refi.push_back(in && refi ? refi[src] : 0.0f);
+    }
+    for (int c = 0; c < nch; ++c) for (int r = 0; r < new_nrh; ++r) for (int p = 0; p < new_npol; ++p)
+      for (int m = 0; m < new_nmom; ++m) {
+        float v = 0.0f;
+        if (pmom && r < nrh && p < npol && m < nmom)   // Fortran pmom(nrh,nch,nbin,nmom,npol)
+          v = pmom[((((size_t)p * nmom + m) * nbin + b) * nch + c) * nrh + r];
+        t->pmom.push_back(v);
+      }
+    std::vector<int> hi(aa::NRH_MAP);
+    std::vector<float> ha(aa::NRH_MAP);
+    build_rh_map(rh, nrh, hi.data(), ha.data());
+    t->rhi.insert(t->rhi.end(), hi.begin(), hi.end());
+    t->rha.insert(t->rha.end(), ha.begin(), ha.end());
+  }
+  t->dirty = true;
+  return 0;
+}
+
+int aa_mie_query_idx(const aa_mie* t, const char* qname) {
+  if (!t || !qname) return -1;
+  std::string n = norm_name(qname, std::strlen(qname));
+  for (size_t i = 0; i < t->names.size(); ++i) if (t->names[i] == n) return (int)i;
+  return -1;
+}
+
+int aa_mie_dims(const aa_mie* t, int* nch, int* nmom_min, int* npol_max, int* ntracer) {
+  if (!t) return -1;
+  int mm = 1 << 30;
+  for (int v : t->nmom_t) mm = std::min(mm, v);
+  if (nch) *nch = t->nch;
+  if (nmom_min) *nmom_min = t->names.empty() ? 0 : mm;
+  if (npol_max) *npol_max = t->npol;
+  if (ntracer) *ntracer = (int)t->names.size();
+  return 0;
+}
+
+int aa_mie_query(const aa_mie* tc, int idx, float channel, int n, const float* q_mass, const float* rh, float* tau,
+                 float* ssa, float* gasym, float* bext, float* bsca, float* bbck, float* refr, float* refi,
+                 int nmom, int npol, float* pmom) {
+  aa_mie* t = const_cast<aa_mie*>(tc);
+  if (!t || idx < 0 || idx >= (int)t->names.size() || n < 0 || !q_mass || !rh) return -1;
+  int ich = (int)lrintf(channel) - 1;
+  if (ich < 0 || ich >= t->nch) return -1;
+  if (pmom && (nmom > t->nmom_t[idx] || npol > t->npol_t[idx])) return -1;
+  if (n == 0) return 0;
+  if (upload(t)) return -2;
+  float* outs[9] = {tau, ssa, gasym, bext, bsca, bbck, refr, refi, pmom};
+  size_t bytes[9];
+  for (int i = 0; i < 8; ++i) bytes[i] = (size_t)n * 4;
+  bytes[8] = (size_t)n * (pmom ? nmom * npol : 0) * 4;
+  void* dq = nullptr; void* drh = nullptr; void* dout[9] = {nullptr};
+  AA_CHECK(hipMalloc(&dq, (size_t)n * 4));
+  AA_CHECK(hipMalloc(&drh, (size_t)n * 4));
+  AA_CHECK(hipMemcpy(dq, q_mass, (size_t)n * 4, hipMemcpyHostToDevice));
+  AA_CHECK(hipMemcpy(drh, rh, (size_t)n * 4, hipMemcpyHostToDevice));
+  for (int i = 0; i < 9; ++i) if (outs[i] && bytes[i]) AA_CHECK(hipMalloc(&dout[i], bytes[i]));
+  int threads = 256;
+  hipLaunchKernelGGL(aa::mie_query_kernel, dim3((n + threads - 1) / threads), dim3(threads), 0, 0, t->dev, idx, ich,
+                     n, (const float*)dq, (const float*)drh, (float*)dout[0], (float*)dout[1], (float*)dout[2],
+                     (float*)dout[3], (float*)dout[4], (float*)dout[5], (float*)dout[6], (float*)dout[7], nmom, npol,
+                     (float*)dout[8]);
+  AA_CHECK(hipGetLastError());
+  AA_CHECK(hipDeviceSynchronize());
+  for (int i = 0; i < 9; ++i) if (dout[i]) AA_CHECK(hipMemcpy(outs[i], dout[i], bytes[i], hipMemcpyDeviceToHost));
+  for (int i = 0; i < 9; ++i) if (dout[i]) (void)hipFree(dout[i]);
+  (void)hipFree(dq); (void)hipFree(drh);
+  return 0;
+}
+
+int aa_mie_destroy(aa_mie* t) {
+  if (!t) return -1;
+  free_dev(t);
+  delete t;
+  return 0;
+}
+
+// Host-pointer VLIDORT_getAOPvector.
+static int get_aop_host(aa_mie* t, int km, int nobs, int nch, int nq, const float* channels, const char* vname,
+                        int verbose, const float* qm, const float* rh, int nmom, int npol, float* tau, float* ssa,
+                        float* g, float* pmom, int* rc) {
+  if (!t || km <= 0 || nobs < 0 || nch <= 0 || nq <= 0 || nq > aa::MAXTR || !channels || !vname || !qm || !rh ||
+      !tau || !ssa || !g || !rc || npol > 6)
+    return -1;
+  *rc = 0;
+  const size_t ne = (size_t)km * nch * nobs;
+  std::fill(tau, tau + ne, 0.0f); std::fill(ssa, ssa + ne, 0.0f); std::fill(g, g + ne, 0.0f);
+  if (pmom) std::fill(pmom, pmom + ne * nmom * npol, 0.0f);
+  // channel matching in float32 (GEO/VLIDORT_MieMod.F90:210-225)
+  std::vector<int> chan(nch, -1);
+  for (int n = 0; n < nch; ++n)
+    for (int m = 0; m < t->nch; ++m) {
+      volatile float scaled = 1.0e9f * t->channels_m[m];
+      volatile float diff = channels[n] - scaled;
+      if (std::fabs(diff) < 1.0f) { chan[n] = m; break; }
+    }
+  for (int n = 0; n < nch; ++n) if (chan[n] < 0) { *rc = 99; return 0; }
+  // tracer resolution and the reference's early returns
+  std::vector<int> tidx(nq, -1);
+  int ntr_eff = nq, normalize = 1, rcv = 0;
+  for (int iq = 0; iq < nq; ++iq) {
+    char nm[17];
+    for (int c = 0; c < 16; ++c) nm[c] = vname[(size_t)c * nq + iq];
+    nm[16] = 0;
+    int id = aa_mie_query_idx(t, nm);
+    rcv = (id < 0) ? 1 : 0;
+    if (id < 0) continue;
+    if (verbose == 1) std::printf("[+] Adding %s contribution\n", t->names[id].c_str());
+    if (nmom > t->nmom_t[id]) { ntr_eff = iq; normalize = 0; rcv = 666; break; }
+    int tp = t->npol_t[id];
+    if (!(npol <= tp || (npol == 6 && tp == 4))) { ntr_eff = iq; normalize = 0; rcv = 777; break; }
+    tidx[iq] = id;
+  }
+  if (nobs == 0) { *rc = rcv; return 0; }
+  if (upload(t)) return -2;
+  void *dq, *drh, *dt, *ds, *dg, *dp = nullptr, *dti, *dci;
+  AA_CHECK(hipMalloc(&dq, (size_t)km * nq * nobs * 4));
+  AA_CHECK(hipMalloc(&drh, (size_t)km * nobs * 4));
+  AA_CHECK(hipMalloc(&dt, ne * 4)); AA_CHECK(hipMalloc(&ds, ne * 4)); AA_CHECK(hipMalloc(&dg, ne * 4));
+  if (pmom && nmom > 0) AA_CHECK(hipMalloc(&dp, ne * nmom * npol * 4));
+  AA_CHECK(hipMalloc(&dti, nq * 4)); AA_CHECK(hipMalloc(&dci, nch * 4));
+  AA_CHECK(hipMemcpy(dq, qm, (size_t)km * nq * nobs * 4, hipMemcpyHostToDevice));
+  AA_CHECK(hipMemcpy(drh, rh, (size_t)km * nobs * 4, hipMemcpyHostToDevice));
+  AA_CHECK(hipMemcpy(dti, tidx.data(), nq * 4, hipMemcpyHostToDevice));
+  AA_CHECK(hipMemcpy(dci, chan.data(), nch * 4, hipMemcpyHostToDevice));
+  int e = aa::launch_get_aop(t, km, nobs, nch, nq, (const int*)dti, (const int*)dci, (const float*)dq,
+                             (const float*)drh, nmom, npol, ntr_eff, normalize, (float*)dt, (float*)ds, (float*)dg,
+                             (float*)dp, 0);
+  if (e) return e;
+  AA_CHECK(hipDeviceSynchronize());
+  AA_CHECK(hipMemcpy(tau, dt, ne * 4, hipMemcpyDeviceToHost));
+  AA_CHECK(hipMemcpy(ssa, ds, ne * 4, hipMemcpyDeviceToHost));
+  AA_CHECK(hipMemcpy(g, dg, ne * 4, hipMemcpyDeviceToHost));
+  if (dp) AA_CHECK(hipMemcpy(pmom, dp, ne * nmom * npol * 4, hipMemcpyDeviceToHost));
+  for (void* p : {dq, drh, dt, ds, dg, dp, dti, dci}) if (p) (void)hipFree(p);
+  *rc = rcv;
+  return 0;
+}
+
+int aa_get_aop_vector(aa_mie* t, int km, int nobs, int nch, int nq, const float* channels, const char* vname,
+                      int verbose, const float* qm, const float* rh, int nmom, int npol, float* tau, float* ssa,
+                      float* g, float* pmom, int* rc) {
+  if (!pmom) return -1;
+  return get_aop_host(t, km, nobs, nch, nq, channels, vname, verbose, qm, rh, nmom, npol, tau, ssa, g, pmom, rc);
+}
+
+int aa_get_aop_scalar(aa_mie* t, int km, int nobs, int nch, int nq, const float* channels, const char* vname,
+                      int verbose, const float* qm, const float* rh, float* tau, float* ssa, float* g, int* rc) {
+  return get_aop_host(t, km, nobs, nch, nq, channels, vname, verbose, qm, rh, 0, 1, tau, ssa, g, nullptr, rc);
+}
+
+int aa_get_aop_vector_dev(aa_mie* t, int km, int nobs, int nch, int nq, const int* tracer_idx, const int* chan_idx,
+                          const float* d_qm, const float* d_rh, int nmom, int npol, float* d_tau, float* d_ssa,
+                          float* d_g, float* d_pmom, void* stream) {
+  if (!t || km <= 0 || nobs < 0 || nch <= 0 || nq <= 0 || nq > aa::MAXTR || npol > 6) return -1;
+  if (upload(t)) return -2;
+  return aa::launch_get_aop(t, km, nobs, nch, nq, tracer_idx, chan_idx, d_qm, d_rh, nmom, npol, nq, 1, d_tau, d_ssa,
+                            d_g, d_pmom, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,790 @@
+// Fourier-term kernels of the discrete-ordinate radiance path.
+//
+//   k_layer: one thread per (item, layer).  Homogeneous solutions
+//            (VLIDORT_QHOM_SOLUTION + ASYMTX, REG/vlidort_solutions.f90:150-1047,
+//            REG/vlidort_aux.f90:386-959), their normalisation
+//            (VLIDORT_QHOM_NORMS :1051-1163), user-angle solutions and
+//            multipliers (VLIDORT_UHOM_SOLUTION :1167-1505, HMULT :1651-1790),
+//            Green's-function beam solution (VLIDORT_GBEAM_SOLUTION :2649-2932)
+//            and the layer's upwelling source coefficients
+//            (REG/vlidort_PostProcessing.f90:139-451).
+//   k_bvp:   one wavefront per item.  The banded boundary-value problem
+//            (REG/vlidort_bvproblem.f90:147-2311): matrix rows are generated on
+//            the fly from the layer solutions, factorised by partial-pivoting
+//            band elimination with the same pivot choices as DGBTF2
+//            (REG/lapack_tools.f90:156-357) while the right-hand side is carried
+//            along (DGBTRS forward part, :7354); back-substitution then yields
+//            the integration constants column by column and the TOA source
+//            recursion (REG/vlidort_intensity.f90:145-643) and the Fourier
+//            convergence test (REG/vlidort_converge.f90:825-1231) are folded in.
+//
+// Band LU layout: lanes hold the columns of the active window (width
+// 2*KL+1, circular over Q = 64*CPL-1 virtual lanes, the last virtual lane
+// holds the right-hand side); every lane keeps the KL+1 candidate rows of its
+// column in registers.  The pivot search runs in the lane owning the pivot
+// column and the multipliers are broadcast with readlane, so a step needs no
+// LDS and no barrier.  Eliminated rows (U and the transformed right-hand
+// side) go to a per-item buffer that the back-substitution reads in reverse.
+#pragma once
+#include "rt_work.h"
+#include "rt_device.h"
+
+namespace aa {
+
+namespace {
+
+constexpr double DMS[4] = {1.0, 1.0, -1.0, -1.0};   // DMAT diagonal
+constexpr double VLIDORT_ACCURACY_D = (double)0.0001f;   // V283/VLIDORT_Mod.F90:232
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_readlane(lo, lane);
+  hi = __builtin_amdgcn_readlane(hi, lane);
+  return __hiloint2double(hi, lo);
+}
+
+// ------------------------------------------------------------------ k_layer
+template <int N, int S>
+__global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __restrict__ active,
+                                              const int* __restrict__ nactive) {
+  constexpr int n = N * S, LM = 2 * N;
+  constexpr LOff O(N, S);
+  constexpr QOff QO(N, S);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int km = W.km;
+  const int a = t / km, layer = t - a * km;
+  if (a >= *nactive) return;
+  const int it = active[a];
+  const int L = W.L_it[it];
+  if (layer >= L) return;
+  const int NMOM = W.nmoments;
+  const double* __restrict__ Qt = W.quad;
+  auto XQ = [&](int l, int i, int s1, int s2) -> double { return Qt[QO.xq(M, l, i, s1, s2)]; };
+  auto pisign = [&](int l) -> double { return (((l - M) & 1) == 0) ? 1.0 : -1.0; };
+  auto inl = [&](int l) -> bool { return l >= M && l <= NMOM; };
+  double qs[N], qw[N], qhw[N], qsw[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    qs[i] = Qt[QO.qs(i)]; qw[i] = Qt[QO.qw(i)];
+    qhw[i] = 0.5 * qw[i]; qsw[i] = qs[i] * qw[i];
+  }
+  const size_t il = (size_t)it * km + layer;
+  const double* trr = W.tr + il * TR_NFIELDS;
+  const double dtau = trr[TR_DTAU], tduser = trr[TR_TDUSER], avsec = trr[TR_AVSEC], itrans = trr[TR_ITRANS];
+  const double tdmubar = trr[TR_TDMUBAR], itransu = trr[TR_ITRANSU], emultup = trr[TR_EMULTUP], sigp = trr[TR_SIGP];
+  const bool lsc = (M < 3) ? true : (((W.lsc[il] >> M) & 1u) != 0u);
+  const bool beam = (layer + 1 <= W.beam_cutoff[it]) && lsc;
+  const double usec = W.usec[it];
+  double og[LM][S][S];
+  {
+    const double* ogp = W.og + il * (size_t)(LM * S * S);
+#pragma unroll
+    for (int l = 0; l < LM; ++l)
+#pragma unroll
+      for (int o1 = 0; o1 < S; ++o1)
+#pragma unroll
+        for (int o2 = 0; o2 < S; ++o2) og[l][o1][o2] = ogp[(l * S + o1) * S + o2];
+  }
+
+  // ---- SAB / DAB (REG/vlidort_solutions.f90:439-475)
+  // (dp, dm) of stream pair (i,o1), (j,o2)
+  auto dpdm = [&](int i, int o1, int j, int o2, double& dp, double& dm) {
+    dp = 0.0; dm = 0.0;
+#pragma unroll
+    for (int l = 0; l < LM; ++l) {
+      if (!inl(l)) continue;
+      double sp = 0.0, sm = 0.0;
+#pragma unroll
+      for (int o = 0; o < S; ++o) {
+        double h2p = 0.0, h2m = 0.0;
+#pragma unroll
+        for (int o3 = 0; o3 < S; ++o3) {
+          const double xqv = XQ(l, j, o3, o2);
+          h2p += og[l][o][o3] * xqv;
+          h2m += og[l][o][o3] * (DMS[o3] * xqv * pisign(l));
+        }
+        const double xi = XQ(l, i, o1, o);
+        sp += xi * h2p; sm += xi * h2m;
+      }
+      dp += sp; dm += sm;
+    }
+  };
+  double sab[n][n];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const double xinv = -1.0 / qs[i];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const double fac = xinv * qhw[j];
+#pragma unroll
+      for (int o1 = 0; o1 < S; ++o1)
+#pragma unroll
+        for (int o2 = 0; o2 < S; ++o2) {
+          double dp, dm; dpdm(i, o1, j, o2, dp, dm);
+          sab[i * S + o1][j * S + o2] = fac * (dp + dm);
+        }
+    }
+#pragma unroll
+    for (int o1 = 0; o1 < S; ++o1) sab[i * S + o1][i * S + o1] -= xinv;
+  }
+
+  // ---- eigensystem of DAB.SAB with the reference's tolerance retries (:545-575)
+  double evec[n][n], eval[n];
+  int fail = 0, ier = 0;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    const double tol = attempt == 0 ? ASYMTX_TOL : (attempt == 1 ? 1.0e-10 : 1.0e-8);
+    double am[n][n];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double xinv = -1.0 / qs[i];
+#pragma unroll
+      for (int o1 = 0; o1 < S; ++o1) {
+        const int r = i * S + o1;
+        double drow[n];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          const double fac = xinv * qhw[j];
+#pragma unroll
+          for (int o2 = 0; o2 < S; ++o2) {
+            double dp, dm; dpdm(i, o1, j, o2, dp, dm);
+            drow[j * S + o2] = fac * (dp - dm);
+          }
+        }
+        drow[r] -= xinv;
+#pragma unroll
+        for (int c = 0; c < n; ++c) {
+          double s = 0.0;
+#pragma unroll
+          for (int o = 0; o < S; ++o)
+#pragma unroll
+            for (int j = 0; j < N; ++j) s += drow[j * S + o] * sab[j * S + o][c];
+          am[r][c] = s;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < n; ++r)
+#pragma unroll
+      for (int c = 0; c < n; ++c) evec[r][c] = 0.0;
+    ier = asymtx_dev<n>(tol, am, evec, eval, &fail);
+    if (fail || ier == 0) break;
+  }
+  if (fail || ier) { W.status[it] = 4; return; }
+
+  // ---- normalised eigenvectors, X+ / X- (:580-700)
+#pragma unroll
+  for (int c = 0; c < n; ++c) {
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < n; ++r) s += evec[r][c] * evec[r][c];
+    const double nr = sqrt(s);
+#pragma unroll
+    for (int r = 0; r < n; ++r) evec[r][c] = evec[r][c] / nr;
+  }
+  double keig[n], tdel[n];
+  double xp[n][n], xn[n][n];   // X(i,o,k) for i < N at row i*S+o, column k
+#pragma unroll
+  for (int k = 0; k < n; ++k) {
+    const double kv = sqrt(eval[k]);
+    keig[k] = kv;
+    const double sc_r = 1.0 / kv;
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < n; ++r) s += evec[r][k] * evec[r][k];
+    const double norm_r = sqrt(s);
+    double sum[n];
+#pragma unroll
+    for (int r = 0; r < n; ++r) sum[r] = evec[r][k] / norm_r;
+#pragma unroll
+    for (int r = 0; r < n; ++r) {
+      double h = 0.0;
+#pragma unroll
+      for (int o = 0; o < S; ++o)
+#pragma unroll
+        for (int j = 0; j < N; ++j) h += sab[r][j * S + o] * sum[j * S + o];
+      const double dif = h * sc_r;
+      xp[r][k] = 0.5 * (sum[r] + dif);
+      xn[r][k] = 0.5 * (sum[r] - dif);
+    }
+    const double help = kv * dtau;
+    tdel[k] = (help > MAX_TAU_PATH) ? 0.0 : exp(-help);
+  }
+  // X for the lower half-space streams (i + N)
+  auto XPF = [&](int i, int o, int k) -> double {
+    if (i < N) return xp[i * S + o][k];
+    const int ii = i - N;
+    return (o < 2) ? xn[ii * S + o][k] : -xn[ii * S + o][k];
+  };
+  auto XNF = [&](int i, int o, int k) -> double {
+    if (i < N) return xn[i * S + o][k];
+    const int ii = i - N;
+    return (o < 2) ? xp[ii * S + o][k] : -xp[ii * S + o][k];
+  };
+
+  // ---- PI matrices at the user stream and the solar cosine
+  const int obs = (W.item0 + it) / W.nch;
+  const double ustream = cos(DEG_TO_RAD * W.vza[obs]);
+  double xup[LM][S][S], x0p[LM][S][S];
+  pimatrix_x<S, LM>(M, NMOM, ustream, xup);
+  pimatrix_x<S, LM>(M, NMOM, W.csza[it], x0p);
+
+  // ---- UHOM + HMULT (:1167-1505, :1651-1790)
+  double updn[n][S], upup[n][S], hm1[n], hm2[n];
+#pragma unroll
+  for (int k = 0; k < n; ++k) {
+    const double rho_p = usec + keig[k], rho_m = usec - keig[k];
+    const double zeta_p = 1.0 / rho_p;
+    const bool hsingo = fabs(rho_m) < TAYLOR_SMALL;
+    const double zeta_m = hsingo ? rho_m : 1.0 / rho_m;
+    if (!lsc) {
+#pragma unroll
+      for (int o = 0; o < S; ++o) { updn[k][o] = 0.0; upup[k][o] = 0.0; }
+    } else {
+      double gaux[LM][S];
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        double hs[S];
+#pragma unroll
+        for (int o1 = 0; o1 < S; ++o1) {
+          double sps = 0.0;
+          if (inl(l)) {
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+              const double a5 = qhw[j];
+              double sp = 0.0, sm = 0.0;
+#pragma unroll
+              for (int o = 0; o < S; ++o) {
+                const double xqv = XQ(l, j, o1, o);
+                sp += xqv * a5 * xp[j * S + o][k];
+                sm += (DMS[o1] * xqv * pisign(l)) * a5 * xn[j * S + o][k];
+              }
+              sps = sps + sp + sm;
+            }
+          }
+          hs[o1] = sps;
+        }
+#pragma unroll
+        for (int o1 = 0; o1 < S; ++o1) {
+          double s = 0.0;
+#pragma unroll
+          for (int o = 0; o < S; ++o) s += og[l][o1][o] * hs[o];
+          gaux[l][o1] = s;
+        }
+      }
+#pragma unroll
+      for (int o1 = 0; o1 < S; ++o1) {
+        double sdn = 0.0, sup = 0.0;
+#pragma unroll
+        for (int o = 0; o < S; ++o)
+#pragma unroll
+          for (int l = 0; l < LM; ++l) {
+            if (!inl(l)) continue;
+            const double xum = DMS[o1] * xup[l][o1][o] * DMS[o] * pisign(l);
+            const double xupre = DMS[o1] * xup[l][o1][o];
+            sdn += xum * gaux[l][o];
+            sup += xupre * gaux[l][o];
+          }
+        updn[k][o1] = sdn; upup[k][o1] = sup;
+      }
+    }
+    const double udel = tduser;
+    const double zdel = tdel[k], zudel = zdel * udel;
+    const double th2 = 1.0 - zudel, th1 = zdel - udel;
+    hm1[k] = hsingo ? taylor1(zeta_m, dtau, udel, usec) : usec * th1 * zeta_m;
+    hm2[k] = usec * th2 * zeta_p;
+  }
+
+  // ---- norms (:1051-1163)
+  double normv[n];
+#pragma unroll
+  for (int k = 0; k < n; ++k) {
+    double nv = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double t1 = 0.0, t2 = 0.0;
+#pragma unroll
+      for (int o = 0; o < S; ++o) { t1 += xp[j * S + o][k] * xp[j * S + o][k]; t2 += xn[j * S + o][k] * xn[j * S + o][k]; }
+      nv = nv + qsw[j] * (t1 - t2);
+    }
+    normv[k] = nv;
+  }
+
+  // ---- Green's-function beam solution (:2649-2932)
+  double wu[2 * N][S], wl[2 * N][S];
+  double aterm[n], bterm[n], gamp[n], gamm[n];
+#pragma unroll
+  for (int k = 0; k < n; ++k) { aterm[k] = 0.0; bterm[k] = 0.0; gamp[k] = 0.0; gamm[k] = 0.0; }
+#pragma unroll
+  for (int i = 0; i < 2 * N; ++i)
+#pragma unroll
+    for (int o = 0; o < S; ++o) { wu[i][o] = 0.0; wl[i][o] = 0.0; }
+  if (beam) {
+    const int g = W.item0 + it;
+    const int ch = g - obs * W.nch;
+    const double F1 = (W.flux ? W.flux[(size_t)obs * W.nch + ch] : 1.0) / PI4;
+    const double secbar = avsec, cst = itrans, wdel = tdmubar;
+    double cfunc[n], dfunc[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+      const double zdel = tdel[k], zwdel = zdel * wdel;
+      gamp[k] = secbar + keig[k];
+      gamm[k] = secbar - keig[k];
+      const double eps = gamm[k];
+      if (fabs(eps) < TAYLOR_SMALL) cfunc[k] = taylor1(eps, dtau, wdel, cst);
+      else cfunc[k] = cst * (zdel - wdel) / gamm[k];
+      dfunc[k] = cst * (1.0 - zwdel) / gamp[k];
+    }
+    double hq[LM][S];
+#pragma unroll
+    for (int o1 = 0; o1 < S; ++o1)
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        double gsum = 0.0;
+#pragma unroll
+        for (int o2 = 0; o2 < S; ++o2) gsum = gsum + og[l][o1][o2] * x0p[l][o2][0];
+        hq[l][o1] = gsum;
+      }
+    double dpi[N][S], dmi[N][S];
+#pragma unroll
+    for (int o1 = 0; o1 < S; ++o1)
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        double sp = 0.0, sm = 0.0;
+#pragma unroll
+        for (int o = 0; o < S; ++o)
+#pragma unroll
+          for (int l = 0; l < LM; ++l) {
+            if (!inl(l)) continue;
+            const double xqv = XQ(l, i, o1, o);
+            sp += xqv * hq[l][o];
+            sm += (xqv * DMS[o] * pisign(l)) * hq[l][o];
+          }
+        dpi[i][o1] = F1 * sp; dmi[i][o1] = F1 * sm;
+      }
+    double gup[n], gdn[n];
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+      double av = 0.0, bv = 0.0;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        double h1 = 0.0, h2 = 0.0, h1b = 0.0, h2b = 0.0;
+#pragma unroll
+        for (int o = 0; o < S; ++o) h1 += dpi[i][o] * xp[i * S + o][k];
+#pragma unroll
+        for (int o = 0; o < S; ++o) h1b += dmi[i][o] * xn[i * S + o][k];
+        h1 = h1 + h1b;
+#pragma unroll
+        for (int o = 0; o < S; ++o) h2 += dmi[i][o] * xp[i * S + o][k];
+#pragma unroll
+        for (int o = 0; o < S; ++o) h2b += dpi[i][o] * xn[i * S + o][k];
+        h2 = h2 + h2b;
+        av += qw[i] * h1; bv += qw[i] * h2;
+      }
+      aterm[k] = av / normv[k];
+      bterm[k] = bv / normv[k];
+      gdn[k] = cfunc[k] * aterm[k];
+      gup[k] = dfunc[k] * bterm[k];
+    }
+#pragma unroll
+    for (int o1 = 0; o1 < S; ++o1)
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        double su = 0.0, su1 = 0.0, sl1 = 0.0, sl = 0.0;
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+          su += gup[k] * xn[i * S + o1][k]; su1 += gup[k] * xp[i * S + o1][k];
+          sl1 += gdn[k] * xn[i * S + o1][k]; sl += gdn[k] * xp[i * S + o1][k];
+        }
+        wu[i][o1] = su; wu[i + N][o1] = (o1 > 1) ? -su1 : su1;
+        wl[i + N][o1] = (o1 > 1) ? -sl1 : sl1; wl[i][o1] = sl;
+      }
+  }
+
+  // ---- write the layer record
+  double* rec = W.lbuf + il * (size_t)W.lstride;
+#pragma unroll
+  for (int i = 0; i < 2 * N; ++i)
+#pragma unroll
+    for (int o = 0; o < S; ++o)
+#pragma unroll
+      for (int k = 0; k < n; ++k) rec[O.xp(i, o, k)] = XPF(i, o, k);
+#pragma unroll
+  for (int i = 0; i < 2 * N; ++i)
+#pragma unroll
+    for (int o = 0; o < S; ++o)
+#pragma unroll
+      for (int k = 0; k < n; ++k) rec[O.xn(i, o, k)] = XNF(i, o, k);
+#pragma unroll
+  for (int k = 0; k < n; ++k) rec[O.tdel(k)] = tdel[k];
+#pragma unroll
+  for (int i = 0; i < 2 * N; ++i)
+#pragma unroll
+    for (int o = 0; o < S; ++o) { rec[O.wu(i, o)] = wu[i][o]; rec[O.wl(i, o)] = wl[i][o]; }
+  // upwelling source: src(o) = sum_k lcon_k A(o,k) + mcon_k B(o,k) + C(o)
+#pragma unroll
+  for (int o = 0; o < S; ++o) {
+    double spar = 0.0;
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+      rec[O.coef(o, k)] = lsc ? updn[k][o] * hm2[k] : 0.0;
+      rec[O.coef(o, n + k)] = lsc ? upup[k][o] * hm1[k] : 0.0;
+      if (beam) {
+        const double wdel = tdmubar, itw = -itrans * wdel;
+        double sd;
+        if (fabs(gamm[k]) < TAYLOR_SMALL) sd = itransu * taylor2(gamm[k], sigp, dtau, 1.0, wdel * tduser, 1.0);
+        else sd = (itrans * hm2[k] - emultup) / gamm[k];
+        const double su = (itw * hm1[k] + emultup) / gamp[k];
+        spar = spar + updn[k][o] * sd * aterm[k] + upup[k][o] * su * bterm[k];
+      }
+    }
+    rec[O.csrc(o)] = beam ? spar : 0.0;
+  }
+  // ---- Lambertian surface terms of the bottom layer at Fourier 0
+  //      (BVP_SURFACE_SETUP_HOM/BEAM, REG/vlidort_bvproblem.f90; GET_BOA_SOURCE,
+  //      REG/vlidort_intensity.f90:1096-1496)
+  if (layer == L - 1 && M == 0) {
+    const int g = W.item0 + it;
+    const int ch = g - obs * W.nch;
+    const double albedo = W.albedo[(size_t)ch * W.nobs + obs];
+    if (albedo != 0.0) {
+      constexpr BOff BO(n);
+      double* bb = W.bbuf + (size_t)it * W.bstride;
+      const double kmult = 2.0 * albedo;
+#pragma unroll
+      for (int k = 0; k < n; ++k) {
+        double sp = 0.0, sm = 0.0, ba = 0.0, bbv = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          sp += qsw[j] * xp[j * S][k]; sm += qsw[j] * xn[j * S][k];
+          ba += qsw[j] * xp[j * S][k] * tdel[k]; bbv += qsw[j] * xn[j * S][k];
+        }
+        bb[BO.r2p(k)] = sp * kmult; bb[BO.r2m(k)] = sm * kmult;
+        bb[BO.bcoef(k)] = ba; bb[BO.bcoef(n + k)] = bbv;
+      }
+      double rb = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) rb += qsw[j] * wl[j][0];
+      bb[BO.r2beam()] = rb * kmult;
+      bb[BO.bc()] = rb;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ k_bvp
+template <int N, int S>
+__global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restrict__ active,
+                                            const int* __restrict__ nactive, int* __restrict__ next,
+                                            int* __restrict__ nnext) {
+  constexpr int n = N * S, n2 = 2 * n, KL = 3 * n - 1, R = KL + 1, WD = 2 * KL + 1;
+  constexpr int CPL = (WD + 1 > 64) ? 2 : 1;
+  constexpr int P = 64 * CPL, Q = P - 1;
+  constexpr int PF = 4;    // back-substitution: right-hand sides loaded PF steps ahead
+  static_assert(WD + PF + 1 <= P, "window does not fit the wavefront");
+  constexpr LOff O(N, S);
+  constexpr BOff BO(n);
+  constexpr int BIG = 0x3fffffff;
+  const int ai = blockIdx.x;
+  if (ai >= *nactive) return;
+  const int it = active[ai];
+  const int lane = threadIdx.x;
+  const int km = W.km;
+  const int g = W.item0 + it, obs = g / W.nch, ch = g - obs * W.nch;
+  const size_t oi = (size_t)ch * W.nobs + obs;
+  if (W.status[it] != 0) {
+    if (lane == 0) {
+      W.radiance[oi] = W.missing; W.reflectance[oi] = W.missing;
+      if (S > 1) { W.qout[oi] = W.missing; W.uout[oi] = W.missing; }
+      W.nfourier[oi] = 0;
+    }
+    return;
+  }
+  const int L = W.L_it[it];
+  const int NT = n2 * L;
+  const int LS = W.lstride, UW = W.ustride;
+  const double* __restrict__ lb = W.lbuf + (size_t)it * km * LS;
+  const double* __restrict__ bb = W.bbuf + (size_t)it * W.bstride;
+  double* __restrict__ ub = W.ubuf + (size_t)it * (size_t)n2 * km * UW;
+  const double albedo = W.albedo[oi];
+  const bool surf = (M == 0) && (albedo != 0.0);
+  const double tsb = W.tsb[it];
+  const double ff = W.flux ? W.flux[(size_t)obs * W.nch + ch] : 1.0;
+  double rf_direct = 0.0;
+  if (surf && tsb != 0.0) {   // DIRECTRADIANCE (REG/vlidort_miscsetups.f90:1296-1614), Lambertian
+    double x0 = 4.0 * W.csza[it] / 1.0;
+    x0 = ff * x0 / PI4;
+    rf_direct = x0 * tsb * albedo;
+  }
+  const double r2beam = surf ? bb[BO.r2beam()] : 0.0;
+
+  // BVP matrix element (row r, column c), 0-based (BVP_MATRIX_SETUP,
+  // REG/vlidort_bvproblem.f90:627-1176)
+  auto elem = [&](int r, int c) -> double {
+    if (r < n) {
+      const int i = r / S, o1 = r - (r / S) * S;
+      if (c < n) return lb[O.xp(i, o1, c)];
+      if (c < n2) return lb[O.xn(i, o1, c - n)] * lb[O.tdel(c - n)];
+      return 0.0;
+    }
+    if (r >= NT - n) {
+      const int rr = r - (NT - n);
+      const int i = rr / S, o1 = rr - (rr / S) * S;
+      const int q = c - (NT - n2);
+      if (q < 0 || q >= n2) return 0.0;
+      const double* lr = lb + (size_t)(L - 1) * LS;
+      if (q < n) {
+        double xpn = lr[O.xp(i + N, o1, q)];
+        if (surf && o1 == 0) xpn = xpn - bb[BO.r2p(q)];
+        return lr[O.tdel(q)] * xpn;
+      }
+      double xmn = lr[O.xn(i + N, o1, q - n)];
+      if (surf && o1 == 0) xmn = xmn - bb[BO.r2m(q - n)];
+      return xmn;
+    }
+    const int tt = r - n;
+    const int nl0 = tt / n2, irow = tt - nl0 * n2;
+    const int i = irow / S, o1 = irow - (irow / S) * S;
+    const int q = c - nl0 * n2;
+    if (q < 0 || q >= 2 * n2) return 0.0;
+    const double* lu = lb + (size_t)nl0 * LS;
+    const double* ll = lu + LS;
+    if (q < n) return lu[O.tdel(q)] * lu[O.xp(i, o1, q)];
+    if (q < n2) return lu[O.xn(i, o1, q - n)];
+    if (q < 3 * n) return -ll[O.xp(i, o1, q - n2)];
+    return -ll[O.tdel(q - 3 * n)] * ll[O.xn(i, o1, q - 3 * n)];
+  };
+  // right-hand side (BVP_COLUMN_SETUP + BVP_SURFACE_SETUP_BEAM)
+  auto rhs = [&](int r) -> double {
+    if (r < n) { const int i = r / S, o1 = r - (r / S) * S; return -lb[O.wu(i, o1)]; }
+    if (r >= NT - n) {
+      const int rr = r - (NT - n);
+      const int i = rr / S, o1 = rr - (rr / S) * S;
+      double v = -lb[(size_t)(L - 1) * LS + O.wl(i + N, o1)];
+      if (surf && o1 == 0) v = v + r2beam;
+      if (surf && o1 == 0) v = v + rf_direct;
+      return v;
+    }
+    const int tt = r - n;
+    const int nl0 = tt / n2, irow = tt - nl0 * n2;
+    const int i = irow / S, o1 = irow - (irow / S) * S;
+    return lb[(size_t)(nl0 + 1) * LS + O.wu(i, o1)] - lb[(size_t)nl0 * LS + O.wl(i, o1)];
+  };
+  // value of row r at the column held by virtual lane v when the window starts
+  // at column j0 (jm0 = j0 mod Q)
+  auto gen = [&](int v, int r, int j0, int jm0) -> double {
+    if (r >= NT) return 0.0;
+    if (v == Q) return rhs(r);
+    int off = v - jm0;
+    if (off < 0) off += Q;
+    const int c = j0 + off;
+    if (off >= WD || c >= NT) return 0.0;
+    return elem(r, c);
+  };
+
+  // ---------------- band LU with partial pivoting + forward substitution
+  double val[CPL][R];
+  int pos[R];
+#pragma unroll
+  for (int s = 0; s < R; ++s) pos[s] = (s < NT) ? s : BIG;
+#pragma unroll
+  for (int cs = 0; cs < CPL; ++cs) {
+    const int v = lane + 64 * cs;
+#pragma unroll
+    for (int s = 0; s < R; ++s) val[cs][s] = gen(v, s, 0, 0);
+  }
+  bool singular = false;
+  int jm = 0;   // j mod Q
+  for (int j = 0; j < NT; ++j) {
+    const int jm1 = (jm + 1 == Q) ? 0 : jm + 1;
+    const int rnew = j + R;
+    double nv[CPL];
+#pragma unroll
+    for (int cs = 0; cs < CPL; ++cs) nv[cs] = gen(lane + 64 * cs, rnew, j + 1, jm1);
+    // pivot column j lives in virtual lane jm
+    const int lj = jm & 63, csj = jm >> 6;
+    double x[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) x[s] = (CPL == 1 || csj == 0) ? val[0][s] : val[CPL - 1][s];
+    double best = -1.0;
+    int bs = 0, bpos = BIG;
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      const double av = fabs(x[s]);
+      const bool better = (pos[s] < BIG) && (av > best || (av == best && pos[s] < bpos));
+      if (better) { best = av; bpos = pos[s]; bs = s; }
+    }
+    const int bsu = __builtin_amdgcn_readlane(bs, lj);
+    const int jp = __builtin_amdgcn_readlane(bpos, lj);
+    double xpv = 0.0;
+#pragma unroll
+    for (int s = 0; s < R; ++s) if (s == bsu) xpv = x[s];
+    const double pv = readlane_d(xpv, lj);
+    if (pv == 0.0) singular = true;
+    const double rp = 1.0 / pv;
+    double lm[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) lm[s] = (s == bsu) ? 0.0 : readlane_d(x[s], lj) * rp;
+    // eliminate, store the pivot row (U row j and transformed RHS)
+#pragma unroll
+    for (int cs = 0; cs < CPL; ++cs) {
+      double u = 0.0;
+#pragma unroll
+      for (int s = 0; s < R; ++s) if (s == bsu) u = val[cs][s];
+#pragma unroll
+      for (int s = 0; s < R; ++s) val[cs][s] = val[cs][s] - lm[s] * u;
+      const int v = lane + 64 * cs;
+      if (v == Q) ub[(size_t)j * UW + WD] = u;
+      else {
+        int off = v - jm;
+        if (off < 0) off += Q;
+        if (off < WD && j + off < NT) ub[(size_t)j * UW + off] = u;
+      }
+    }
+    // row interchange bookkeeping: the row that sat at position j moves to jp
+#pragma unroll
+    for (int s = 0; s < R; ++s) if (pos[s] == j && s != bsu) pos[s] = jp;
+    // the pivot slot takes the next row; a column entering the window starts at zero
+#pragma unroll
+    for (int s = 0; s < R; ++s) if (s == bsu) pos[s] = (rnew < NT) ? rnew : BIG;
+#pragma unroll
+    for (int cs = 0; cs < CPL; ++cs) {
+      const int v = lane + 64 * cs;
+      bool entering = false;
+      if (v != Q) {
+        int off1 = v - jm1;
+        if (off1 < 0) off1 += Q;
+        entering = (off1 == WD - 1);
+      }
+#pragma unroll
+      for (int s = 0; s < R; ++s) val[cs][s] = (s == bsu) ? nv[cs] : (entering ? 0.0 : val[cs][s]);
+    }
+    jm = jm1;
+  }
+
+  // ---------------- back-substitution fused with the TOA source recursion
+  // lanes hold pending right-hand sides: virtual lane v <-> rows i = v mod P
+  double bvv[CPL];
+#pragma unroll
+  for (int cs = 0; cs < CPL; ++cs) {
+    const int v = lane + 64 * cs;
+    const int i = (NT - 1) - (((NT - 1) - v) & (P - 1));
+    bvv[cs] = (i >= 0 && i >= NT - WD - PF) ? ub[(size_t)i * UW + WD] : 0.0;
+  }
+  const double kmult = 2.0 * albedo;
+  double acc[S], cum[S];
+#pragma unroll
+  for (int o = 0; o < S; ++o) { acc[o] = 0.0; cum[o] = 0.0; }
+  double bacc = 0.0;
+  const double* trr = W.tr + (size_t)it * km * TR_NFIELDS;
+  for (int j = NT - 1; j >= 0; --j) {
+    const int jv = j & (P - 1), lj = jv & 63, csj = jv >> 6;
+    // coefficients of row j of U for the pending rows (issued before the chain)
+    double uij[CPL];
+    const int lo = (j - WD + 1 > 0) ? j - WD + 1 : 0;
+#pragma unroll
+    for (int cs = 0; cs < CPL; ++cs) {
+      const int v = lane + 64 * cs;
+      const int i = j - ((j - v) & (P - 1));
+      uij[cs] = (i >= lo && i < j) ? ub[(size_t)i * UW + (j - i)] : 0.0;
+    }
+    const int inew = j - WD - PF;
+    double ynew = 0.0;
+    if (inew >= 0) ynew = ub[(size_t)inew * UW + WD];
+    const double bsel = (CPL == 1 || csj == 0) ? bvv[0] : bvv[CPL - 1];
+    const double bj = readlane_d(bsel, lj);
+    const double xj = bj / ub[(size_t)j * UW];
+#pragma unroll
+    for (int cs = 0; cs < CPL; ++cs) bvv[cs] = bvv[cs] - xj * uij[cs];
+    if (inew >= 0) {
+      const int vn = inew & (P - 1);
+#pragma unroll
+      for (int cs = 0; cs < CPL; ++cs) if (lane + 64 * cs == vn) bvv[cs] = ynew;
+    }
+    // TOA source: layer nl, constant index q (lcon for q < n, mcon after)
+    const int nl = j / n2, q = j - nl * n2;
+    const double* lr = lb + (size_t)nl * LS;
+#pragma unroll
+    for (int o = 0; o < S; ++o) acc[o] += xj * lr[O.coef(o, q)];
+    if (surf && nl == L - 1) bacc += xj * bb[BO.bcoef(q)];
+    if (q == 0) {
+      if (nl == L - 1) {
+#pragma unroll
+        for (int o = 0; o < S; ++o) cum[o] = 0.0;
+        if (surf) cum[0] = kmult * (bacc + bb[BO.bc()]);
+      }
+      const double tdu = trr[(size_t)nl * TR_NFIELDS + TR_TDUSER];
+#pragma unroll
+      for (int o = 0; o < S; ++o) {
+        const double src = acc[o] + lr[O.csrc(o)];
+        cum[o] = src + tdu * cum[o];
+        acc[o] = 0.0;
+      }
+    }
+  }
+
+  // ---------------- Fourier convergence (VLIDORT_CONVERGE_OBSGEO)
+  if (lane != 0) return;
+  const double flux_multiplier = (M == 0) ? 1.0 : 2.0;
+  double* stk = W.stokes + (size_t)it * 3;
+  bool done = false;
+  if (singular) {
+    W.status[it] = 4;
+    W.radiance[oi] = W.missing; W.reflectance[oi] = W.missing;
+    if (S > 1) { W.qout[oi] = W.missing; W.uout[oi] = W.missing; }
+    W.nfourier[oi] = 0;
+    return;
+  }
+  double sf[S];
+#pragma unroll
+  for (int o = 0; o < S; ++o) sf[o] = flux_multiplier * cum[o];
+  int testconv = W.testconv[it], fsaved = W.fsaved[it];
+  if (M == 0) {
+#pragma unroll
+    for (int o = 0; o < S; ++o) stk[o] = sf[o] + W.ss[(size_t)it * 3 + o] + W.db[(size_t)it * 3 + o];
+  } else {
+    const double azm = W.raa[obs] * (double)M * DEG_TO_RAD;
+    const double azmfac[3] = {cos(azm), cos(azm), sin(azm)};
+    double tazm[S], tnew[S];
+#pragma unroll
+    for (int o = 0; o < S; ++o) { tazm[o] = azmfac[o] * sf[o]; tnew[o] = stk[o] + tazm[o]; }
+    int count = 0;
+    if (tazm[0] != 0.0) { const double accur = fabs(tazm[0] / tnew[0]); if (accur < VLIDORT_ACCURACY_D) count++; }
+    else count++;
+#pragma unroll
+    for (int o = 0; o < S; ++o) stk[o] = tnew[o];
+    if (count == 1) {
+      testconv++;
+      if (testconv == 2) { done = true; fsaved = M; }
+    } else { testconv = 0; fsaved = 2 * N - 1; }
+  }
+  W.testconv[it] = testconv;
+  W.fsaved[it] = fsaved;
+  if (M >= W.nmoments) done = true;
+  if (done) {
+    const float pif = 4.0f * atanf(1.0f);
+    const double pi = (double)pif;
+    const double I = stk[0];
+    W.radiance[oi] = I;
+    W.reflectance[oi] = (pi * I) / (cos(W.sza[obs] * pi / (double)180.0f) * ff);
+    if (S > 1) { W.qout[oi] = stk[1]; W.uout[oi] = stk[2]; }
+    W.nfourier[oi] = fsaved + 1;
+  } else {
+    const int slot = atomicAdd(nnext, 1);
+    next[slot] = it;
+  }
+}
+
+template <int N, int S>
+int fourier_loop(const RTWork& W, int nitems, int* const lists[2], int* counts, hipStream_t st) {
+  const int nl = (int)(((size_t)nitems * W.km + 63) / 64);
+  for (int M = 0; M <= W.nmoments; ++M) {
+    const int* in = lists[M & 1];
+    int* out = lists[(M + 1) & 1];
+    hipLaunchKernelGGL((k_layer<N, S>), dim3(nl), dim3(64), 0, st, W, M, in, counts + M);
+    hipLaunchKernelGGL((k_bvp<N, S>), dim3(nitems), dim3(64), 0, st, W, M, in, counts + M, out, counts + M + 1);
+  }
+  AA_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+}  // namespace aa

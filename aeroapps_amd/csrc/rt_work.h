@@ -1,0 +1,132 @@
+// Device work layout of the discrete-ordinate radiance pipeline.
+//
+// A call processes nobs*nch items (one item = one column at one channel),
+// item g = obs*nch + ch, in chunks of up to `cap` items.  Per-item arrays are
+// indexed by the chunk-local item `it`; per-layer arrays are item-major
+// ([it][layer][field]) so that the wavefront solving one item reads its layer
+// data contiguously.
+#pragma once
+#include "aa_common.h"
+
+namespace aa {
+
+// Per-(item, layer) transmittance/beam record written by the setup stage.
+enum TrField {
+  TR_DTAU = 0,     // delta-M scaled layer optical thickness
+  TR_TDUSER,       // exp(-dtau / mu_user)
+  TR_AVSEC,        // average secant of the solar beam in the layer
+  TR_ITRANS,       // solar beam transmittance to the layer top
+  TR_TDMUBAR,      // exp(-dtau * avsec)
+  TR_ITRANSU,      // itrans / mu_user
+  TR_EMULTUP,      // upwelling beam multiplier at the user angle
+  TR_SIGP,         // avsec + 1/mu_user
+  TR_NFIELDS
+};
+
+struct RTWork {
+  // ---- call description
+  int km = 0;          // layers allocated per column (Fortran km)
+  int nch = 0, nobs = 0;
+  int nmom = 0;        // Legendre moments given per layer (nMom)
+  int npol = 0;
+  int N = 0, S = 1;    // streams per hemisphere, Stokes components
+  int nmoments = 0;    // min(2N-1, nmom): Fourier terms and scaled moments
+  int plane_parallel = 0;
+  double missing = 1e15;
+  int item0 = 0, cap = 0;   // first global item of the chunk, chunk capacity
+  // ---- inputs (device, Fortran layouts of the driver arguments)
+  const double* rot = nullptr;     // (km,nobs,nch)
+  const double* depol = nullptr;   // (nch)
+  const double* alpha = nullptr;   // (km,nobs,nch) or null
+  const void* tau = nullptr;       // (km,nch,nobs)  double or float
+  const void* ssa = nullptr;       // (km,nch,nobs)
+  const void* pmom = nullptr;      // (km,nch,nobs,nmom,npol)
+  int optics_f32 = 0;              // tau/ssa/pmom are float (device Mie output)
+  int opt_obs0 = 0;                // first observation held in tau/ssa/pmom
+  int opt_nobs = 0;                // observations held in tau/ssa/pmom (their nobs extent)
+  const double* he = nullptr;      // (km+1,nobs) metres
+  const double* albedo = nullptr;  // (nobs,nch)
+  const double* sza = nullptr;     // (nobs)
+  const double* raa = nullptr;
+  const double* vza = nullptr;
+  const double* flux = nullptr;    // (nch,nobs) or null (1)
+  const int* nlayers = nullptr;    // (nobs) or null (km)
+  const int* latch = nullptr;      // first global item of the SZA==0 nadir latch
+  // ---- per item (chunk-local)
+  int* status = nullptr;           // 0 ok, else the element is MISSING
+  int* L_it = nullptr;             // layers of the item's column
+  int* beam_cutoff = nullptr;
+  int* testconv = nullptr;
+  int* fsaved = nullptr;
+  int* nfour = nullptr;            // Fourier terms done
+  double* csza = nullptr;
+  double* usec = nullptr;
+  double* tsb = nullptr;           // transmittance of the whole slant solar path (0 if cut off)
+  double* ss = nullptr;            // [it][3] first-order single scatter at TOA
+  double* db = nullptr;            // [it][3] first-order direct bounce at TOA
+  double* stokes = nullptr;        // [it][3] Fourier-summed TOA Stokes vector
+  // ---- per (item, layer)
+  double* deltau = nullptr;        // [it][km] unscaled
+  double* fac1 = nullptr;          // [it][km] 1 - f*omega
+  double* exact = nullptr;         // [it][km][3] FO phase function x TMS factor
+  double* og = nullptr;            // [it][km][LM][S][S] omega * scaled Greek
+  unsigned* lsc = nullptr;         // [it][km] bit m: layer scatters at Fourier m
+  double* tr = nullptr;            // [it][km][TR_NFIELDS]
+  double* lev = nullptr;           // [it][km+1][4]: tauslant, suntau, atten, FO mult(enhanced)
+  double* lostrans = nullptr;      // [it][km+1]
+  double* lbuf = nullptr;          // [it][km][lstride] homogeneous + particular solutions
+  double* bbuf = nullptr;          // [it][bstride] surface terms of the bottom layer
+  double* ubuf = nullptr;          // [it][2*n*km][ustride] band-LU U rows + transformed RHS
+  int lstride = 0, bstride = 0, ustride = 0;
+  // ---- quadrature and PI tables of the quadrature streams (device)
+  const double* quad = nullptr;    // [qs N][qw N][xq: M][l][i][S][S]
+  // ---- outputs (device, (nobs,nch) Fortran)
+  double* radiance = nullptr;
+  double* reflectance = nullptr;
+  double* qout = nullptr;
+  double* uout = nullptr;
+  int* nfourier = nullptr;
+};
+
+// Offsets inside one lbuf record (per item and layer) for n = N*S.
+struct LOff {
+  int n, S, N;
+  __host__ __device__ constexpr LOff(int N_, int S_) : n(N_ * S_), S(S_), N(N_) {}
+  __host__ __device__ constexpr int xp(int i, int o, int k) const { return (i * S + o) * n + k; }
+  __host__ __device__ constexpr int xn(int i, int o, int k) const { return 2 * n * n + (i * S + o) * n + k; }
+  __host__ __device__ constexpr int tdel(int k) const { return 4 * n * n + k; }
+  __host__ __device__ constexpr int wu(int i, int o) const { return 4 * n * n + n + i * S + o; }
+  __host__ __device__ constexpr int wl(int i, int o) const { return 4 * n * n + 3 * n + i * S + o; }
+  // source coefficients: src[o] = sum_q x(q) * coef(o, q) + csrc(o), q < 2n (lcon then mcon)
+  __host__ __device__ constexpr int coef(int o, int q) const { return 4 * n * n + 5 * n + o * 2 * n + q; }
+  __host__ __device__ constexpr int csrc(int o) const { return 4 * n * n + 5 * n + 2 * n * S + o; }
+  __host__ __device__ constexpr int size() const { return (4 * n * n + 5 * n + 2 * n * S + S + 1) & ~1; }
+};
+
+// Offsets inside one bbuf record: surface reflection terms of the bottom
+// layer at Fourier 0 (Lambertian).
+struct BOff {
+  int n;
+  __host__ __device__ constexpr BOff(int n_) : n(n_) {}
+  __host__ __device__ constexpr int r2p(int k) const { return k; }
+  __host__ __device__ constexpr int r2m(int k) const { return n + k; }
+  __host__ __device__ constexpr int bcoef(int q) const { return 2 * n + q; }   // q < 2n
+  __host__ __device__ constexpr int r2beam() const { return 4 * n; }
+  __host__ __device__ constexpr int bc() const { return 4 * n + 1; }
+  __host__ __device__ constexpr int size() const { return 4 * n + 2; }
+};
+
+// Quadrature table layout for (N, S): qs[N], qw[N], then for each Fourier term
+// M < 2N the normalised PI matrices xq[l][i][s1][s2] (l < 2N, i < N).
+struct QOff {
+  int N, S;
+  __host__ __device__ constexpr QOff(int N_, int S_) : N(N_), S(S_) {}
+  __host__ __device__ constexpr int qs(int i) const { return i; }
+  __host__ __device__ constexpr int qw(int i) const { return N + i; }
+  __host__ __device__ constexpr int xq(int M, int l, int i, int s1, int s2) const {
+    return 2 * N + ((((M * 2 * N + l) * N + i) * S + s1) * S + s2);
+  }
+  __host__ __device__ constexpr int size() const { return 2 * N + 2 * N * 2 * N * N * S * S; }
+};
+
+}  // namespace aa

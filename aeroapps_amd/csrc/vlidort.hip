@@ -1,0 +1,420 @@
+// C-ABI of the discrete-ordinate radiance path: the drop-in replacements of
+// VLIDORT_Scalar_Lambert / VLIDORT_Vector_Lambert
+// (src/Components/rtms/geo_vlidort/VLIDORT_LAMBERT.F90:34-342) and the fused
+// device path (Mie optics -> radiance on resident data).
+//
+// Host side: a grow-only device arena per GPU, the quadrature/PI tables of the
+// quadrature streams (identical for every column of a call), and the chunked
+// pipeline setup -> Fourier terms.
+#include <algorithm>
+#include <climits>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "rt_work.h"
+#include "rt_device.h"
+#include "../../include/aeroapps_amd.h"
+
+struct aa_mie;
+
+namespace aa {
+
+int rt_setup(const RTWork& W, int nitems, int* d_active, int* d_nactive, hipStream_t st);
+int rt_latch(int nitems, int nch, int nobs, const double* sza, const double* vza, const double* raa,
+             const double* albedo, double missing, int* d_latch, hipStream_t st);
+#define AA_DECL(N, S) \
+  int rt_fourier_##N##_##S(const RTWork& W, int nitems, int* const lists[2], int* counts, hipStream_t st);
+AA_DECL(1, 1) AA_DECL(2, 1) AA_DECL(3, 1) AA_DECL(4, 1) AA_DECL(5, 1) AA_DECL(6, 1) AA_DECL(7, 1) AA_DECL(8, 1)
+AA_DECL(1, 3) AA_DECL(2, 3) AA_DECL(3, 3) AA_DECL(4, 3)
+#undef AA_DECL
+
+// Fourier-term kernels for the call's (streams, Stokes) pair; -4 when the
+// pair has no compiled instantiation.
+int rt_fourier(const RTWork& W, int nitems, int* const lists[2], int* counts, hipStream_t st) {
+  if (W.S == 1) {
+    switch (W.N) {
+      case 1: return rt_fourier_1_1(W, nitems, lists, counts, st);
+      case 2: return rt_fourier_2_1(W, nitems, lists, counts, st);
+      case 3: return rt_fourier_3_1(W, nitems, lists, counts, st);
+      case 4: return rt_fourier_4_1(W, nitems, lists, counts, st);
+      case 5: return rt_fourier_5_1(W, nitems, lists, counts, st);
+      case 6: return rt_fourier_6_1(W, nitems, lists, counts, st);
+      case 7: return rt_fourier_7_1(W, nitems, lists, counts, st);
+      case 8: return rt_fourier_8_1(W, nitems, lists, counts, st);
+      default: return -4;
+    }
+  }
+  switch (W.N) {
+    case 1: return rt_fourier_1_3(W, nitems, lists, counts, st);
+    case 2: return rt_fourier_2_3(W, nitems, lists, counts, st);
+    case 3: return rt_fourier_3_3(W, nitems, lists, counts, st);
+    case 4: return rt_fourier_4_3(W, nitems, lists, counts, st);
+    default: return -4;
+  }
+}
+int mie_upload(aa_mie* t);
+int launch_get_aop(const aa_mie* t, int km, int nobs, int nch, int nq, const int* d_tracer_idx,
+                   const int* d_chan_idx, const float* d_qm, const float* d_rh, int nmom, int npol, int ntr_eff,
+                   int normalize, float* d_tau, float* d_ssa, float* d_g, float* d_pmom, hipStream_t st);
+
+namespace {
+
+// ---------------------------------------------------------------- device arena
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0;
+};
+Arena g_arena[64];
+
+int arena_reserve(size_t bytes, char** out) {
+  int dev = 0;
+  AA_CHECK(hipGetDevice(&dev));
+  Arena& a = g_arena[dev & 63];
+  if (a.cap < bytes) {
+    if (a.base) AA_CHECK(hipFree(a.base));
+    a.base = nullptr; a.cap = 0;
+    size_t want = bytes + bytes / 8;
+    AA_CHECK(hipMalloc((void**)&a.base, want));
+    a.cap = want;
+  }
+  *out = a.base;
+  return 0;
+}
+
+// Sequential carving of one arena block (a first pass with base == nullptr
+// measures the block).
+struct Carver {
+  char* base;
+  size_t off = 0;
+  explicit Carver(char* b) : base(b) {}
+  template <class T>
+  T* take(size_t count) {
+    off = (off + 255) & ~(size_t)255;
+    T* p = base ? (T*)(base + off) : nullptr;
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
+size_t env_bytes(const char* name, size_t dflt) {
+  const char* s = std::getenv(name);
+  if (!s || !*s) return dflt;
+  return (size_t)std::strtoull(s, nullptr, 10);
+}
+
+// Quadrature streams and their PI matrices for every Fourier term
+// (GETQUAD2 REG/vlidort_aux.f90:960; PIMATRIX_SETUP REG/vlidort_miscsetups.f90:2273).
+template <int S, int LM>
+void quad_table_t(int N, int nmoments, std::vector<double>& q) {
+  QOff QO(N, S);
+  q.assign(QO.size(), 0.0);
+  std::vector<double> qs(N), qw(N);
+  if (N == 1) { qs[0] = std::sqrt(1.0 / 3.0); qw[0] = 1.0; }
+  else getquad2(0.0, 1.0, N, qs.data(), qw.data());
+  for (int i = 0; i < N; ++i) { q[QO.qs(i)] = qs[i]; q[QO.qw(i)] = qw[i]; }
+  for (int M = 0; M < LM; ++M) {
+    if (M > nmoments) break;
+    for (int i = 0; i < N; ++i) {
+      double pi[LM][S][S];
+      pimatrix_x<S, LM>(M, nmoments, qs[i], pi);
+      for (int l = 0; l < LM; ++l)
+        for (int a = 0; a < S; ++a)
+          for (int b = 0; b < S; ++b) q[QO.xq(M, l, i, a, b)] = pi[l][a][b];
+    }
+  }
+}
+
+void quad_table(int N, int S, int nmoments, std::vector<double>& q) {
+  if (S == 1) {
+    switch (N) {
+      case 1: quad_table_t<1, 2>(N, nmoments, q); break;
+      case 2: quad_table_t<1, 4>(N, nmoments, q); break;
+      case 3: quad_table_t<1, 6>(N, nmoments, q); break;
+      case 4: quad_table_t<1, 8>(N, nmoments, q); break;
+      case 5: quad_table_t<1, 10>(N, nmoments, q); break;
+      case 6: quad_table_t<1, 12>(N, nmoments, q); break;
+      case 7: quad_table_t<1, 14>(N, nmoments, q); break;
+      default: quad_table_t<1, 16>(N, nmoments, q); break;
+    }
+  } else {
+    switch (N) {
+      case 1: quad_table_t<3, 2>(N, nmoments, q); break;
+      case 2: quad_table_t<3, 4>(N, nmoments, q); break;
+      case 3: quad_table_t<3, 6>(N, nmoments, q); break;
+      default: quad_table_t<3, 8>(N, nmoments, q); break;
+    }
+  }
+}
+
+int max_streams(int S) { return S == 1 ? MAX_STREAMS : 4; }
+
+// Per-item work of one chunk, carved from the arena.
+size_t carve_work(Carver& C, RTWork& W, int cap) {
+  const int n = W.N * W.S, LM = 2 * W.N, km = W.km;
+  W.lstride = LOff(W.N, W.S).size();
+  W.bstride = BOff(n).size();
+  W.ustride = 6 * n;   // band row width 2*(3n-1)+1 plus the transformed RHS
+  W.cap = cap;
+  W.status = C.take<int>(cap);
+  W.L_it = C.take<int>(cap);
+  W.beam_cutoff = C.take<int>(cap);
+  W.testconv = C.take<int>(cap);
+  W.fsaved = C.take<int>(cap);
+  W.nfour = C.take<int>(cap);
+  W.csza = C.take<double>(cap);
+  W.usec = C.take<double>(cap);
+  W.tsb = C.take<double>(cap);
+  W.ss = C.take<double>((size_t)cap * 3);
+  W.db = C.take<double>((size_t)cap * 3);
+  W.stokes = C.take<double>((size_t)cap * 3);
+  W.deltau = C.take<double>((size_t)cap * km);
+  W.fac1 = C.take<double>((size_t)cap * km);
+  W.exact = C.take<double>((size_t)cap * km * 3);
+  W.og = C.take<double>((size_t)cap * km * LM * W.S * W.S);
+  W.lsc = C.take<unsigned>((size_t)cap * km);
+  W.tr = C.take<double>((size_t)cap * km * TR_NFIELDS);
+  W.lev = C.take<double>((size_t)cap * (km + 1) * 4);
+  W.lostrans = C.take<double>((size_t)cap * (km + 1));
+  W.lbuf = C.take<double>((size_t)cap * km * W.lstride);
+  W.bbuf = C.take<double>((size_t)cap * W.bstride);
+  W.ubuf = C.take<double>((size_t)cap * (2 * n * km) * W.ustride);
+  return C.off;
+}
+
+size_t bytes_per_item(const RTWork& W0) {
+  RTWork W = W0;
+  Carver a(nullptr), b(nullptr);
+  size_t s1 = carve_work(a, W, 1), s2 = carve_work(b, W, 2);
+  return s2 - s1 + 256 * 24;
+}
+
+// Device int holding the first item of the SZA==0 nadir latch for the
+// current call (one per GPU, allocated once).
+int* g_latch[64] = {nullptr};
+
+int compute_latch(RTWork& W, hipStream_t st) {
+  int dev = 0;
+  AA_CHECK(hipGetDevice(&dev));
+  int*& lp = g_latch[dev & 63];
+  if (!lp) AA_CHECK(hipMalloc((void**)&lp, sizeof(int)));
+  static const int big = INT_MAX;
+  AA_CHECK(hipMemcpyAsync(lp, &big, sizeof(int), hipMemcpyHostToDevice, st));
+  const int nitems = W.nobs * W.nch;
+  if (nitems > 0 && rt_latch(nitems, W.nch, W.nobs, W.sza, W.vza, W.raa, W.albedo, W.missing, lp, st)) return AA_EHIP;
+  W.latch = lp;
+  return 0;
+}
+
+// Runs the radiance pipeline over items [g0, g1) (item g = obs*nch + ch).
+// W carries the inputs and outputs (device pointers) and the latch.
+int run_items(RTWork W, int g0, int g1, hipStream_t st) {
+  const int nitems = g1 - g0;
+  if (nitems <= 0) return 0;
+  std::vector<double> qtab;
+  quad_table(W.N, W.S, W.nmoments, qtab);
+  // chunk capacity from the work budget
+  size_t freeb = 0, totalb = 0;
+  AA_CHECK(hipMemGetInfo(&freeb, &totalb));
+  const size_t per = bytes_per_item(W);
+  size_t budget = env_bytes("AA_WORK_BYTES", (size_t)24 << 30);
+  budget = std::min(budget, (size_t)(0.6 * (double)freeb));
+  int cap = (int)std::max<size_t>(1, std::min<size_t>((size_t)nitems, budget / per));
+  const size_t envcap = env_bytes("AA_CHUNK_ITEMS", 0);
+  if (envcap) cap = (int)std::min<size_t>((size_t)cap, envcap);
+  // arena: work + quad table + active lists + counters
+  auto carve_all = [&](char* base, RTWork& w, double** dq, int** l0, int** l1, int** cnt) {
+    Carver C(base);
+    carve_work(C, w, cap);
+    *dq = C.take<double>(qtab.size());
+    *l0 = C.take<int>(cap);
+    *l1 = C.take<int>(cap);
+    *cnt = C.take<int>(W.nmoments + 2);
+    return C.off;
+  };
+  double* dq; int *l0, *l1, *cnt;
+  RTWork tmp = W;
+  size_t total = carve_all(nullptr, tmp, &dq, &l0, &l1, &cnt);
+  char* base = nullptr;
+  if (arena_reserve(total, &base)) return AA_EHIP;
+  carve_all(base, W, &dq, &l0, &l1, &cnt);
+  AA_CHECK(hipMemcpyAsync(dq, qtab.data(), qtab.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  W.quad = dq;
+  int* lists[2] = {l0, l1};
+  for (int item0 = g0; item0 < g1; item0 += cap) {
+    const int ni = std::min(cap, g1 - item0);
+    W.item0 = item0;
+    AA_CHECK(hipMemsetAsync(cnt, 0, sizeof(int) * (W.nmoments + 2), st));
+    if (rt_setup(W, ni, lists[0], cnt, st)) return AA_EHIP;
+    int e = rt_fourier(W, ni, lists, cnt, st);
+    if (e) return e == -4 ? AA_EDIMS : AA_EHIP;
+  }
+  // the qtab host vector must outlive the async copy
+  AA_CHECK(hipStreamSynchronize(st));
+  return 0;
+}
+
+// Device copy helper for the host-pointer drivers.
+struct DevBuf {
+  std::vector<void*> ptrs;
+  ~DevBuf() { for (void* p : ptrs) (void)hipFree(p); }
+  template <class T>
+  T* up(const T* h, size_t n) {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+    ptrs.push_back(p);
+    if (h && n && hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return (T*)p;
+  }
+  template <class T>
+  T* alloc(size_t n) { return up<T>(nullptr, n); }
+};
+
+int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane_parallel, int nmom, int npol,
+                 const double* rot, const double* depol, const double* alpha, const double* tau, const double* ssa,
+                 const double* pmom, const double* he, const double* albedo, const double* sza, const double* raa,
+                 const double* vza, const double* flux_factor, double missing, int precision, double* radiance,
+                 double* reflectance, double* q, double* u, int* nfourier, int* rc) {
+  if (!rc) return AA_EARG;
+  *rc = 0;
+  if (km <= 0 || nch <= 0 || nobs < 0 || nmom <= 0 || npol <= 0) return AA_EARG;
+  if (!rot || !depol || !tau || !ssa || !pmom || !he || !albedo || !sza || !raa || !vza || !radiance ||
+      !reflectance)
+    return AA_EARG;
+  if (vector && (!q || !u || !flux_factor || npol < 6)) return AA_EARG;
+  if (precision != 0) return AA_EARG;
+  const int S = vector ? 3 : 1;
+  if (nstreams < 1 || nstreams > max_streams(S)) { *rc = 2; return AA_EDIMS; }
+  if (km > MAX_LAYERS) { *rc = 3; return AA_EDIMS; }
+  if (nobs == 0) return 0;
+  RTWork W;
+  W.km = km; W.nch = nch; W.nobs = nobs; W.nmom = nmom; W.npol = npol;
+  W.N = nstreams; W.S = S; W.nmoments = std::min(2 * nstreams - 1, nmom);
+  W.plane_parallel = plane_parallel ? 1 : 0;
+  W.missing = missing;
+  const size_t nl = (size_t)km * nobs * nch;
+  DevBuf B;
+  W.rot = B.up(rot, nl);
+  W.depol = B.up(depol, (size_t)nch);
+  W.alpha = (vector && alpha) ? B.up(alpha, nl) : nullptr;
+  W.tau = B.up(tau, nl);
+  W.ssa = B.up(ssa, nl);
+  W.pmom = B.up(pmom, nl * nmom * npol);
+  W.optics_f32 = 0; W.opt_obs0 = 0; W.opt_nobs = nobs;
+  W.he = B.up(he, (size_t)(km + 1) * nobs);
+  W.albedo = B.up(albedo, (size_t)nobs * nch);
+  W.sza = B.up(sza, (size_t)nobs);
+  W.raa = B.up(raa, (size_t)nobs);
+  W.vza = B.up(vza, (size_t)nobs);
+  W.flux = vector ? B.up(flux_factor, (size_t)nch * nobs) : nullptr;
+  const size_t no = (size_t)nobs * nch;
+  W.radiance = B.alloc<double>(no);
+  W.reflectance = B.alloc<double>(no);
+  W.qout = vector ? B.alloc<double>(no) : nullptr;
+  W.uout = vector ? B.alloc<double>(no) : nullptr;
+  W.nfourier = B.alloc<int>(no);
+  for (void* p : B.ptrs) if (!p) return AA_EHIP;
+  int e = compute_latch(W, 0);
+  if (!e) e = run_items(W, 0, nobs * nch, 0);
+  if (e) return e;
+  AA_CHECK(hipDeviceSynchronize());
+  AA_CHECK(hipMemcpy(radiance, W.radiance, no * sizeof(double), hipMemcpyDeviceToHost));
+  AA_CHECK(hipMemcpy(reflectance, W.reflectance, no * sizeof(double), hipMemcpyDeviceToHost));
+  if (vector) {
+    AA_CHECK(hipMemcpy(q, W.qout, no * sizeof(double), hipMemcpyDeviceToHost));
+    AA_CHECK(hipMemcpy(u, W.uout, no * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  if (nfourier) AA_CHECK(hipMemcpy(nfourier, W.nfourier, no * sizeof(int), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // namespace
+}  // namespace aa
+
+extern "C" {
+
+int aa_vlidort_scalar_lambert(int km, int nch, int nobs, const double* channels, int nstreams, int plane_parallel,
+                              int nmom, int npol, const double* rot, const double* depol, const double* tau,
+                              const double* ssa, const double* g, const double* pmom, const double* pe,
+                              const double* he, const double* te, const double* albedo,
+                              const double* solar_zenith, const double* relat_azymuth, const double* sensor_zenith,
+                              double missing, int verbose, int precision, double* radiance, double* reflectance,
+                              int* nfourier, int* rc) {
+  (void)channels; (void)g; (void)pe; (void)te; (void)verbose;
+  return aa::lambert_host(false, km, nch, nobs, nstreams, plane_parallel, nmom, npol, rot, depol, nullptr, tau, ssa,
+                          pmom, he, albedo, solar_zenith, relat_azymuth, sensor_zenith, nullptr, missing, precision,
+                          radiance, reflectance, nullptr, nullptr, nfourier, rc);
+}
+
+int aa_vlidort_vector_lambert(int km, int nch, int nobs, const double* channels, int nstreams, int plane_parallel,
+                              int nmom, int npol, const double* rot, const double* depol, const double* alpha,
+                              const double* tau, const double* ssa, const double* pmom, const double* pe,
+                              const double* he, const double* te, const double* albedo,
+                              const double* solar_zenith, const double* relat_azymuth, const double* sensor_zenith,
+                              const double* flux_factor, double missing, int verbose, int precision,
+                              double* radiance, double* reflectance, double* q, double* u, int* nfourier, int* rc) {
+  (void)channels; (void)pe; (void)te; (void)verbose;
+  return aa::lambert_host(true, km, nch, nobs, nstreams, plane_parallel, nmom, npol, rot, depol, alpha, tau, ssa,
+                          pmom, he, albedo, solar_zenith, relat_azymuth, sensor_zenith, flux_factor, missing,
+                          precision, radiance, reflectance, q, u, nfourier, rc);
+}
+
+int aa_column_radiance_dev(aa_mie* t, int km, int nobs, int nch, int nq, const int* tracer_idx, const int* chan_idx,
+                           const float* d_qm, const float* d_rh, int nmom, int npol, int nstreams, int vector,
+                           int precision, const int* d_nlayers, const int* d_nstreams_col, const double* d_rot,
+                           const double* d_depol, const double* d_pe, const double* d_he, const double* d_te,
+                           const double* d_albedo, const double* d_sza, const double* d_raa, const double* d_vza,
+                           const double* d_flux_factor, double missing, double* d_radiance, double* d_reflectance,
+                           double* d_q, double* d_u, int* d_nfourier, void* stream) {
+  (void)d_pe; (void)d_te;
+  using namespace aa;
+  if (!t || km <= 0 || km > MAX_LAYERS || nobs < 0 || nch <= 0 || nq <= 0 || nmom <= 0 || npol <= 0) return AA_EARG;
+  if (!tracer_idx || !chan_idx || !d_qm || !d_rh || !d_rot || !d_depol || !d_he || !d_albedo || !d_sza || !d_raa ||
+      !d_vza || !d_radiance || !d_reflectance || !d_nfourier)
+    return AA_EARG;
+  if (d_nstreams_col) return AA_EARG;   // per-column stream counts go through the batch scheduler
+  if (precision != 0) return AA_EARG;
+  const int S = vector ? 3 : 1;
+  if (vector && (!d_q || !d_u || npol < 6)) return AA_EARG;
+  if (nstreams < 1 || nstreams > max_streams(S)) return AA_EDIMS;
+  if (nobs == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  if (mie_upload(t)) return AA_EHIP;
+  RTWork W;
+  W.km = km; W.nch = nch; W.nobs = nobs; W.nmom = nmom; W.npol = npol;
+  W.N = nstreams; W.S = S; W.nmoments = std::min(2 * nstreams - 1, nmom);
+  W.missing = missing;
+  W.rot = d_rot; W.depol = d_depol; W.alpha = nullptr;
+  W.he = d_he; W.albedo = d_albedo; W.sza = d_sza; W.raa = d_raa; W.vza = d_vza;
+  W.flux = vector ? d_flux_factor : nullptr;
+  W.nlayers = d_nlayers;
+  W.radiance = d_radiance; W.reflectance = d_reflectance; W.qout = d_q; W.uout = d_u; W.nfourier = d_nfourier;
+  W.optics_f32 = 1;
+  // optics for a block of observations at a time
+  const size_t per_obs = (size_t)km * nch * (3 + (size_t)nmom * npol) * sizeof(float);
+  size_t ob = std::max<size_t>(1, env_bytes("AA_OPTICS_BYTES", (size_t)4 << 30) / per_obs);
+  ob = std::min<size_t>(ob, (size_t)nobs);
+  float *dt = nullptr, *ds = nullptr, *dg = nullptr, *dp = nullptr;
+  AA_CHECK(hipMalloc((void**)&dt, ob * km * nch * sizeof(float)));
+  AA_CHECK(hipMalloc((void**)&ds, ob * km * nch * sizeof(float)));
+  AA_CHECK(hipMalloc((void**)&dg, ob * km * nch * sizeof(float)));
+  AA_CHECK(hipMalloc((void**)&dp, ob * km * nch * (size_t)nmom * npol * sizeof(float)));
+  int rcode = compute_latch(W, st);
+  for (int o0 = 0; o0 < nobs && !rcode; o0 += (int)ob) {
+    const int no = std::min((int)ob, nobs - o0);
+    rcode = launch_get_aop(t, km, no, nch, nq, tracer_idx, chan_idx, d_qm + (size_t)o0 * km * nq,
+                           d_rh + (size_t)o0 * km, nmom, npol, nq, 1, dt, ds, dg, dp, st);
+    if (rcode) break;
+    RTWork Wc = W;
+    Wc.tau = dt; Wc.ssa = ds; Wc.pmom = dp;
+    Wc.opt_obs0 = o0; Wc.opt_nobs = no;
+    rcode = run_items(Wc, o0 * nch, (o0 + no) * nch, st);
+  }
+  AA_CHECK(hipStreamSynchronize(st));
+  (void)hipFree(dt); (void)hipFree(ds); (void)hipFree(dg); (void)hipFree(dp);
+  return rcode;
+}
+
+const char* aa_version(void) { return "aeroapps_amd 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+"""Python binding of the radiance drivers, with the call contract of the
+reference's f2py module ``VLIDORT_POLAR_`` (scalar_lambert / vector_lambert,
+src/Components/rtms/leo_vlidort/VLIDORT_POLAR_py.F90:457-671, called from
+src/Components/rtms/leo_vlidort/polar_vlidort.py): arrays are given in their
+Fortran shapes (e.g. tau[km, nch, nobs]), the dimensions are taken from them,
+and the outputs come back as (nobs, nch) arrays plus the return code.
+
+Extra keyword arguments: ``precision`` (0 = FP64, the drop-in) and
+``return_nfourier`` (also return the Fourier terms used per element).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from ._lib import check, lib, _dp, _ip
+
+
+def _f64(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def _ptr(a, kind=_dp):
+    return a.ctypes.data_as(kind)
+
+
+def scalar_lambert(channels, nstreams, plane_parallel, ROT, depol, tau, ssa, g, pmom, pe, he, te, albedo,
+                   solar_zenith, relat_azymuth, sensor_zenith, MISSING, verbose=0, precision=0,
+                   return_nfourier=False):
+    """VLIDORT_Scalar_Lambert (src/Components/rtms/geo_vlidort/VLIDORT_LAMBERT.F90:34-168)."""
+    channels, ROT, depol, tau, ssa, g, pmom, pe, he, te, albedo, sza, raa, vza = map(
+        _f64, (channels, ROT, depol, tau, ssa, g, pmom, pe, he, te, albedo, solar_zenith, relat_azymuth,
+               sensor_zenith))
+    km, nch, nobs = tau.shape
+    if pmom.ndim == 4:
+        pmom = _f64(pmom[..., None])
+    nmom, npol = pmom.shape[3], pmom.shape[4]
+    rad = np.zeros((nobs, nch), order="F")
+    refl = np.zeros((nobs, nch), order="F")
+    nf = np.zeros((nobs, nch), dtype=np.int32, order="F")
+    rc = np.zeros(1, dtype=np.int32)
+    e = lib().aa_vlidort_scalar_lambert(km, nch, nobs, _ptr(channels), int(nstreams), int(bool(plane_parallel)),
+                                        nmom, npol, _ptr(ROT), _ptr(depol), _ptr(tau), _ptr(ssa), _ptr(g),
+                                        _ptr(pmom), _ptr(pe), _ptr(he), _ptr(te), _ptr(albedo), _ptr(sza),
+                                        _ptr(raa), _ptr(vza), float(MISSING), int(verbose), int(precision),
+                                        _ptr(rad), _ptr(refl), _ptr(nf, _ip), _ptr(rc, _ip))
+    check(e, "aa_vlidort_scalar_lambert")
+    if return_nfourier:
+        return rad, refl, int(rc[0]), nf
+    return rad, refl, int(rc[0])
+
+
+def vector_lambert(channels, nstreams, plane_parallel, ROT, depol, alpha, tau, ssa, pmom, pe, he, te, albedo,
+                   solar_zenith, relat_azymuth, sensor_zenith, flux_factor, MISSING, verbose=0, precision=0,
+                   return_nfourier=False):
+    """VLIDORT_Vector_Lambert (src/Components/rtms/geo_vlidort/VLIDORT_LAMBERT.F90:172-342)."""
+    (channels, ROT, depol, alpha, tau, ssa, pmom, pe, he, te, albedo, sza, raa, vza, ff) = map(
+        _f64, (channels, ROT, depol, alpha, tau, ssa, pmom, pe, he, te, albedo, solar_zenith, relat_azymuth,
+               sensor_zenith, flux_factor))
+    km, nch, nobs = tau.shape
+    nmom, npol = pmom.shape[3], pmom.shape[4]
+    out = [np.zeros((nobs, nch), order="F") for _ in range(4)]
+    nf = np.zeros((nobs, nch), dtype=np.int32, order="F")
+    rc = np.zeros(1, dtype=np.int32)
+    e = lib().aa_vlidort_vector_lambert(km, nch, nobs, _ptr(channels), int(nstreams), int(bool(plane_parallel)),
+                                        nmom, npol, _ptr(ROT), _ptr(depol), _ptr(alpha), _ptr(tau), _ptr(ssa),
+                                        _ptr(pmom), _ptr(pe), _ptr(he), _ptr(te), _ptr(albedo), _ptr(sza),
+                                        _ptr(raa), _ptr(vza), _ptr(ff), float(MISSING), int(verbose),
+                                        int(precision), *[_ptr(o) for o in out], _ptr(nf, _ip), _ptr(rc, _ip))
+    check(e, "aa_vlidort_vector_lambert")
+    I, R, Q, U = out
+    if return_nfourier:
+        return I, R, Q, U, int(rc[0]), nf
+    return I, R, Q, U, int(rc[0])

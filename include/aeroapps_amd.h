@@ -1,0 +1,160 @@
+/* C-ABI of the MI355X aerosol-optics + discrete-ordinate radiance library.
+ *
+ * Every entry point takes plain pointers and sizes.  Host-pointer entry points
+ * ("drop-in" surface) accept arrays in the reference's Fortran column-major
+ * layout and are synchronous, like the Fortran routines they replace.  The
+ * "_dev" entry points take device pointers in the same layouts and enqueue on a
+ * caller stream (a hipStream_t passed as void*); they return once the work is
+ * enqueued.
+ *
+ * Return value of every function: 0 on success, otherwise an AA_E* code below
+ * (an argument/setup error detected before any kernel ran).  Per-element
+ * failures follow the reference convention instead: the element's outputs are
+ * set to MISSING and the call still returns 0.
+ *
+ * Reference paths are relative to the GEOS-ESM/AeroApps source tree:
+ *   CB   = src/Shared/GMAO_Shared/Chem_Base
+ *   GEO  = src/Components/rtms/geo_vlidort
+ *   V283 = src/Components/rtms/RTSI/VLIDORT2p8p3
+ */
+#ifndef AEROAPPS_AMD_H
+#define AEROAPPS_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  AA_OK = 0,
+  AA_EARG = -1,        /* bad size / null pointer / unsupported option */
+  AA_EHIP = -2,        /* HIP runtime error (allocation, launch) */
+  AA_ENOTABLE = -3,    /* table handle has no species table */
+  AA_EDIMS = -4        /* dimension beyond the compiled maxima (streams, layers, moments) */
+};
+
+/* Reference return codes reproduced in the rc out-arguments. */
+#define AA_RC_CHANNEL 99    /* GEO/VLIDORT_MieMod.F90:225  channel not in the Mie tables       */
+#define AA_RC_NMOM 666      /* GEO/VLIDORT_MieMod.F90:260  table has fewer moments than asked   */
+#define AA_RC_NPOL 777      /* GEO/VLIDORT_MieMod.F90:276  inconsistent number of polarizations */
+
+/* ------------------------------------------------------------------ Chem_Mie
+ * Device-resident optics tables.  Replaces the Chem_Mie object of
+ * CB/Chem_MieMod.F90:65-99 (type Chem_Mie) built by Chem_MieCreate
+ * (CB/Chem_MieMod.F90:124-411) from Chem_MieTableRead tables
+ * (CB/Chem_MieTableMod.F90:238-719).  The tables are uploaded once and stay
+ * resident until aa_mie_destroy. */
+typedef struct aa_mie aa_mie;
+
+/* Empty table set for nch channels given in metres (Chem_Mie%channels).
+ * Replaces Chem_MieCreate (CB/Chem_MieMod.F90:124). */
+int aa_mie_create(aa_mie** out, int nch, const float* channels_m);
+
+/* Adds one species table already interpolated to the nch channels (the result
+ * of Chem_MieTableRead's polint step, CB/Chem_MieTableMod.F90:560-589).  Arrays
+ * are Fortran-ordered float32: rh(nrh), bext/bsca/bbck/g/refr/refi(nrh,nch,nbin),
+ * pmom(nrh,nch,nbin,nmom,npol).  bin_names is nbin names of 16 characters
+ * (blank padded), the tracer names served by this table (Chem_Mie%vname).
+ * The 991-slot RH hash (CB/Chem_MieTableMod.F90:612-627) is built here. */
+int aa_mie_add_table(aa_mie* t, int nrh, int nbin, int nmom, int npol, const float* rh,
+                     const float* bext, const float* bsca, const float* bbck, const float* g,
+                     const float* refr, const float* refi, const float* pmom, const char* bin_names);
+
+/* Tracer index (0-based) of qname, or -1.  Replaces Chem_MieQueryIdx
+ * (CB/Chem_MieMod.F90:720-777): case-insensitive, "XX::" qualifier stripped. */
+int aa_mie_query_idx(const aa_mie* t, const char* qname);
+
+/* Chem_MieQuery over n (q_mass, rh) pairs of one tracer and channel
+ * (CB/Chem_MieMod.F90:789-965).  channel is the 1-based table channel number
+ * (Chem_MieQuery's real CHANNEL argument, rounded with nint).  Outputs may be
+ * NULL when not wanted; pmom is (n, nmom, npol) Fortran order and only filled
+ * for nmom <= the table's moments.  Host pointers, synchronous. */
+int aa_mie_query(const aa_mie* t, int idx, float channel, int n, const float* q_mass, const float* rh,
+                 float* tau, float* ssa, float* gasym, float* bext, float* bsca, float* bbck,
+                 float* refr, float* refi, int nmom, int npol, float* pmom);
+
+/* Number of channels / moments / polarizations of the table set. */
+int aa_mie_dims(const aa_mie* t, int* nch, int* nmom_min, int* npol_max, int* ntracer);
+
+int aa_mie_destroy(aa_mie* t);
+
+/* Replaces VLIDORT_getAOPvector (GEO/VLIDORT_MieMod.F90:124-336).  Fortran
+ * arrays: channels(nch) [nm], vname(nq,16) (character array as passed by f2py,
+ * i.e. vname[c*nq + iq]), qm(km,nq,nobs), rh(km,nobs) -> tau,ssa,g(km,nch,nobs),
+ * pmom(km,nch,nobs,nmom,npol); rc as in the reference (0, 1 when the last
+ * tracer name was not found, 99, 666, 777).  Host pointers, synchronous. */
+int aa_get_aop_vector(aa_mie* t, int km, int nobs, int nch, int nq, const float* channels,
+                      const char* vname, int verbose, const float* qm, const float* rh, int nmom, int npol,
+                      float* tau, float* ssa, float* g, float* pmom, int* rc);
+
+/* Replaces VLIDORT_getAOPscalar (GEO/VLIDORT_MieMod.F90:10-120). */
+int aa_get_aop_scalar(aa_mie* t, int km, int nobs, int nch, int nq, const float* channels,
+                      const char* vname, int verbose, const float* qm, const float* rh,
+                      float* tau, float* ssa, float* g, int* rc);
+
+/* Device-pointer form of aa_get_aop_vector: same layouts, tracer indices
+ * already resolved (tracer_idx[nq], -1 = skipped) and channel indices
+ * resolved (chan_idx[nch], 0-based), enqueued on stream. */
+int aa_get_aop_vector_dev(aa_mie* t, int km, int nobs, int nch, int nq, const int* tracer_idx,
+                          const int* chan_idx, const float* d_qm, const float* d_rh, int nmom, int npol,
+                          float* d_tau, float* d_ssa, float* d_g, float* d_pmom, void* stream);
+
+/* ------------------------------------------------------------------ VLIDORT drivers
+ * Replace VLIDORT_Scalar_Lambert (GEO/VLIDORT_LAMBERT.F90:34-168) and
+ * VLIDORT_Vector_Lambert single-geometry (GEO/VLIDORT_LAMBERT.F90:172-342),
+ * which run VLIDORT_Run_Scalar / VLIDORT_Run_Vector (V283/VLIDORT_ScatMod.F90:
+ * 975-1247 / 284-617) over columns x channels with the fixed options of
+ * VLIDORT_Init (V283/VLIDORT_Mod.F90:65-383).  Fortran arrays:
+ *   channels(nch), ROT(km,nobs,nch), depol(nch), [alpha(km,nobs,nch)],
+ *   tau,ssa,[g](km,nch,nobs), pmom(km,nch,nobs,nMom,nPol), pe,he,te(km+1,nobs),
+ *   albedo(nobs,nch), solar_zenith/relat_azymuth/sensor_zenith(nobs),
+ *   [flux_factor(nch,nobs)] -> radiance,reflectance,[Q,U](nobs,nch).
+ * nfourier (may be NULL) receives the number of Fourier terms used per
+ * (obs, channel), 0 where MISSING.  precision: 0 = FP64 throughout (the
+ * drop-in), 1 = mixed (FP32 optics/phase-matrix/source assembly, FP64
+ * eigensystem and boundary-value solve).  Host pointers, synchronous. */
+int aa_vlidort_scalar_lambert(int km, int nch, int nobs, const double* channels, int nstreams,
+                              int plane_parallel, int nmom, int npol, const double* rot, const double* depol,
+                              const double* tau, const double* ssa, const double* g, const double* pmom,
+                              const double* pe, const double* he, const double* te, const double* albedo,
+                              const double* solar_zenith, const double* relat_azymuth,
+                              const double* sensor_zenith, double missing, int verbose, int precision,
+                              double* radiance, double* reflectance, int* nfourier, int* rc);
+
+int aa_vlidort_vector_lambert(int km, int nch, int nobs, const double* channels, int nstreams,
+                              int plane_parallel, int nmom, int npol, const double* rot, const double* depol,
+                              const double* alpha, const double* tau, const double* ssa, const double* pmom,
+                              const double* pe, const double* he, const double* te, const double* albedo,
+                              const double* solar_zenith, const double* relat_azymuth,
+                              const double* sensor_zenith, const double* flux_factor, double missing,
+                              int verbose, int precision, double* radiance, double* reflectance, double* q,
+                              double* u, int* nfourier, int* rc);
+
+/* ------------------------------------------------------------------ fused device path
+ * The whole hot path on resident data: Mie optics (tables in t) -> layer
+ * optics -> VLIDORT radiance, for nobs columns x nch channels, device pointers
+ * in the Fortran layouts above (qm(km,nq,nobs), rh(km,nobs) float32; rot,
+ * pe/he/te, albedo, angles, flux_factor float64).  nlayers (may be NULL,
+ * int[nobs]) gives a per-column layer count <= km (terrain truncation: the
+ * column is layers 1..nlayers counted from the top, its surface is edge
+ * nlayers+1); nstreams_col (may be NULL, int[nobs]) a per-column stream count
+ * (overrides nstreams).  vector
+ * selects NSTOKES=3.  Outputs radiance/reflectance/[q,u](nobs,nch) float64,
+ * nfourier(nobs,nch) int32.  Work buffers come from the library's device pool;
+ * enqueued on stream. */
+int aa_column_radiance_dev(aa_mie* t, int km, int nobs, int nch, int nq, const int* tracer_idx,
+                           const int* chan_idx, const float* d_qm, const float* d_rh, int nmom, int npol,
+                           int nstreams, int vector, int precision, const int* d_nlayers,
+                           const int* d_nstreams_col, const double* d_rot, const double* d_depol,
+                           const double* d_pe, const double* d_he, const double* d_te,
+                           const double* d_albedo, const double* d_sza, const double* d_raa,
+                           const double* d_vza, const double* d_flux_factor, double missing,
+                           double* d_radiance, double* d_reflectance, double* d_q, double* d_u,
+                           int* d_nfourier, void* stream);
+
+/* Library version string. */
+const char* aa_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AEROAPPS_AMD_H */

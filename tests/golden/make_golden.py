@@ -1,0 +1,88 @@
+"""Writes the radiance golden fixtures from the reference VLIDORT 2.8.3 core.
+
+The reference is built from its own sources by oracle/Makefile.ref into
+oracle/_ref/libvlidort_ref.so and driven through oracle/ref_driver.f90 (the
+VLIDORT_Scalar/Vector_Lambert driver set-up).  Inputs come from the seeded
+generator aeroapps_amd.synthetic and the float32 Chem_Mie restatement
+oracle.chem_mie; both inputs and reference outputs are stored, so the tests
+do not depend on regenerating them bit for bit.
+
+Run here (the reference is only available in this container):
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from aeroapps_amd import synthetic as S  # noqa: E402
+import oracle  # noqa: E402
+from oracle import chem_mie as CM  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+CASES = {
+    # name: (ncol, channels, nstreams, nmom, vector, plane_parallel, edge)
+    "scalar_n4_edge": (12, (550.0,), 4, 32, False, False, True),
+    "scalar_n6_4ch": (6, (354.0, 388.0, 550.0, 870.0), 6, 64, False, False, False),
+    "scalar_n2": (6, (550.0,), 2, 32, False, False, False),
+    "scalar_n8": (3, (550.0,), 8, 32, False, False, False),
+    "scalar_n4_pp": (4, (550.0,), 4, 32, False, True, False),
+    "vector_n4": (4, (550.0, 870.0), 4, 32, True, False, False),
+    "vector_n2": (4, (550.0,), 2, 32, True, False, False),
+}
+
+
+def make_inputs(ncol, channels, nmom, vector, edge, seed=7, col0=0):
+    npol = 6 if vector else 1
+    cols = S.make_columns(ncol, 72, channels, seed=seed, col0=col0)
+    tabs = S.make_mie_tables(channels, nmom=nmom, npol=npol)
+    tau, ssa, g, pm, rc = CM.get_aop_vector(tabs, channels, tabs[0].lambda_m, S.TRACERS, cols.qm, cols.rh, nmom, npol)
+    assert rc == 0
+    sza, vza, raa, albedo = cols.sza.copy(), cols.vza.copy(), cols.raa.copy(), cols.albedo.copy()
+    if edge:
+        vza[2] = 0.0            # nadir view: FO nadir-LOS geometry
+        sza[4] = S.MISSING      # missing geometry -> MISSING output
+        sza[7] = 0.0            # overhead sun: latches nadir FO mode for later columns
+        albedo[:, 9] = 0.0      # no surface term
+        albedo[:, 10] = S.MISSING
+        raa[11] = 0.0
+    return dict(km=72, channels=np.asarray(channels, float), nmom=nmom, npol=npol,
+                tau=tau, ssa=ssa, g=g, pmom=pm, rot=cols.rot, depol=cols.depol, pe=cols.pe, he=cols.he,
+                te=cols.te, albedo=albedo, sza=sza, vza=vza, raa=raa, flux_factor=cols.flux_factor)
+
+
+def run_driver(drv, d, nstreams, vector, plane_parallel):
+    f = lambda a: np.asarray(a, dtype=np.float64)
+    km = d["km"]
+    if vector:
+        alpha = np.zeros_like(d["rot"])
+        I, R, Q, U, nf, rc = drv.vector_lambert(km, d["channels"], nstreams, d["nmom"], d["rot"], d["depol"], alpha,
+                                                f(d["tau"]), f(d["ssa"]), f(d["pmom"]), d["pe"], d["he"], d["te"],
+                                                d["albedo"], d["sza"], d["raa"], d["vza"], d["flux_factor"],
+                                                plane_parallel=plane_parallel)
+        return dict(radiance=I, reflectance=R, q=Q, u=U, nfourier=nf, rc=rc)
+    I, R, nf, rc = drv.scalar_lambert(km, d["channels"], nstreams, d["nmom"], d["rot"], d["depol"], f(d["tau"]),
+                                      f(d["ssa"]), f(d["g"]), f(d["pmom"]), d["pe"], d["he"], d["te"], d["albedo"],
+                                      d["sza"], d["raa"], d["vza"], plane_parallel=plane_parallel)
+    return dict(radiance=I, reflectance=R, nfourier=nf, rc=rc)
+
+
+def main():
+    ref = oracle.ref()
+    if ref is None:
+        raise SystemExit("reference library not built: make -C oracle -f Makefile.ref")
+    for name, (ncol, ch, N, nmom, vector, pp, edge) in CASES.items():
+        d = make_inputs(ncol, ch, nmom, vector, edge)
+        out = run_driver(ref, d, N, vector, pp)
+        meta = dict(nstreams=N, vector=int(vector), plane_parallel=int(pp))
+        np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **d, **{"out_" + k: v for k, v in out.items()},
+                            **{"meta_" + k: v for k, v in meta.items()})
+        print(name, "rc", out["rc"], "nfourier", np.unique(out["nfourier"]), "I[:3]", out["radiance"].ravel()[:3])
+
+
+if __name__ == "__main__":
+    main()

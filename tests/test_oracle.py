@@ -1,0 +1,100 @@
+"""The oracle (C++ restatement, oracle/vlidort_oracle.cpp) against the golden
+fixtures written by the reference VLIDORT build (tests/golden/make_golden.py),
+and the Chem_Mie restatement against direct definitions."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import chem_mie as CM
+from aeroapps_amd import synthetic as S
+from _fixtures import RTOL_FP64, golden_names, load, max_rel
+
+
+def _port():
+    p = oracle.port()
+    if p is None:
+        pytest.fail("oracle/liboracle_vlidort.so not built (make -C oracle)")
+    return p
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_oracle_matches_reference_fixture(name):
+    d = load(name)
+    P = _port()
+    f = lambda a: np.asarray(a, dtype=np.float64)
+    N, pp = d["meta_nstreams"], bool(d["meta_plane_parallel"])
+    if d["meta_vector"]:
+        alpha = np.zeros_like(d["rot"])
+        I, R, Q, U, nf, rc = P.vector_lambert(d["km"], d["channels"], N, d["nmom"], d["rot"], d["depol"], alpha,
+                                              f(d["tau"]), f(d["ssa"]), f(d["pmom"]), d["pe"], d["he"], d["te"],
+                                              d["albedo"], d["sza"], d["raa"], d["vza"], d["flux_factor"],
+                                              plane_parallel=pp)
+        assert max_rel(Q, d["out_q"]) < RTOL_FP64 * 10
+        assert max_rel(U, d["out_u"]) < RTOL_FP64 * 10
+    else:
+        I, R, nf, rc = P.scalar_lambert(d["km"], d["channels"], N, d["nmom"], d["rot"], d["depol"], f(d["tau"]),
+                                        f(d["ssa"]), f(d["g"]), f(d["pmom"]), d["pe"], d["he"], d["te"],
+                                        d["albedo"], d["sza"], d["raa"], d["vza"], plane_parallel=pp)
+    assert rc == d["out_rc"]
+    assert max_rel(I, d["out_radiance"]) < RTOL_FP64
+    assert max_rel(R, d["out_reflectance"]) < RTOL_FP64
+    assert (nf == d["out_nfourier"]).all()
+
+
+def test_rh_map_exhaustive():
+    """Every one of the 991 hash slots against a direct search of the RH grid
+    (CB/Chem_MieTableMod.F90:612-627)."""
+    rh = (0.99 * np.arange(36) / 35).astype(np.float32)
+    rhi, rha = CM.rh_map(rh)
+    thr = [int(np.float32(r) * np.float32(1000.0)) for r in rh]
+    for j in range(991):
+        cand = [i for i in range(36) if j >= thr[i]]
+        i = max(cand)
+        assert rhi[j] == i + 1
+        if i + 1 < 36:
+            exp = np.float32(np.float32(np.float32(j) / np.float32(1000.0)) - rh[i]) / np.float32(rh[i + 1] - rh[i])
+            assert rha[j] == np.float32(exp)
+        else:
+            assert rha[j] == 0
+
+
+def test_get_aop_vector_brute_force():
+    """Tracer sum of the restatement against a per-element loop written from
+    the reference's equations (GEO/VLIDORT_MieMod.F90:283-334) on tiny inputs."""
+    chans = (550.0, 870.0)
+    nmom, npol = 8, 6
+    tabs = S.make_mie_tables(chans, nmom=nmom, npol=npol, seed=11)
+    cols = S.make_columns(3, 5, chans, seed=3)
+    tau, ssa, g, pm, rc = CM.get_aop_vector(tabs, chans, tabs[0].lambda_m, S.TRACERS, cols.qm, cols.rh, nmom, npol)
+    assert rc == 0
+    f32 = np.float32
+    names = []
+    for t in tabs:
+        names += [(t, b) for b in range(len(t.tracers))]
+    for o in range(3):
+        for ic in range(2):
+            for k in range(5):
+                T = Ssum = Gs = f32(0)
+                P = np.zeros((npol, nmom), f32)
+                for (t, b) in names:
+                    rhv = min(cols.rh[o, k], f32(0.99))
+                    isnap = int((rhv + f32(0.001)) * f32(1000.0))
+                    isnap = max(isnap, 1)
+                    rhi, rha = CM.rh_map(t.rh)
+                    irh = rhi[isnap - 1]
+                    arh = rha[isnap - 1]
+                    i0, i1 = irh - 1, min(irh, 35)
+                    lin = lambda a: f32(f32(a[i0] * f32(f32(1.0) - arh)) + f32(a[i1] * arh))
+                    bext = lin(t.bext[b, ic]); bsca = lin(t.bsca[b, ic]); gg = lin(t.g[b, ic])
+                    ti = f32(bext * cols.qm[o, S.TRACERS.index(t.tracers[b]), k])
+                    si = f32(bsca / bext)
+                    T = f32(T + ti); Ssum = f32(Ssum + f32(si * ti)); Gs = f32(Gs + f32(f32(gg * si) * ti))
+                    for p in range(npol):
+                        for m in range(nmom):
+                            pv = lin(t.pmom[p, m, b, ic])
+                            P[p, m] = f32(P[p, m] + f32(f32(pv * si) * ti))
+                assert tau[o, ic, k] == T
+                w = f32(Ssum / T)
+                assert ssa[o, ic, k] == w
+                assert g[o, ic, k] == f32(Gs / f32(w * T))
+                assert np.array_equal(pm[:, :, o, ic, k], (P / f32(w * T)).astype(f32))

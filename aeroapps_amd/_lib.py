@@ -40,6 +40,8 @@ SIGNATURES = {
                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _cd, _vp, _vp, _vp, _vp, _vp,
                                      _vp]),
     "aa_version": (ctypes.c_char_p, []),
+    "aa_profile_start": (_ci, []),
+    "aa_profile_read": (_ci, [_dp, ctypes.POINTER(ctypes.c_longlong), _ci]),
 }
 
 ERRORS = {-1: "AA_EARG (bad argument)", -2: "AA_EHIP (HIP runtime error)", -3: "AA_ENOTABLE (no table)",

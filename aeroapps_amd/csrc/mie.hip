@@ -15,6 +15,7 @@
 // layers of the same column: every access is coalesced.  The tables (< 1 MB)
 // stay in L2.
 #include "aa_common.h"
+#include "prof.h"
 
 #include <algorithm>
 #include <cctype>
@@ -289,9 +290,11 @@ int launch_get_aop(const aa_mie* t, int km, int nobs, int nch, int nq, const int
   if (ne == 0) return 0;
   int threads = 256;
   long blocks = (ne + threads - 1) / threads;
+  prof_begin(PROF_MIE, st);
   hipLaunchKernelGGL(get_aop_kernel, dim3((unsigned)blocks), dim3(threads), 0, st, t->dev, km, nobs, nch, nq,
                      d_tracer_idx, d_chan_idx, d_qm, d_rh, nmom, npol, ntr_eff, normalize, d_tau, d_ssa, d_g,
                      d_pmom);
+  prof_end(PROF_MIE, st);
   AA_CHECK(hipGetLastError());
   return 0;
 }

@@ -28,6 +28,7 @@
 #pragma once
 #include "rt_work.h"
 #include "rt_device.h"
+#include "prof.h"
 
 namespace aa {
 
@@ -779,8 +780,12 @@ int fourier_loop(const RTWork& W, int nitems, int* const lists[2], int* counts, 
   for (int M = 0; M <= W.nmoments; ++M) {
     const int* in = lists[M & 1];
     int* out = lists[(M + 1) & 1];
+    prof_begin(PROF_LAYER, st);
     hipLaunchKernelGGL((k_layer<N, S>), dim3(nl), dim3(64), 0, st, W, M, in, counts + M);
+    prof_end(PROF_LAYER, st);
+    prof_begin(PROF_BVP, st);
     hipLaunchKernelGGL((k_bvp<N, S>), dim3(nitems), dim3(64), 0, st, W, M, in, counts + M, out, counts + M + 1);
+    prof_end(PROF_BVP, st);
   }
   AA_CHECK(hipGetLastError());
   return 0;

@@ -19,6 +19,7 @@
 // thread per item for the vertical recursions.  All FP64.
 #include "rt_work.h"
 #include "rt_device.h"
+#include "prof.h"
 
 namespace aa {
 
@@ -681,7 +682,10 @@ int rt_setup(const RTWork& W, int nitems, int* d_active, int* d_nactive, hipStre
   const int B = 256;
   const int gi = (nitems + B - 1) / B;
   const int gl = (int)(((size_t)nitems * W.km + B - 1) / B);
+  prof_begin(PROF_CHECK, st);
   hipLaunchKernelGGL(k_check, dim3(gi), dim3(B), 0, st, W, nitems);
+  prof_end(PROF_CHECK, st);
+  prof_begin(PROF_OPTICS, st);
   if (W.S == 1) {
     if (W.optics_f32) hipLaunchKernelGGL((k_optics<1, float>), dim3(gl), dim3(B), 0, st, W, nitems);
     else hipLaunchKernelGGL((k_optics<1, double>), dim3(gl), dim3(B), 0, st, W, nitems);
@@ -689,9 +693,14 @@ int rt_setup(const RTWork& W, int nitems, int* d_active, int* d_nactive, hipStre
     if (W.optics_f32) hipLaunchKernelGGL((k_optics<3, float>), dim3(gl), dim3(B), 0, st, W, nitems);
     else hipLaunchKernelGGL((k_optics<3, double>), dim3(gl), dim3(B), 0, st, W, nitems);
   }
+  prof_end(PROF_OPTICS, st);
+  prof_begin(PROF_GEOM, st);
   hipLaunchKernelGGL(k_geom, dim3(gl), dim3(B), 0, st, W, nitems);
+  prof_end(PROF_GEOM, st);
+  prof_begin(PROF_ITEM, st);
   if (W.S == 1) hipLaunchKernelGGL((k_item<1>), dim3(gi), dim3(B), 0, st, W, nitems, d_active, d_nactive);
   else hipLaunchKernelGGL((k_item<3>), dim3(gi), dim3(B), 0, st, W, nitems, d_active, d_nactive);
+  prof_end(PROF_ITEM, st);
   AA_CHECK(hipGetLastError());
   return 0;
 }

@@ -311,3 +311,107 @@ WORKLOADS: Dict[str, dict] = {
     "cfg3_uneven_300k": dict(ncol=300_000, km=72, channels=(550.0,), nstreams=None, nmom=32, npol=1, vector=False,
                              uneven_layers=(30, 72), stream_choices=(2, 4, 6, 8)),
 }
+
+
+# ---------------------------------------------------------------------------
+# Device-side generator (torch) for large workloads.  The integer hash is the
+# same SplitMix64 chain as ``uniform`` (two's-complement int64 arithmetic with
+# logical shifts), so the uniforms are bit-identical to the host generator; the
+# float transforms run on the GPU and agree with the host generator to within
+# float rounding.
+
+def _to_i64(v: int) -> int:
+    v &= 0xFFFFFFFFFFFFFFFF
+    return v - (1 << 64) if v >= (1 << 63) else v
+
+
+def _srl(x, k: int):
+    """Logical right shift of an int64 tensor."""
+    import torch
+    return (x >> k) & ((1 << (64 - k)) - 1)
+
+
+def _splitmix64_t(x):
+    x = x + _to_i64(0x9E3779B97F4A7C15)
+    x = x ^ _srl(x, 30)
+    x = x * _to_i64(0xBF58476D1CE4E5B9)
+    x = x ^ _srl(x, 27)
+    x = x * _to_i64(0x94D049BB133111EB)
+    x = x ^ _srl(x, 31)
+    return x
+
+
+def uniform_t(seed: int, field: int, *idx):
+    """torch version of ``uniform`` (idx are int64 tensors, broadcast)."""
+    import torch
+    key = np.uint64((seed * 0x100000001B3 + field * 0x9E3779B1) & 0xFFFFFFFFFFFFFFFF)
+    h = int(_splitmix64(np.asarray(key, dtype=np.uint64)))
+    arrs = torch.broadcast_tensors(*idx)
+    out = torch.full(arrs[0].shape, _to_i64(h), dtype=torch.int64, device=arrs[0].device)
+    for a in arrs:
+        out = _splitmix64_t(out ^ a)
+    return _srl(out, 11).to(torch.float64) * (1.0 / 9007199254740992.0)
+
+
+def make_columns_torch(ncol: int, km: int, channels_nm: Sequence[float], seed: int = 7, col0: int = 0,
+                       device="cuda", uneven_layers: Tuple[int, int] | None = None) -> Dict[str, object]:
+    """Device tensors of ``make_columns`` (same shapes, reversed-Fortran C order)."""
+    import torch
+    dev = torch.device(device)
+    f32, f64 = torch.float32, torch.float64
+    cols = torch.arange(col0, col0 + ncol, dtype=torch.int64, device=dev)
+    c2 = cols[:, None]
+    kk = torch.arange(km, device=dev, dtype=torch.int64)[None, :]
+    nq = len(TRACERS)
+    delp = (200.0 + 1800.0 * uniform_t(seed, _F_DELP, c2, kk)).to(f32)
+    c3 = cols[:, None, None]
+    q3 = torch.arange(nq, device=dev, dtype=torch.int64)[None, :, None]
+    k3 = torch.arange(km, device=dev, dtype=torch.int64)[None, None, :]
+    u1 = uniform_t(seed, _F_Q1, c3, q3, k3)
+    u2 = uniform_t(seed, _F_Q2, c3, q3, k3)
+    z = torch.sqrt(-2.0 * torch.log1p(-u1)) * torch.cos(2.0 * math.pi * u2)
+    del u1, u2
+    q = 1.0e-9 * torch.exp(math.log(1.5) * z)
+    del z
+    qm = (q.to(f32) * delp[:, None, :] / torch.tensor(9.81, dtype=f32, device=dev)).to(f32)
+    del q
+    rh = (0.99 * uniform_t(seed, _F_RH, c2, kk)).to(f32)
+    # edge variables (getEdgeVars, float32)
+    pe = torch.empty((ncol, km + 1), dtype=f32, device=dev)
+    pe[:, 0] = 1.0
+    for k in range(km):
+        pe[:, k + 1] = pe[:, k] + delp[:, k]
+    pm_true = 0.5 * (pe[:, :-1].to(f64) + pe[:, 1:].to(f64))
+    tstd = torch.clamp(288.15 * (pm_true / 101325.0) ** 0.190263, min=216.65)
+    airdens = (pm_true / (287.0 * tstd)).to(f32)
+    grav = torch.tensor(9.80616, dtype=f32, device=dev)
+    ze = torch.empty((ncol, km + 1), dtype=f32, device=dev)
+    ze[:, km] = 0.0
+    for k in range(km - 1, -1, -1):
+        ze[:, k] = ze[:, k + 1] + delp[:, k] / (airdens[:, k] * grav)
+    pmid = (pe[:, :-1] + pe[:, 1:]) / 2.0
+    tm = pmid / (airdens * torch.tensor(287.0, dtype=f32, device=dev))
+    te = torch.empty((ncol, km + 1), dtype=f32, device=dev)
+    te[:, 0] = tm[:, 0]
+    for k in range(1, km):
+        a = torch.log(pe[:, k] / pmid[:, k - 1]) / torch.log(pmid[:, k] / pmid[:, k - 1])
+        te[:, k] = tm[:, k - 1] + a * (tm[:, k] - tm[:, k - 1])
+    te[:, km] = tm[:, km - 1] * (pe[:, km] / pmid[:, km - 1]) ** (2.0 / 7.0)
+    sza = 70.0 * uniform_t(seed, _F_SZA, cols)
+    vza = 60.0 * uniform_t(seed, _F_VZA, cols)
+    raa = 180.0 * uniform_t(seed, _F_RAA, cols)
+    ch = [float(w) for w in channels_nm]
+    dp64 = delp.to(f64)
+    frac = dp64 / dp64.sum(dim=1, keepdim=True)
+    a550 = _rayleigh_xsec(0.55)
+    rot = torch.stack([0.097 * frac * (_rayleigh_xsec(w * 1e-3) / a550) for w in ch], dim=0).contiguous()
+    nch = len(ch)
+    if uneven_layers is None:
+        nl = torch.full((ncol,), km, dtype=torch.int32, device=dev)
+    else:
+        lo, hi = uneven_layers
+        nl = (lo + torch.floor((hi - lo + 1) * uniform_t(seed, _F_NLAY, cols))).to(torch.int32)
+    return dict(km=km, channels=np.asarray(ch), qm=qm.contiguous(), rh=rh.contiguous(), rot=rot,
+                depol=torch.full((nch,), 0.03, dtype=f64, device=dev), pe=pe.to(f64), he=ze.to(f64), te=te.to(f64),
+                albedo=torch.full((nch, ncol), 0.05, dtype=f64, device=dev), sza=sza, vza=vza, raa=raa,
+                flux_factor=torch.ones((ncol, nch), dtype=f64, device=dev), nlayers=nl)

@@ -1,0 +1,257 @@
+"""Benchmark of the aerosol-optics + discrete-ordinate radiance path on one GPU.
+
+Workload (default): 500,000 columns x 72 layers, 4 channels (354, 388, 550,
+870 nm), 12 streams (6 per hemisphere), 64 Legendre moments, 15 tracers,
+Lambertian albedo 0.05 -- seeded synthetic inputs generated on the device.  A
+step is one pass of the fused device path (Chem_Mie optics -> layer optics ->
+radiance) over every column, with inputs resident in HBM.
+
+Prints one JSON line (metric, value in columns/s, ms_per_step, the dominant
+kernel's roofline figures measured with HIP events on its launch stream, and
+the host-CPU reference timed on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from aeroapps_amd import synthetic as S  # noqa: E402
+
+METRIC = "columns/sec at 72 layers/12 streams, mixed vs FP64 GPU; p99 granule latency"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+PROF_NAMES = ["mie", "check", "optics", "geom", "item", "layer", "bvp"]
+
+
+# ----------------------------------------------------------------- algorithmic work
+def flops_per_term(N, S_, L, nmoments, M):
+    """Algorithmic FP64 flops of one (column, channel) at Fourier term M:
+    eigensystem (F_b), band LU + solve (F_c), source (F_d)."""
+    n = S_ * N
+    NT = 2 * n * L
+    kl = ku = 3 * n - 1
+    Lm = nmoments - M + 1
+    Fb = L * (4 * S_ * n * n * Lm + 4 * N * S_ ** 3 * Lm + 2 * n ** 3 + 25 * n ** 3)
+    Fc = 2 * NT * kl * (kl + ku) + 2 * NT * (2 * kl + ku)
+    Fd = L * (8 * n * S_ + 10 * n * n)
+    return Fb, Fc, Fd
+
+
+def bytes_per_column(km, nq, nch, vector):
+    """Compulsory HBM bytes of the fused path per column (inputs + outputs)."""
+    b = 4 * km * nq + 4 * km + 8 * km * nch + 24 * (km + 1) + 24 + 8 * nch
+    if vector:
+        b += 8 * nch
+    b += 8 * nch * (2 + 2 * int(vector))
+    return b
+
+
+# ----------------------------------------------------------------- CPU baseline
+def _cpu_worker(args):
+    (c0, nc, wl, kind) = args
+    import oracle
+    from oracle import chem_mie as CM
+    drv = oracle.ref() if kind == "reference" else oracle.port()
+    ch = wl["channels"]
+    t0 = time.perf_counter()
+    cols = S.make_columns(nc, wl["km"], ch, seed=7, col0=c0)
+    tabs = S.make_mie_tables(ch, nmom=wl["nmom"], npol=wl["npol"])
+    gen = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tau, ssa, g, pm, rc = CM.get_aop_vector(tabs, ch, tabs[0].lambda_m, S.TRACERS, cols.qm, cols.rh, wl["nmom"],
+                                            wl["npol"])
+    f = lambda a: np.asarray(a, dtype=np.float64)
+    drv.scalar_lambert(wl["km"], cols.channels, wl["nstreams"], wl["nmom"], cols.rot, cols.depol, f(tau), f(ssa),
+                       f(g), f(pm), cols.pe, cols.he, cols.te, cols.albedo, cols.sza, cols.raa, cols.vza)
+    return time.perf_counter() - t0, gen
+
+
+def cpu_baseline(wl, ncols, procs):
+    import oracle
+    kind = "reference" if os.path.exists(oracle.REF_SO) else "port"
+    if kind == "port" and not os.path.exists(oracle.PORT_SO):
+        return None
+    per = max(1, ncols // procs)
+    jobs = [(i * per, per, wl, kind) for i in range(procs)]
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(procs) as pool:
+        res = pool.map(_cpu_worker, jobs)
+    wall = time.perf_counter() - t0
+    busy = max(r[0] for r in res)
+    ncol = per * procs
+    return {
+        "value": ncol / busy,
+        "unit": "columns/s",
+        "cores": procs,
+        "kind": kind,
+        "sample": (f"{ncol} columns ({per} per process, {procs} processes, one core each) of the same workload: "
+                   f"float32 Chem_Mie restatement + {'reference VLIDORT 2.8.3 (flang -O3)' if kind == 'reference' else 'C++ restatement'} "
+                   f"scalar Lambertian driver, {len(wl['channels'])} channels, N={wl['nstreams']}, nmom={wl['nmom']}; "
+                   f"slowest process {busy:.1f} s, pool wall {wall:.1f} s"),
+    }
+
+
+# ----------------------------------------------------------------- GPU run
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg1_scalar_500k")
+    ap.add_argument("--ncol", type=int, default=0, help="override the column count")
+    ap.add_argument("--precision", type=int, default=0)
+    ap.add_argument("--cpu-cols", type=int, default=512)
+    ap.add_argument("--cpu-procs", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the path has no cross-GPU step (every column is independent); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    wl = dict(S.WORKLOADS[args.workload])
+    ncol = args.ncol or wl["ncol"]
+    km, nch, nq = wl["km"], len(wl["channels"]), len(S.TRACERS)
+    N, nmom, npol, vector = wl["nstreams"], wl["nmom"], wl["npol"], wl["vector"]
+
+    # host-CPU baseline first, before this process touches the GPU (fork-safe)
+    cpu = None
+    if not args.no_cpu:
+        procs = args.cpu_procs or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(wl, args.cpu_cols, procs)
+
+    import torch
+    from aeroapps_amd import _lib
+    from aeroapps_amd import chem_mie as AM
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    dev = torch.device("cuda:0")
+    L = _lib.lib()
+
+    # inputs, generated on the device in slices
+    t0 = time.perf_counter()
+    parts = []
+    step = 100_000
+    for c0 in range(0, ncol, step):
+        parts.append(S.make_columns_torch(min(step, ncol - c0), km, wl["channels"], seed=7, col0=c0, device=dev))
+    cat = lambda k, dim: torch.cat([p[k] for p in parts], dim=dim).contiguous()
+    qm = cat("qm", 0)            # [nobs, nq, km]  == Fortran qm(km,nq,nobs)
+    rh = cat("rh", 0)            # [nobs, km]
+    rot = cat("rot", 1)          # [nch, nobs, km]
+    he = cat("he", 0)            # [nobs, km+1]
+    pe = cat("pe", 0)
+    te = cat("te", 0)
+    albedo = cat("albedo", 1)    # [nch, nobs]
+    sza, vza, raa = cat("sza", 0), cat("vza", 0), cat("raa", 0)
+    flux = cat("flux_factor", 0)  # [nobs, nch]
+    depol = parts[0]["depol"]
+    del parts
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+
+    tabs = S.make_mie_tables(wl["channels"], nmom=nmom, npol=npol)
+    mie = AM.MieTables.from_synthetic(tabs)
+    tr_idx = torch.tensor([mie.query_idx(n) for n in S.TRACERS], dtype=torch.int32, device=dev)
+    ch_idx = torch.arange(nch, dtype=torch.int32, device=dev)
+    rad = torch.empty((nch, ncol), dtype=torch.float64, device=dev)
+    refl = torch.empty_like(rad)
+    qo = torch.empty_like(rad) if vector else None
+    uo = torch.empty_like(rad) if vector else None
+    nf = torch.empty((nch, ncol), dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    ptr = lambda t: t.data_ptr() if t is not None else None
+
+    def run_step():
+        e = L.aa_column_radiance_dev(mie.handle, km, ncol, nch, nq, ptr(tr_idx), ptr(ch_idx), ptr(qm), ptr(rh), nmom,
+                                     npol, N, int(vector), args.precision, None, None, ptr(rot), ptr(depol), ptr(pe),
+                                     ptr(he), ptr(te), ptr(albedo), ptr(sza), ptr(raa), ptr(vza), ptr(flux),
+                                     S.MISSING, ptr(rad), ptr(refl), ptr(qo), ptr(uo), ptr(nf), stream.cuda_stream)
+        _lib.check(e, "aa_column_radiance_dev")
+
+    for i in range(args.warmup):
+        run_step()
+        torch.cuda.synchronize()
+        print(f"warmup step {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    L.aa_profile_start()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        run_step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    import ctypes
+    msa = (ctypes.c_double * 8)()
+    lna = (ctypes.c_longlong * 8)()
+    _lib.check(L.aa_profile_read(msa, lna, 8), "aa_profile_read")
+    kms = {PROF_NAMES[k]: msa[k] for k in range(7)}
+    kln = {PROF_NAMES[k]: lna[k] for k in range(7)}
+
+    ms_per_step = el * 1000.0 / args.steps
+    value = ncol / (ms_per_step / 1000.0)
+
+    # algorithmic work of the timed region from the Fourier-term counts
+    nfh = nf.cpu().numpy().ravel()
+    ok = nfh > 0
+    nmoments = min(2 * N - 1, nmom)
+    Sx = 3 if vector else 1
+    fb = fc = fd = 0.0
+    counts = np.bincount(nfh[ok], minlength=2 * N + 1)
+    for terms, cnt in enumerate(counts):
+        for M in range(terms):
+            b, c, d = flops_per_term(N, Sx, km, nmoments, M)
+            fb += cnt * b; fc += cnt * c; fd += cnt * d
+    fb *= args.steps; fc *= args.steps; fd *= args.steps
+    dom = max(("layer", "bvp"), key=lambda k: kms[k])
+    dom_flops = fc + fd if dom == "bvp" else fb
+    dom_ms = kms[dom]
+    dom_launches = max(1, kln[dom])
+    achieved = dom_flops / (dom_ms / 1000.0) / 1e12
+    path_bytes = bytes_per_column(km, nq, nch, vector) * ncol * args.steps
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "columns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == 0 else "f32+f64",
+        "data": "synthetic (seeded, generated on device)",
+        "config": {"workload": f"{args.workload}: {ncol} columns x {km} layers, {nch} channels "
+                               f"{list(wl['channels'])} nm, {2 * N} streams, {nmom} moments, {nq} tracers, "
+                               f"{'vector' if vector else 'scalar'} Lambertian",
+                   "precision": "fp64" if args.precision == 0 else "mixed",
+                   "path": "aa_column_radiance_dev (Chem_Mie optics + VLIDORT radiance, device-resident)"},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "kernel": f"k_{dom}<{N},{Sx}>", "kernel_avg_ms": round(dom_ms / dom_launches, 4),
+                     "kernel_launches": int(dom_launches),
+                     "note": "FP64 VALU work (no MFMA); peak is MI355X FP64 (vector = matrix rate)"},
+        "hbm_path": {"bytes_per_column": bytes_per_column(km, nq, nch, vector),
+                     "achieved_GBs": round(path_bytes / el / 1e9, 3),
+                     "frac": round(path_bytes / el / 1e9 / PEAK_HBM_GBS, 6)},
+        "kernel_ms": {k: round(v, 3) for k, v in kms.items()},
+        "fourier_terms_mean": round(float(nfh[ok].mean()), 3),
+        "cpu_baseline": cpu,
+        "gen_s": round(gen_s, 2),
+    }
+    if cpu:
+        out["gpu_vs_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -93,7 +93,7 @@ int aa_get_aop_scalar(aa_mie* t, int km, int nobs, int nch, int nq, const float*
 
 /* Device-pointer form of aa_get_aop_vector: same layouts, tracer indices
  * already resolved (tracer_idx[nq], -1 = skipped) and channel indices
- * resolved (chan_idx[nch], 0-based), enqueued on stream. */
+ * resolved (chan_idx[nch], 0-based), both DEVICE arrays; enqueued on stream. */
 int aa_get_aop_vector_dev(aa_mie* t, int km, int nobs, int nch, int nq, const int* tracer_idx,
                           const int* chan_idx, const float* d_qm, const float* d_rh, int nmom, int npol,
                           float* d_tau, float* d_ssa, float* d_g, float* d_pmom, void* stream);
@@ -153,6 +153,17 @@ int aa_column_radiance_dev(aa_mie* t, int km, int nobs, int nch, int nq, const i
 
 /* Library version string. */
 const char* aa_version(void);
+
+/* ------------------------------------------------------------------ measurement hook
+ * No counterpart in the reference.  aa_profile_start() makes every later
+ * kernel launch of the library record a pair of HIP events on its launch
+ * stream; aa_profile_read() synchronises the device, returns per-class total
+ * milliseconds and launch counts for nkinds classes (0 Chem_Mie optics,
+ * 1 input checks, 2 layer optics/delta-M, 3 Chapman + first-order geometry,
+ * 4 per-column setup, 5 per-layer eigensystems, 6 boundary-value problem +
+ * source + convergence) and switches recording off. */
+int aa_profile_start(void);
+int aa_profile_read(double* ms, long long* launches, int nkinds);
 
 #ifdef __cplusplus
 }

@@ -37,12 +37,63 @@ namespace {
 constexpr double DMS[4] = {1.0, 1.0, -1.0, -1.0};   // DMAT diagonal
 constexpr double VLIDORT_ACCURACY_D = (double)0.0001f;   // V283/VLIDORT_Mod.F90:232
 
+// Row slots of one band column live in up to five vectors of 8 doubles
+// (P##0..P##4).  With a wave-uniform slot index these macros compile to
+// indexed register moves; the vectors must stay plain local variables (no
+// references or lambdas) for that to happen.
+typedef double d8v __attribute__((ext_vector_type(8)));
+#define AA_SLOT_GET(P, r, s)                                 \
+  {                                                          \
+    const int k_ = (s) >> 3, e_ = (s) & 7;                   \
+    r = P##0[e_];                                            \
+    if (NV > 1 && k_ == 1) r = P##1[e_];                     \
+    if (NV > 2 && k_ == 2) r = P##2[e_];                     \
+    if (NV > 3 && k_ == 3) r = P##3[e_];                     \
+    if (NV > 4 && k_ == 4) r = P##4[e_];                     \
+  }
+#define AA_SLOT_SET(P, s, x)                                 \
+  {                                                          \
+    const int k_ = (s) >> 3, e_ = (s) & 7;                   \
+    if (k_ == 0) P##0[e_] = (x);                             \
+    if (NV > 1 && k_ == 1) P##1[e_] = (x);                   \
+    if (NV > 2 && k_ == 2) P##2[e_] = (x);                   \
+    if (NV > 3 && k_ == 3) P##3[e_] = (x);                   \
+    if (NV > 4 && k_ == 4) P##4[e_] = (x);                   \
+  }
+#define AA_XV(K) d8v{xc[K * 8 + 0], xc[K * 8 + 1], xc[K * 8 + 2], xc[K * 8 + 3], \
+                     xc[K * 8 + 4], xc[K * 8 + 5], xc[K * 8 + 6], xc[K * 8 + 7]}
+#define AA_SLOT_AXPY(P, w)                                   \
+  {                                                          \
+    P##0 = P##0 - AA_XV(0) * (w);                            \
+    if (NV > 1) P##1 = P##1 - AA_XV(1) * (w);                \
+    if (NV > 2) P##2 = P##2 - AA_XV(2) * (w);                \
+    if (NV > 3) P##3 = P##3 - AA_XV(3) * (w);                \
+    if (NV > 4) P##4 = P##4 - AA_XV(4) * (w);                \
+  }
+
 __device__ __forceinline__ double readlane_d(double v, int lane) {
   int lo = __double2loint(v), hi = __double2hiint(v);
   lo = __builtin_amdgcn_readlane(lo, lane);
   hi = __builtin_amdgcn_readlane(hi, lane);
   return __hiloint2double(hi, lo);
 }
+
+// Lane exchange of a double within 16-lane rows (DPP) and across the two rows
+// of a 32-lane half (swizzle); used by the max-reductions of the band LU.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+  int lo = __double2loint(x), hi = __double2hiint(x);
+  lo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double swz16_d(double x) {
+  int lo = __double2loint(x), hi = __double2hiint(x);
+  lo = __builtin_amdgcn_ds_swizzle(lo, 0x401F);
+  hi = __builtin_amdgcn_ds_swizzle(hi, 0x401F);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 
 // ------------------------------------------------------------------ k_layer
 template <int N, int S>
@@ -582,143 +633,218 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
   };
 
   // ---------------- band LU with partial pivoting + forward substitution
-  double val[CPL][R];
-  int pos[R];
+  // Step j: the lane owning column j copies it to LDS; lanes 0..R-1 read it
+  // back transposed (lane = row slot) and a butterfly finds the pivot: largest
+  // |a(r,j)|, ties to the smallest row position (IDAMAX order of DGBTF2).  Every
+  // lane then eliminates its column with the broadcast column:
+  //   a(s,c) -= a(s,j) * (a(p,c) / a(p,j)).
+  // The column leaving the window is zeroed exactly (its own scale is 1), so a
+  // lane re-entering as a new column starts from zeros.  The row entering the
+  // window after step j (position j+R) is independent of the pivoting and is
+  // loaded PD steps ahead.
+  constexpr int PD = 4;
+  constexpr int NV = (R + 7) / 8;
+  static_assert(NV <= 5 && R <= 64, "candidate rows exceed the register layout");
+  __shared__ double colbuf[NV * 8];
+  d8v a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;   // column slot 0
+  d8v b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0, b4 = 0.0;   // column slot 1 (two columns per lane)
 #pragma unroll
-  for (int s = 0; s < R; ++s) pos[s] = (s < NT) ? s : BIG;
-#pragma unroll
-  for (int cs = 0; cs < CPL; ++cs) {
-    const int v = lane + 64 * cs;
-#pragma unroll
-    for (int s = 0; s < R; ++s) val[cs][s] = gen(v, s, 0, 0);
+  for (int s = 0; s < R; ++s) {
+    AA_SLOT_SET(a, s, gen(lane, s, 0, 0));
+    if (CPL > 1) AA_SLOT_SET(b, s, gen(lane + 64, s, 0, 0));
   }
+  int posl = (lane < R && lane < NT) ? lane : BIG;   // row position held by slot `lane`
+  if (lane < NV * 8) colbuf[lane] = 0.0;
+  auto fetch_row = [&](int j, double (&dst)[CPL]) {
+    const int j1 = j + 1;
+    const int jm1x = j1 % Q;
+#pragma unroll
+    for (int cs = 0; cs < CPL; ++cs) dst[cs] = gen(lane + 64 * cs, j + R, j1, jm1x);
+  };
+  double nvq[PD][CPL];
+#pragma unroll
+  for (int d = 0; d < PD; ++d) fetch_row(d, nvq[d]);
   bool singular = false;
   int jm = 0;   // j mod Q
-  for (int j = 0; j < NT; ++j) {
-    const int jm1 = (jm + 1 == Q) ? 0 : jm + 1;
-    const int rnew = j + R;
-    double nv[CPL];
+  __syncthreads();
+  for (int jb = 0; jb < NT; jb += PD) {
+    double nvn[PD][CPL];
 #pragma unroll
-    for (int cs = 0; cs < CPL; ++cs) nv[cs] = gen(lane + 64 * cs, rnew, j + 1, jm1);
-    // pivot column j lives in virtual lane jm
-    const int lj = jm & 63, csj = jm >> 6;
-    double x[R];
+    for (int d = 0; d < PD; ++d) fetch_row(jb + PD + d, nvn[d]);
 #pragma unroll
-    for (int s = 0; s < R; ++s) x[s] = (CPL == 1 || csj == 0) ? val[0][s] : val[CPL - 1][s];
-    double best = -1.0;
-    int bs = 0, bpos = BIG;
+    for (int d = 0; d < PD; ++d) {
+      const int j = jb + d;
+      if (j >= NT) break;
+      const int jm1 = (jm + 1 == Q) ? 0 : jm + 1;
+      const int rnew = j + R;
+      const int lj = jm & 63, csj = jm >> 6;
+      // column j to LDS
+      if (lane == lj) {
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-      const double av = fabs(x[s]);
-      const bool better = (pos[s] < BIG) && (av > best || (av == best && pos[s] < bpos));
-      if (better) { best = av; bpos = pos[s]; bs = s; }
-    }
-    const int bsu = __builtin_amdgcn_readlane(bs, lj);
-    const int jp = __builtin_amdgcn_readlane(bpos, lj);
-    double xpv = 0.0;
-#pragma unroll
-    for (int s = 0; s < R; ++s) if (s == bsu) xpv = x[s];
-    const double pv = readlane_d(xpv, lj);
-    if (pv == 0.0) singular = true;
-    const double rp = 1.0 / pv;
-    double lm[R];
-#pragma unroll
-    for (int s = 0; s < R; ++s) lm[s] = (s == bsu) ? 0.0 : readlane_d(x[s], lj) * rp;
-    // eliminate, store the pivot row (U row j and transformed RHS)
-#pragma unroll
-    for (int cs = 0; cs < CPL; ++cs) {
-      double u = 0.0;
-#pragma unroll
-      for (int s = 0; s < R; ++s) if (s == bsu) u = val[cs][s];
-#pragma unroll
-      for (int s = 0; s < R; ++s) val[cs][s] = val[cs][s] - lm[s] * u;
-      const int v = lane + 64 * cs;
-      if (v == Q) ub[(size_t)j * UW + WD] = u;
-      else {
-        int off = v - jm;
-        if (off < 0) off += Q;
-        if (off < WD && j + off < NT) ub[(size_t)j * UW + off] = u;
+        for (int s = 0; s < R; ++s) {
+          double x;
+          if (CPL == 1 || csj == 0) AA_SLOT_GET(a, x, s) else AA_SLOT_GET(b, x, s)
+          colbuf[s] = x;
+        }
       }
-    }
-    // row interchange bookkeeping: the row that sat at position j moves to jp
-#pragma unroll
-    for (int s = 0; s < R; ++s) if (pos[s] == j && s != bsu) pos[s] = jp;
-    // the pivot slot takes the next row; a column entering the window starts at zero
-#pragma unroll
-    for (int s = 0; s < R; ++s) if (s == bsu) pos[s] = (rnew < NT) ? rnew : BIG;
-#pragma unroll
-    for (int cs = 0; cs < CPL; ++cs) {
-      const int v = lane + 64 * cs;
-      bool entering = false;
-      if (v != Q) {
-        int off1 = v - jm1;
-        if (off1 < 0) off1 += Q;
-        entering = (off1 == WD - 1);
+      __syncthreads();
+      // pivot search, lane = row slot: max |a(s,j)| over the candidate rows,
+      // ties resolved to the smallest row position
+      const double xt = (lane < R) ? colbuf[lane] : 0.0;
+      double am = (lane < R && posl < BIG) ? fabs(xt) : -1.0;
+      am = dmax(am, dpp_d<0xB1>(am));
+      am = dmax(am, dpp_d<0x4E>(am));
+      am = dmax(am, dpp_d<0x141>(am));
+      am = dmax(am, dpp_d<0x140>(am));
+      if (R > 16) am = dmax(am, swz16_d(am));
+      double mx = readlane_d(am, 0);
+      if (R > 32) mx = dmax(mx, readlane_d(am, 32));
+      unsigned long long cand = __ballot(lane < R && posl < BIG && fabs(xt) == mx);
+      int bsu = (int)__builtin_ctzll(cand);
+      if (__popcll(cand) > 1) {   // equal magnitudes: smallest position wins
+        int best = BIG;
+        for (unsigned long long m = cand; m; m &= m - 1) {
+          const int l = (int)__builtin_ctzll(m);
+          const int pl = __builtin_amdgcn_readlane(posl, l);
+          if (pl < best) { best = pl; bsu = l; }
+        }
       }
+      const int jp = __builtin_amdgcn_readlane(posl, bsu);
+      const int sj = (int)__builtin_ctzll(__ballot(lane < R && posl == j));
+      const double pv = readlane_d(xt, bsu);
+      if (pv == 0.0) singular = true;
+      const double rp = 1.0 / pv;
+      double xc[NV * 8];
 #pragma unroll
-      for (int s = 0; s < R; ++s) val[cs][s] = (s == bsu) ? nv[cs] : (entering ? 0.0 : val[cs][s]);
+      for (int s = 0; s < NV * 8; ++s) xc[s] = colbuf[s];
+      // eliminate, store the pivot row (U row j and transformed RHS), and
+      // put the next row into the pivot slot
+      {
+        double u;
+        AA_SLOT_GET(a, u, bsu);
+        const int v = lane;
+        const double w = (v == jm) ? 1.0 : u * rp;
+        AA_SLOT_AXPY(a, w);
+        if (v == Q) ub[(size_t)j * UW + WD] = u;
+        else {
+          int off = v - jm;
+          if (off < 0) off += Q;
+          if (off < WD && j + off < NT) ub[(size_t)j * UW + off] = u;
+        }
+        AA_SLOT_SET(a, bsu, nvq[d][0]);
+      }
+      if (CPL > 1) {
+        double u;
+        AA_SLOT_GET(b, u, bsu);
+        const int v = lane + 64;
+        const double w = (v == jm) ? 1.0 : u * rp;
+        AA_SLOT_AXPY(b, w);
+        if (v == Q) ub[(size_t)j * UW + WD] = u;
+        else {
+          int off = v - jm;
+          if (off < 0) off += Q;
+          if (off < WD && j + off < NT) ub[(size_t)j * UW + off] = u;
+        }
+        AA_SLOT_SET(b, bsu, nvq[d][CPL - 1]);
+      }
+      // row interchange bookkeeping: the row at position j moves to position jp
+      if (lane == sj && sj != bsu) posl = jp;
+      if (lane == bsu) posl = (rnew < NT) ? rnew : BIG;
+      __syncthreads();
+      jm = jm1;
     }
-    jm = jm1;
+#pragma unroll
+    for (int d = 0; d < PD; ++d)
+#pragma unroll
+      for (int cs = 0; cs < CPL; ++cs) nvq[d][cs] = nvn[d][cs];
   }
 
   // ---------------- back-substitution fused with the TOA source recursion
-  // lanes hold pending right-hand sides: virtual lane v <-> rows i = v mod P
-  double bvv[CPL];
+  // Row-oriented: lanes hold the solution values of the columns in the window
+  // (j, j+WD-1] (same circular lane map as the elimination) and load row j of
+  // U coalesced; x_j = (y_j - sum_c U(j,c) x_c) / U(j,j) with a wave
+  // reduction.  Rows are loaded PB steps ahead.
+  constexpr int PB = 8;
+  double xv[CPL];
 #pragma unroll
-  for (int cs = 0; cs < CPL; ++cs) {
-    const int v = lane + 64 * cs;
-    const int i = (NT - 1) - (((NT - 1) - v) & (P - 1));
-    bvv[cs] = (i >= 0 && i >= NT - WD - PF) ? ub[(size_t)i * UW + WD] : 0.0;
-  }
+  for (int cs = 0; cs < CPL; ++cs) xv[cs] = 0.0;
   const double kmult = 2.0 * albedo;
   double acc[S], cum[S];
 #pragma unroll
   for (int o = 0; o < S; ++o) { acc[o] = 0.0; cum[o] = 0.0; }
   double bacc = 0.0;
   const double* trr = W.tr + (size_t)it * km * TR_NFIELDS;
-  for (int j = NT - 1; j >= 0; --j) {
-    const int jv = j & (P - 1), lj = jv & 63, csj = jv >> 6;
-    // coefficients of row j of U for the pending rows (issued before the chain)
-    double uij[CPL];
-    const int lo = (j - WD + 1 > 0) ? j - WD + 1 : 0;
+  // U(j, c) for this lane's column(s) when the solve is at row j
+  auto fetch_u = [&](int j, double (&dst)[CPL]) {
+    const int jmq = (j >= 0) ? j % Q : 0;
 #pragma unroll
     for (int cs = 0; cs < CPL; ++cs) {
       const int v = lane + 64 * cs;
-      const int i = j - ((j - v) & (P - 1));
-      uij[cs] = (i >= lo && i < j) ? ub[(size_t)i * UW + (j - i)] : 0.0;
+      int off = v - jmq;
+      if (off < 0) off += Q;
+      dst[cs] = (j >= 0 && v != Q && off >= 1 && off < WD && j + off < NT) ? ub[(size_t)j * UW + off] : 0.0;
     }
-    const int inew = j - WD - PF;
-    double ynew = 0.0;
-    if (inew >= 0) ynew = ub[(size_t)inew * UW + WD];
-    const double bsel = (CPL == 1 || csj == 0) ? bvv[0] : bvv[CPL - 1];
-    const double bj = readlane_d(bsel, lj);
-    const double xj = bj / ub[(size_t)j * UW];
+  };
+  double uq[PB][CPL], dq[PB], yq[PB];
 #pragma unroll
-    for (int cs = 0; cs < CPL; ++cs) bvv[cs] = bvv[cs] - xj * uij[cs];
-    if (inew >= 0) {
-      const int vn = inew & (P - 1);
+  for (int d = 0; d < PB; ++d) {
+    const int j = NT - 1 - d;
+    fetch_u(j, uq[d]);
+    dq[d] = (j >= 0) ? ub[(size_t)j * UW] : 1.0;
+    yq[d] = (j >= 0) ? ub[(size_t)j * UW + WD] : 0.0;
+  }
+  for (int jb = NT - 1; jb >= 0; jb -= PB) {
+    double un[PB][CPL], dn[PB], yn[PB];
 #pragma unroll
-      for (int cs = 0; cs < CPL; ++cs) if (lane + 64 * cs == vn) bvv[cs] = ynew;
+    for (int d = 0; d < PB; ++d) {
+      const int j = jb - PB - d;
+      fetch_u(j, un[d]);
+      dn[d] = (j >= 0) ? ub[(size_t)j * UW] : 1.0;
+      yn[d] = (j >= 0) ? ub[(size_t)j * UW + WD] : 0.0;
     }
-    // TOA source: layer nl, constant index q (lcon for q < n, mcon after)
-    const int nl = j / n2, q = j - nl * n2;
-    const double* lr = lb + (size_t)nl * LS;
 #pragma unroll
-    for (int o = 0; o < S; ++o) acc[o] += xj * lr[O.coef(o, q)];
-    if (surf && nl == L - 1) bacc += xj * bb[BO.bcoef(q)];
-    if (q == 0) {
-      if (nl == L - 1) {
+    for (int d = 0; d < PB; ++d) {
+      const int j = jb - d;
+      if (j < 0) break;
+      double pr = 0.0;
 #pragma unroll
-        for (int o = 0; o < S; ++o) cum[o] = 0.0;
-        if (surf) cum[0] = kmult * (bacc + bb[BO.bc()]);
+      for (int cs = 0; cs < CPL; ++cs) pr += uq[d][cs] * xv[cs];
+      pr += dpp_d<0xB1>(pr);
+      pr += dpp_d<0x4E>(pr);
+      pr += dpp_d<0x141>(pr);
+      pr += dpp_d<0x140>(pr);
+      pr += swz16_d(pr);
+      const double sum = readlane_d(pr, 0) + readlane_d(pr, 32);
+      const double xj = (yq[d] - sum) / dq[d];
+      const int jv = j % Q;
+#pragma unroll
+      for (int cs = 0; cs < CPL; ++cs) if (lane + 64 * cs == jv) xv[cs] = xj;
+      // TOA source: layer nl, constant index q (lcon for q < n, mcon after)
+      const int nl = j / n2, q = j - nl * n2;
+      const double* lr = lb + (size_t)nl * LS;
+#pragma unroll
+      for (int o = 0; o < S; ++o) acc[o] += xj * lr[O.coef(o, q)];
+      if (surf && nl == L - 1) bacc += xj * bb[BO.bcoef(q)];
+      if (q == 0) {
+        if (nl == L - 1) {
+#pragma unroll
+          for (int o = 0; o < S; ++o) cum[o] = 0.0;
+          if (surf) cum[0] = kmult * (bacc + bb[BO.bc()]);
+        }
+        const double tdu = trr[(size_t)nl * TR_NFIELDS + TR_TDUSER];
+#pragma unroll
+        for (int o = 0; o < S; ++o) {
+          const double src = acc[o] + lr[O.csrc(o)];
+          cum[o] = src + tdu * cum[o];
+          acc[o] = 0.0;
+        }
       }
-      const double tdu = trr[(size_t)nl * TR_NFIELDS + TR_TDUSER];
+    }
 #pragma unroll
-      for (int o = 0; o < S; ++o) {
-        const double src = acc[o] + lr[O.csrc(o)];
-        cum[o] = src + tdu * cum[o];
-        acc[o] = 0.0;
-      }
+    for (int d = 0; d < PB; ++d) {
+      dq[d] = dn[d]; yq[d] = yn[d];
+#pragma unroll
+      for (int cs = 0; cs < CPL; ++cs) uq[d][cs] = un[d][cs];
     }
   }
 

@@ -1,0 +1,98 @@
+! Drop-in replacement of module VLIDORT_LAMBERT
+! (src/Components/rtms/geo_vlidort/VLIDORT_LAMBERT.F90:1-342) for the
+! Lambertian clear-sky drivers: same module, procedure names and dummy
+! arguments; the work runs on the GPU through the C-ABI.  The GPU status of
+! the call (argument / HIP errors) is reported in rc when the reference rc
+! would be 0: rc = 1000 - (C-ABI error code).
+module VLIDORT_LAMBERT
+  use iso_c_binding
+  use aeroapps_amd_capi
+  implicit none
+  private
+  public VLIDORT_Scalar_Lambert
+  public VLIDORT_Vector_Lambert
+
+  interface VLIDORT_Vector_Lambert
+    module procedure VLIDORT_Vector_Lambert_SingleGeom
+  end interface
+
+contains
+
+  subroutine VLIDORT_Scalar_Lambert(km, nch, nobs, channels, nstreams, plane_parallel, nMom, &
+                                    nPol, ROT, depol, tau, ssa, g, pmom, pe, he, te, albedo, &
+                                    solar_zenith, relat_azymuth, sensor_zenith, &
+                                    MISSING, verbose, radiance_VL, reflectance_VL, rc)
+    integer, intent(in) :: km, nch, nobs
+    logical, intent(in) :: plane_parallel
+    integer, target, intent(in) :: nMom, nPol
+    integer, intent(in) :: nstreams
+    real*8, target, intent(in) :: channels(nch)
+    real*8, target, intent(in) :: ROT(km,nobs,nch), depol(nch)
+    real*8, target, intent(in) :: tau(km,nch,nobs), ssa(km,nch,nobs), g(km,nch,nobs)
+    real*8, target, intent(in) :: pmom(km,nch,nobs,nMom,nPol)
+    real*8, target, intent(in) :: pe(km+1,nobs), he(km+1,nobs), te(km+1,nobs)
+    real*8, target, intent(in) :: MISSING
+    real*8, target, intent(in) :: solar_zenith(nobs), relat_azymuth(nobs), sensor_zenith(nobs)
+    real*8, target, intent(in) :: albedo(nobs,nch)
+    integer, intent(in) :: verbose
+    real*8, intent(out) :: radiance_VL(nobs,nch)
+    integer, intent(out) :: rc
+    real*8, intent(out) :: reflectance_VL(nobs,nch)
+    integer(c_int) :: e, crc
+    integer(c_int) :: pp
+    pp = 0
+    if (plane_parallel) pp = 1
+    e = aa_vlidort_scalar_lambert(km, nch, nobs, channels, nstreams, pp, nMom, nPol, ROT, depol, tau, ssa, g, &
+                                  pmom, pe, he, te, albedo, solar_zenith, relat_azymuth, sensor_zenith, MISSING, &
+                                  verbose, 0, radiance_VL, reflectance_VL, c_null_ptr, crc)
+    rc = crc
+    if (rc == 0 .and. e /= 0) rc = 1000 - e
+  end subroutine VLIDORT_Scalar_Lambert
+
+  subroutine VLIDORT_Vector_Lambert_SingleGeom(km, nch, nobs, channels, nstreams, plane_parallel, nMom, &
+                                               nPol, ROT, depol, alpha, tau, ssa, pmom, pe, he, te, albedo, &
+                                               solar_zenith, relat_azymuth, sensor_zenith, flux_factor, &
+                                               MISSING, verbose, radiance_VL, reflectance_VL, Q, U, rc, &
+                                               DO_2OS_CORRECTION, DO_BOA)
+    integer, intent(in) :: km, nch, nobs
+    logical, intent(in) :: plane_parallel
+    integer, target, intent(in) :: nMom, nPol
+    integer, intent(in) :: nstreams
+    real*8, target, intent(in) :: channels(nch)
+    real*8, target, intent(in) :: ROT(km,nobs,nch), depol(nch), alpha(km,nobs,nch)
+    real*8, target, intent(in) :: tau(km,nch,nobs), ssa(km,nch,nobs)
+    real*8, target, intent(in) :: pmom(km,nch,nobs,nMom,nPol)
+    real*8, target, intent(in) :: MISSING
+    real*8, target, intent(in) :: pe(km+1,nobs), he(km+1,nobs), te(km+1,nobs)
+    real*8, target, intent(in) :: solar_zenith(nobs), relat_azymuth(nobs), sensor_zenith(nobs)
+    real*8, intent(in) :: flux_factor(nch,nobs)
+    real*8, target, intent(in) :: albedo(nobs,nch)
+    integer, intent(in) :: verbose
+    real*8, intent(out) :: radiance_VL(nobs,nch), reflectance_VL(nobs,nch), Q(nobs,nch), U(nobs,nch)
+    integer, intent(out) :: rc
+    logical, optional, intent(in) :: DO_2OS_CORRECTION, DO_BOA
+    integer(c_int) :: e, crc, pp
+    rc = 0
+    ! the second-order correction and BOA output are not on the accelerated path
+    if (present(DO_2OS_CORRECTION)) then
+      if (DO_2OS_CORRECTION) then
+        rc = 1001
+        return
+      end if
+    end if
+    if (present(DO_BOA)) then
+      if (DO_BOA) then
+        rc = 1001
+        return
+      end if
+    end if
+    pp = 0
+    if (plane_parallel) pp = 1
+    e = aa_vlidort_vector_lambert(km, nch, nobs, channels, nstreams, pp, nMom, nPol, ROT, depol, alpha, tau, ssa, &
+                                  pmom, pe, he, te, albedo, solar_zenith, relat_azymuth, sensor_zenith, &
+                                  flux_factor, MISSING, verbose, 0, radiance_VL, reflectance_VL, Q, U, c_null_ptr, crc)
+    rc = crc
+    if (rc == 0 .and. e /= 0) rc = 1000 - e
+  end subroutine VLIDORT_Vector_Lambert_SingleGeom
+
+end module VLIDORT_LAMBERT

@@ -274,7 +274,7 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
   };
 
   // ---- PI matrices at the user stream and the solar cosine
-  const int obs = (W.item0 + it) / W.nch;
+  const int obs = global_item(W, it) / W.nch;
   const double ustream = cos(DEG_TO_RAD * W.vza[obs]);
   double xup[LM][S][S], x0p[LM][S][S];
   pimatrix_x<S, LM>(M, NMOM, ustream, xup);
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
 #pragma unroll
     for (int o = 0; o < S; ++o) { wu[i][o] = 0.0; wl[i][o] = 0.0; }
   if (beam) {
-    const int g = W.item0 + it;
+    const int g = global_item(W, it);
     const int ch = g - obs * W.nch;
     const double F1 = (W.flux ? W.flux[(size_t)obs * W.nch + ch] : 1.0) / PI4;
     const double secbar = avsec, cst = itrans, wdel = tdmubar;
@@ -495,7 +495,7 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
   //      (BVP_SURFACE_SETUP_HOM/BEAM, REG/vlidort_bvproblem.f90; GET_BOA_SOURCE,
   //      REG/vlidort_intensity.f90:1096-1496)
   if (layer == L - 1 && M == 0) {
-    const int g = W.item0 + it;
+    const int g = global_item(W, it);
     const int ch = g - obs * W.nch;
     const double albedo = W.albedo[(size_t)ch * W.nobs + obs];
     if (albedo != 0.0) {
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
   const int it = active[ai];
   const int lane = threadIdx.x;
   const int km = W.km;
-  const int g = W.item0 + it, obs = g / W.nch, ch = g - obs * W.nch;
+  const int g = global_item(W, it), obs = g / W.nch, ch = g - obs * W.nch;
   const size_t oi = (size_t)ch * W.nobs + obs;
   if (W.status[it] != 0) {
     if (lane == 0) {

@@ -146,7 +146,7 @@ struct ItemIdx {
   int g, obs, ch;
 };
 __device__ __forceinline__ ItemIdx item_of(const RTWork& W, int it) {
-  ItemIdx r; r.g = W.item0 + it; r.obs = r.g / W.nch; r.ch = r.g - r.obs * W.nch; return r;
+  ItemIdx r; r.g = global_item(W, it); r.obs = r.g / W.nch; r.ch = r.g - r.obs * W.nch; return r;
 }
 
 // ---------------------------------------------------------------- kernels
@@ -455,7 +455,7 @@ __global__ void k_geom(RTWork W, int nitems) {
   // layer extinction per km; levels below the surface (a ray tangent under
   // the ground) contribute nothing
   auto ext = [&](int k) { return k <= L ? dts(k - 1) / (G.h(k - 1) - G.h(k)) : 0.0; };
-  const bool latched = (W.item0 + it) >= *W.latch;
+  const bool latched = global_item(W, it) >= *W.latch;
   const bool planpar = W.plane_parallel != 0;
   const bool enhanced = !planpar && !latched;
   double suntau = 0.0, multenh = 0.0, lostrans = 0.0;
@@ -622,7 +622,7 @@ __global__ void k_item(RTWork W, int nitems, int* active, int* nactive) {
   // ---- first-order TOA recursion (FOC/FO_VectorSS_RTCalcs_I.f90 SSV_Integral_I_UP)
   const double sza = W.sza[I.obs], vza = W.vza[I.obs];
   const double dtr = DEG_TO_RAD;
-  const bool latched = (W.item0 + it) >= *W.latch;
+  const bool latched = global_item(W, it) >= *W.latch;
   const bool planpar = W.plane_parallel != 0;
   const bool enhanced = !planpar && !latched;
   double mu0, mu1;

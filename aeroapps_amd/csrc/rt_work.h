@@ -33,7 +33,8 @@ struct RTWork {
   int nmoments = 0;    // min(2N-1, nmom): Fourier terms and scaled moments
   int plane_parallel = 0;
   double missing = 1e15;
-  int item0 = 0, cap = 0;   // first global item of the chunk, chunk capacity
+  int item0 = 0, cap = 0;   // first item of the chunk, chunk capacity
+  const int* item_list = nullptr;   // optional: chunk item `it` is global item item_list[item0 + it]
   // ---- inputs (device, Fortran layouts of the driver arguments)
   const double* rot = nullptr;     // (km,nobs,nch)
   const double* depol = nullptr;   // (nch)
@@ -87,6 +88,11 @@ struct RTWork {
   double* uout = nullptr;
   int* nfourier = nullptr;
 };
+
+// Global item (obs*nch + ch) of chunk-local item `it`.
+__device__ __forceinline__ int global_item(const RTWork& W, int it) {
+  return W.item_list ? W.item_list[W.item0 + it] : W.item0 + it;
+}
 
 // Offsets inside one lbuf record (per item and layer) for n = N*S.
 struct LOff {

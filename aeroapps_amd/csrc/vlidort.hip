@@ -7,6 +7,9 @@
 // quadrature streams (identical for every column of a call), and the chunked
 // pipeline setup -> Fourier terms.
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <climits>
 #include <cstdlib>
 #include <cstring>
@@ -60,19 +63,26 @@ int launch_get_aop(const aa_mie* t, int km, int nobs, int nch, int nq, const int
 
 namespace {
 
-// ---------------------------------------------------------------- device arena
+// ---------------------------------------------------------------- device arenas
+// One grow-only arena per (device, stream, purpose), so calls enqueued on
+// different streams never share work memory and can overlap on the GPU.
 struct Arena {
   char* base = nullptr;
   size_t cap = 0;
 };
-Arena g_arena[64];
+std::mutex g_arena_mu;
+std::map<std::tuple<int, hipStream_t, int>, Arena> g_arenas;
 
-int arena_reserve(size_t bytes, char** out) {
+int arena_reserve(hipStream_t st, int purpose, size_t bytes, char** out) {
   int dev = 0;
   AA_CHECK(hipGetDevice(&dev));
-  Arena& a = g_arena[dev & 63];
+  std::lock_guard<std::mutex> lk(g_arena_mu);
+  Arena& a = g_arenas[std::make_tuple(dev, st, purpose)];
   if (a.cap < bytes) {
-    if (a.base) AA_CHECK(hipFree(a.base));
+    if (a.base) {
+      AA_CHECK(hipStreamSynchronize(st));
+      AA_CHECK(hipFree(a.base));
+    }
     a.base = nullptr; a.cap = 0;
     size_t want = bytes + bytes / 8;
     AA_CHECK(hipMalloc((void**)&a.base, want));
@@ -81,6 +91,7 @@ int arena_reserve(size_t bytes, char** out) {
   *out = a.base;
   return 0;
 }
+enum ArenaPurpose { AR_WORK = 0, AR_OPTICS = 1, AR_CALL = 2 };
 
 // Sequential carving of one arena block (a first pass with base == nullptr
 // measures the block).
@@ -149,6 +160,40 @@ void quad_table(int N, int S, int nmoments, std::vector<double>& q) {
 
 int max_streams(int S) { return S == 1 ? MAX_STREAMS : 4; }
 
+// Quadrature tables are built once per (N, S, nmoments) and kept for the
+// process lifetime (the host copy source of every asynchronous upload).
+const std::vector<double>& quad_table_cached(int N, int S, int nmoments) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int>, std::vector<double>> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_tuple(N, S, nmoments);
+  auto it = cache.find(key);
+  if (it == cache.end()) {
+    std::vector<double> q;
+    quad_table(N, S, nmoments, q);
+    it = cache.emplace(key, std::move(q)).first;
+  }
+  return it->second;
+}
+
+// Library streams (default priority) used to run stream-count buckets of one
+// call concurrently; created once per (device, caller stream, bucket slot), so
+// calls on different caller streams never share them.
+hipStream_t bucket_stream(hipStream_t caller, int k) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, hipStream_t, int>, hipStream_t> pool;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_tuple(dev, caller, k);
+  auto it = pool.find(key);
+  if (it != pool.end()) return it->second;
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  pool[key] = st;
+  return st;
+}
+
 // Per-item work of one chunk, carved from the arena.
 size_t carve_work(Carver& C, RTWork& W, int cap) {
   const int n = W.N * W.S, LM = 2 * W.N, km = W.km;
@@ -189,15 +234,9 @@ size_t bytes_per_item(const RTWork& W0) {
   return s2 - s1 + 256 * 24;
 }
 
-// Device int holding the first item of the SZA==0 nadir latch for the
-// current call (one per GPU, allocated once).
-int* g_latch[64] = {nullptr};
-
-int compute_latch(RTWork& W, hipStream_t st) {
-  int dev = 0;
-  AA_CHECK(hipGetDevice(&dev));
-  int*& lp = g_latch[dev & 63];
-  if (!lp) AA_CHECK(hipMalloc((void**)&lp, sizeof(int)));
+// First item of the SZA==0 nadir latch for the current call, in the call
+// arena of the stream.
+int compute_latch(RTWork& W, int* lp, hipStream_t st) {
   static const int big = INT_MAX;
   AA_CHECK(hipMemcpyAsync(lp, &big, sizeof(int), hipMemcpyHostToDevice, st));
   const int nitems = W.nobs * W.nch;
@@ -206,13 +245,13 @@ int compute_latch(RTWork& W, hipStream_t st) {
   return 0;
 }
 
-// Runs the radiance pipeline over items [g0, g1) (item g = obs*nch + ch).
-// W carries the inputs and outputs (device pointers) and the latch.
-int run_items(RTWork W, int g0, int g1, hipStream_t st) {
+// Runs the radiance pipeline over items [g0, g1) (item g = obs*nch + ch), or
+// over d_items[g0..g1) when d_items (device) is given.  W carries the inputs
+// and outputs (device pointers) and the latch.
+int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nullptr) {
   const int nitems = g1 - g0;
   if (nitems <= 0) return 0;
-  std::vector<double> qtab;
-  quad_table(W.N, W.S, W.nmoments, qtab);
+  const std::vector<double>& qtab = quad_table_cached(W.N, W.S, W.nmoments);
   // chunk capacity from the work budget
   size_t freeb = 0, totalb = 0;
   AA_CHECK(hipMemGetInfo(&freeb, &totalb));
@@ -236,11 +275,12 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st) {
   RTWork tmp = W;
   size_t total = carve_all(nullptr, tmp, &dq, &l0, &l1, &cnt);
   char* base = nullptr;
-  if (arena_reserve(total, &base)) return AA_EHIP;
+  if (arena_reserve(st, AR_WORK, total, &base)) return AA_EHIP;
   carve_all(base, W, &dq, &l0, &l1, &cnt);
   AA_CHECK(hipMemcpyAsync(dq, qtab.data(), qtab.size() * sizeof(double), hipMemcpyHostToDevice, st));
   W.quad = dq;
   int* lists[2] = {l0, l1};
+  W.item_list = d_items;
   for (int item0 = g0; item0 < g1; item0 += cap) {
     const int ni = std::min(cap, g1 - item0);
     W.item0 = item0;
@@ -249,8 +289,6 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st) {
     int e = rt_fourier(W, ni, lists, cnt, st);
     if (e) return e == -4 ? AA_EDIMS : AA_EHIP;
   }
-  // the qtab host vector must outlive the async copy
-  AA_CHECK(hipStreamSynchronize(st));
   return 0;
 }
 
@@ -314,7 +352,13 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
   W.uout = vector ? B.alloc<double>(no) : nullptr;
   W.nfourier = B.alloc<int>(no);
   for (void* p : B.ptrs) if (!p) return AA_EHIP;
-  int e = compute_latch(W, 0);
+  int* lp = nullptr;
+  {
+    char* cb = nullptr;
+    if (arena_reserve(0, AR_CALL, 256, &cb)) return AA_EHIP;
+    lp = (int*)cb;
+  }
+  int e = compute_latch(W, lp, 0);
   if (!e) e = run_items(W, 0, nobs * nch, 0);
   if (e) return e;
   AA_CHECK(hipDeviceSynchronize());
@@ -372,14 +416,57 @@ int aa_column_radiance_dev(aa_mie* t, int km, int nobs, int nch, int nq, const i
   if (!tracer_idx || !chan_idx || !d_qm || !d_rh || !d_rot || !d_depol || !d_he || !d_albedo || !d_sza || !d_raa ||
       !d_vza || !d_radiance || !d_reflectance || !d_nfourier)
     return AA_EARG;
-  if (d_nstreams_col) return AA_EARG;   // per-column stream counts go through the batch scheduler
   if (precision != 0) return AA_EARG;
   const int S = vector ? 3 : 1;
   if (vector && (!d_q || !d_u || npol < 6)) return AA_EARG;
-  if (nstreams < 1 || nstreams > max_streams(S)) return AA_EDIMS;
+  if (!d_nstreams_col && (nstreams < 1 || nstreams > max_streams(S))) return AA_EDIMS;
   if (nobs == 0) return 0;
   hipStream_t st = (hipStream_t)stream;
   if (mie_upload(t)) return AA_EHIP;
+  // stream counts per column -> buckets (items grouped per optics block, then per N)
+  std::vector<int> nstr(nobs, nstreams);
+  if (d_nstreams_col) {
+    AA_CHECK(hipMemcpyAsync(nstr.data(), d_nstreams_col, sizeof(int) * nobs, hipMemcpyDeviceToHost, st));
+    AA_CHECK(hipStreamSynchronize(st));
+    for (int v : nstr) if (v < 1 || v > max_streams(S)) return AA_EDIMS;
+  }
+  const size_t per_obs = (size_t)km * nch * (3 + (size_t)nmom * npol) * sizeof(float);
+  size_t ob = std::max<size_t>(1, env_bytes("AA_OPTICS_BYTES", (size_t)4 << 30) / per_obs);
+  ob = std::min<size_t>(ob, (size_t)nobs);
+  const int nblk = (int)((nobs + ob - 1) / ob);
+  // host item lists: for block b and stream count N, items [off, off+cnt)
+  struct Bucket { int blk, N, off, cnt; };
+  std::vector<Bucket> buckets;
+  std::vector<int> hitems;
+  if (d_nstreams_col) {
+    hitems.reserve((size_t)nobs * nch);
+    for (int b = 0; b < nblk; ++b) {
+      const int o0 = (int)(b * ob), o1 = std::min(nobs, (int)((b + 1) * ob));
+      for (int N = 1; N <= max_streams(S); ++N) {
+        const int off = (int)hitems.size();
+        for (int o = o0; o < o1; ++o)
+          if (nstr[o] == N) for (int c = 0; c < nch; ++c) hitems.push_back(o * nch + c);
+        const int cnt = (int)hitems.size() - off;
+        if (cnt) buckets.push_back({b, N, off, cnt});
+      }
+    }
+  }
+  // call arena: latch + item list; optics arena: float optics of one block
+  char* cb = nullptr;
+  const size_t call_bytes = 256 + sizeof(int) * std::max<size_t>(1, hitems.size());
+  if (arena_reserve(st, AR_CALL, call_bytes, &cb)) return AA_EHIP;
+  int* lp = (int*)cb;
+  int* d_items = (int*)(cb + 256);
+  if (!hitems.empty())
+    AA_CHECK(hipMemcpyAsync(d_items, hitems.data(), sizeof(int) * hitems.size(), hipMemcpyHostToDevice, st));
+  char* obase = nullptr;
+  const size_t n3 = ob * km * nch;
+  const size_t obytes = (3 * n3 + n3 * (size_t)nmom * npol) * sizeof(float) + 1024;
+  if (arena_reserve(st, AR_OPTICS, obytes, &obase)) return AA_EHIP;
+  float* dt = (float*)obase;
+  float* ds = dt + n3;
+  float* dg = ds + n3;
+  float* dp = dg + n3;
   RTWork W;
   W.km = km; W.nch = nch; W.nobs = nobs; W.nmom = nmom; W.npol = npol;
   W.N = nstreams; W.S = S; W.nmoments = std::min(2 * nstreams - 1, nmom);
@@ -390,28 +477,43 @@ int aa_column_radiance_dev(aa_mie* t, int km, int nobs, int nch, int nq, const i
   W.nlayers = d_nlayers;
   W.radiance = d_radiance; W.reflectance = d_reflectance; W.qout = d_q; W.uout = d_u; W.nfourier = d_nfourier;
   W.optics_f32 = 1;
-  // optics for a block of observations at a time
-  const size_t per_obs = (size_t)km * nch * (3 + (size_t)nmom * npol) * sizeof(float);
-  size_t ob = std::max<size_t>(1, env_bytes("AA_OPTICS_BYTES", (size_t)4 << 30) / per_obs);
-  ob = std::min<size_t>(ob, (size_t)nobs);
-  float *dt = nullptr, *ds = nullptr, *dg = nullptr, *dp = nullptr;
-  AA_CHECK(hipMalloc((void**)&dt, ob * km * nch * sizeof(float)));
-  AA_CHECK(hipMalloc((void**)&ds, ob * km * nch * sizeof(float)));
-  AA_CHECK(hipMalloc((void**)&dg, ob * km * nch * sizeof(float)));
-  AA_CHECK(hipMalloc((void**)&dp, ob * km * nch * (size_t)nmom * npol * sizeof(float)));
-  int rcode = compute_latch(W, st);
-  for (int o0 = 0; o0 < nobs && !rcode; o0 += (int)ob) {
-    const int no = std::min((int)ob, nobs - o0);
+  W.tau = dt; W.ssa = ds; W.pmom = dp;
+  int rcode = compute_latch(W, lp, st);
+  for (int b = 0; b < nblk && !rcode; ++b) {
+    const int o0 = (int)(b * ob), no = std::min((int)ob, nobs - o0);
     rcode = launch_get_aop(t, km, no, nch, nq, tracer_idx, chan_idx, d_qm + (size_t)o0 * km * nq,
                            d_rh + (size_t)o0 * km, nmom, npol, nq, 1, dt, ds, dg, dp, st);
     if (rcode) break;
     RTWork Wc = W;
-    Wc.tau = dt; Wc.ssa = ds; Wc.pmom = dp;
     Wc.opt_obs0 = o0; Wc.opt_nobs = no;
-    rcode = run_items(Wc, o0 * nch, (o0 + no) * nch, st);
+    if (!d_nstreams_col) {
+      rcode = run_items(Wc, o0 * nch, (o0 + no) * nch, st);
+      continue;
+    }
+    // stream-count buckets of this block run concurrently on library streams
+    hipEvent_t ready;
+    AA_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    AA_CHECK(hipEventRecord(ready, st));
+    std::vector<hipEvent_t> done;
+    int k = 0;
+    for (const Bucket& bk : buckets) {
+      if (bk.blk != b) continue;
+      hipStream_t bs = bucket_stream(st, k++);
+      if (!bs) { rcode = AA_EHIP; break; }
+      AA_CHECK(hipStreamWaitEvent(bs, ready, 0));
+      RTWork Wb = Wc;
+      Wb.N = bk.N; Wb.nmoments = std::min(2 * bk.N - 1, nmom);
+      rcode = run_items(Wb, bk.off, bk.off + bk.cnt, bs, d_items);
+      hipEvent_t e;
+      AA_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      AA_CHECK(hipEventRecord(e, bs));
+      done.push_back(e);
+      if (rcode) break;
+    }
+    for (hipEvent_t e : done) { AA_CHECK(hipStreamWaitEvent(st, e, 0)); (void)hipEventDestroy(e); }
+    (void)hipEventDestroy(ready);
   }
   AA_CHECK(hipStreamSynchronize(st));
-  (void)hipFree(dt); (void)hipFree(ds); (void)hipFree(dg); (void)hipFree(dp);
   return rcode;
 }
 

@@ -114,6 +114,9 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=512)
     ap.add_argument("--cpu-procs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving-granules", type=int, default=40,
+                    help="granules of the serving-loop phase (0 skips it)")
+    ap.add_argument("--serving-load", type=float, default=0.7, help="offered load as a fraction of peak")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the path has no cross-GPU step (every column is independent); "
@@ -250,6 +253,24 @@ def main():
     }
     if cpu:
         out["gpu_vs_cpu"] = round(value / cpu["value"], 1)
+
+    # serving loop: Poisson granules at a fraction of the measured peak
+    if args.serving_granules > 0 and not vector:
+        from aeroapps_amd import serving as SV
+        del qm, rh, rot, he, pe, te, albedo, sza, vza, raa, flux, rad, refl, nf
+        torch.cuda.empty_cache()
+        pool = SV.ColumnPool(torch, 200_000, km, wl["channels"], dev)
+        grans = SV.make_granules(args.serving_granules, pool.ncol, wl["channels"])
+        mean_cols = float(np.mean([g.ncol for g in grans]))
+        rate = args.serving_load * value / mean_cols
+        arr = SV.poisson_arrivals(len(grans), rate)
+        srv = SV.Server(torch, mie, pool, N, nmom, slots=4, hbm_budget=200e9, arena_bytes=28e9)
+        st = srv.serve(grans, arr)
+        out["serving"] = {"granules": st["granules"], "columns": st["columns"],
+                          "offered_load": args.serving_load, "arrival_rate_per_s": round(rate, 4),
+                          "p50_latency_s": round(st["p50_s"], 3), "p99_latency_s": round(st["p99_s"], 3),
+                          "throughput_cols_per_s": round(st["throughput_cols_per_s"], 1), "slots": st["slots"],
+                          "granule_columns": "log-uniform 1000-200000, 1-4 channels, 72 layers, 12 streams"}
     print(json.dumps(out))
 
 

@@ -26,6 +26,7 @@
 // LDS and no barrier.  Eliminated rows (U and the transformed right-hand
 // side) go to a per-item buffer that the back-substitution reads in reverse.
 #pragma once
+#include <type_traits>
 #include "rt_work.h"
 #include "rt_device.h"
 #include "prof.h"
@@ -96,10 +97,14 @@ __device__ __forceinline__ double swz16_d(double x) {
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 
 // ------------------------------------------------------------------ k_layer
-template <int N, int S>
+template <int N, int S, bool MIX>
 __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __restrict__ active,
                                               const int* __restrict__ nactive) {
   constexpr int n = N * S, LM = 2 * N;
+  // MIX: SAB/DAB come from the FP32 MFMA contraction, the source-assembly
+  // contractions accumulate in FP32; eigensystem, exponentials and
+  // normalisations stay FP64.
+  using TR = typename std::conditional<MIX, float, double>::type;
   constexpr LOff O(N, S);
   constexpr QOff QO(N, S);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -161,6 +166,7 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
       dp += sp; dm += sm;
     }
   };
+  const float* s32 = MIX ? W.sab32 + il * (size_t)(2 * n * n) : nullptr;
   double sab[n][n];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
@@ -172,8 +178,12 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
       for (int o1 = 0; o1 < S; ++o1)
 #pragma unroll
         for (int o2 = 0; o2 < S; ++o2) {
-          double dp, dm; dpdm(i, o1, j, o2, dp, dm);
-          sab[i * S + o1][j * S + o2] = fac * (dp + dm);
+          if (MIX) {
+            sab[i * S + o1][j * S + o2] = (double)s32[(i * S + o1) * n + j * S + o2];
+          } else {
+            double dp, dm; dpdm(i, o1, j, o2, dp, dm);
+            sab[i * S + o1][j * S + o2] = fac * (dp + dm);
+          }
         }
     }
 #pragma unroll
@@ -198,8 +208,12 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
           const double fac = xinv * qhw[j];
 #pragma unroll
           for (int o2 = 0; o2 < S; ++o2) {
-            double dp, dm; dpdm(i, o1, j, o2, dp, dm);
-            drow[j * S + o2] = fac * (dp - dm);
+            if (MIX) {
+              drow[j * S + o2] = (double)s32[n * n + r * n + j * S + o2];
+            } else {
+              double dp, dm; dpdm(i, o1, j, o2, dp, dm);
+              drow[j * S + o2] = fac * (dp - dm);
+            }
           }
         }
         drow[r] -= xinv;
@@ -292,23 +306,23 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
 #pragma unroll
       for (int o = 0; o < S; ++o) { updn[k][o] = 0.0; upup[k][o] = 0.0; }
     } else {
-      double gaux[LM][S];
+      TR gaux[LM][S];
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
-        double hs[S];
+        TR hs[S];
 #pragma unroll
         for (int o1 = 0; o1 < S; ++o1) {
-          double sps = 0.0;
+          TR sps = 0.0;
           if (inl(l)) {
 #pragma unroll
             for (int j = 0; j < N; ++j) {
-              const double a5 = qhw[j];
-              double sp = 0.0, sm = 0.0;
+              const TR a5 = (TR)qhw[j];
+              TR sp = 0.0, sm = 0.0;
 #pragma unroll
               for (int o = 0; o < S; ++o) {
-                const double xqv = XQ(l, j, o1, o);
-                sp += xqv * a5 * xp[j * S + o][k];
-                sm += (DMS[o1] * xqv * pisign(l)) * a5 * xn[j * S + o][k];
+                const TR xqv = (TR)XQ(l, j, o1, o);
+                sp += xqv * a5 * (TR)xp[j * S + o][k];
+                sm += ((TR)DMS[o1] * xqv * (TR)pisign(l)) * a5 * (TR)xn[j * S + o][k];
               }
               sps = sps + sp + sm;
             }
@@ -317,22 +331,22 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
         }
 #pragma unroll
         for (int o1 = 0; o1 < S; ++o1) {
-          double s = 0.0;
+          TR s = 0.0;
 #pragma unroll
-          for (int o = 0; o < S; ++o) s += og[l][o1][o] * hs[o];
+          for (int o = 0; o < S; ++o) s += (TR)og[l][o1][o] * hs[o];
           gaux[l][o1] = s;
         }
       }
 #pragma unroll
       for (int o1 = 0; o1 < S; ++o1) {
-        double sdn = 0.0, sup = 0.0;
+        TR sdn = 0.0, sup = 0.0;
 #pragma unroll
         for (int o = 0; o < S; ++o)
 #pragma unroll
           for (int l = 0; l < LM; ++l) {
             if (!inl(l)) continue;
-            const double xum = DMS[o1] * xup[l][o1][o] * DMS[o] * pisign(l);
-            const double xupre = DMS[o1] * xup[l][o1][o];
+            const TR xum = (TR)(DMS[o1] * xup[l][o1][o] * DMS[o] * pisign(l));
+            const TR xupre = (TR)(DMS[o1] * xup[l][o1][o]);
             sdn += xum * gaux[l][o];
             sup += xupre * gaux[l][o];
           }
@@ -386,54 +400,54 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
       else cfunc[k] = cst * (zdel - wdel) / gamm[k];
       dfunc[k] = cst * (1.0 - zwdel) / gamp[k];
     }
-    double hq[LM][S];
+    TR hq[LM][S];
 #pragma unroll
     for (int o1 = 0; o1 < S; ++o1)
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
-        double gsum = 0.0;
+        TR gsum = 0.0;
 #pragma unroll
-        for (int o2 = 0; o2 < S; ++o2) gsum = gsum + og[l][o1][o2] * x0p[l][o2][0];
+        for (int o2 = 0; o2 < S; ++o2) gsum = gsum + (TR)og[l][o1][o2] * (TR)x0p[l][o2][0];
         hq[l][o1] = gsum;
       }
-    double dpi[N][S], dmi[N][S];
+    TR dpi[N][S], dmi[N][S];
 #pragma unroll
     for (int o1 = 0; o1 < S; ++o1)
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        double sp = 0.0, sm = 0.0;
+        TR sp = 0.0, sm = 0.0;
 #pragma unroll
         for (int o = 0; o < S; ++o)
 #pragma unroll
           for (int l = 0; l < LM; ++l) {
             if (!inl(l)) continue;
-            const double xqv = XQ(l, i, o1, o);
+            const TR xqv = (TR)XQ(l, i, o1, o);
             sp += xqv * hq[l][o];
-            sm += (xqv * DMS[o] * pisign(l)) * hq[l][o];
+            sm += (xqv * (TR)DMS[o] * (TR)pisign(l)) * hq[l][o];
           }
-        dpi[i][o1] = F1 * sp; dmi[i][o1] = F1 * sm;
+        dpi[i][o1] = (TR)F1 * sp; dmi[i][o1] = (TR)F1 * sm;
       }
     double gup[n], gdn[n];
 #pragma unroll
     for (int k = 0; k < n; ++k) {
-      double av = 0.0, bv = 0.0;
+      TR av = 0.0, bv = 0.0;
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        double h1 = 0.0, h2 = 0.0, h1b = 0.0, h2b = 0.0;
+        TR h1 = 0.0, h2 = 0.0, h1b = 0.0, h2b = 0.0;
 #pragma unroll
-        for (int o = 0; o < S; ++o) h1 += dpi[i][o] * xp[i * S + o][k];
+        for (int o = 0; o < S; ++o) h1 += dpi[i][o] * (TR)xp[i * S + o][k];
 #pragma unroll
-        for (int o = 0; o < S; ++o) h1b += dmi[i][o] * xn[i * S + o][k];
+        for (int o = 0; o < S; ++o) h1b += dmi[i][o] * (TR)xn[i * S + o][k];
         h1 = h1 + h1b;
 #pragma unroll
-        for (int o = 0; o < S; ++o) h2 += dmi[i][o] * xp[i * S + o][k];
+        for (int o = 0; o < S; ++o) h2 += dmi[i][o] * (TR)xp[i * S + o][k];
 #pragma unroll
-        for (int o = 0; o < S; ++o) h2b += dpi[i][o] * xn[i * S + o][k];
+        for (int o = 0; o < S; ++o) h2b += dpi[i][o] * (TR)xn[i * S + o][k];
         h2 = h2 + h2b;
-        av += qw[i] * h1; bv += qw[i] * h2;
+        av += (TR)qw[i] * h1; bv += (TR)qw[i] * h2;
       }
-      aterm[k] = av / normv[k];
-      bterm[k] = bv / normv[k];
+      aterm[k] = (double)av / normv[k];
+      bterm[k] = (double)bv / normv[k];
       gdn[k] = cfunc[k] * aterm[k];
       gup[k] = dfunc[k] * bterm[k];
     }
@@ -900,14 +914,75 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
   }
 }
 
+// ------------------------------------------------------------------ k_sabdab
+// Mixed precision: SAB and DAB of every active (item, layer) at Fourier term M
+// as one FP32 GEMM on the matrix cores,
+//   C[row][pair] = sum_k Y_M[row][k] * og[pair][k],
+// rows = 2n^2 (SAB then DAB entries), k = (l, o, o3) over the omega-Greek
+// coefficients (REG/vlidort_solutions.f90:439-475).  One wavefront computes
+// 16 pairs x all rows with v_mfma_f32_16x16x4f32 tiles.
+typedef float f4v __attribute__((ext_vector_type(4)));
+template <int N, int S>
+__global__ __launch_bounds__(64) void k_sabdab(RTWork W, int M, const int* __restrict__ active,
+                                               const int* __restrict__ nactive) {
+  constexpr int n = N * S, LM = 2 * N;
+  constexpr YOff YO(N, S);
+  constexpr int ROWS = YO.rows(), KK = YO.K;
+  constexpr int RT = (ROWS + 15) / 16, KT = (KK + 3) / 4;
+  const int lane = threadIdx.x;
+  const int km = W.km;
+  const long p = (long)blockIdx.x * 16 + (lane & 15);   // pair of this lane's B column / C column
+  const int a = (int)(p / km), layer = (int)(p - (long)a * km);
+  const int na = *nactive;
+  bool valid = a < na;
+  int it = 0;
+  if (valid) { it = active[a]; valid = layer < W.L_it[it]; }
+  if (__ballot(valid) == 0ull) return;
+  const size_t il = (size_t)it * km + layer;
+  const double* ogp = W.og + il * (size_t)(LM * S * S);
+  const float* Y = W.ytab;
+  const int kq = lane >> 4;
+  float bk[KT];
+#pragma unroll
+  for (int ks = 0; ks < KT; ++ks) {
+    const int k = 4 * ks + kq;
+    bk[ks] = (valid && k < KK) ? (float)ogp[k] : 0.0f;
+  }
+  float* out = W.sab32 + il * (size_t)ROWS;
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt) {
+    f4v c = {0.0f, 0.0f, 0.0f, 0.0f};
+    const int row = rt * 16 + (lane & 15);
+#pragma unroll
+    for (int ks = 0; ks < KT; ++ks) {
+      const int k = 4 * ks + kq;
+      const float av = (row < ROWS && k < KK) ? Y[YO.at(M, row, k)] : 0.0f;
+      c = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bk[ks], c, 0, 0, 0);
+    }
+    if (valid) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = rt * 16 + 4 * kq + v;
+        if (r < ROWS) out[r] = c[v];
+      }
+    }
+  }
+}
+
 template <int N, int S>
 int fourier_loop(const RTWork& W, int nitems, int* const lists[2], int* counts, hipStream_t st) {
   const int nl = (int)(((size_t)nitems * W.km + 63) / 64);
+  const int ns = (int)(((size_t)nitems * W.km + 15) / 16);
   for (int M = 0; M <= W.nmoments; ++M) {
     const int* in = lists[M & 1];
     int* out = lists[(M + 1) & 1];
     prof_begin(PROF_LAYER, st);
-    hipLaunchKernelGGL((k_layer<N, S>), dim3(nl), dim3(64), 0, st, W, M, in, counts + M);
+    if (W.precision == 1) {
+      hipLaunchKernelGGL((k_sabdab<N, S>), dim3(ns), dim3(64), 0, st, W, M, in, counts + M);
+      hipLaunchKernelGGL((k_layer<N, S, true>), dim3(nl), dim3(64), 0, st, W, M, in, counts + M);
+    } else {
+      hipLaunchKernelGGL((k_layer<N, S, false>), dim3(nl), dim3(64), 0, st, W, M, in, counts + M);
+    }
     prof_end(PROF_LAYER, st);
     prof_begin(PROF_BVP, st);
     hipLaunchKernelGGL((k_bvp<N, S>), dim3(nitems), dim3(64), 0, st, W, M, in, counts + M, out, counts + M + 1);

@@ -81,6 +81,10 @@ struct RTWork {
   int lstride = 0, bstride = 0, ustride = 0;
   // ---- quadrature and PI tables of the quadrature streams (device)
   const double* quad = nullptr;    // [qs N][qw N][xq: M][l][i][S][S]
+  // ---- mixed precision: FP32 phase-matrix contraction (SAB/DAB) by MFMA
+  int precision = 0;               // 0 FP64, 1 mixed
+  const float* ytab = nullptr;     // [M][2n^2 rows][2N*S*S] contraction weights
+  float* sab32 = nullptr;          // [it][km][2n^2] SAB rows then DAB rows (FP32)
   // ---- outputs (device, (nobs,nch) Fortran)
   double* radiance = nullptr;
   double* reflectance = nullptr;
@@ -120,6 +124,16 @@ struct BOff {
   __host__ __device__ constexpr int r2beam() const { return 4 * n; }
   __host__ __device__ constexpr int bc() const { return 4 * n + 1; }
   __host__ __device__ constexpr int size() const { return 4 * n + 2; }
+};
+
+// Rows/columns of the SAB/DAB contraction table for (N, S): rows r*n+c (SAB)
+// and n*n + r*n + c (DAB), columns (l*S + o)*S + o3 over l < 2N.
+struct YOff {
+  int n, K;
+  __host__ __device__ constexpr YOff(int N_, int S_) : n(N_ * S_), K(2 * N_ * S_ * S_) {}
+  __host__ __device__ constexpr int rows() const { return 2 * n * n; }
+  __host__ __device__ constexpr int at(int M, int row, int k) const { return (M * rows() + row) * K + k; }
+  __host__ __device__ constexpr int size(int LM) const { return LM * rows() * K; }
 };
 
 // Quadrature table layout for (N, S): qs[N], qw[N], then for each Fourier term

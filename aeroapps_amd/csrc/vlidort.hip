@@ -176,6 +176,44 @@ const std::vector<double>& quad_table_cached(int N, int S, int nmoments) {
   return it->second;
 }
 
+// Contraction weights of the mixed-precision SAB/DAB GEMM for every Fourier
+// term: SAB(r,c) = sum_k Y[r*n+c][k] og[k], DAB(r,c) = sum_k Y[n^2+r*n+c][k] og[k]
+// with k = (l,o,o3) (restates the SAB/DAB sums of REG/vlidort_solutions.f90:
+// 439-475 with the quadrature PI matrices folded in).
+const std::vector<float>& ytab_cached(int N, int S, int nmoments) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int>, std::vector<float>> cache;
+  const std::vector<double>& q = quad_table_cached(N, S, nmoments);
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_tuple(N, S, nmoments);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  const int n = N * S, LM = 2 * N;
+  QOff QO(N, S);
+  YOff YO(N, S);
+  std::vector<float> y((size_t)YO.size(LM), 0.0f);
+  const double DM[4] = {1.0, 1.0, -1.0, -1.0};
+  for (int M = 0; M < LM && M <= nmoments; ++M)
+    for (int i = 0; i < N; ++i)
+      for (int o1 = 0; o1 < S; ++o1)
+        for (int j = 0; j < N; ++j)
+          for (int o2 = 0; o2 < S; ++o2) {
+            const int r = i * S + o1, c = j * S + o2;
+            const double fac = (-1.0 / q[QO.qs(i)]) * (0.5 * q[QO.qw(j)]);
+            for (int l = M; l < LM && l <= nmoments; ++l) {
+              const double ps = (((l - M) & 1) == 0) ? 1.0 : -1.0;
+              for (int o = 0; o < S; ++o)
+                for (int o3 = 0; o3 < S; ++o3) {
+                  const int k = (l * S + o) * S + o3;
+                  const double xx = q[QO.xq(M, l, i, o1, o)] * q[QO.xq(M, l, j, o3, o2)];
+                  y[YO.at(M, r * n + c, k)] = (float)(fac * xx * (1.0 + DM[o3] * ps));
+                  y[YO.at(M, n * n + r * n + c, k)] = (float)(fac * xx * (1.0 - DM[o3] * ps));
+                }
+            }
+          }
+  return cache.emplace(key, std::move(y)).first->second;
+}
+
 // Library streams (default priority) used to run stream-count buckets of one
 // call concurrently; created once per (device, caller stream, bucket slot), so
 // calls on different caller streams never share them.
@@ -224,6 +262,7 @@ size_t carve_work(Carver& C, RTWork& W, int cap) {
   W.lbuf = C.take<double>((size_t)cap * km * W.lstride);
   W.bbuf = C.take<double>((size_t)cap * W.bstride);
   W.ubuf = C.take<double>((size_t)cap * (2 * n * km) * W.ustride);
+  W.sab32 = (W.precision == 1) ? C.take<float>((size_t)cap * km * 2 * n * n) : nullptr;
   return C.off;
 }
 
@@ -252,6 +291,8 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   const int nitems = g1 - g0;
   if (nitems <= 0) return 0;
   const std::vector<double>& qtab = quad_table_cached(W.N, W.S, W.nmoments);
+  static const std::vector<float> no_ytab;
+  const std::vector<float>& ytab = (W.precision == 1) ? ytab_cached(W.N, W.S, W.nmoments) : no_ytab;
   // chunk capacity from the work budget
   size_t freeb = 0, totalb = 0;
   AA_CHECK(hipMemGetInfo(&freeb, &totalb));
@@ -262,10 +303,12 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   const size_t envcap = env_bytes("AA_CHUNK_ITEMS", 0);
   if (envcap) cap = (int)std::min<size_t>((size_t)cap, envcap);
   // arena: work + quad table + active lists + counters
+  float* dy = nullptr;
   auto carve_all = [&](char* base, RTWork& w, double** dq, int** l0, int** l1, int** cnt) {
     Carver C(base);
     carve_work(C, w, cap);
     *dq = C.take<double>(qtab.size());
+    dy = C.take<float>(std::max<size_t>(1, ytab.size()));
     *l0 = C.take<int>(cap);
     *l1 = C.take<int>(cap);
     *cnt = C.take<int>(W.nmoments + 2);
@@ -279,6 +322,10 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   carve_all(base, W, &dq, &l0, &l1, &cnt);
   AA_CHECK(hipMemcpyAsync(dq, qtab.data(), qtab.size() * sizeof(double), hipMemcpyHostToDevice, st));
   W.quad = dq;
+  if (!ytab.empty()) {
+    AA_CHECK(hipMemcpyAsync(dy, ytab.data(), ytab.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    W.ytab = dy;
+  }
   int* lists[2] = {l0, l1};
   W.item_list = d_items;
   for (int item0 = g0; item0 < g1; item0 += cap) {
@@ -320,7 +367,7 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
       !reflectance)
     return AA_EARG;
   if (vector && (!q || !u || !flux_factor || npol < 6)) return AA_EARG;
-  if (precision != 0) return AA_EARG;
+  if (precision != 0 && precision != 1) return AA_EARG;
   const int S = vector ? 3 : 1;
   if (nstreams < 1 || nstreams > max_streams(S)) { *rc = 2; return AA_EDIMS; }
   if (km > MAX_LAYERS) { *rc = 3; return AA_EDIMS; }
@@ -330,6 +377,7 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
   W.N = nstreams; W.S = S; W.nmoments = std::min(2 * nstreams - 1, nmom);
   W.plane_parallel = plane_parallel ? 1 : 0;
   W.missing = missing;
+  W.precision = precision;
   const size_t nl = (size_t)km * nobs * nch;
   DevBuf B;
   W.rot = B.up(rot, nl);
@@ -416,7 +464,7 @@ int aa_column_radiance_dev(aa_mie* t, int km, int nobs, int nch, int nq, const i
   if (!tracer_idx || !chan_idx || !d_qm || !d_rh || !d_rot || !d_depol || !d_he || !d_albedo || !d_sza || !d_raa ||
       !d_vza || !d_radiance || !d_reflectance || !d_nfourier)
     return AA_EARG;
-  if (precision != 0) return AA_EARG;
+  if (precision != 0 && precision != 1) return AA_EARG;
   const int S = vector ? 3 : 1;
   if (vector && (!d_q || !d_u || npol < 6)) return AA_EARG;
   if (!d_nstreams_col && (nstreams < 1 || nstreams > max_streams(S))) return AA_EDIMS;
@@ -477,6 +525,7 @@ int aa_column_radiance_dev(aa_mie* t, int km, int nobs, int nch, int nq, const i
   W.nlayers = d_nlayers;
   W.radiance = d_radiance; W.reflectance = d_reflectance; W.qout = d_q; W.uout = d_u; W.nfourier = d_nfourier;
   W.optics_f32 = 1;
+  W.precision = precision;
   W.tau = dt; W.ssa = ds; W.pmom = dp;
   int rcode = compute_latch(W, lp, st);
   for (int b = 0; b < nblk && !rcode; ++b) {

@@ -74,8 +74,10 @@ class Server:
     """Admission + continuous batching over `slots` concurrent GPU streams."""
 
     def __init__(self, torch, mie, pool: ColumnPool, nstreams: int, nmom: int, slots: int = 4,
-                 hbm_budget: float = 200e9, arena_bytes: float = 28e9, keep_results: bool = False):
+                 hbm_budget: float = 200e9, arena_bytes: float = 28e9, keep_results: bool = False,
+                 precision: int = 0):
         self.torch = torch
+        self.precision = precision
         self.mie = mie
         self.pool = pool
         self.nstreams = nstreams
@@ -117,7 +119,7 @@ class Server:
             p = lambda t: t.data_ptr()
             e = _lib.lib().aa_column_radiance_dev(
                 self.mie.handle, km, g.ncol, nch, len(S.TRACERS), p(self.tr_idx), p(ch_idx), p(qm), p(rh),
-                self.nmom, 1, self.nstreams, 0, 0, None, None, p(rot), p(depol), p(pe), p(he), p(te), p(alb),
+                self.nmom, 1, self.nstreams, 0, self.precision, None, None, p(rot), p(depol), p(pe), p(he), p(te), p(alb),
                 p(sza), p(raa), p(vza), None, S.MISSING, p(rad), p(refl), None, None, p(nf), st.cuda_stream)
             _lib.check(e, "aa_column_radiance_dev")
             if self.keep:

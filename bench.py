@@ -106,11 +106,11 @@ def cpu_baseline(wl, ncols, procs):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cfg1_scalar_500k")
     ap.add_argument("--ncol", type=int, default=0, help="override the column count")
-    ap.add_argument("--precision", type=int, default=0)
+    ap.add_argument("--fp64-only", action="store_true", help="skip the mixed-precision run")
     ap.add_argument("--cpu-cols", type=int, default=512)
     ap.add_argument("--cpu-procs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -174,51 +174,64 @@ def main():
     stream = torch.cuda.current_stream(dev)
     ptr = lambda t: t.data_ptr() if t is not None else None
 
-    def run_step():
+    def run_step(precision):
         e = L.aa_column_radiance_dev(mie.handle, km, ncol, nch, nq, ptr(tr_idx), ptr(ch_idx), ptr(qm), ptr(rh), nmom,
-                                     npol, N, int(vector), args.precision, None, None, ptr(rot), ptr(depol), ptr(pe),
+                                     npol, N, int(vector), precision, None, None, ptr(rot), ptr(depol), ptr(pe),
                                      ptr(he), ptr(te), ptr(albedo), ptr(sza), ptr(raa), ptr(vza), ptr(flux),
                                      S.MISSING, ptr(rad), ptr(refl), ptr(qo), ptr(uo), ptr(nf), stream.cuda_stream)
         _lib.check(e, "aa_column_radiance_dev")
 
-    for i in range(args.warmup):
-        run_step()
-        torch.cuda.synchronize()
-        print(f"warmup step {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    L.aa_profile_start()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        run_step()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
     import ctypes
-    msa = (ctypes.c_double * 8)()
-    lna = (ctypes.c_longlong * 8)()
-    _lib.check(L.aa_profile_read(msa, lna, 8), "aa_profile_read")
-    kms = {PROF_NAMES[k]: msa[k] for k in range(7)}
-    kln = {PROF_NAMES[k]: lna[k] for k in range(7)}
-
-    ms_per_step = el * 1000.0 / args.steps
-    value = ncol / (ms_per_step / 1000.0)
-
-    # algorithmic work of the timed region from the Fourier-term counts
-    nfh = nf.cpu().numpy().ravel()
-    ok = nfh > 0
     nmoments = min(2 * N - 1, nmom)
     Sx = 3 if vector else 1
+
+    def timed(precision):
+        for i in range(args.warmup):
+            run_step(precision)
+            torch.cuda.synchronize()
+            print(f"precision {precision}: warmup step {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        L.aa_profile_start()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            run_step(precision)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        msa = (ctypes.c_double * 8)()
+        lna = (ctypes.c_longlong * 8)()
+        _lib.check(L.aa_profile_read(msa, lna, 8), "aa_profile_read")
+        kms = {PROF_NAMES[k]: msa[k] for k in range(7)}
+        kln = {PROF_NAMES[k]: lna[k] for k in range(7)}
+        ms_per_step = el * 1000.0 / args.steps
+        nfh = nf.cpu().numpy().ravel().copy()
+        radh = rad.cpu().numpy().ravel().copy()
+        print(f"precision {precision}: {args.steps} timed steps done, {ms_per_step:.1f} ms/step", file=sys.stderr,
+              flush=True)
+        return dict(el=el, kms=kms, kln=kln, ms_per_step=ms_per_step, value=ncol / (ms_per_step / 1000.0), nf=nfh,
+                    rad=radh)
+
+    r64 = timed(0)
+    rmx = timed(1) if not args.fp64_only else None
+    main_run = rmx if rmx is not None else r64
+
+    # algorithmic work of the timed region from the Fourier-term counts
+    nfh = main_run["nf"]
+    ok = nfh > 0
     fb = fc = fd = 0.0
     counts = np.bincount(nfh[ok], minlength=2 * N + 1)
     for terms, cnt in enumerate(counts):
         for M in range(terms):
-            b, c, d = flops_per_term(N, Sx, km, nmoments, M)
-            fb += cnt * b; fc += cnt * c; fd += cnt * d
+            b_, c_, d_ = flops_per_term(N, Sx, km, nmoments, M)
+            fb += cnt * b_; fc += cnt * c_; fd += cnt * d_
     fb *= args.steps; fc *= args.steps; fd *= args.steps
+    kms, kln = main_run["kms"], main_run["kln"]
     dom = max(("layer", "bvp"), key=lambda k: kms[k])
     dom_flops = fc + fd if dom == "bvp" else fb
     dom_ms = kms[dom]
     dom_launches = max(1, kln[dom])
     achieved = dom_flops / (dom_ms / 1000.0) / 1e12
+    el = main_run["el"]
+    value = main_run["value"]
     path_bytes = bytes_per_column(km, nq, nch, vector) * ncol * args.steps
     out = {
         "metric": METRIC,
@@ -227,16 +240,17 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(main_run["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == 0 else "f32+f64",
+        "dtype": "f32+f64" if rmx is not None else "f64",
         "data": "synthetic (seeded, generated on device)",
         "config": {"workload": f"{args.workload}: {ncol} columns x {km} layers, {nch} channels "
                                f"{list(wl['channels'])} nm, {2 * N} streams, {nmom} moments, {nq} tracers, "
                                f"{'vector' if vector else 'scalar'} Lambertian",
-                   "precision": "fp64" if args.precision == 0 else "mixed",
+                   "precision": "mixed (FP32 optics/phase-matrix contraction on MFMA/source assembly; FP64 "
+                                "eigensystem and band elimination)" if rmx is not None else "fp64",
                    "path": "aa_column_radiance_dev (Chem_Mie optics + VLIDORT radiance, device-resident)"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
@@ -248,11 +262,21 @@ def main():
                      "frac": round(path_bytes / el / 1e9 / PEAK_HBM_GBS, 6)},
         "kernel_ms": {k: round(v, 3) for k, v in kms.items()},
         "fourier_terms_mean": round(float(nfh[ok].mean()), 3),
+        "fp64": {"value": round(r64["value"], 1), "ms_per_step": round(r64["ms_per_step"], 3),
+                 "kernel_ms": {k: round(v, 3) for k, v in r64["kms"].items()}},
         "cpu_baseline": cpu,
         "gen_s": round(gen_s, 2),
     }
+    if rmx is not None:
+        same = (rmx["nf"] == r64["nf"]) & ok
+        rel = np.abs(rmx["rad"] - r64["rad"])[same] / np.abs(r64["rad"][same])
+        out["mixed_vs_fp64"] = {"speedup": round(rmx["value"] / r64["value"], 3),
+                                "max_rel_err_radiance": float(rel.max()) if rel.size else None,
+                                "p99_rel_err_radiance": float(np.percentile(rel, 99)) if rel.size else None,
+                                "fourier_count_mismatch_frac": float(1 - same.sum() / ok.sum())}
     if cpu:
         out["gpu_vs_cpu"] = round(value / cpu["value"], 1)
+        out["fp64_vs_cpu"] = round(r64["value"] / cpu["value"], 1)
 
     # serving loop: Poisson granules at a fraction of the measured peak
     if args.serving_granules > 0 and not vector:
@@ -264,10 +288,12 @@ def main():
         mean_cols = float(np.mean([g.ncol for g in grans]))
         rate = args.serving_load * value / mean_cols
         arr = SV.poisson_arrivals(len(grans), rate)
-        srv = SV.Server(torch, mie, pool, N, nmom, slots=4, hbm_budget=200e9, arena_bytes=28e9)
+        srv = SV.Server(torch, mie, pool, N, nmom, slots=4, hbm_budget=200e9, arena_bytes=28e9,
+                        precision=1 if rmx is not None else 0)
         st = srv.serve(grans, arr)
         out["serving"] = {"granules": st["granules"], "columns": st["columns"],
                           "offered_load": args.serving_load, "arrival_rate_per_s": round(rate, 4),
+                          "precision": "mixed" if rmx is not None else "fp64",
                           "p50_latency_s": round(st["p50_s"], 3), "p99_latency_s": round(st["p99_s"], 3),
                           "throughput_cols_per_s": round(st["throughput_cols_per_s"], 1), "slots": st["slots"],
                           "granule_columns": "log-uniform 1000-200000, 1-4 channels, 72 layers, 12 streams"}

@@ -8,23 +8,30 @@
 //            Green's-function beam solution (VLIDORT_GBEAM_SOLUTION :2649-2932)
 //            and the layer's upwelling source coefficients
 //            (REG/vlidort_PostProcessing.f90:139-451).
-//   k_bvp:   one wavefront per item.  The banded boundary-value problem
-//            (REG/vlidort_bvproblem.f90:147-2311): matrix rows are generated on
-//            the fly from the layer solutions, factorised by partial-pivoting
-//            band elimination with the same pivot choices as DGBTF2
+//   k_bvp:   the banded boundary-value problem (REG/vlidort_bvproblem.f90:
+//            147-2311) for 64/G items per wavefront (G lanes per item): matrix
+//            rows are built from the layer solutions, factorised by partial
+//            pivoting with the pivot choices of DGBTF2
 //            (REG/lapack_tools.f90:156-357) while the right-hand side is carried
-//            along (DGBTRS forward part, :7354); back-substitution then yields
-//            the integration constants column by column and the TOA source
-//            recursion (REG/vlidort_intensity.f90:145-643) and the Fourier
-//            convergence test (REG/vlidort_converge.f90:825-1231) are folded in.
+//            along (DGBTRS forward part, :7354); back-substitution in DTBSV's
+//            column order then yields the integration constants layer by layer,
+//            with the TOA source recursion (REG/vlidort_intensity.f90:145-643)
+//            and the Fourier convergence test
+//            (REG/vlidort_converge.f90:825-1231) folded in.
 //
-// Band LU layout: lanes hold the columns of the active window (width
-// 2*KL+1, circular over Q = 64*CPL-1 virtual lanes, the last virtual lane
-// holds the right-hand side); every lane keeps the KL+1 candidate rows of its
-// column in registers.  The pivot search runs in the lane owning the pivot
-// column and the multipliers are broadcast with readlane, so a step needs no
-// LDS and no barrier.  Eliminated rows (U and the transformed right-hand
-// side) go to a per-item buffer that the back-substitution reads in reverse.
+// Panel LU: with the unknowns ordered layer by layer (2n per layer) the n TOA
+// rows touch layer 0 only, the 2n rows of boundary k touch layers k and k+1,
+// and the n BOA rows touch the bottom layer.  DGBTF2's band window at a column
+// of layer k therefore holds, besides structural zeros, exactly the n rows left
+// over from layer k-1 and the 2n rows of boundary k: one panel of 3n rows over
+// the 4n columns of layers k, k+1 and the right-hand side.  One lane holds one
+// row in registers; the pivot search is a lane reduction, the pivot row is
+// broadcast through LDS, and the rows of the next boundary move into the lanes
+// whose rows became pivots.  Row positions are tracked as DGBTF2 numbers them
+// (ties go to the smallest position), so pivots, interchanges and updated
+// values are those of the band factorisation without its arithmetic on
+// structural zeros.  U rows (with 1/pivot) go to a per-item buffer that the
+// back-substitution reads layer by layer.
 #pragma once
 #include <type_traits>
 #include "rt_work.h"
@@ -537,36 +544,78 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
 }
 
 // ------------------------------------------------------------------ k_bvp
+// Group reductions over the G lanes that hold one item's rows (G = 16, 32 or
+// 64; every lane of the group ends with the same value).
+template <int G>
+__device__ __forceinline__ double grp_max(double x) {
+  x = dmax(x, dpp_d<0xB1>(x));    // quad_perm [1,0,3,2]
+  x = dmax(x, dpp_d<0x4E>(x));    // quad_perm [2,3,0,1]
+  x = dmax(x, dpp_d<0x141>(x));   // row_half_mirror
+  x = dmax(x, dpp_d<0x140>(x));   // row_mirror
+  if (G >= 32) x = dmax(x, swz16_d(x));
+  if (G == 64) x = dmax(readlane_d(x, 0), readlane_d(x, 32));
+  return x;
+}
+template <int G>
+__device__ __forceinline__ double grp_sum(double x) {
+  x += dpp_d<0xB1>(x);
+  x += dpp_d<0x4E>(x);
+  x += dpp_d<0x141>(x);
+  x += dpp_d<0x140>(x);
+  if (G >= 32) x += swz16_d(x);
+  if (G == 64) x = readlane_d(x, 0) + readlane_d(x, 32);
+  return x;
+}
+template <int G>
+__device__ __forceinline__ int grp_min_i(int x) {
+  x = min(x, __builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, false));
+  x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, false));
+  x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x141, 0xF, 0xF, false));
+  x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x140, 0xF, 0xF, false));
+  if (G >= 32) x = min(x, __builtin_amdgcn_ds_swizzle(x, 0x401F));
+  if (G == 64) x = min(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 32));
+  return x;
+}
+
+// Lanes per item in k_bvp: one lane per row of a layer panel (3n rows).
+__host__ __device__ constexpr int bvp_group(int n) { return 3 * n <= 16 ? 16 : (3 * n <= 32 ? 32 : 64); }
+
 template <int N, int S>
 __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restrict__ active,
                                             const int* __restrict__ nactive, int* __restrict__ next,
                                             int* __restrict__ nnext) {
-  constexpr int n = N * S, n2 = 2 * n, KL = 3 * n - 1, R = KL + 1, WD = 2 * KL + 1;
-  constexpr int CPL = (WD + 1 > 64) ? 2 : 1;
-  constexpr int P = 64 * CPL, Q = P - 1;
-  constexpr int PF = 4;    // back-substitution: right-hand sides loaded PF steps ahead
-  static_assert(WD + PF + 1 <= P, "window does not fit the wavefront");
+  constexpr int n = N * S, n2 = 2 * n, NC = 4 * n, RH = NC, UW = NC + 2;
+  constexpr int G = bvp_group(n), IPW = 64 / G;
+  static_assert(3 * n <= 64, "panel rows exceed the wavefront");
   constexpr LOff O(N, S);
   constexpr BOff BO(n);
   constexpr int BIG = 0x3fffffff;
-  const int ai = blockIdx.x;
-  if (ai >= *nactive) return;
-  const int it = active[ai];
-  const int lane = threadIdx.x;
+  __shared__ __align__(16) double us[IPW][n2][UW];   // U rows of the current panel (cols, RHS, 1/pivot)
+  __shared__ double xs[IPW][n2];       // integration constants of the current/next layer
+  __shared__ int pp[IPW];              // row position of the current pivot
+  const int lane = threadIdx.x, g = lane / G, t = lane - g * G;
+  const unsigned long long gm = (G == 64) ? ~0ull : (((1ull << (G & 63)) - 1ull) << (g * G));
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const int ai = blockIdx.x * IPW + g;
+  bool valid = ai < *nactive;
+  if (__ballot(valid) == 0ull) return;
+  const int it = valid ? active[ai] : active[blockIdx.x * IPW];
   const int km = W.km;
-  const int g = global_item(W, it), obs = g / W.nch, ch = g - obs * W.nch;
+  const int gi = global_item(W, it), obs = gi / W.nch, ch = gi - obs * W.nch;
   const size_t oi = (size_t)ch * W.nobs + obs;
-  if (W.status[it] != 0) {
-    if (lane == 0) {
+  if (valid && W.status[it] != 0) {
+    if (t == 0) {
       W.radiance[oi] = W.missing; W.reflectance[oi] = W.missing;
       if (S > 1) { W.qout[oi] = W.missing; W.uout[oi] = W.missing; }
       W.nfourier[oi] = 0;
     }
-    return;
+    valid = false;
   }
-  const int L = W.L_it[it];
-  const int NT = n2 * L;
-  const int LS = W.lstride, UW = W.ustride;
+  const int L = valid ? W.L_it[it] : 0;
+  int Lmax = 0;
+#pragma unroll
+  for (int q = 0; q < IPW; ++q) Lmax = max(Lmax, __builtin_amdgcn_readlane(L, q * G));
+  const int LS = W.lstride;
   const double* __restrict__ lb = W.lbuf + (size_t)it * km * LS;
   const double* __restrict__ bb = W.bbuf + (size_t)it * W.bstride;
   double* __restrict__ ub = W.ubuf + (size_t)it * (size_t)n2 * km * UW;
@@ -582,292 +631,195 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
   }
   const double r2beam = surf ? bb[BO.r2beam()] : 0.0;
 
-  // BVP matrix element (row r, column c), 0-based (BVP_MATRIX_SETUP,
-  // REG/vlidort_bvproblem.f90:627-1176)
-  auto elem = [&](int r, int c) -> double {
-    if (r < n) {
-      const int i = r / S, o1 = r - (r / S) * S;
-      if (c < n) return lb[O.xp(i, o1, c)];
-      if (c < n2) return lb[O.xn(i, o1, c - n)] * lb[O.tdel(c - n)];
-      return 0.0;
-    }
-    if (r >= NT - n) {
-      const int rr = r - (NT - n);
-      const int i = rr / S, o1 = rr - (rr / S) * S;
-      const int q = c - (NT - n2);
-      if (q < 0 || q >= n2) return 0.0;
-      const double* lr = lb + (size_t)(L - 1) * LS;
-      if (q < n) {
-        double xpn = lr[O.xp(i + N, o1, q)];
-        if (surf && o1 == 0) xpn = xpn - bb[BO.r2p(q)];
-        return lr[O.tdel(q)] * xpn;
-      }
-      double xmn = lr[O.xn(i + N, o1, q - n)];
-      if (surf && o1 == 0) xmn = xmn - bb[BO.r2m(q - n)];
-      return xmn;
-    }
-    const int tt = r - n;
-    const int nl0 = tt / n2, irow = tt - nl0 * n2;
-    const int i = irow / S, o1 = irow - (irow / S) * S;
-    const int q = c - nl0 * n2;
-    if (q < 0 || q >= 2 * n2) return 0.0;
-    const double* lu = lb + (size_t)nl0 * LS;
-    const double* ll = lu + LS;
-    if (q < n) return lu[O.tdel(q)] * lu[O.xp(i, o1, q)];
-    if (q < n2) return lu[O.xn(i, o1, q - n)];
-    if (q < 3 * n) return -ll[O.xp(i, o1, q - n2)];
-    return -ll[O.tdel(q - 3 * n)] * ll[O.xn(i, o1, q - 3 * n)];
-  };
-  // right-hand side (BVP_COLUMN_SETUP + BVP_SURFACE_SETUP_BEAM)
-  auto rhs = [&](int r) -> double {
-    if (r < n) { const int i = r / S, o1 = r - (r / S) * S; return -lb[O.wu(i, o1)]; }
-    if (r >= NT - n) {
-      const int rr = r - (NT - n);
-      const int i = rr / S, o1 = rr - (rr / S) * S;
-      double v = -lb[(size_t)(L - 1) * LS + O.wl(i + N, o1)];
-      if (surf && o1 == 0) v = v + r2beam;
-      if (surf && o1 == 0) v = v + rf_direct;
-      return v;
-    }
-    const int tt = r - n;
-    const int nl0 = tt / n2, irow = tt - nl0 * n2;
-    const int i = irow / S, o1 = irow - (irow / S) * S;
-    return lb[(size_t)(nl0 + 1) * LS + O.wu(i, o1)] - lb[(size_t)nl0 * LS + O.wl(i, o1)];
-  };
-  // value of row r at the column held by virtual lane v when the window starts
-  // at column j0 (jm0 = j0 mod Q)
-  auto gen = [&](int v, int r, int j0, int jm0) -> double {
-    if (r >= NT) return 0.0;
-    if (v == Q) return rhs(r);
-    int off = v - jm0;
-    if (off < 0) off += Q;
-    const int c = j0 + off;
-    if (off >= WD || c >= NT) return 0.0;
-    return elem(r, c);
-  };
-
-  // ---------------- band LU with partial pivoting + forward substitution
-  // Step j: the lane owning column j copies it to LDS; lanes 0..R-1 read it
-  // back transposed (lane = row slot) and a butterfly finds the pivot: largest
-  // |a(r,j)|, ties to the smallest row position (IDAMAX order of DGBTF2).  Every
-  // lane then eliminates its column with the broadcast column:
-  //   a(s,c) -= a(s,j) * (a(p,c) / a(p,j)).
-  // The column leaving the window is zeroed exactly (its own scale is 1), so a
-  // lane re-entering as a new column starts from zeros.  The row entering the
-  // window after step j (position j+R) is independent of the pivoting and is
-  // loaded PD steps ahead.
-  constexpr int PD = 4;
-  constexpr int NV = (R + 7) / 8;
-  static_assert(NV <= 5 && R <= 64, "candidate rows exceed the register layout");
-  __shared__ double colbuf[NV * 8];
-  d8v a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;   // column slot 0
-  d8v b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0, b4 = 0.0;   // column slot 1 (two columns per lane)
+  // ---------------- panel LU with partial pivoting + forward substitution
+  // Lane t of the group holds one row of the panel: a[0..2n) the columns of
+  // layer k, a[2n..4n) those of layer k+1, a[4n] the right-hand side.
+  double a[NC + 1];
 #pragma unroll
-  for (int s = 0; s < R; ++s) {
-    AA_SLOT_SET(a, s, gen(lane, s, 0, 0));
-    if (CPL > 1) AA_SLOT_SET(b, s, gen(lane + 64, s, 0, 0));
+  for (int c = 0; c <= NC; ++c) a[c] = 0.0;
+  bool act = false;
+  int posl = BIG;
+  bool sing = false;
+  if (valid && t < n) {   // TOA rows (BVP_MATRIX_SETUP, upper boundary), layer 0
+    const int i = t / S, o1 = t - (t / S) * S;
+#pragma unroll
+    for (int q = 0; q < n; ++q) a[q] = lb[O.xp(i, o1, q)];
+#pragma unroll
+    for (int q = 0; q < n; ++q) a[n + q] = lb[O.xn(i, o1, q)] * lb[O.tdel(q)];
+    a[RH] = -lb[O.wu(i, o1)];
+    act = true;
+    posl = t;
   }
-  int posl = (lane < R && lane < NT) ? lane : BIG;   // row position held by slot `lane`
-  if (lane < NV * 8) colbuf[lane] = 0.0;
-  auto fetch_row = [&](int j, double (&dst)[CPL]) {
-    const int j1 = j + 1;
-    const int jm1x = j1 % Q;
+
+  // one panel: eliminate its 2n columns; LAST: the bottom layer (no layer k+1)
+  auto lu_panel = [&](auto lastc, int k) {
+    constexpr bool LAST = decltype(lastc)::value;
+    constexpr int NCOL = LAST ? n2 : NC;
+    const int j0 = n2 * k;
 #pragma unroll
-    for (int cs = 0; cs < CPL; ++cs) dst[cs] = gen(lane + 64 * cs, j + R, j1, jm1x);
-  };
-  double nvq[PD][CPL];
+    for (int i = 0; i < n2; ++i) {
+      const int j = j0 + i;
+      const double xi = a[i];
+      const double ax = fabs(xi);
+      // pivot: largest |a(r,j)| among the candidate rows, ties to the smallest
+      // row position (IDAMAX order of DGBTF2)
+      const double mx = grp_max<G>(act ? ax : -1.0);
+      const unsigned long long eb = __ballot(act && ax == mx) & gm;
+      const unsigned long long ab = __ballot(act) & gm;
+      const unsigned long long cand = eb ? eb : ab;
+      const bool cnd = (cand >> lane) & 1ull;
+      bool ispiv;
+      if (__ballot(__popcll(cand) > 1) != 0ull) {
+        const int kp = grp_min_i<G>(cnd ? posl : BIG);
+        ispiv = cnd && posl == kp;
+      } else {
+        ispiv = cnd && (cand & below) == 0ull;
+      }
+      if (ispiv) {
 #pragma unroll
-  for (int d = 0; d < PD; ++d) fetch_row(d, nvq[d]);
-  bool singular = false;
-  int jm = 0;   // j mod Q
-  __syncthreads();
-  for (int jb = 0; jb < NT; jb += PD) {
-    double nvn[PD][CPL];
-#pragma unroll
-    for (int d = 0; d < PD; ++d) fetch_row(jb + PD + d, nvn[d]);
-#pragma unroll
-    for (int d = 0; d < PD; ++d) {
-      const int j = jb + d;
-      if (j >= NT) break;
-      const int jm1 = (jm + 1 == Q) ? 0 : jm + 1;
-      const int rnew = j + R;
-      const int lj = jm & 63, csj = jm >> 6;
-      // column j to LDS
-      if (lane == lj) {
-#pragma unroll
-        for (int s = 0; s < R; ++s) {
-          double x;
-          if (CPL == 1 || csj == 0) AA_SLOT_GET(a, x, s) else AA_SLOT_GET(b, x, s)
-          colbuf[s] = x;
-        }
+        for (int c = i; c < NCOL; ++c) us[g][i][c] = a[c];
+        us[g][i][RH] = a[RH];
+        pp[g] = posl;
       }
       __syncthreads();
-      // pivot search, lane = row slot: max |a(s,j)| over the candidate rows,
-      // ties resolved to the smallest row position
-      const double xt = (lane < R) ? colbuf[lane] : 0.0;
-      double am = (lane < R && posl < BIG) ? fabs(xt) : -1.0;
-      am = dmax(am, dpp_d<0xB1>(am));
-      am = dmax(am, dpp_d<0x4E>(am));
-      am = dmax(am, dpp_d<0x141>(am));
-      am = dmax(am, dpp_d<0x140>(am));
-      if (R > 16) am = dmax(am, swz16_d(am));
-      double mx = readlane_d(am, 0);
-      if (R > 32) mx = dmax(mx, readlane_d(am, 32));
-      unsigned long long cand = __ballot(lane < R && posl < BIG && fabs(xt) == mx);
-      int bsu = (int)__builtin_ctzll(cand);
-      if (__popcll(cand) > 1) {   // equal magnitudes: smallest position wins
-        int best = BIG;
-        for (unsigned long long m = cand; m; m &= m - 1) {
-          const int l = (int)__builtin_ctzll(m);
-          const int pl = __builtin_amdgcn_readlane(posl, l);
-          if (pl < best) { best = pl; bsu = l; }
-        }
-      }
-      const int jp = __builtin_amdgcn_readlane(posl, bsu);
-      const int sj = (int)__builtin_ctzll(__ballot(lane < R && posl == j));
-      const double pv = readlane_d(xt, bsu);
-      if (pv == 0.0) singular = true;
+      const double pv = us[g][i][i];
+      const int ppos = pp[g];
       const double rp = 1.0 / pv;
-      double xc[NV * 8];
+      const double l = (act && !ispiv) ? xi * rp : 0.0;
 #pragma unroll
-      for (int s = 0; s < NV * 8; ++s) xc[s] = colbuf[s];
-      // eliminate, store the pivot row (U row j and transformed RHS), and
-      // put the next row into the pivot slot
-      {
-        double u;
-        AA_SLOT_GET(a, u, bsu);
-        const int v = lane;
-        const double w = (v == jm) ? 1.0 : u * rp;
-        AA_SLOT_AXPY(a, w);
-        if (v == Q) ub[(size_t)j * UW + WD] = u;
-        else {
-          int off = v - jm;
-          if (off < 0) off += Q;
-          if (off < WD && j + off < NT) ub[(size_t)j * UW + off] = u;
-        }
-        AA_SLOT_SET(a, bsu, nvq[d][0]);
+      for (int c = i + 1; c < NCOL; ++c) a[c] = fma(-l, us[g][i][c], a[c]);
+      a[RH] = fma(-l, us[g][i][RH], a[RH]);
+      sing = sing || (pv == 0.0);
+      if (ispiv) {
+        us[g][i][RH + 1] = rp;
+        act = false;
+      } else if (act && posl == j) {
+        posl = ppos;   // row interchange: the row at position j moves to the pivot's position
       }
-      if (CPL > 1) {
-        double u;
-        AA_SLOT_GET(b, u, bsu);
-        const int v = lane + 64;
-        const double w = (v == jm) ? 1.0 : u * rp;
-        AA_SLOT_AXPY(b, w);
-        if (v == Q) ub[(size_t)j * UW + WD] = u;
-        else {
-          int off = v - jm;
-          if (off < 0) off += Q;
-          if (off < WD && j + off < NT) ub[(size_t)j * UW + off] = u;
-        }
-        AA_SLOT_SET(b, bsu, nvq[d][CPL - 1]);
-      }
-      // row interchange bookkeeping: the row at position j moves to position jp
-      if (lane == sj && sj != bsu) posl = jp;
-      if (lane == bsu) posl = (rnew < NT) ? rnew : BIG;
-      __syncthreads();
-      jm = jm1;
     }
+    __syncthreads();
+    double2* dst = reinterpret_cast<double2*>(ub + (size_t)j0 * UW);
+    const double2* src = reinterpret_cast<const double2*>(&us[g][0][0]);
+    for (int e = t; e < n2 * UW / 2; e += G) dst[e] = src[e];
+    if (!LAST) {
 #pragma unroll
-    for (int d = 0; d < PD; ++d)
-#pragma unroll
-      for (int cs = 0; cs < CPL; ++cs) nvq[d][cs] = nvn[d][cs];
-  }
-
-  // ---------------- back-substitution fused with the TOA source recursion
-  // Row-oriented: lanes hold the solution values of the columns in the window
-  // (j, j+WD-1] (same circular lane map as the elimination) and load row j of
-  // U coalesced; x_j = (y_j - sum_c U(j,c) x_c) / U(j,j) with a wave
-  // reduction.  Rows are loaded PB steps ahead.
-  constexpr int PB = 8;
-  double xv[CPL];
-#pragma unroll
-  for (int cs = 0; cs < CPL; ++cs) xv[cs] = 0.0;
-  const double kmult = 2.0 * albedo;
-  double acc[S], cum[S];
-#pragma unroll
-  for (int o = 0; o < S; ++o) { acc[o] = 0.0; cum[o] = 0.0; }
-  double bacc = 0.0;
-  const double* trr = W.tr + (size_t)it * km * TR_NFIELDS;
-  // U(j, c) for this lane's column(s) when the solve is at row j
-  auto fetch_u = [&](int j, double (&dst)[CPL]) {
-    const int jmq = (j >= 0) ? j % Q : 0;
-#pragma unroll
-    for (int cs = 0; cs < CPL; ++cs) {
-      const int v = lane + 64 * cs;
-      int off = v - jmq;
-      if (off < 0) off += Q;
-      dst[cs] = (j >= 0 && v != Q && off >= 1 && off < WD && j + off < NT) ? ub[(size_t)j * UW + off] : 0.0;
+      for (int c = 0; c < n2; ++c) { a[c] = a[c + n2]; a[c + n2] = 0.0; }
     }
   };
-  double uq[PB][CPL], dq[PB], yq[PB];
+
+  for (int k = 0; k < Lmax; ++k) {
+    const bool here = valid && k < L;
+    const bool lastp = here && k == L - 1;
+    // the rows of boundary k (or the BOA rows) go to the lanes retired so far
+    const unsigned long long rb = __ballot(here && !act) & gm;
+    const int rank = __popcll(rb & below);
+    if (here && !act && rank < (lastp ? n : n2)) {
+      const double* lu = lb + (size_t)k * LS;
+      if (!lastp) {   // boundary between layers k and k+1
+        const double* ll = lu + LS;
+        const int i = rank / S, o1 = rank - (rank / S) * S;
 #pragma unroll
-  for (int d = 0; d < PB; ++d) {
-    const int j = NT - 1 - d;
-    fetch_u(j, uq[d]);
-    dq[d] = (j >= 0) ? ub[(size_t)j * UW] : 1.0;
-    yq[d] = (j >= 0) ? ub[(size_t)j * UW + WD] : 0.0;
-  }
-  for (int jb = NT - 1; jb >= 0; jb -= PB) {
-    double un[PB][CPL], dn[PB], yn[PB];
+        for (int q = 0; q < n; ++q) a[q] = lu[O.tdel(q)] * lu[O.xp(i, o1, q)];
 #pragma unroll
-    for (int d = 0; d < PB; ++d) {
-      const int j = jb - PB - d;
-      fetch_u(j, un[d]);
-      dn[d] = (j >= 0) ? ub[(size_t)j * UW] : 1.0;
-      yn[d] = (j >= 0) ? ub[(size_t)j * UW + WD] : 0.0;
-    }
+        for (int q = 0; q < n; ++q) a[n + q] = lu[O.xn(i, o1, q)];
 #pragma unroll
-    for (int d = 0; d < PB; ++d) {
-      const int j = jb - d;
-      if (j < 0) break;
-      double pr = 0.0;
+        for (int q = 0; q < n; ++q) a[n2 + q] = -ll[O.xp(i, o1, q)];
 #pragma unroll
-      for (int cs = 0; cs < CPL; ++cs) pr += uq[d][cs] * xv[cs];
-      pr += dpp_d<0xB1>(pr);
-      pr += dpp_d<0x4E>(pr);
-      pr += dpp_d<0x141>(pr);
-      pr += dpp_d<0x140>(pr);
-      pr += swz16_d(pr);
-      const double sum = readlane_d(pr, 0) + readlane_d(pr, 32);
-      const double xj = (yq[d] - sum) / dq[d];
-      const int jv = j % Q;
+        for (int q = 0; q < n; ++q) a[3 * n + q] = -ll[O.tdel(q)] * ll[O.xn(i, o1, q)];
+        a[RH] = ll[O.wu(i, o1)] - lu[O.wl(i, o1)];
+      } else {        // BOA rows with the Lambertian reflection (BVP_SURFACE_SETUP)
+        const int i = rank / S, o1 = rank - (rank / S) * S;
+        const bool sr = surf && o1 == 0;
 #pragma unroll
-      for (int cs = 0; cs < CPL; ++cs) if (lane + 64 * cs == jv) xv[cs] = xj;
-      // TOA source: layer nl, constant index q (lcon for q < n, mcon after)
-      const int nl = j / n2, q = j - nl * n2;
-      const double* lr = lb + (size_t)nl * LS;
-#pragma unroll
-      for (int o = 0; o < S; ++o) acc[o] += xj * lr[O.coef(o, q)];
-      if (surf && nl == L - 1) bacc += xj * bb[BO.bcoef(q)];
-      if (q == 0) {
-        if (nl == L - 1) {
-#pragma unroll
-          for (int o = 0; o < S; ++o) cum[o] = 0.0;
-          if (surf) cum[0] = kmult * (bacc + bb[BO.bc()]);
+        for (int q = 0; q < n; ++q) {
+          double xpn = lu[O.xp(i + N, o1, q)];
+          if (sr) xpn = xpn - bb[BO.r2p(q)];
+          a[q] = lu[O.tdel(q)] * xpn;
         }
-        const double tdu = trr[(size_t)nl * TR_NFIELDS + TR_TDUSER];
 #pragma unroll
-        for (int o = 0; o < S; ++o) {
-          const double src = acc[o] + lr[O.csrc(o)];
-          cum[o] = src + tdu * cum[o];
-          acc[o] = 0.0;
+        for (int q = 0; q < n; ++q) {
+          double xmn = lu[O.xn(i + N, o1, q)];
+          if (sr) xmn = xmn - bb[BO.r2m(q)];
+          a[n + q] = xmn;
+        }
+        double v = -lu[O.wl(i + N, o1)];
+        if (sr) v = v + r2beam;
+        if (sr) v = v + rf_direct;
+        a[RH] = v;
+      }
+      act = true;
+      posl = n + n2 * k + rank;
+    }
+    if (__ballot(here && !lastp) != 0ull) {
+      if (here && !lastp) lu_panel(std::integral_constant<bool, false>(), k);
+    }
+    if (__ballot(lastp) != 0ull) {
+      if (lastp) lu_panel(std::integral_constant<bool, true>(), k);
+    }
+  }
+
+  // ---------------- back-substitution (DTBSV column order) fused with the
+  // TOA source recursion (REG/vlidort_intensity.f90:145-643): lane t < 2n
+  // holds U row t of the panel and ends with x_t, the t-th integration
+  // constant of the layer (LCON for t < n, MCON after).
+  const double kmult = 2.0 * albedo;
+  double cum[S];
+#pragma unroll
+  for (int o = 0; o < S; ++o) cum[o] = 0.0;
+  const double* trr = W.tr + (size_t)it * km * TR_NFIELDS;
+  for (int k = Lmax - 1; k >= 0; --k) {
+    const bool here = valid && k < L;
+    if (__ballot(here) == 0ull) continue;
+    const bool lastp = k == L - 1;
+    const bool own = here && t < n2;
+    double u[UW];
+    if (own) {
+      const double2* src = reinterpret_cast<const double2*>(ub + ((size_t)k * n2 + t) * UW);
+#pragma unroll
+      for (int e = 0; e < UW / 2; ++e) { const double2 v = src[e]; u[2 * e] = v.x; u[2 * e + 1] = v.y; }
+    } else {
+#pragma unroll
+      for (int e = 0; e < UW; ++e) u[e] = 0.0;
+    }
+    double tv = u[RH];
+    if (here && !lastp) {
+#pragma unroll
+      for (int c = n2 - 1; c >= 0; --c) tv = fma(-u[n2 + c], xs[g][c], tv);
+    }
+    __syncthreads();
+    double xt = 0.0;
+#pragma unroll
+    for (int i = n2 - 1; i >= 0; --i) {
+      if (own && t == i) { xt = tv * u[RH + 1]; xs[g][i] = xt; }
+      __syncthreads();
+      const double xv = xs[g][i];
+      if (t < i) tv = fma(-u[i], xv, tv);
+    }
+    if (here) {
+      const double* lr = lb + (size_t)k * LS;
+      double sv[S];
+#pragma unroll
+      for (int o = 0; o < S; ++o) sv[o] = grp_sum<G>(own ? xt * lr[O.coef(o, t)] : 0.0);
+      if (lastp) {
+#pragma unroll
+        for (int o = 0; o < S; ++o) cum[o] = 0.0;
+        if (surf) {
+          const double bv = grp_sum<G>(own ? xt * bb[BO.bcoef(t)] : 0.0);
+          cum[0] = kmult * (bv + bb[BO.bc()]);
         }
       }
-    }
+      const double tdu = trr[(size_t)k * TR_NFIELDS + TR_TDUSER];
 #pragma unroll
-    for (int d = 0; d < PB; ++d) {
-      dq[d] = dn[d]; yq[d] = yn[d];
-#pragma unroll
-      for (int cs = 0; cs < CPL; ++cs) uq[d][cs] = un[d][cs];
+      for (int o = 0; o < S; ++o) cum[o] = (sv[o] + lr[O.csrc(o)]) + tdu * cum[o];
     }
   }
 
   // ---------------- Fourier convergence (VLIDORT_CONVERGE_OBSGEO)
-  if (lane != 0) return;
+  if (t != 0 || !valid) return;
   const double flux_multiplier = (M == 0) ? 1.0 : 2.0;
   double* stk = W.stokes + (size_t)it * 3;
   bool done = false;
-  if (singular) {
+  if (sing) {
     W.status[it] = 4;
     W.radiance[oi] = W.missing; W.reflectance[oi] = W.missing;
     if (S > 1) { W.qout[oi] = W.missing; W.uout[oi] = W.missing; }
@@ -973,6 +925,7 @@ template <int N, int S>
 int fourier_loop(const RTWork& W, int nitems, int* const lists[2], int* counts, hipStream_t st) {
   const int nl = (int)(((size_t)nitems * W.km + 63) / 64);
   const int ns = (int)(((size_t)nitems * W.km + 15) / 16);
+  const int nb = (nitems + 64 / bvp_group(N * S) - 1) / (64 / bvp_group(N * S));
   for (int M = 0; M <= W.nmoments; ++M) {
     const int* in = lists[M & 1];
     int* out = lists[(M + 1) & 1];
@@ -985,7 +938,7 @@ int fourier_loop(const RTWork& W, int nitems, int* const lists[2], int* counts, 
     }
     prof_end(PROF_LAYER, st);
     prof_begin(PROF_BVP, st);
-    hipLaunchKernelGGL((k_bvp<N, S>), dim3(nitems), dim3(64), 0, st, W, M, in, counts + M, out, counts + M + 1);
+    hipLaunchKernelGGL((k_bvp<N, S>), dim3(nb), dim3(64), 0, st, W, M, in, counts + M, out, counts + M + 1);
     prof_end(PROF_BVP, st);
   }
   AA_CHECK(hipGetLastError());

@@ -237,7 +237,7 @@ size_t carve_work(Carver& C, RTWork& W, int cap) {
   const int n = W.N * W.S, LM = 2 * W.N, km = W.km;
   W.lstride = LOff(W.N, W.S).size();
   W.bstride = BOff(n).size();
-  W.ustride = 6 * n;   // band row width 2*(3n-1)+1 plus the transformed RHS
+  W.ustride = 4 * n + 2;   // U row of a layer panel: 4n columns, transformed RHS, 1/pivot
   W.cap = cap;
   W.status = C.take<int>(cap);
   W.L_it = C.take<int>(cap);

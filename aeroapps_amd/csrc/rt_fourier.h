@@ -86,8 +86,8 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
   return __hiloint2double(hi, lo);
 }
 
-// Lane exchange of a double within 16-lane rows (DPP) and across the two rows
-// of a 32-lane half (swizzle); used by the max-reductions of the band LU.
+// Lane exchange of a double within 16-lane rows (DPP); used by the lane
+// reductions of the boundary-value solve.
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double x) {
   int lo = __double2loint(x), hi = __double2hiint(x);
@@ -95,13 +95,6 @@ __device__ __forceinline__ double dpp_d(double x) {
   hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
 }
-__device__ __forceinline__ double swz16_d(double x) {
-  int lo = __double2loint(x), hi = __double2hiint(x);
-  lo = __builtin_amdgcn_ds_swizzle(lo, 0x401F);
-  hi = __builtin_amdgcn_ds_swizzle(hi, 0x401F);
-  return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 
 // ------------------------------------------------------------------ k_layer
 template <int N, int S, bool MIX>
@@ -546,15 +539,14 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
 // ------------------------------------------------------------------ k_bvp
 // Group reductions over the G lanes that hold one item's rows (G = 16, 32 or
 // 64; every lane of the group ends with the same value).
-template <int G>
-__device__ __forceinline__ double grp_max(double x) {
-  x = dmax(x, dpp_d<0xB1>(x));    // quad_perm [1,0,3,2]
-  x = dmax(x, dpp_d<0x4E>(x));    // quad_perm [2,3,0,1]
-  x = dmax(x, dpp_d<0x141>(x));   // row_half_mirror
-  x = dmax(x, dpp_d<0x140>(x));   // row_mirror
-  if (G >= 32) x = dmax(x, swz16_d(x));
-  if (G == 64) x = dmax(readlane_d(x, 0), readlane_d(x, 32));
-  return x;
+// Values of the two 16-lane rows of each 32-lane half, in both rows
+// (v_permlane16_swap on x against itself): for every lane, {r.x, r.y} holds
+// the value of its own row and of the partner row at the same position.
+__device__ __forceinline__ void xrow16_d(double x, double& p, double& q) {
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(x), (unsigned)__double2loint(x), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(x), (unsigned)__double2hiint(x), false, false);
+  p = __hiloint2double((int)hi[0], (int)lo[0]);
+  q = __hiloint2double((int)hi[1], (int)lo[1]);
 }
 template <int G>
 __device__ __forceinline__ double grp_sum(double x) {
@@ -562,7 +554,7 @@ __device__ __forceinline__ double grp_sum(double x) {
   x += dpp_d<0x4E>(x);
   x += dpp_d<0x141>(x);
   x += dpp_d<0x140>(x);
-  if (G >= 32) x += swz16_d(x);
+  if (G >= 32) { double p, q; xrow16_d(x, p, q); x = p + q; }
   if (G == 64) x = readlane_d(x, 0) + readlane_d(x, 32);
   return x;
 }
@@ -572,9 +564,67 @@ __device__ __forceinline__ int grp_min_i(int x) {
   x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, false));
   x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x141, 0xF, 0xF, false));
   x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x140, 0xF, 0xF, false));
-  if (G >= 32) x = min(x, __builtin_amdgcn_ds_swizzle(x, 0x401F));
+  if (G >= 32) {
+    const auto r = __builtin_amdgcn_permlane16_swap((unsigned)x, (unsigned)x, false, false);
+    x = min((int)r[0], (int)r[1]);
+  }
   if (G == 64) x = min(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 32));
   return x;
+}
+// unsigned max; the DPP moves use 0 (the identity) as the old value so they
+// fold into v_max_u32's DPP form
+template <int G>
+__device__ __forceinline__ unsigned grp_max_u(unsigned x) {
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false));
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false));
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false));
+  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false));
+  if (G >= 32) {
+    const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    x = max((unsigned)r[0], (unsigned)r[1]);
+  }
+  if (G == 64) x = max((unsigned)__builtin_amdgcn_readlane((int)x, 0), (unsigned)__builtin_amdgcn_readlane((int)x, 32));
+  return x;
+}
+
+// Lowest set lane of each group's bits in the wave mask B (lq[q], -1 if none,
+// wave-uniform) and, per lane, that of its own group g; `multi` reports a
+// group with more than one bit.
+template <int G>
+__device__ __forceinline__ int grp_lowest(unsigned long long B, int g, bool& multi, int (&lq)[64 / G]) {
+  constexpr int IPW = 64 / G;
+  constexpr unsigned long long GM = (G == 64) ? ~0ull : ((1ull << (G & 63)) - 1ull);
+  int sel = -1;
+  multi = false;
+#pragma unroll
+  for (int q = 0; q < IPW; ++q) {
+    const unsigned long long bq = (B >> (q * G)) & GM;
+    lq[q] = bq ? (int)__builtin_ctzll(bq) + q * G : -1;
+    multi = multi || (__builtin_popcountll(bq) > 1);
+    sel = (g == q) ? lq[q] : sel;
+  }
+  return sel;
+}
+// Value of x at lane lq[g] of the lane's own group (wave-uniform lanes).
+template <int G>
+__device__ __forceinline__ double grp_pick_d(double x, int g, const int (&lq)[64 / G]) {
+  double r = 0.0;
+#pragma unroll
+  for (int q = 0; q < 64 / G; ++q) {
+    const double v = readlane_d(x, lq[q] < 0 ? 0 : lq[q]);
+    r = (g == q) ? v : r;
+  }
+  return r;
+}
+template <int G>
+__device__ __forceinline__ int grp_pick_i(int x, int g, const int (&lq)[64 / G]) {
+  int r = 0;
+#pragma unroll
+  for (int q = 0; q < 64 / G; ++q) {
+    const int v = __builtin_amdgcn_readlane(x, lq[q] < 0 ? 0 : lq[q]);
+    r = (g == q) ? v : r;
+  }
+  return r;
 }
 
 // Lanes per item in k_bvp: one lane per row of a layer panel (3n rows).
@@ -592,7 +642,6 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
   constexpr int BIG = 0x3fffffff;
   __shared__ __align__(16) double us[IPW][n2][UW];   // U rows of the current panel (cols, RHS, 1/pivot)
   __shared__ double xs[IPW][n2];       // integration constants of the current/next layer
-  __shared__ int pp[IPW];              // row position of the current pivot
   const int lane = threadIdx.x, g = lane / G, t = lane - g * G;
   const unsigned long long gm = (G == 64) ? ~0ull : (((1ull << (G & 63)) - 1ull) << (g * G));
   const unsigned long long below = (1ull << lane) - 1ull;
@@ -660,38 +709,47 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
     for (int i = 0; i < n2; ++i) {
       const int j = j0 + i;
       const double xi = a[i];
-      const double ax = fabs(xi);
       // pivot: largest |a(r,j)| among the candidate rows, ties to the smallest
-      // row position (IDAMAX order of DGBTF2)
-      const double mx = grp_max<G>(act ? ax : -1.0);
-      const unsigned long long eb = __ballot(act && ax == mx) & gm;
-      const unsigned long long ab = __ballot(act) & gm;
-      const unsigned long long cand = eb ? eb : ab;
-      const bool cnd = (cand >> lane) & 1ull;
-      bool ispiv;
-      if (__ballot(__popcll(cand) > 1) != 0ull) {
-        const int kp = grp_min_i<G>(cnd ? posl : BIG);
-        ispiv = cnd && posl == kp;
-      } else {
-        ispiv = cnd && (cand & below) == 0ull;
+      // row position (IDAMAX order of DGBTF2).  |a| orders as its bit
+      // pattern: the high words decide unless two candidates share one.
+      const unsigned hi = act ? ((unsigned)__double2hiint(xi) & 0x7fffffffu) : 0u;
+      const unsigned mh = grp_max_u<G>(hi);
+      bool eq = act && hi == mh;
+      bool multi;
+      int lq[IPW];
+      int sel = grp_lowest<G>(__ballot(eq), g, multi, lq);
+      if (multi) {   // wave-uniform: some group has equal high words
+        const unsigned lo = eq ? (unsigned)__double2loint(xi) : 0u;
+        const unsigned ml = grp_max_u<G>(lo);
+        eq = eq && lo == ml;
+        sel = grp_lowest<G>(__ballot(eq), g, multi, lq);
+        if (multi) {
+          const int kp = grp_min_i<G>(eq ? posl : BIG);
+          eq = eq && posl == kp;
+          sel = grp_lowest<G>(__ballot(eq), g, multi, lq);
+        }
       }
-      if (ispiv) {
+      const bool ispiv = lane == sel;
+      // pivot value and position by lane reads: the reciprocal is ready while
+      // the pivot row makes its way through LDS
+      const double pv = grp_pick_d<G>(xi, g, lq);
+      const int ppos = grp_pick_i<G>(posl, g, lq);
+      const double rp = 1.0 / pv;
+      if (ispiv) {   // aligned pairs from the even column at or before i
 #pragma unroll
-        for (int c = i; c < NCOL; ++c) us[g][i][c] = a[c];
-        us[g][i][RH] = a[RH];
-        pp[g] = posl;
+        for (int c = i & ~1; c < NCOL; c += 2)
+          *reinterpret_cast<double2*>(&us[g][i][c]) = make_double2(a[c], a[c + 1]);
+        *reinterpret_cast<double2*>(&us[g][i][RH]) = make_double2(a[RH], rp);
       }
       __syncthreads();
-      const double pv = us[g][i][i];
-      const int ppos = pp[g];
-      const double rp = 1.0 / pv;
-      const double l = (act && !ispiv) ? xi * rp : 0.0;
+      if (act && !ispiv) {   // only the rows still to be reduced read the pivot row
+        const double l = xi * rp;
 #pragma unroll
-      for (int c = i + 1; c < NCOL; ++c) a[c] = fma(-l, us[g][i][c], a[c]);
-      a[RH] = fma(-l, us[g][i][RH], a[RH]);
+        for (int c = i + 1; c < NCOL; ++c) a[c] = fma(-l, us[g][i][c], a[c]);
+        a[RH] = fma(-l, us[g][i][RH], a[RH]);
+      }
       sing = sing || (pv == 0.0);
       if (ispiv) {
-        us[g][i][RH + 1] = rp;
         act = false;
       } else if (act && posl == j) {
         posl = ppos;   // row interchange: the row at position j moves to the pivot's position
@@ -761,45 +819,75 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
   // ---------------- back-substitution (DTBSV column order) fused with the
   // TOA source recursion (REG/vlidort_intensity.f90:145-643): lane t < 2n
   // holds U row t of the panel and ends with x_t, the t-th integration
-  // constant of the layer (LCON for t < n, MCON after).
+  // constant of the layer (LCON for t < n, MCON after).  The U rows and source
+  // coefficients of the next panel up are loaded one panel ahead; x_i is
+  // broadcast within the group by lane reads.
   const double kmult = 2.0 * albedo;
   double cum[S];
 #pragma unroll
   for (int o = 0; o < S; ++o) cum[o] = 0.0;
   const double* trr = W.tr + (size_t)it * km * TR_NFIELDS;
+  double u[UW], cf[S], cs[S], tdu = 0.0;
+#define AA_BS_LOAD(K)                                                                        \
+  {                                                                                          \
+    const int k_ = (K);                                                                      \
+    if (valid && k_ >= 0 && k_ < L && t < n2) {                                              \
+      const double2* src_ = reinterpret_cast<const double2*>(ub + ((size_t)k_ * n2 + t) * UW); \
+      _Pragma("unroll") for (int e = 0; e < UW / 2; ++e) {                                   \
+        const double2 v_ = src_[e];                                                          \
+        u[2 * e] = v_.x;                                                                     \
+        u[2 * e + 1] = v_.y;                                                                 \
+      }                                                                                      \
+      const double* lr_ = lb + (size_t)k_ * LS;                                              \
+      _Pragma("unroll") for (int o = 0; o < S; ++o) {                                        \
+        cf[o] = lr_[O.coef(o, t)];                                                           \
+        cs[o] = lr_[O.csrc(o)];                                                              \
+      }                                                                                      \
+      tdu = trr[(size_t)k_ * TR_NFIELDS + TR_TDUSER];                                        \
+    } else {                                                                                 \
+      _Pragma("unroll") for (int e = 0; e < UW; ++e) u[e] = 0.0;                             \
+      _Pragma("unroll") for (int o = 0; o < S; ++o) { cf[o] = 0.0; cs[o] = 0.0; }            \
+      tdu = 0.0;                                                                             \
+    }                                                                                        \
+  }
+  AA_BS_LOAD(Lmax - 1);
   for (int k = Lmax - 1; k >= 0; --k) {
+    double uc[UW], cfc[S], csc[S];
+#pragma unroll
+    for (int e = 0; e < UW; ++e) uc[e] = u[e];
+#pragma unroll
+    for (int o = 0; o < S; ++o) { cfc[o] = cf[o]; csc[o] = cs[o]; }
+    const double tduc = tdu;
+    AA_BS_LOAD(k - 1);
     const bool here = valid && k < L;
     if (__ballot(here) == 0ull) continue;
     const bool lastp = k == L - 1;
     const bool own = here && t < n2;
-    double u[UW];
-    if (own) {
-      const double2* src = reinterpret_cast<const double2*>(ub + ((size_t)k * n2 + t) * UW);
-#pragma unroll
-      for (int e = 0; e < UW / 2; ++e) { const double2 v = src[e]; u[2 * e] = v.x; u[2 * e + 1] = v.y; }
-    } else {
-#pragma unroll
-      for (int e = 0; e < UW; ++e) u[e] = 0.0;
-    }
-    double tv = u[RH];
+    double tv = uc[RH];
     if (here && !lastp) {
 #pragma unroll
-      for (int c = n2 - 1; c >= 0; --c) tv = fma(-u[n2 + c], xs[g][c], tv);
+      for (int c = n2 - 1; c >= 0; --c) tv = fma(-uc[n2 + c], xs[g][c], tv);
     }
-    __syncthreads();
     double xt = 0.0;
 #pragma unroll
     for (int i = n2 - 1; i >= 0; --i) {
-      if (own && t == i) { xt = tv * u[RH + 1]; xs[g][i] = xt; }
-      __syncthreads();
-      const double xv = xs[g][i];
-      if (t < i) tv = fma(-u[i], xv, tv);
+      const double cand = tv * uc[RH + 1];
+      double xv = 0.0;
+#pragma unroll
+      for (int q = 0; q < IPW; ++q) {
+        const double v = readlane_d(cand, q * G + i);
+        xv = (g == q) ? v : xv;
+      }
+      if (t == i) xt = xv;
+      if (t < i) tv = fma(-uc[i], xv, tv);
     }
+    __syncthreads();
+    if (own) xs[g][t] = xt;
+    __syncthreads();
     if (here) {
-      const double* lr = lb + (size_t)k * LS;
       double sv[S];
 #pragma unroll
-      for (int o = 0; o < S; ++o) sv[o] = grp_sum<G>(own ? xt * lr[O.coef(o, t)] : 0.0);
+      for (int o = 0; o < S; ++o) sv[o] = grp_sum<G>(own ? xt * cfc[o] : 0.0);
       if (lastp) {
 #pragma unroll
         for (int o = 0; o < S; ++o) cum[o] = 0.0;
@@ -808,11 +896,11 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
           cum[0] = kmult * (bv + bb[BO.bc()]);
         }
       }
-      const double tdu = trr[(size_t)k * TR_NFIELDS + TR_TDUSER];
 #pragma unroll
-      for (int o = 0; o < S; ++o) cum[o] = (sv[o] + lr[O.csrc(o)]) + tdu * cum[o];
+      for (int o = 0; o < S; ++o) cum[o] = (sv[o] + csc[o]) + tduc * cum[o];
     }
   }
+#undef AA_BS_LOAD
 
   // ---------------- Fourier convergence (VLIDORT_CONVERGE_OBSGEO)
   if (t != 0 || !valid) return;

@@ -36,14 +36,20 @@ PROF_NAMES = ["mie", "check", "optics", "geom", "item", "layer", "bvp"]
 # ----------------------------------------------------------------- algorithmic work
 def flops_per_term(N, S_, L, nmoments, M):
     """Algorithmic FP64 flops of one (column, channel) at Fourier term M:
-    eigensystem (F_b), band LU + solve (F_c), source (F_d)."""
+    eigensystem (F_b), boundary-value solve (F_c) and source (F_d).  F_c counts
+    the structurally non-zero work of the pivoted factorisation (one panel of
+    3n rows x (4n columns + RHS) per layer, 2n x (2n + RHS) for the bottom
+    layer) and of the back-substitution; an FMA counts as 2 flops."""
     n = S_ * N
-    NT = 2 * n * L
-    kl = ku = 3 * n - 1
+    n2 = 2 * n
     Lm = nmoments - M + 1
     Fb = L * (4 * S_ * n * n * Lm + 4 * N * S_ ** 3 * Lm + 2 * n ** 3 + 25 * n ** 3)
-    Fc = 2 * NT * kl * (kl + ku) + 2 * NT * (2 * kl + ku)
-    Fd = L * (8 * n * S_ + 10 * n * n)
+    mid = sum((3 * n - 1 - i) * (1 + 2 * (4 * n - i)) for i in range(n2))
+    last = sum((2 * n - 1 - i) * (1 + 2 * (2 * n - i)) for i in range(n2))
+    bs_mid = 2 * n2 * n2 + n2 * (n2 - 1) + n2
+    bs_last = n2 * (n2 - 1) + n2
+    Fc = (L - 1) * (mid + bs_mid) + last + bs_last
+    Fd = L * S_ * (2 * n2 + 3)
     return Fb, Fc, Fd
 
 

@@ -587,56 +587,49 @@ __device__ __forceinline__ unsigned grp_max_u(unsigned x) {
   return x;
 }
 
-// Lowest set lane of each group's bits in the wave mask B (lq[q], -1 if none,
-// wave-uniform) and, per lane, that of its own group g; `multi` reports a
-// group with more than one bit.
-template <int G>
-__device__ __forceinline__ int grp_lowest(unsigned long long B, int g, bool& multi, int (&lq)[64 / G]) {
+// Lanes per item in k_bvp (G) and panel rows per lane (SL): the back-
+// substitution puts one unknown of a layer on each lane (2n <= G) and the 3n
+// rows of a panel fill SL row slots per lane.
+__host__ __device__ constexpr int bvp_group(int n) { return 2 * n <= 16 ? 16 : (2 * n <= 32 ? 32 : 64); }
+__host__ __device__ constexpr int bvp_slots(int n) { return (3 * n + bvp_group(n) - 1) / bvp_group(n); }
+
+// Pivot of each group from per-slot candidate flags: lq[q]/sq[q] are the lane
+// and slot of group q's candidate (lowest lane, slot 0 first; -1 if none);
+// `multi` reports a group with more than one candidate.  Returns the lane of
+// the lane's own group and sets `slot`.
+template <int G, int SL>
+__device__ __forceinline__ int grp_pivot(const bool (&eq)[SL], int g, bool& multi, int& slot) {
   constexpr int IPW = 64 / G;
   constexpr unsigned long long GM = (G == 64) ? ~0ull : ((1ull << (G & 63)) - 1ull);
+  unsigned long long B[SL];
+#pragma unroll
+  for (int s = 0; s < SL; ++s) B[s] = __ballot(eq[s]);
   int sel = -1;
+  slot = 0;
   multi = false;
 #pragma unroll
   for (int q = 0; q < IPW; ++q) {
-    const unsigned long long bq = (B >> (q * G)) & GM;
-    lq[q] = bq ? (int)__builtin_ctzll(bq) + q * G : -1;
-    multi = multi || (__builtin_popcountll(bq) > 1);
-    sel = (g == q) ? lq[q] : sel;
+    int lq = -1, sq = 0, cnt = 0;
+#pragma unroll
+    for (int s = SL - 1; s >= 0; --s) {
+      const unsigned long long bq = (B[s] >> (q * G)) & GM;
+      cnt += __builtin_popcountll(bq);
+      if (bq) { lq = (int)__builtin_ctzll(bq) + q * G; sq = s; }
+    }
+    multi = multi || cnt > 1;
+    sel = (g == q) ? lq : sel;
+    slot = (g == q) ? sq : slot;
   }
   return sel;
 }
-// Value of x at lane lq[g] of the lane's own group (wave-uniform lanes).
-template <int G>
-__device__ __forceinline__ double grp_pick_d(double x, int g, const int (&lq)[64 / G]) {
-  double r = 0.0;
-#pragma unroll
-  for (int q = 0; q < 64 / G; ++q) {
-    const double v = readlane_d(x, lq[q] < 0 ? 0 : lq[q]);
-    r = (g == q) ? v : r;
-  }
-  return r;
-}
-template <int G>
-__device__ __forceinline__ int grp_pick_i(int x, int g, const int (&lq)[64 / G]) {
-  int r = 0;
-#pragma unroll
-  for (int q = 0; q < 64 / G; ++q) {
-    const int v = __builtin_amdgcn_readlane(x, lq[q] < 0 ? 0 : lq[q]);
-    r = (g == q) ? v : r;
-  }
-  return r;
-}
-
-// Lanes per item in k_bvp: one lane per row of a layer panel (3n rows).
-__host__ __device__ constexpr int bvp_group(int n) { return 3 * n <= 16 ? 16 : (3 * n <= 32 ? 32 : 64); }
 
 template <int N, int S>
 __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restrict__ active,
                                             const int* __restrict__ nactive, int* __restrict__ next,
                                             int* __restrict__ nnext) {
   constexpr int n = N * S, n2 = 2 * n, NC = 4 * n, RH = NC, UW = NC + 2;
-  constexpr int G = bvp_group(n), IPW = 64 / G;
-  static_assert(3 * n <= 64, "panel rows exceed the wavefront");
+  constexpr int G = bvp_group(n), IPW = 64 / G, SL = bvp_slots(n);
+  static_assert(2 * n <= 64 && SL <= 2, "panel rows exceed the wavefront");
   constexpr LOff O(N, S);
   constexpr BOff BO(n);
   constexpr int BIG = 0x3fffffff;
@@ -681,132 +674,189 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
   const double r2beam = surf ? bb[BO.r2beam()] : 0.0;
 
   // ---------------- panel LU with partial pivoting + forward substitution
-  // Lane t of the group holds one row of the panel: a[0..2n) the columns of
-  // layer k, a[2n..4n) those of layer k+1, a[4n] the right-hand side.
-  double a[NC + 1];
+  // Each lane of the group holds SL rows of the panel: a[s][0..2n) the columns
+  // of layer k, a[s][2n..4n) those of layer k+1, a[s][4n] the right-hand side.
+  double a[SL][NC + 1];
+  bool act[SL];
+  int posl[SL];
 #pragma unroll
-  for (int c = 0; c <= NC; ++c) a[c] = 0.0;
-  bool act = false;
-  int posl = BIG;
+  for (int sl = 0; sl < SL; ++sl) {
+#pragma unroll
+    for (int c = 0; c <= NC; ++c) a[sl][c] = 0.0;
+    act[sl] = false;
+    posl[sl] = BIG;
+  }
   bool sing = false;
   if (valid && t < n) {   // TOA rows (BVP_MATRIX_SETUP, upper boundary), layer 0
     const int i = t / S, o1 = t - (t / S) * S;
 #pragma unroll
-    for (int q = 0; q < n; ++q) a[q] = lb[O.xp(i, o1, q)];
+    for (int q = 0; q < n; ++q) a[0][q] = lb[O.xp(i, o1, q)];
 #pragma unroll
-    for (int q = 0; q < n; ++q) a[n + q] = lb[O.xn(i, o1, q)] * lb[O.tdel(q)];
-    a[RH] = -lb[O.wu(i, o1)];
-    act = true;
-    posl = t;
+    for (int q = 0; q < n; ++q) a[0][n + q] = lb[O.xn(i, o1, q)] * lb[O.tdel(q)];
+    a[0][RH] = -lb[O.wu(i, o1)];
+    act[0] = true;
+    posl[0] = t;
   }
 
   // one panel: eliminate its 2n columns; LAST: the bottom layer (no layer k+1)
-  auto lu_panel = [&](auto lastc, int k) {
+  auto lu_panel = [&](auto lastc, int k) __attribute__((always_inline)) {
     constexpr bool LAST = decltype(lastc)::value;
     constexpr int NCOL = LAST ? n2 : NC;
     const int j0 = n2 * k;
 #pragma unroll
     for (int i = 0; i < n2; ++i) {
       const int j = j0 + i;
-      const double xi = a[i];
       // pivot: largest |a(r,j)| among the candidate rows, ties to the smallest
       // row position (IDAMAX order of DGBTF2).  |a| orders as its bit
       // pattern: the high words decide unless two candidates share one.
-      const unsigned hi = act ? ((unsigned)__double2hiint(xi) & 0x7fffffffu) : 0u;
-      const unsigned mh = grp_max_u<G>(hi);
-      bool eq = act && hi == mh;
+      unsigned hs[SL], key = 0u;
+      bool eq[SL];
+#pragma unroll
+      for (int sl = 0; sl < SL; ++sl) {
+        hs[sl] = act[sl] ? ((unsigned)__double2hiint(a[sl][i]) & 0x7fffffffu) : 0u;
+        key = max(key, hs[sl]);
+      }
+      const unsigned mh = grp_max_u<G>(key);
+#pragma unroll
+      for (int sl = 0; sl < SL; ++sl) eq[sl] = act[sl] && hs[sl] == mh;
       bool multi;
-      int lq[IPW];
-      int sel = grp_lowest<G>(__ballot(eq), g, multi, lq);
+      int psl;
+      int sel = grp_pivot<G, SL>(eq, g, multi, psl);
       if (multi) {   // wave-uniform: some group has equal high words
-        const unsigned lo = eq ? (unsigned)__double2loint(xi) : 0u;
-        const unsigned ml = grp_max_u<G>(lo);
-        eq = eq && lo == ml;
-        sel = grp_lowest<G>(__ballot(eq), g, multi, lq);
+        unsigned ls[SL], k2 = 0u;
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl) {
+          ls[sl] = eq[sl] ? (unsigned)__double2loint(a[sl][i]) : 0u;
+          k2 = max(k2, ls[sl]);
+        }
+        const unsigned ml = grp_max_u<G>(k2);
+#pragma unroll
+        for (int sl = 0; sl < SL; ++sl) eq[sl] = eq[sl] && ls[sl] == ml;
+        sel = grp_pivot<G, SL>(eq, g, multi, psl);
         if (multi) {
-          const int kp = grp_min_i<G>(eq ? posl : BIG);
-          eq = eq && posl == kp;
-          sel = grp_lowest<G>(__ballot(eq), g, multi, lq);
+          int k3 = BIG;
+#pragma unroll
+          for (int sl = 0; sl < SL; ++sl) k3 = min(k3, eq[sl] ? posl[sl] : BIG);
+          const int kp = grp_min_i<G>(k3);
+#pragma unroll
+          for (int sl = 0; sl < SL; ++sl) eq[sl] = eq[sl] && posl[sl] == kp;
+          sel = grp_pivot<G, SL>(eq, g, multi, psl);
         }
       }
-      const bool ispiv = lane == sel;
-      // pivot value and position by lane reads: the reciprocal is ready while
-      // the pivot row makes its way through LDS
-      const double pv = grp_pick_d<G>(xi, g, lq);
-      const int ppos = grp_pick_i<G>(posl, g, lq);
-      const double rp = 1.0 / pv;
-      if (ispiv) {   // aligned pairs from the even column at or before i
+      bool ispiv[SL];
 #pragma unroll
-        for (int c = i & ~1; c < NCOL; c += 2)
-          *reinterpret_cast<double2*>(&us[g][i][c]) = make_double2(a[c], a[c + 1]);
-        *reinterpret_cast<double2*>(&us[g][i][RH]) = make_double2(a[RH], rp);
+      for (int sl = 0; sl < SL; ++sl) ispiv[sl] = lane == sel && psl == sl;
+      // pivot value and position from the pivot lane (LDS crossbar), so the
+      // reciprocal is ready while the pivot row makes its way through LDS
+      const int src = 4 * (sel < 0 ? lane : sel);
+      double myv = a[0][i];
+      int myp = posl[0];
+      if (SL > 1) {
+        const double v1 = a[SL - 1][i];
+        const int p1 = posl[SL - 1];
+        myv = (psl == 1) ? v1 : myv;
+        myp = (psl == 1) ? p1 : myp;
+      }
+      const double pv = __hiloint2double(__builtin_amdgcn_ds_bpermute(src, __double2hiint(myv)),
+                                         __builtin_amdgcn_ds_bpermute(src, __double2loint(myv)));
+      const int ppos = __builtin_amdgcn_ds_bpermute(src, myp);
+      const double rp = 1.0 / pv;
+#pragma unroll
+      for (int sl = 0; sl < SL; ++sl) {
+        if (ispiv[sl]) {   // aligned pairs from the even column at or before i
+#pragma unroll
+          for (int c = 0; c < NCOL; c += 2)
+            if (c >= (i & ~1)) *reinterpret_cast<double2*>(&us[g][i][c]) = make_double2(a[sl][c], a[sl][c + 1]);
+          *reinterpret_cast<double2*>(&us[g][i][RH]) = make_double2(a[sl][RH], rp);
+        }
       }
       __syncthreads();
-      if (act && !ispiv) {   // only the rows still to be reduced read the pivot row
-        const double l = xi * rp;
+      double u[NC + 1];
 #pragma unroll
-        for (int c = i + 1; c < NCOL; ++c) a[c] = fma(-l, us[g][i][c], a[c]);
-        a[RH] = fma(-l, us[g][i][RH], a[RH]);
+      for (int c = 0; c < NCOL; ++c)
+        if (c > i) u[c] = us[g][i][c];
+      u[RH] = us[g][i][RH];
+#pragma unroll
+      for (int sl = 0; sl < SL; ++sl) {
+        const double l = (act[sl] && !ispiv[sl]) ? a[sl][i] * rp : 0.0;
+#pragma unroll
+        for (int c = 0; c < NCOL; ++c)
+          if (c > i) a[sl][c] = fma(-l, u[c], a[sl][c]);
+        a[sl][RH] = fma(-l, u[RH], a[sl][RH]);
+        if (ispiv[sl]) {
+          act[sl] = false;
+        } else if (act[sl] && posl[sl] == j) {
+          posl[sl] = ppos;   // row interchange: the row at position j moves to the pivot's position
+        }
       }
       sing = sing || (pv == 0.0);
-      if (ispiv) {
-        act = false;
-      } else if (act && posl == j) {
-        posl = ppos;   // row interchange: the row at position j moves to the pivot's position
-      }
     }
     __syncthreads();
     double2* dst = reinterpret_cast<double2*>(ub + (size_t)j0 * UW);
-    const double2* src = reinterpret_cast<const double2*>(&us[g][0][0]);
-    for (int e = t; e < n2 * UW / 2; e += G) dst[e] = src[e];
+    const double2* srcu = reinterpret_cast<const double2*>(&us[g][0][0]);
+    for (int e = t; e < n2 * UW / 2; e += G) dst[e] = srcu[e];
     if (!LAST) {
 #pragma unroll
-      for (int c = 0; c < n2; ++c) { a[c] = a[c + n2]; a[c + n2] = 0.0; }
+      for (int sl = 0; sl < SL; ++sl)
+#pragma unroll
+        for (int c = 0; c < n2; ++c) { a[sl][c] = a[sl][c + n2]; a[sl][c + n2] = 0.0; }
+    }
+  };
+
+  // row `rank` of boundary k (between layers k and k+1) or, for the bottom
+  // layer, of the BOA rows with the Lambertian reflection (BVP_SURFACE_SETUP)
+  auto load_row = [&](double (&r)[NC + 1], int k, bool lastp, int rank) __attribute__((always_inline)) {
+    const double* lu = lb + (size_t)k * LS;
+    const int i = rank / S, o1 = rank - (rank / S) * S;
+    if (!lastp) {
+      const double* ll = lu + LS;
+#pragma unroll
+      for (int q = 0; q < n; ++q) r[q] = lu[O.tdel(q)] * lu[O.xp(i, o1, q)];
+#pragma unroll
+      for (int q = 0; q < n; ++q) r[n + q] = lu[O.xn(i, o1, q)];
+#pragma unroll
+      for (int q = 0; q < n; ++q) r[n2 + q] = -ll[O.xp(i, o1, q)];
+#pragma unroll
+      for (int q = 0; q < n; ++q) r[3 * n + q] = -ll[O.tdel(q)] * ll[O.xn(i, o1, q)];
+      r[RH] = ll[O.wu(i, o1)] - lu[O.wl(i, o1)];
+    } else {
+      const bool sr = surf && o1 == 0;
+#pragma unroll
+      for (int q = 0; q < n; ++q) {
+        double xpn = lu[O.xp(i + N, o1, q)];
+        if (sr) xpn = xpn - bb[BO.r2p(q)];
+        r[q] = lu[O.tdel(q)] * xpn;
+      }
+#pragma unroll
+      for (int q = 0; q < n; ++q) {
+        double xmn = lu[O.xn(i + N, o1, q)];
+        if (sr) xmn = xmn - bb[BO.r2m(q)];
+        r[n + q] = xmn;
+      }
+      double v = -lu[O.wl(i + N, o1)];
+      if (sr) v = v + r2beam;
+      if (sr) v = v + rf_direct;
+      r[RH] = v;
     }
   };
 
   for (int k = 0; k < Lmax; ++k) {
     const bool here = valid && k < L;
     const bool lastp = here && k == L - 1;
-    // the rows of boundary k (or the BOA rows) go to the lanes retired so far
-    const unsigned long long rb = __ballot(here && !act) & gm;
-    const int rank = __popcll(rb & below);
-    if (here && !act && rank < (lastp ? n : n2)) {
-      const double* lu = lb + (size_t)k * LS;
-      if (!lastp) {   // boundary between layers k and k+1
-        const double* ll = lu + LS;
-        const int i = rank / S, o1 = rank - (rank / S) * S;
+    // the rows of boundary k (or the BOA rows) go to the row slots retired so
+    // far, slot 0 of every lane before slot 1
+    const int nnew = lastp ? n : n2;
+    int base = 0;
 #pragma unroll
-        for (int q = 0; q < n; ++q) a[q] = lu[O.tdel(q)] * lu[O.xp(i, o1, q)];
-#pragma unroll
-        for (int q = 0; q < n; ++q) a[n + q] = lu[O.xn(i, o1, q)];
-#pragma unroll
-        for (int q = 0; q < n; ++q) a[n2 + q] = -ll[O.xp(i, o1, q)];
-#pragma unroll
-        for (int q = 0; q < n; ++q) a[3 * n + q] = -ll[O.tdel(q)] * ll[O.xn(i, o1, q)];
-        a[RH] = ll[O.wu(i, o1)] - lu[O.wl(i, o1)];
-      } else {        // BOA rows with the Lambertian reflection (BVP_SURFACE_SETUP)
-        const int i = rank / S, o1 = rank - (rank / S) * S;
-        const bool sr = surf && o1 == 0;
-#pragma unroll
-        for (int q = 0; q < n; ++q) {
-          double xpn = lu[O.xp(i + N, o1, q)];
-          if (sr) xpn = xpn - bb[BO.r2p(q)];
-          a[q] = lu[O.tdel(q)] * xpn;
-        }
-#pragma unroll
-        for (int q = 0; q < n; ++q) {
-          double xmn = lu[O.xn(i + N, o1, q)];
-          if (sr) xmn = xmn - bb[BO.r2m(q)];
-          a[n + q] = xmn;
-        }
-        double v = -lu[O.wl(i + N, o1)];
-        if (sr) v = v + r2beam;
-        if (sr) v = v + rf_direct;
-        a[RH] = v;
+    for (int sl = 0; sl < SL; ++sl) {
+      const unsigned long long fb = __ballot(here && !act[sl]) & gm;
+      const int rank = base + __popcll(fb & below);
+      if (here && !act[sl] && rank < nnew) {
+        load_row(a[sl], k, lastp, rank);
+        act[sl] = true;
+        posl[sl] = n + n2 * k + rank;
       }
-      act = true;
-      posl = n + n2 * k + rank;
+      base += __popcll(fb);
     }
     if (__ballot(here && !lastp) != 0ull) {
       if (here && !lastp) lu_panel(std::integral_constant<bool, false>(), k);

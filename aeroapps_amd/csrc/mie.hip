@@ -51,108 +51,144 @@ __device__ __forceinline__ float lin(float lo, float hi, float arh) {
   return __fadd_rn(__fmul_rn(lo, __fsub_rn(1.0f, arh)), __fmul_rn(hi, arh));
 }
 
-constexpr int MCHUNK = 8;   // moments per accumulation pass
 constexpr int MAXPOL = 6;
 constexpr int MAXTR = 32;
 
 // VLIDORT_getAOPvector for a range of tracers [0, ntr_eff) (ntr_eff < nq only
 // when the reference would have returned early with rc 666/777, in which case
 // the partial, un-normalised sums are what the reference leaves behind).
-__global__ void __launch_bounds__(256) get_aop_kernel(MieDev T, int km, int nobs, int nch, int nq,
-                                                      const int* __restrict__ tracer_idx,
-                                                      const int* __restrict__ chan_idx,
-                                                      const float* __restrict__ qm, const float* __restrict__ rh,
-                                                      int nmom, int npol, int ntr_eff, int normalize,
-                                                      float* __restrict__ tau_o, float* __restrict__ ssa_o,
-                                                      float* __restrict__ g_o, float* __restrict__ pmom_o) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long)km * nobs) return;
-  const int k = (int)(e % km);
-  const int i = (int)(e / km);
-  const float rhv = rh[(long)i * km + k];
-  for (int n = 0; n < nch; ++n) {
-    const int ich = chan_idx[n];
-    float tau = 0.0f, ssa = 0.0f, g = 0.0f;
-    // pass 1: tau, ssa*tau, g*ssa*tau in tracer order
-    for (int iq = 0; iq < ntr_eff; ++iq) {
-      const int tr = tracer_idx[iq];
-      if (tr < 0) continue;
-      SnapF s = rh_snap(T, tr, rhv);
-      const long tb = ((long)tr * T.nch + ich) * T.nrh;
-      float bext_in = lin(T.bext[tb + s.irh], T.bext[tb + s.irhp1], s.arh);
-      float bsca_in = lin(T.bsca[tb + s.irh], T.bsca[tb + s.irhp1], s.arh);
-      float g_in = lin(T.g[tb + s.irh], T.g[tb + s.irhp1], s.arh);
-      float q = qm[((long)i * nq + iq) * km + k];
-      float tau_ = __fmul_rn(bext_in, q);
-      float ssa_ = __fdiv_rn(bsca_in, bext_in);
-      tau = __fadd_rn(tau, tau_);
-      ssa = __fadd_rn(ssa, __fmul_rn(ssa_, tau_));
-      g = __fadd_rn(g, __fmul_rn(__fmul_rn(g_in, ssa_), tau_));
-    }
-    // normalisation (GEO/VLIDORT_MieMod.F90:320-334); note g and pmom use the
-    // already-normalised ssa
-    float st = 0.0f;
-    if (normalize) {
-      if (tau > 0.0f) ssa = __fdiv_rn(ssa, tau);
-      st = __fmul_rn(ssa, tau);
-      if (st > 0.0f) g = __fdiv_rn(g, st);
-    }
-    const long o = ((long)i * nch + n) * km + k;
-    tau_o[o] = tau; ssa_o[o] = ssa; g_o[o] = g;
-    if (pmom_o == nullptr) continue;
-    // pass 2: moments in chunks, all polarizations of a chunk together so the
-    // spherical P22 = P11 / P44 = P33 extension sees the running sums
-    const long ostride = (long)nobs * nch * km;   // stride of one moment
-    for (int m0 = 0; m0 < nmom; m0 += MCHUNK) {
-      float acc[MAXPOL][MCHUNK];
-#pragma unroll
-      for (int p = 0; p < MAXPOL; ++p)
-#pragma unroll
-        for (int j = 0; j < MCHUNK; ++j) acc[p][j] = 0.0f;
-      for (int iq = 0; iq < ntr_eff; ++iq) {
+//
+// One wavefront per (column, channel), KC layers at a time:
+//  1. lane = (layer, tracer): RH snap and the tracer's tau_i, ssa_i, g_i
+//     (independent per tracer, so all of them at once), into LDS;
+//  2. lane = layer: the tracer-order sums tau, ssa*tau, g*ssa*tau and the
+//     normalisation, exactly in the reference's order;
+//  3. lane = moment: the phase-moment sums.  The two table rows of a (tracer,
+//     channel, node) are contiguous in the moment index, so every table read
+//     is a 256-byte coalesced load from L2; the KC layers give KC independent
+//     accumulations per tracer.  Per output element the order is the
+//     reference's (tracer order; (pm*ssa_i)*tau_i).
+constexpr int MIE_KC_MAX = 8;
+__global__ void __launch_bounds__(64) get_aop_kernel(MieDev T, int km, int nobs, int nch, int nq,
+                                                     const int* __restrict__ tracer_idx,
+                                                     const int* __restrict__ chan_idx,
+                                                     const float* __restrict__ qm, const float* __restrict__ rh,
+                                                     int nmom, int npol, int ntr_eff, int normalize,
+                                                     float* __restrict__ tau_o, float* __restrict__ ssa_o,
+                                                     float* __restrict__ g_o, float* __restrict__ pmom_o) {
+  __shared__ float4 info[64];          // (node, arh, ssa_i, tau_i) of lane (layer, tracer)
+  __shared__ float gin[64];            // g_i of lane (layer, tracer)
+  __shared__ float stl[MIE_KC_MAX];    // ssa*tau of the chunk's layers (normalisation)
+  const int lane = threadIdx.x;
+  const int i = blockIdx.x / nch, n = blockIdx.x - (blockIdx.x / nch) * nch;
+  const int ich = chan_idx[n];
+  const int ntr = ntr_eff > 0 ? ntr_eff : 1;
+  const int KC = min(MIE_KC_MAX, 64 / ntr);
+  const long ostride = (long)nobs * nch * km;   // stride of one moment
+  const long obase = ((long)i * nch + n) * km;
+  const long rowstride = (long)T.npol * T.nmom;
+  const int npass = (npol == 6) ? 4 : npol;
+  for (int k0 = 0; k0 < km; k0 += KC) {
+    const int kc = min(KC, km - k0);
+    // ---- 1. lane = (layer, tracer)
+    {
+      const int kk = lane / ntr, iq = lane - (lane / ntr) * ntr;
+      if (kk < kc && iq < ntr_eff) {
+        const int k = k0 + kk;
         const int tr = tracer_idx[iq];
-        if (tr < 0) continue;
-        const int tpol = T.npol_t[tr];
-        const bool sph = (npol == 6 && tpol == 4);
-        const int npol_ = sph ? 4 : npol;
-        SnapF s = rh_snap(T, tr, rhv);
-        const long pb = ((long)tr * T.nch + ich) * T.nrh;
-        // tau_ and ssa_ of this tracer, recomputed exactly as in pass 1
-        const float bext_in = lin(T.bext[pb + s.irh], T.bext[pb + s.irhp1], s.arh);
-        const float bsca_in = lin(T.bsca[pb + s.irh], T.bsca[pb + s.irhp1], s.arh);
-        const float wt = __fmul_rn(bext_in, qm[((long)i * nq + iq) * km + k]);
-        const float ws = __fdiv_rn(bsca_in, bext_in);
-        const float* plo = T.pmom + (pb + s.irh) * (long)T.npol * T.nmom;
-        const float* phi = T.pmom + (pb + s.irhp1) * (long)T.npol * T.nmom;
+        if (tr >= 0) {
+          SnapF s = rh_snap(T, tr, rh[(long)i * km + k]);
+          const long tb = ((long)tr * T.nch + ich) * T.nrh;
+          const float bext_in = lin(T.bext[tb + s.irh], T.bext[tb + s.irhp1], s.arh);
+          const float bsca_in = lin(T.bsca[tb + s.irh], T.bsca[tb + s.irhp1], s.arh);
+          const float g_in = lin(T.g[tb + s.irh], T.g[tb + s.irhp1], s.arh);
+          const float q = qm[((long)i * nq + iq) * km + k];
+          info[lane] = make_float4(__int_as_float(s.irh), s.arh, __fdiv_rn(bsca_in, bext_in), __fmul_rn(bext_in, q));
+          gin[lane] = g_in;
+        }
+      }
+    }
+    __syncthreads();
+    // ---- 2. lane = layer: sums in tracer order, normalisation
+    //      (GEO/VLIDORT_MieMod.F90:320-334; g and pmom use the normalised ssa)
+    if (lane < kc) {
+      float tau = 0.0f, ssa = 0.0f, g = 0.0f;
+      for (int iq = 0; iq < ntr_eff; ++iq) {
+        if (tracer_idx[iq] < 0) continue;
+        const float4 inf = info[lane * ntr + iq];
+        const float ssa_ = inf.z, tau_ = inf.w;
+        tau = __fadd_rn(tau, tau_);
+        ssa = __fadd_rn(ssa, __fmul_rn(ssa_, tau_));
+        g = __fadd_rn(g, __fmul_rn(__fmul_rn(gin[lane * ntr + iq], ssa_), tau_));
+      }
+      float st = 0.0f;
+      if (normalize) {
+        if (tau > 0.0f) ssa = __fdiv_rn(ssa, tau);
+        st = __fmul_rn(ssa, tau);
+        if (st > 0.0f) g = __fdiv_rn(g, st);
+      }
+      const long o = obase + k0 + lane;
+      tau_o[o] = tau; ssa_o[o] = ssa; g_o[o] = g;
+      stl[lane] = st;
+    }
+    __syncthreads();
+    // ---- 3. lane = moment.  Polarisations run in pairs (a, b) where the
+    //      spherical extension ties b to a's running sum (P22 = P11, P44 = P33).
+    if (pmom_o != nullptr) {
+      for (int m0 = 0; m0 < nmom; m0 += 64) {
+        const int m = m0 + lane;
+        const bool mon = m < nmom;
+        const int mr = mon ? m : 0;
+        for (int ps = 0; ps < npass; ++ps) {
+          int pa = ps, pb = -1;
+          if (npol == 6) {
+            pa = (ps == 0) ? 0 : (ps == 1) ? 2 : (ps == 2) ? 1 : 3;
+            pb = (ps == 0) ? 4 : (ps == 1) ? 5 : -1;
+          }
+          float acc_a[MIE_KC_MAX], acc_b[MIE_KC_MAX];
 #pragma unroll
-        for (int p = 0; p < MAXPOL; ++p) {
-          if (p >= npol_) continue;
+          for (int kk = 0; kk < MIE_KC_MAX; ++kk) { acc_a[kk] = 0.0f; acc_b[kk] = 0.0f; }
+          for (int iq = 0; iq < ntr_eff; ++iq) {
+            const int tr = tracer_idx[iq];
+            if (tr < 0) continue;
+            const bool sph = (npol == 6 && T.npol_t[tr] == 4);
+            const int nrh = T.nrh_t[tr];
+            const long pb0 = ((long)tr * T.nch + ich) * T.nrh;
 #pragma unroll
-          for (int j = 0; j < MCHUNK; ++j) {
-            int m = m0 + j;
-            if (m >= nmom) continue;
-            float pm = lin(plo[p * T.nmom + m], phi[p * T.nmom + m], s.arh);
-            acc[p][j] = __fadd_rn(acc[p][j], __fmul_rn(__fmul_rn(pm, ws), wt));
+            for (int kk = 0; kk < MIE_KC_MAX; ++kk) {
+              if (kk >= kc) break;
+              const float4 inf = info[kk * ntr + iq];
+              const int irh = __float_as_int(inf.x);
+              const int irhp1 = (irh + 1 > nrh - 1) ? nrh - 1 : irh + 1;
+              const float* plo = T.pmom + (pb0 + irh) * rowstride + mr;
+              const float* phi = T.pmom + (pb0 + irhp1) * rowstride + mr;
+              const float pma = lin(plo[pa * T.nmom], phi[pa * T.nmom], inf.y);
+              acc_a[kk] = __fadd_rn(acc_a[kk], __fmul_rn(__fmul_rn(pma, inf.z), inf.w));
+              if (pb >= 0) {
+                if (sph) {
+                  acc_b[kk] = acc_a[kk];
+                } else {
+                  const float pmb = lin(plo[pb * T.nmom], phi[pb * T.nmom], inf.y);
+                  acc_b[kk] = __fadd_rn(acc_b[kk], __fmul_rn(__fmul_rn(pmb, inf.z), inf.w));
+                }
+              }
+            }
+          }
+          if (mon) {
+#pragma unroll
+            for (int kk = 0; kk < MIE_KC_MAX; ++kk) {
+              if (kk >= kc) break;
+              const float st = stl[kk];
+              float va = acc_a[kk], vb = acc_b[kk];
+              if (normalize && st > 0.0f) { va = __fdiv_rn(va, st); vb = __fdiv_rn(vb, st); }
+              pmom_o[((long)pa * nmom + m) * ostride + obase + k0 + kk] = va;
+              if (pb >= 0) pmom_o[((long)pb * nmom + m) * ostride + obase + k0 + kk] = vb;
+            }
           }
         }
-        if (sph) {
-#pragma unroll
-          for (int j = 0; j < MCHUNK; ++j) { acc[4][j] = acc[0][j]; acc[5][j] = acc[2][j]; }
-        }
-      }
-#pragma unroll
-      for (int p = 0; p < MAXPOL; ++p) {
-        if (p >= npol) continue;
-#pragma unroll
-        for (int j = 0; j < MCHUNK; ++j) {
-          int m = m0 + j;
-          if (m >= nmom) continue;
-          float v = acc[p][j];
-          if (normalize && st > 0.0f) v = __fdiv_rn(v, st);
-          pmom_o[((long)p * nmom + m) * ostride + o] = v;
-        }
       }
     }
+    __syncthreads();
   }
 }
 
@@ -288,10 +324,10 @@ int launch_get_aop(const aa_mie* t, int km, int nobs, int nch, int nq, const int
                    int normalize, float* d_tau, float* d_ssa, float* d_g, float* d_pmom, hipStream_t st) {
   long ne = (long)km * nobs;
   if (ne == 0) return 0;
-  int threads = 256;
-  long blocks = (ne + threads - 1) / threads;
+  const long blocks = (long)nobs * nch;
+  if (ntr_eff > MAXTR || blocks > 0x7fffffffL) return -1;
   prof_begin(PROF_MIE, st);
-  hipLaunchKernelGGL(get_aop_kernel, dim3((unsigned)blocks), dim3(threads), 0, st, t->dev, km, nobs, nch, nq,
+  hipLaunchKernelGGL(get_aop_kernel, dim3((unsigned)blocks), dim3(64), 0, st, t->dev, km, nobs, nch, nq,
                      d_tracer_idx, d_chan_idx, d_qm, d_rh, nmom, npol, ntr_eff, normalize, d_tau, d_ssa, d_g,
                      d_pmom);
   prof_end(PROF_MIE, st);

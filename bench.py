@@ -31,6 +31,10 @@ METRIC = "columns/sec at 72 layers/12 streams, mixed vs FP64 GPU; p99 granule la
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 PROF_NAMES = ["mie", "check", "optics", "geom", "item", "layer", "bvp"]
+# HBM bytes per item-term (read, written) of the radiance kernels at N=6
+# scalar, from FETCH_SIZE/WRITE_SIZE counter runs scaled by a copy-kernel
+# calibration (profiles/hbm_traffic_40k_columns_4ch_12streams.md)
+TRAFFIC_PER_ITEM_TERM = {("bvp", 6, 1): (394.5e3, 180.2e3), ("layer", 6, 1): (17.3e3, 125.5e3)}
 
 
 # ----------------------------------------------------------------- algorithmic work
@@ -236,6 +240,12 @@ def main():
     dom_ms = kms[dom]
     dom_launches = max(1, kln[dom])
     achieved = dom_flops / (dom_ms / 1000.0) / 1e12
+    item_terms = float(nfh[ok].sum()) * args.steps
+    tr_key = (dom, N, Sx)
+    traffic = None
+    if tr_key in TRAFFIC_PER_ITEM_TERM:
+        rd, wr = TRAFFIC_PER_ITEM_TERM[tr_key]
+        traffic = round((rd + wr) * item_terms / dom_launches, 0)
     el = main_run["el"]
     value = main_run["value"]
     path_bytes = bytes_per_column(km, nq, nch, vector) * ncol * args.steps
@@ -259,7 +269,8 @@ def main():
                                 "eigensystem and band elimination)" if rmx is not None else "fp64",
                    "path": "aa_column_radiance_dev (Chem_Mie optics + VLIDORT radiance, device-resident)"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+                     "traffic_GBs": (round(traffic * dom_launches / (dom_ms / 1000.0) / 1e9, 1) if traffic else None),
                      "kernel": f"k_{dom}<{N},{Sx}>", "kernel_avg_ms": round(dom_ms / dom_launches, 4),
                      "kernel_launches": int(dom_launches),
                      "note": "FP64 VALU work (no MFMA); peak is MI355X FP64 (vector = matrix rate)"},

@@ -39,6 +39,12 @@ SIGNATURES = {
     "aa_column_radiance_dev": (_ci, [_vp, _ci, _ci, _ci, _ci, _vp, _vp, _vp, _vp, _ci, _ci, _ci, _ci, _ci, _vp, _vp,
                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _cd, _vp, _vp, _vp, _vp, _vp,
                                      _vp]),
+    "aa_rot_calc": (_ci, [_ci, _ci, _ci, _dp, _dp, _dp, _dp, _cd, _ci, _dp, _dp, _ip]),
+    "aa_rot_calc_dev": (_ci, [_ci, _ci, _ci, _dp, _vp, _vp, _vp, _vp, _dp, _vp]),
+    "aa_edge_vars": (_ci, [_ci, _ci, _fp, _fp, _cf, _fp, _fp, _fp]),
+    "aa_edge_vars_dev": (_ci, [_ci, _ci, _vp, _vp, _cf, _vp, _vp, _vp, _vp]),
+    "aa_calc_qm": (_ci, [_ci, _ci, _ci, _fp, _fp, _fp]),
+    "aa_calc_qm_dev": (_ci, [_ci, _ci, _ci, _vp, _vp, _vp, _vp]),
     "aa_version": (ctypes.c_char_p, []),
     "aa_profile_start": (_ci, []),
     "aa_profile_read": (_ci, [_dp, ctypes.POINTER(ctypes.c_longlong), _ci]),

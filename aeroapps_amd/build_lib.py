@@ -21,9 +21,10 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-pass-failed",
 ]
-# Chem_Mie arithmetic must not contract (float32 bit-exactness); the radiance
-# kernels keep the compiler default.
-NO_CONTRACT = {"mie.hip"}
+# Chem_Mie arithmetic and the input preparation must not contract (the
+# reference's operation order, bit-exactness); the radiance kernels keep the
+# compiler default.
+NO_CONTRACT = {"mie.hip", "prep.hip"}
 
 FOURIER_PARTS = [(n, 1) for n in range(1, 9)] + [(n, 3) for n in range(1, 5)]
 

@@ -131,3 +131,43 @@ def get_aop_scalar(mie: MieTables, channels, vname: Sequence[str], verbose, qm, 
     check(lib().aa_get_aop_scalar(mie.handle, km, nobs, nch, nq, _p(ch), _vname(vname), int(verbose), _p(qm), _p(rh),
                                   _p(tau), _p(ssa), _p(g), _p(rc, _ip)), "aa_get_aop_scalar")
     return tau, ssa, g, int(rc[0])
+
+
+def getEdgeVars(aer, ptop=1.0):
+    """mieobs.getEdgeVars (src/Components/misc/scat/mieobs.py:60-76 ->
+    getEdgeVars, src/Shared/GMAO_Shared/Chem_Base/MieObs_py.F90:27-83): edge
+    pressure [Pa], height above the surface [m] and temperature [K], each
+    [km+1, nobs], from ``aer.AIRDENS`` and ``aer.DELP`` given as (km, nobs) or
+    (nobs, km) float32 arrays."""
+    ad, dp = np.asarray(aer.AIRDENS, dtype=np.float32), np.asarray(aer.DELP, dtype=np.float32)
+    if needs_transpose(aer):
+        ad, dp = ad.T, dp.T
+    return edge_vars(ad, dp, ptop)
+
+
+def needs_transpose(aer) -> bool:
+    """True when the arrays of ``aer`` are (nobs, km): the surface pressure
+    ``aer.PS`` has as many entries as DELP's first dimension
+    (src/Components/misc/scat/mieobs.py:353-364)."""
+    return np.shape(aer.PS)[0] == np.shape(aer.DELP)[0]
+
+
+def edge_vars(airdens, delp, ptop=1.0):
+    """getEdgeVars on (km, nobs) float32 arrays; returns (pe, ze, te)."""
+    ad, dp = _f32(airdens), _f32(delp)
+    km, nobs = ad.shape
+    pe = np.zeros((km + 1, nobs), dtype=np.float32, order="F")
+    ze = np.zeros_like(pe)
+    te = np.zeros_like(pe)
+    check(lib().aa_edge_vars(km, nobs, _p(ad), _p(dp), float(ptop), _p(pe), _p(ze), _p(te)), "aa_edge_vars")
+    return pe, ze, te
+
+
+def calc_qm(q, delp):
+    """calc_qm (src/Components/rtms/geo_vlidort/geo_vlidort.F90:2124-2144):
+    qm[km, nq, nobs] = q * delp / 9.81 (float32), q[km, nq, nobs], delp[km, nobs]."""
+    q, dp = _f32(q), _f32(delp)
+    km, nq, nobs = q.shape
+    qm = np.zeros((km, nq, nobs), dtype=np.float32, order="F")
+    check(lib().aa_calc_qm(km, nq, nobs, _p(q), _p(dp), _p(qm)), "aa_calc_qm")
+    return qm

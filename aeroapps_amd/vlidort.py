@@ -71,3 +71,24 @@ def vector_lambert(channels, nstreams, plane_parallel, ROT, depol, alpha, tau, s
     if return_nfourier:
         return I, R, Q, U, int(rc[0]), nf
     return I, R, Q, U, int(rc[0])
+
+
+def rot_calc(channels, pe, he, te, MISSING, verbose=0):
+    """ROT_CALC (src/Components/rtms/leo_vlidort/VLIDORT_POLAR_py.F90:9-41 ->
+    VLIDORT_ROT_CALC, src/Components/rtms/geo_vlidort/VLIDORT_ROT.F90:19-85):
+    Rayleigh optical thickness ROT[km, nobs, nch] and depolarisation ratio
+    depol[nch] from edge pressure/height/temperature pe, he, te [km+1, nobs]
+    and channels [nm].  Returns (ROT, depol, rc)."""
+    channels, pe, he, te = map(_f64, (channels, pe, he, te))
+    if pe.ndim == 1:
+        pe, he, te = (_f64(a[:, None]) for a in (pe, he, te))
+    km, nobs = pe.shape[0] - 1, pe.shape[1]
+    nch = channels.size
+    rot = np.zeros((km, nobs, nch), order="F")
+    depol = np.zeros(nch)
+    rc = np.zeros(1, dtype=np.int32)
+    e = lib().aa_rot_calc(km, nch, nobs, _ptr(channels), _ptr(pe), _ptr(he), _ptr(te), float(MISSING),
+                          int(verbose), _ptr(rot), _ptr(depol), _ptr(rc, _ip))
+    if e != -4:
+        check(e, "aa_rot_calc")
+    return rot, depol, int(rc[0])

@@ -129,6 +129,40 @@ int aa_vlidort_vector_lambert(int km, int nch, int nobs, const double* channels,
                               int verbose, int precision, double* radiance, double* reflectance, double* q,
                               double* u, int* nfourier, int* rc);
 
+/* ------------------------------------------------------------------ input preparation
+ * The steps either side of the radiance path in a geo_vlidort run
+ * (GEO/geo_vlidort_AI.F90:399-430): met fields -> edge variables and tracer
+ * column masses -> (optics) -> Rayleigh optical thickness -> radiance. */
+
+/* VLIDORT_ROT_CALC (GEO/VLIDORT_ROT.F90:19-85; f2py VLIDORT_POLAR_.rot_calc,
+ * src/Components/rtms/leo_vlidort/VLIDORT_POLAR_py.F90:9-41): Rayleigh optical
+ * thickness rot(km,nobs,nch) of each layer and the depolarisation ratio
+ * depol(nch) from edge pressure pe [Pa], height he [m] and temperature te [K]
+ * (km+1,nobs), channels [nm] (VLIDORT_Rayleigh, coef_depol, F_air,
+ * V283/VLIDORT_ScatMod.F90:143-281).  rc = 3 (and AA_EDIMS) when km exceeds
+ * the layer dimension, as the drivers' VLIDORT_Init. */
+int aa_rot_calc(int km, int nch, int nobs, const double* channels, const double* pe, const double* he,
+                const double* te, double missing, int verbose, double* rot, double* depol, int* rc);
+/* Device form: d_pe/d_he/d_te/d_rot device pointers, channels and depol host
+ * arrays (nch), enqueued on stream. */
+int aa_rot_calc_dev(int km, int nch, int nobs, const double* channels, const double* d_pe, const double* d_he,
+                    const double* d_te, double* d_rot, double* depol, void* stream);
+
+/* getEdgeVars (CB/MieObs_py.F90:27-83; f2py MieObs_.getedgevars behind
+ * mieobs.getEdgeVars, src/Components/misc/scat/mieobs.py:60-76): edge
+ * pressure pe [Pa], height above the surface ze [m] and temperature te [K]
+ * (km+1,nobs) from air density airdens and pressure thickness delp (km,nobs),
+ * top pressure ptop; float32 as the reference. */
+int aa_edge_vars(int km, int nobs, const float* airdens, const float* delp, float ptop, float* pe, float* ze,
+                 float* te);
+int aa_edge_vars_dev(int km, int nobs, const float* d_airdens, const float* d_delp, float ptop, float* d_pe,
+                     float* d_ze, float* d_te, void* stream);
+
+/* calc_qm (GEO/geo_vlidort.F90:2124-2144): tracer mass per layer
+ * qm(km,nq,nobs) = q(km,nq,nobs) * delp(km,nobs) / 9.81, float32. */
+int aa_calc_qm(int km, int nq, int nobs, const float* q, const float* delp, float* qm);
+int aa_calc_qm_dev(int km, int nq, int nobs, const float* d_q, const float* d_delp, float* d_qm, void* stream);
+
 /* ------------------------------------------------------------------ fused device path
  * The whole hot path on resident data: Mie optics (tables in t) -> layer
  * optics -> VLIDORT radiance, for nobs columns x nch channels, device pointers

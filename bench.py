@@ -127,6 +127,8 @@ def main():
     ap.add_argument("--serving-granules", type=int, default=40,
                     help="granules of the serving-loop phase (0 skips it)")
     ap.add_argument("--serving-load", type=float, default=0.7, help="offered load as a fraction of peak")
+    ap.add_argument("--host-cols", type=int, default=20000,
+                    help="columns of the host-pointer (PCIe-inclusive) drop-in phase (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the path has no cross-GPU step (every column is independent); "
@@ -137,6 +139,9 @@ def main():
     ncol = args.ncol or wl["ncol"]
     km, nch, nq = wl["km"], len(wl["channels"]), len(S.TRACERS)
     N, nmom, npol, vector = wl["nstreams"], wl["nmom"], wl["npol"], wl["vector"]
+    uneven = wl.get("uneven_layers")
+    if N is None:   # per-column stream counts (uneven batch)
+        N = max(wl["stream_choices"])
 
     # host-CPU baseline first, before this process touches the GPU (fork-safe)
     cpu = None
@@ -156,7 +161,8 @@ def main():
     parts = []
     step = 100_000
     for c0 in range(0, ncol, step):
-        parts.append(S.make_columns_torch(min(step, ncol - c0), km, wl["channels"], seed=7, col0=c0, device=dev))
+        parts.append(S.make_columns_torch(min(step, ncol - c0), km, wl["channels"], seed=7, col0=c0, device=dev,
+                                          uneven_layers=uneven))
     cat = lambda k, dim: torch.cat([p[k] for p in parts], dim=dim).contiguous()
     qm = cat("qm", 0)            # [nobs, nq, km]  == Fortran qm(km,nq,nobs)
     rh = cat("rh", 0)            # [nobs, km]
@@ -168,6 +174,14 @@ def main():
     sza, vza, raa = cat("sza", 0), cat("vza", 0), cat("raa", 0)
     flux = cat("flux_factor", 0)  # [nobs, nch]
     depol = parts[0]["depol"]
+    nl_h = torch.cat([p["nlayers"] for p in parts]).cpu().numpy()
+    d_nl = torch.cat([p["nlayers"] for p in parts]).contiguous() if uneven else None
+    if wl["nstreams"] is None:
+        nstr_h = S.stream_choice(ncol, wl["stream_choices"], seed=7)
+        d_nstr = torch.tensor(nstr_h, dtype=torch.int32, device=dev)
+    else:
+        nstr_h = np.full(ncol, N, dtype=np.int32)
+        d_nstr = None
     del parts
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
@@ -186,7 +200,8 @@ def main():
 
     def run_step(precision):
         e = L.aa_column_radiance_dev(mie.handle, km, ncol, nch, nq, ptr(tr_idx), ptr(ch_idx), ptr(qm), ptr(rh), nmom,
-                                     npol, N, int(vector), precision, None, None, ptr(rot), ptr(depol), ptr(pe),
+                                     npol, N, int(vector), precision, ptr(d_nl), ptr(d_nstr), ptr(rot), ptr(depol),
+                                     ptr(pe),
                                      ptr(he), ptr(te), ptr(albedo), ptr(sza), ptr(raa), ptr(vza), ptr(flux),
                                      S.MISSING, ptr(rad), ptr(refl), ptr(qo), ptr(uo), ptr(nf), stream.cuda_stream)
         _lib.check(e, "aa_column_radiance_dev")
@@ -228,10 +243,13 @@ def main():
     nfh = main_run["nf"]
     ok = nfh > 0
     fb = fc = fd = 0.0
-    counts = np.bincount(nfh[ok], minlength=2 * N + 1)
-    for terms, cnt in enumerate(counts):
-        for M in range(terms):
-            b_, c_, d_ = flops_per_term(N, Sx, km, nmoments, M)
+    # items are (channel, column) in nf's order; each with its column's N and L
+    n_it = np.tile(nstr_h, nch)[ok]
+    l_it = np.tile(nl_h, nch)[ok]
+    combos, cnts = np.unique(np.stack([n_it, l_it, nfh[ok]]), axis=1, return_counts=True)
+    for (n_c, l_c, terms), cnt in zip(combos.T, cnts):
+        for M in range(int(terms)):
+            b_, c_, d_ = flops_per_term(int(n_c), Sx, int(l_c), min(2 * int(n_c) - 1, nmom), M)
             fb += cnt * b_; fc += cnt * c_; fd += cnt * d_
     fb *= args.steps; fc *= args.steps; fd *= args.steps
     kms, kln = main_run["kms"], main_run["kln"]
@@ -243,7 +261,7 @@ def main():
     item_terms = float(nfh[ok].sum()) * args.steps
     tr_key = (dom, N, Sx)
     traffic = None
-    if tr_key in TRAFFIC_PER_ITEM_TERM:
+    if tr_key in TRAFFIC_PER_ITEM_TERM and wl["nstreams"] is not None and not uneven:
         rd, wr = TRAFFIC_PER_ITEM_TERM[tr_key]
         traffic = round((rd + wr) * item_terms / dom_launches, 0)
     el = main_run["el"]
@@ -262,8 +280,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64" if rmx is not None else "f64",
         "data": "synthetic (seeded, generated on device)",
-        "config": {"workload": f"{args.workload}: {ncol} columns x {km} layers, {nch} channels "
-                               f"{list(wl['channels'])} nm, {2 * N} streams, {nmom} moments, {nq} tracers, "
+        "config": {"workload": f"{args.workload}: {ncol} columns x "
+                               f"{f'{uneven[0]}-{uneven[1]}' if uneven else km} layers, {nch} channels "
+                               f"{list(wl['channels'])} nm, "
+                               f"{2 * N if wl['nstreams'] is not None else '/'.join(str(2 * c) for c in wl['stream_choices'])}"
+                               f" streams, {nmom} moments, {nq} tracers, "
                                f"{'vector' if vector else 'scalar'} Lambertian",
                    "precision": "mixed (FP32 optics/phase-matrix contraction on MFMA/source assembly; FP64 "
                                 "eigensystem and band elimination)" if rmx is not None else "fp64",
@@ -271,7 +292,9 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                      "traffic_GBs": (round(traffic * dom_launches / (dom_ms / 1000.0) / 1e9, 1) if traffic else None),
-                     "kernel": f"k_{dom}<{N},{Sx}>", "kernel_avg_ms": round(dom_ms / dom_launches, 4),
+                     "kernel": (f"k_{dom}<{N},{Sx}>" if wl["nstreams"] is not None
+                                else f"k_{dom}<{min(wl['stream_choices'])}..{N},{Sx}>"),
+                     "kernel_avg_ms": round(dom_ms / dom_launches, 4),
                      "kernel_launches": int(dom_launches),
                      "note": "FP64 VALU work (no MFMA); peak is MI355X FP64 (vector = matrix rate)"},
         "hbm_path": {"bytes_per_column": bytes_per_column(km, nq, nch, vector),
@@ -296,7 +319,7 @@ def main():
         out["fp64_vs_cpu"] = round(r64["value"] / cpu["value"], 1)
 
     # serving loop: Poisson granules at a fraction of the measured peak
-    if args.serving_granules > 0 and not vector:
+    if args.serving_granules > 0 and not vector and wl["nstreams"] is not None and not uneven:
         from aeroapps_amd import serving as SV
         del qm, rh, rot, he, pe, te, albedo, sza, vza, raa, flux, rad, refl, nf
         torch.cuda.empty_cache()
@@ -314,6 +337,24 @@ def main():
                           "p50_latency_s": round(st["p50_s"], 3), "p99_latency_s": round(st["p99_s"], 3),
                           "throughput_cols_per_s": round(st["throughput_cols_per_s"], 1), "slots": st["slots"],
                           "granule_columns": "log-uniform 1000-200000, 1-4 channels, 72 layers, 12 streams"}
+    # drop-in host-pointer path (PCIe-inclusive): getAOPvector + VLIDORT_Scalar_Lambert
+    if args.host_cols > 0 and not vector and wl["nstreams"] is not None and not uneven:
+        from aeroapps_amd import vlidort as AV
+        hc = min(args.host_cols, ncol)
+        cols_h = S.make_columns(hc, km, wl["channels"], seed=7)
+        chs = np.asarray(wl["channels"], dtype=np.float32)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tau_h, ssa_h, g_h, pm_h, rc_a = AM.get_aop_vector(mie, chs, S.TRACERS, 0, cols_h.qm.T, cols_h.rh.T, nmom, npol)
+        t1 = time.perf_counter()
+        rad_h, refl_h, rc_l = AV.scalar_lambert(cols_h.channels, N, 0, cols_h.rot.transpose(2, 1, 0), cols_h.depol,
+                                                tau_h, ssa_h, g_h, pm_h, cols_h.pe.T, cols_h.he.T, cols_h.te.T,
+                                                cols_h.albedo.T, cols_h.sza, cols_h.raa, cols_h.vza, S.MISSING, 0)
+        t2 = time.perf_counter()
+        out["host_path"] = {"columns": hc, "getaop_s": round(t1 - t0, 3), "lambert_s": round(t2 - t1, 3),
+                            "columns_per_s": round(hc / (t2 - t0), 1), "rc": [int(rc_a), int(rc_l)],
+                            "note": "host numpy arrays in, host arrays out (FP64 optics widened on the host, "
+                                    "PCIe copies inside the calls); not the headline value"}
     print(json.dumps(out))
 
 

@@ -66,6 +66,7 @@ NRH = 36
 
 # field ids for the counter-based generator
 _F_DELP, _F_Q1, _F_Q2, _F_RH, _F_SZA, _F_VZA, _F_RAA, _F_NLAY, _F_NSTR = range(1, 10)
+_F_FISO, _F_FVOL, _F_FGEO = range(10, 13)
 _F_TBEXT, _F_TSSA, _F_TG, _F_TC12, _F_TC33, _F_TC34, _F_TREFR, _F_TREFI, _F_TREFF = range(20, 29)
 
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
@@ -299,6 +300,23 @@ def stream_choice(ncol: int, choices: Sequence[int] = (2, 4, 6, 8), seed: int = 
     cols = np.arange(col0, col0 + ncol, dtype=np.int64)
     i = np.floor(len(choices) * uniform(seed, _F_NSTR, cols)).astype(np.int64)
     return np.asarray(choices, dtype=np.int32)[i]
+
+
+def make_modis_kernels(ncol: int, nch: int, seed: int = 7, col0: int = 0):
+    """MODIS RTLS kernel weights and Li-sparse parameters per (column, channel):
+    kernel_wt [ncol, nch, 3] = (fiso, fgeo, fvol) in the order of the reference's
+    kernel_wt(3,nch,nobs), param [ncol, nch, 2] = (h/b, b/r) = (2, 1), the
+    MODIS operational values.  Weights span typical MCD43 land values."""
+    cols = np.arange(col0, col0 + ncol, dtype=np.int64)[:, None]
+    chs = np.arange(nch, dtype=np.int64)[None, :]
+    fiso = 0.03 + 0.27 * uniform(seed, _F_FISO, cols, chs)
+    fvol = fiso * (0.1 + 0.5 * uniform(seed, _F_FVOL, cols, chs))
+    fgeo = fiso * (0.02 + 0.18 * uniform(seed, _F_FGEO, cols, chs))
+    kw = np.stack([fiso, fgeo, fvol], axis=-1)
+    par = np.empty((ncol, nch, 2))
+    par[..., 0] = 2.0
+    par[..., 1] = 1.0
+    return kw, par
 
 
 # Named workloads.  nstreams is the per-hemisphere stream count NSTREAMS.

@@ -118,6 +118,59 @@ class LambertDriver:
         return out[0], out[1], out[2], out[3], nf, int(rc[0])
 
 
+class ModisRef:
+    """ctypes binding of the reference harness's MODIS land-surface routines
+    (``ref_vbrdf_modis`` / ``ref_scalar_landmodis`` in ref_driver.f90)."""
+
+    def __init__(self, lib: ctypes.CDLL):
+        ci, cd = ctypes.c_int, ctypes.c_double
+        self.fb = lib.ref_vbrdf_modis
+        self.fb.restype = None
+        self.fb.argtypes = [ci] + [cd] * 8 + [_dp] * 6 + [_ip]
+        self.fl = lib.ref_scalar_landmodis
+        self.fl.restype = None
+        self.fl.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                            _dp, _dp, _dp, _dp, _dp, cd, _dp, _dp, _dp, _ip, _ip]
+
+    def vbrdf_modis(self, nstreams, sza, vza, raa, fiso, fgeo, fvol, p1, p2):
+        """Dict F[M,i,j], F0[M,i], UF[M,j], UF0[M], db, wsa, status for one geometry."""
+        n, nm = nstreams, 2 * nstreams
+        F = np.zeros((n, n, nm))        # reversed Fortran F(0:2N-1, N, N)
+        F0 = np.zeros((n, nm))
+        UF = np.zeros((n, nm))
+        UF0 = np.zeros(nm)
+        db, wsa = np.zeros(1), np.zeros(1)
+        st = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fb, n, sza, vza, raa, fiso, fgeo, fvol, p1, p2, _p(F), _p(F0), _p(UF), _p(UF0),
+                        _p(db), _p(wsa), _p(st, np.int32))
+        return dict(F=F.transpose(2, 1, 0), F0=F0.T, UF=UF.T, UF0=UF0, db=float(db[0]), wsa=float(wsa[0]),
+                    status=int(st[0]))
+
+    def scalar_landmodis(self, km, channels, nstreams, nmom, rot, depol, tau, ssa, g, pmom, pe, he, te,
+                         kernel_wt, param, sza, raa, vza, missing=1e15, plane_parallel=False):
+        """Arrays in reversed-Fortran C order as for scalar_lambert, plus
+        kernel_wt[nobs,nch,3], param[nobs,nch,2]; returns rad, refl, BR [nch,nobs], nf, rc."""
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        channels, rot, depol, tau, ssa, g, pmom, pe, he, te, kernel_wt, param, sza, raa, vza = map(
+            f, (channels, rot, depol, tau, ssa, g, pmom, pe, he, te, kernel_wt, param, sza, raa, vza))
+        nch = len(channels)
+        nobs = len(sza)
+        npol = pmom.shape[0]
+        rad, refl, br = (np.zeros((nch, nobs)) for _ in range(3))
+        nf = np.zeros((nch, nobs), dtype=np.int32)
+        rc = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fl, km, nch, nobs, _p(channels), nstreams, int(plane_parallel), nmom, npol, _p(rot),
+                        _p(depol), _p(tau), _p(ssa), _p(g), _p(pmom), _p(pe), _p(he), _p(te), _p(kernel_wt),
+                        _p(param), _p(sza), _p(raa), _p(vza), missing, _p(rad), _p(refl), _p(br),
+                        _p(nf, np.int32), _p(rc, np.int32))
+        return rad, refl, br, nf, int(rc[0])
+
+
+def ref_modis() -> Optional[ModisRef]:
+    lib = _load(REF_SO)
+    return None if lib is None or not hasattr(lib, "ref_vbrdf_modis") else ModisRef(lib)
+
+
 def ref() -> Optional[LambertDriver]:
     lib = _load(REF_SO)
     return None if lib is None else LambertDriver(lib, "ref")

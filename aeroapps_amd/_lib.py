@@ -39,6 +39,13 @@ SIGNATURES = {
     "aa_column_radiance_dev": (_ci, [_vp, _ci, _ci, _ci, _ci, _vp, _vp, _vp, _vp, _ci, _ci, _ci, _ci, _ci, _vp, _vp,
                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _cd, _vp, _vp, _vp, _vp, _vp,
                                      _vp]),
+    "aa_vlidort_scalar_landmodis": (_ci, [_ci, _ci, _ci, _dp, _ci, _ci, _ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp,
+                                          _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _cd, _ci, _ci, _dp, _dp, _dp, _ip,
+                                          _ip]),
+    "aa_vbrdf_modis": (_ci, [_ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip]),
+    "aa_column_radiance_modis_dev": (_ci, [_vp, _ci, _ci, _ci, _ci, _vp, _vp, _vp, _vp, _ci, _ci, _ci, _vp, _vp,
+                                           _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _cd, _vp, _vp, _vp,
+                                           _vp, _vp]),
     "aa_rot_calc": (_ci, [_ci, _ci, _ci, _dp, _dp, _dp, _dp, _cd, _ci, _dp, _dp, _ip]),
     "aa_rot_calc_dev": (_ci, [_ci, _ci, _ci, _dp, _vp, _vp, _vp, _vp, _dp, _vp]),
     "aa_edge_vars": (_ci, [_ci, _ci, _fp, _fp, _cf, _fp, _fp, _fp]),

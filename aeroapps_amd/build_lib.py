@@ -24,7 +24,7 @@ COMMON = [
 # Chem_Mie arithmetic and the input preparation must not contract (the
 # reference's operation order, bit-exactness); the radiance kernels keep the
 # compiler default.
-NO_CONTRACT = {"mie.hip", "prep.hip"}
+NO_CONTRACT = {"mie.hip", "prep.hip", "brdf.hip"}
 
 FOURIER_PARTS = [(n, 1) for n in range(1, 9)] + [(n, 3) for n in range(1, 5)]
 

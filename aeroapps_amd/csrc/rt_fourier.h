@@ -505,33 +505,71 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
     }
     rec[O.csrc(o)] = beam ? spar : 0.0;
   }
-  // ---- Lambertian surface terms of the bottom layer at Fourier 0
-  //      (BVP_SURFACE_SETUP_HOM/BEAM, REG/vlidort_bvproblem.f90; GET_BOA_SOURCE,
-  //      REG/vlidort_intensity.f90:1096-1496)
-  if (layer == L - 1 && M == 0) {
+  // ---- surface terms of the bottom layer (BVP_SURFACE_SETUP_HOM/BEAM,
+  //      REG/vlidort_bvproblem.f90:440-626, 1284+; GET_BOA_SOURCE,
+  //      REG/vlidort_intensity.f90:1096-1496).  Lambertian: Fourier 0 only,
+  //      the same reflected term in every quadrature row.  MODIS BRDF (scalar):
+  //      every Fourier term, AXBID(i,j) = w_j mu_j BRDF_F(M,i,j) per row, and
+  //      the BOA source weights w_j mu_j USER_BRDF_F(M,j).
+  const bool brdf = (S == 1) && W.brdf != nullptr;
+  if (layer == L - 1 && (brdf || M == 0)) {
     const int g = global_item(W, it);
     const int ch = g - obs * W.nch;
-    const double albedo = W.albedo[(size_t)ch * W.nobs + obs];
-    if (albedo != 0.0) {
-      constexpr BOff BO(n);
+    const double albedo = brdf ? 0.0 : W.albedo[(size_t)ch * W.nobs + obs];
+    if (brdf || albedo != 0.0) {
+      constexpr BOff BO(N, n);
+      constexpr BROff RO(N);
       double* bb = W.bbuf + (size_t)it * W.bstride;
+      const double* rec = brdf ? W.brdf + (size_t)it * W.brdf_stride : nullptr;
+      const double sfac = (M == 0) ? 2.0 : 1.0;
       const double kmult = 2.0 * albedo;
+      double cw[N];
+#pragma unroll
+      for (int j = 0; j < N; ++j) cw[j] = brdf ? qsw[j] * rec[RO.uf(M, j)] : qsw[j];
 #pragma unroll
       for (int k = 0; k < n; ++k) {
-        double sp = 0.0, sm = 0.0, ba = 0.0, bbv = 0.0;
+        double ba = 0.0, bbv = 0.0;
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-          sp += qsw[j] * xp[j * S][k]; sm += qsw[j] * xn[j * S][k];
-          ba += qsw[j] * xp[j * S][k] * tdel[k]; bbv += qsw[j] * xn[j * S][k];
-        }
-        bb[BO.r2p(k)] = sp * kmult; bb[BO.r2m(k)] = sm * kmult;
+        for (int j = 0; j < N; ++j) { ba += cw[j] * xp[j * S][k] * tdel[k]; bbv += cw[j] * xn[j * S][k]; }
         bb[BO.bcoef(k)] = ba; bb[BO.bcoef(n + k)] = bbv;
       }
       double rb = 0.0;
 #pragma unroll
-      for (int j = 0; j < N; ++j) rb += qsw[j] * wl[j][0];
-      bb[BO.r2beam()] = rb * kmult;
+      for (int j = 0; j < N; ++j) rb += cw[j] * wl[j][0];
       bb[BO.bc()] = rb;
+      if (!brdf) {
+        double rq = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) rq += qsw[j] * wl[j][0];
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+          double sp = 0.0, sm = 0.0;
+#pragma unroll
+          for (int j = 0; j < N; ++j) { sp += qsw[j] * xp[j * S][k]; sm += qsw[j] * xn[j * S][k]; }
+#pragma unroll
+          for (int i = 0; i < N; ++i) { bb[BO.r2p(i, k)] = sp * kmult; bb[BO.r2m(i, k)] = sm * kmult; }
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) bb[BO.r2beam(i)] = rq * kmult;
+      } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          double ax[N];
+#pragma unroll
+          for (int j = 0; j < N; ++j) ax[j] = qsw[j] * rec[RO.f(M, i, j)];
+#pragma unroll
+          for (int k = 0; k < n; ++k) {
+            double sp = 0.0, sm = 0.0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) { sp += ax[j] * xp[j * S][k]; sm += ax[j] * xn[j * S][k]; }
+            bb[BO.r2p(i, k)] = sfac * sp; bb[BO.r2m(i, k)] = sfac * sm;
+          }
+          double rbi = 0.0;
+#pragma unroll
+          for (int j = 0; j < N; ++j) rbi += ax[j] * wl[j][0];
+          bb[BO.r2beam(i)] = sfac * rbi;
+        }
+      }
     }
   }
 }
@@ -631,7 +669,7 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
   constexpr int G = bvp_group(n), IPW = 64 / G, SL = bvp_slots(n);
   static_assert(2 * n <= 64 && SL <= 2, "panel rows exceed the wavefront");
   constexpr LOff O(N, S);
-  constexpr BOff BO(n);
+  constexpr BOff BO(N, n);
   constexpr int BIG = 0x3fffffff;
   __shared__ __align__(16) double us[IPW][n2][UW];   // U rows of the current panel (cols, RHS, 1/pivot)
   __shared__ double xs[IPW][n2];       // integration constants of the current/next layer
@@ -647,7 +685,9 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
   const size_t oi = (size_t)ch * W.nobs + obs;
   if (valid && W.status[it] != 0) {
     if (t == 0) {
-      W.radiance[oi] = W.missing; W.reflectance[oi] = W.missing;
+      // status 6: negative direct-bounce BRDF, which the MODIS driver reports as -500
+      const double fill = (W.status[it] == 6) ? -500.0 : W.missing;
+      W.radiance[oi] = fill; W.reflectance[oi] = fill;
       if (S > 1) { W.qout[oi] = W.missing; W.uout[oi] = W.missing; }
       W.nfourier[oi] = 0;
     }
@@ -661,17 +701,22 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
   const double* __restrict__ lb = W.lbuf + (size_t)it * km * LS;
   const double* __restrict__ bb = W.bbuf + (size_t)it * W.bstride;
   double* __restrict__ ub = W.ubuf + (size_t)it * (size_t)n2 * km * UW;
-  const double albedo = W.albedo[oi];
-  const bool surf = (M == 0) && (albedo != 0.0);
+  constexpr BROff RO(N);
+  const bool brdf = (S == 1) && W.brdf != nullptr;
+  const double* __restrict__ brec = brdf ? W.brdf + (size_t)it * W.brdf_stride : nullptr;
+  const double albedo = brdf ? 0.0 : W.albedo[oi];
+  const bool surf = brdf || ((M == 0) && (albedo != 0.0));
   const double tsb = W.tsb[it];
   const double ff = W.flux ? W.flux[(size_t)obs * W.nch + ch] : 1.0;
-  double rf_direct = 0.0;
-  if (surf && tsb != 0.0) {   // DIRECTRADIANCE (REG/vlidort_miscsetups.f90:1296-1614), Lambertian
-    double x0 = 4.0 * W.csza[it] / 1.0;
+  // DIRECTRADIANCE (REG/vlidort_miscsetups.f90:1296-1614): reflected direct
+  // beam ATTN * albedo (Lambertian, Fourier 0) or ATTN * BRDF_F_0(M, i)
+  double attn = 0.0;
+  if (surf && tsb != 0.0) {
+    double x0 = 4.0 * W.csza[it] / ((M == 0) ? 1.0 : 2.0);
     x0 = ff * x0 / PI4;
-    rf_direct = x0 * tsb * albedo;
+    attn = x0 * tsb;
   }
-  const double r2beam = surf ? bb[BO.r2beam()] : 0.0;
+  const double rf_direct = attn * albedo;
 
   // ---------------- panel LU with partial pivoting + forward substitution
   // Each lane of the group holds SL rows of the panel: a[s][0..2n) the columns
@@ -824,18 +869,18 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
 #pragma unroll
       for (int q = 0; q < n; ++q) {
         double xpn = lu[O.xp(i + N, o1, q)];
-        if (sr) xpn = xpn - bb[BO.r2p(q)];
+        if (sr) xpn = xpn - bb[BO.r2p(i, q)];
         r[q] = lu[O.tdel(q)] * xpn;
       }
 #pragma unroll
       for (int q = 0; q < n; ++q) {
         double xmn = lu[O.xn(i + N, o1, q)];
-        if (sr) xmn = xmn - bb[BO.r2m(q)];
+        if (sr) xmn = xmn - bb[BO.r2m(i, q)];
         r[n + q] = xmn;
       }
       double v = -lu[O.wl(i + N, o1)];
-      if (sr) v = v + r2beam;
-      if (sr) v = v + rf_direct;
+      if (sr) v = v + bb[BO.r2beam(i)];
+      if (sr) v = v + (brdf ? attn * brec[RO.f0(M, i)] : rf_direct);
       r[RH] = v;
     }
   };
@@ -872,7 +917,7 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
   // constant of the layer (LCON for t < n, MCON after).  The U rows and source
   // coefficients of the next panel up are loaded one panel ahead; x_i is
   // broadcast within the group by lane reads.
-  const double kmult = 2.0 * albedo;
+  const double kmult = brdf ? ((M == 0) ? 2.0 : 1.0) : 2.0 * albedo;
   double cum[S];
 #pragma unroll
   for (int o = 0; o < S; ++o) cum[o] = 0.0;

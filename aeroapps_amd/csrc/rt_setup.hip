@@ -19,6 +19,7 @@
 // thread per item for the vertical recursions.  All FP64.
 #include "rt_work.h"
 #include "rt_device.h"
+#include "brdf_dev.h"
 #include "prof.h"
 
 namespace aa {
@@ -155,14 +156,23 @@ __device__ __forceinline__ ItemIdx item_of(const RTWork& W, int it) {
 // driver latches DO_FOCORR_NADIR once a column with SZA == 0 is processed
 // (V283/VLIDORT_ScatMod.F90:432-435, 1123-1126); items are processed in
 // (obs, channel) order.
+// The MODIS driver (src/Components/rtms/geo_vlidort/VLIDORT_BRDF_MODIS.F90:
+// 160-170) runs VLIDORT only where the direct-bounce BRDF is not negative, so
+// only such items latch.
 __global__ void k_latch(int nitems, int nch, int nobs, const double* sza, const double* vza, const double* raa,
-                        const double* albedo, double missing, int* latch) {
+                        const double* albedo, const double* kernel_wt, const double* kparam, double missing,
+                        int* latch) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nitems) return;
   int obs = g / nch, ch = g - obs * nch;
   if (sza[obs] != 0.0) return;
   if (is_missing(sza[obs], missing) || is_missing(vza[obs], missing) || is_missing(raa[obs], missing)) return;
-  if (is_missing(albedo[(size_t)ch * nobs + obs], missing)) return;
+  if (albedo && is_missing(albedo[(size_t)ch * nobs + obs], missing)) return;
+  if (kernel_wt) {
+    const double* kw = kernel_wt + ((size_t)obs * nch + ch) * 3;
+    const double* kp = kparam + ((size_t)obs * nch + ch) * 2;
+    if (modis_db(kw[0], kw[1], kw[2], kp[0], kp[1], sza[obs], vza[obs], raa[obs]) < 0.0) return;
+  }
   atomicMin(latch, g);
 }
 
@@ -176,7 +186,7 @@ __global__ void k_check(RTWork W, int nitems) {
   const double sza = W.sza[I.obs], vza = W.vza[I.obs], raa = W.raa[I.obs];
   int st = 0;
   if (is_missing(sza, W.missing) || is_missing(vza, W.missing) || is_missing(raa, W.missing)) st = 1;
-  else if (is_missing(W.albedo[(size_t)I.ch * W.nobs + I.obs], W.missing)) st = 1;
+  else if (W.albedo && is_missing(W.albedo[(size_t)I.ch * W.nobs + I.obs], W.missing)) st = 1;
   int L = W.nlayers ? W.nlayers[I.obs] : W.km;
   if (L < 2 || L > W.km) { if (!st) st = 4; L = 2; }
   if (!st) {
@@ -555,7 +565,9 @@ __global__ void k_item(RTWork W, int nitems, int* active, int* nactive) {
   const size_t oi = (size_t)I.ch * W.nobs + I.obs;
   const int km = W.km;
   if (W.status[it] != 0) {
-    W.radiance[oi] = W.missing; W.reflectance[oi] = W.missing;
+    // status 6: negative direct-bounce BRDF, reported as -500 by the MODIS driver
+    const double fill = (W.status[it] == 6) ? -500.0 : W.missing;
+    W.radiance[oi] = fill; W.reflectance[oi] = fill;
     if (S > 1) { W.qout[oi] = W.missing; W.uout[oi] = W.missing; }
     W.nfourier[oi] = 0;
     return;
@@ -631,10 +643,14 @@ __global__ void k_item(RTWork W, int nitems, int* active, int* nactive) {
     mu0 = (sza == 90.0) ? 0.0 : cos(sza * dtr);
     mu1 = (vza == 90.0) ? 0.0 : cos(vza * dtr);
   }
-  const double albedo = W.albedo[oi];
   const double fvec1 = 1.0;
   double cums[3] = {0, 0, 0};
-  const double cdb = 4.0 * mu0 * albedo * atten(L) * fvec1;
+  // surface term of the direct bounce: M4*albedo*atten*fvec (Lambertian) or
+  // M4*(BRDF . fvec)*atten with the exact direct-bounce BRDF
+  // (FOC/FO_VectorSS_RTCalcs_I.f90:698-706)
+  double cdb;
+  if (W.brdf) cdb = 4.0 * mu0 * (W.brdf[(size_t)it * W.brdf_stride + BROff(W.N).db()] * fvec1) * atten(L);
+  else cdb = 4.0 * mu0 * W.albedo[oi] * atten(L) * fvec1;
   double cdbv[3] = {cdb, 0.0, 0.0};
   const double* lt = W.lostrans + (size_t)it * (km + 1);
   const double* ex = W.exact + (size_t)it * km * 3;
@@ -678,6 +694,8 @@ __global__ void k_item(RTWork W, int nitems, int* active, int* nactive) {
 
 // Runs the setup stage for items [W.item0, W.item0 + nitems) and fills the
 // first active list.
+int rt_brdf(const RTWork& W, int nitems, hipStream_t st);
+
 int rt_setup(const RTWork& W, int nitems, int* d_active, int* d_nactive, hipStream_t st) {
   const int B = 256;
   const int gi = (nitems + B - 1) / B;
@@ -685,6 +703,7 @@ int rt_setup(const RTWork& W, int nitems, int* d_active, int* d_nactive, hipStre
   prof_begin(PROF_CHECK, st);
   hipLaunchKernelGGL(k_check, dim3(gi), dim3(B), 0, st, W, nitems);
   prof_end(PROF_CHECK, st);
+  if (W.kernel_wt && rt_brdf(W, nitems, st)) return -2;
   prof_begin(PROF_OPTICS, st);
   if (W.S == 1) {
     if (W.optics_f32) hipLaunchKernelGGL((k_optics<1, float>), dim3(gl), dim3(B), 0, st, W, nitems);
@@ -706,10 +725,11 @@ int rt_setup(const RTWork& W, int nitems, int* d_active, int* d_nactive, hipStre
 }
 
 int rt_latch(int nitems, int nch, int nobs, const double* sza, const double* vza, const double* raa,
-             const double* albedo, double missing, int* d_latch, hipStream_t st) {
+             const double* albedo, const double* kernel_wt, const double* kparam, double missing, int* d_latch,
+             hipStream_t st) {
   const int B = 256;
   hipLaunchKernelGGL(k_latch, dim3((nitems + B - 1) / B), dim3(B), 0, st, nitems, nch, nobs, sza, vza, raa, albedo,
-                     missing, d_latch);
+                     kernel_wt, kparam, missing, d_latch);
   AA_CHECK(hipGetLastError());
   return 0;
 }

@@ -46,7 +46,15 @@ struct RTWork {
   int opt_obs0 = 0;                // first observation held in tau/ssa/pmom
   int opt_nobs = 0;                // observations held in tau/ssa/pmom (their nobs extent)
   const double* he = nullptr;      // (km+1,nobs) metres
-  const double* albedo = nullptr;  // (nobs,nch)
+  const double* albedo = nullptr;  // (nobs,nch); null for the MODIS BRDF surface
+  // MODIS land surface (BRDF: Lambertian + Ross-thick + Li-sparse kernels);
+  // when kernel_wt is set the surface is the BRDF, else Lambertian.
+  const double* kernel_wt = nullptr;  // (3,nch,nobs): fiso, fgeo, fvol
+  const double* kparam = nullptr;     // (2,nch,nobs): Li-sparse h/b, b/r
+  double* br_out = nullptr;           // (nobs,nch) exact direct-bounce BRDF
+  const double* brdf_tab = nullptr;   // azimuth / scaling quadrature tables (BrTab)
+  double* brdf = nullptr;             // [it][brdf_stride] Fourier components of the BRDF
+  int brdf_stride = 0;
   const double* sza = nullptr;     // (nobs)
   const double* raa = nullptr;
   const double* vza = nullptr;
@@ -114,16 +122,53 @@ struct LOff {
 };
 
 // Offsets inside one bbuf record: surface reflection terms of the bottom
-// layer at Fourier 0 (Lambertian).
+// layer for the current Fourier term, per quadrature row i < N (the
+// Lambertian surface has the same value in every row and only at Fourier 0).
 struct BOff {
-  int n;
-  __host__ __device__ constexpr BOff(int n_) : n(n_) {}
-  __host__ __device__ constexpr int r2p(int k) const { return k; }
-  __host__ __device__ constexpr int r2m(int k) const { return n + k; }
-  __host__ __device__ constexpr int bcoef(int q) const { return 2 * n + q; }   // q < 2n
-  __host__ __device__ constexpr int r2beam() const { return 4 * n; }
-  __host__ __device__ constexpr int bc() const { return 4 * n + 1; }
-  __host__ __device__ constexpr int size() const { return 4 * n + 2; }
+  int N, n;
+  __host__ __device__ constexpr BOff(int N_, int n_) : N(N_), n(n_) {}
+  __host__ __device__ constexpr int r2p(int i, int k) const { return i * n + k; }
+  __host__ __device__ constexpr int r2m(int i, int k) const { return N * n + i * n + k; }
+  __host__ __device__ constexpr int bcoef(int q) const { return 2 * N * n + q; }   // q < 2n
+  __host__ __device__ constexpr int r2beam(int i) const { return 2 * N * n + 2 * n + i; }
+  __host__ __device__ constexpr int bc() const { return 2 * N * n + 2 * n + N; }
+  __host__ __device__ constexpr int size() const { return 2 * N * n + 2 * n + N + 1; }
+};
+
+// Offsets inside one BRDF record (per item, scalar): Fourier components
+// M < 2N of the MODIS BRDF in VLIDORT's arrays (BRDF_F incident j -> reflected
+// i, BRDF_F_0 solar -> i, USER_BRDF_F j -> user, USER_BRDF_F_0 solar -> user)
+// and the exact direct-bounce BRDF.
+struct BROff {
+  int N;
+  __host__ __device__ constexpr BROff(int N_) : N(N_) {}
+  __host__ __device__ constexpr int f(int M, int i, int j) const { return (M * N + i) * N + j; }
+  __host__ __device__ constexpr int f0(int M, int i) const { return 2 * N * N * N + M * N + i; }
+  __host__ __device__ constexpr int uf(int M, int j) const { return 2 * N * N * N + 2 * N * N + M * N + j; }
+  __host__ __device__ constexpr int uf0(int M) const { return 2 * N * N * N + 4 * N * N + M; }
+  __host__ __device__ constexpr int db() const { return 2 * N * N * N + 4 * N * N + 2 * N; }
+  __host__ __device__ constexpr int size() const { return (2 * N * N * N + 4 * N * N + 2 * N + 2) & ~1; }
+};
+
+// Host-built quadrature tables of the BRDF supplement for N streams
+// (BRDF_QUADRATURE_GAUSSIAN half range, 25 azimuth streams; the 24-stream
+// white-sky quadrature; the N quadrature streams with their sines).
+constexpr int BR_NAZ = 25, BR_NSC = 24;
+struct BrTab {
+  int N;
+  __host__ __device__ constexpr BrTab(int N_) : N(N_) {}
+  __host__ __device__ constexpr int x(int k) const { return k; }
+  __host__ __device__ constexpr int cx(int k) const { return BR_NAZ + k; }
+  __host__ __device__ constexpr int sx(int k) const { return 2 * BR_NAZ + k; }
+  __host__ __device__ constexpr int a(int k) const { return 3 * BR_NAZ + k; }
+  __host__ __device__ constexpr int caz(int M, int k) const { return (4 + M) * BR_NAZ + k; }   // M < 2N
+  __host__ __device__ constexpr int sq(int i) const { return (4 + 2 * N) * BR_NAZ + i; }
+  __host__ __device__ constexpr int ssin(int i) const { return (4 + 2 * N) * BR_NAZ + BR_NSC + i; }
+  __host__ __device__ constexpr int ssw(int i) const { return (4 + 2 * N) * BR_NAZ + 2 * BR_NSC + i; }
+  __host__ __device__ constexpr int qs(int i) const { return (4 + 2 * N) * BR_NAZ + 3 * BR_NSC + i; }
+  __host__ __device__ constexpr int qsin(int i) const { return (4 + 2 * N) * BR_NAZ + 3 * BR_NSC + N + i; }
+  __host__ __device__ constexpr int wsa_ross() const { return (4 + 2 * N) * BR_NAZ + 3 * BR_NSC + 2 * N; }
+  __host__ __device__ constexpr int size() const { return (4 + 2 * N) * BR_NAZ + 3 * BR_NSC + 2 * N + 1; }
 };
 
 // Rows/columns of the SAB/DAB contraction table for (N, S): rows r*n+c (SAB)

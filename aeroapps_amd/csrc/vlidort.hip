@@ -25,7 +25,9 @@ namespace aa {
 
 int rt_setup(const RTWork& W, int nitems, int* d_active, int* d_nactive, hipStream_t st);
 int rt_latch(int nitems, int nch, int nobs, const double* sza, const double* vza, const double* raa,
-             const double* albedo, double missing, int* d_latch, hipStream_t st);
+             const double* albedo, const double* kernel_wt, const double* kparam, double missing, int* d_latch,
+             hipStream_t st);
+const std::vector<double>& brdf_table_cached(int N);
 #define AA_DECL(N, S) \
   int rt_fourier_##N##_##S(const RTWork& W, int nitems, int* const lists[2], int* counts, hipStream_t st);
 AA_DECL(1, 1) AA_DECL(2, 1) AA_DECL(3, 1) AA_DECL(4, 1) AA_DECL(5, 1) AA_DECL(6, 1) AA_DECL(7, 1) AA_DECL(8, 1)
@@ -236,7 +238,8 @@ hipStream_t bucket_stream(hipStream_t caller, int k) {
 size_t carve_work(Carver& C, RTWork& W, int cap) {
   const int n = W.N * W.S, LM = 2 * W.N, km = W.km;
   W.lstride = LOff(W.N, W.S).size();
-  W.bstride = BOff(n).size();
+  W.bstride = BOff(W.N, n).size();
+  W.brdf_stride = W.kernel_wt ? BROff(W.N).size() : 0;
   W.ustride = 4 * n + 2;   // U row of a layer panel: 4n columns, transformed RHS, 1/pivot
   W.cap = cap;
   W.status = C.take<int>(cap);
@@ -261,6 +264,7 @@ size_t carve_work(Carver& C, RTWork& W, int cap) {
   W.lostrans = C.take<double>((size_t)cap * (km + 1));
   W.lbuf = C.take<double>((size_t)cap * km * W.lstride);
   W.bbuf = C.take<double>((size_t)cap * W.bstride);
+  W.brdf = W.kernel_wt ? C.take<double>((size_t)cap * W.brdf_stride) : nullptr;
   W.ubuf = C.take<double>((size_t)cap * (2 * n * km) * W.ustride);
   W.sab32 = (W.precision == 1) ? C.take<float>((size_t)cap * km * 2 * n * n) : nullptr;
   return C.off;
@@ -279,7 +283,9 @@ int compute_latch(RTWork& W, int* lp, hipStream_t st) {
   static const int big = INT_MAX;
   AA_CHECK(hipMemcpyAsync(lp, &big, sizeof(int), hipMemcpyHostToDevice, st));
   const int nitems = W.nobs * W.nch;
-  if (nitems > 0 && rt_latch(nitems, W.nch, W.nobs, W.sza, W.vza, W.raa, W.albedo, W.missing, lp, st)) return AA_EHIP;
+  if (nitems > 0 && rt_latch(nitems, W.nch, W.nobs, W.sza, W.vza, W.raa, W.albedo, W.kernel_wt, W.kparam, W.missing,
+                             lp, st))
+    return AA_EHIP;
   W.latch = lp;
   return 0;
 }
@@ -293,6 +299,9 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   const std::vector<double>& qtab = quad_table_cached(W.N, W.S, W.nmoments);
   static const std::vector<float> no_ytab;
   const std::vector<float>& ytab = (W.precision == 1) ? ytab_cached(W.N, W.S, W.nmoments) : no_ytab;
+  static const std::vector<double> no_btab;
+  const std::vector<double>& btab = W.kernel_wt ? brdf_table_cached(W.N) : no_btab;
+  double* dbt = nullptr;
   // chunk capacity from the work budget
   size_t freeb = 0, totalb = 0;
   AA_CHECK(hipMemGetInfo(&freeb, &totalb));
@@ -309,6 +318,7 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
     carve_work(C, w, cap);
     *dq = C.take<double>(qtab.size());
     dy = C.take<float>(std::max<size_t>(1, ytab.size()));
+    dbt = C.take<double>(std::max<size_t>(1, btab.size()));
     *l0 = C.take<int>(cap);
     *l1 = C.take<int>(cap);
     *cnt = C.take<int>(W.nmoments + 2);
@@ -325,6 +335,10 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   if (!ytab.empty()) {
     AA_CHECK(hipMemcpyAsync(dy, ytab.data(), ytab.size() * sizeof(float), hipMemcpyHostToDevice, st));
     W.ytab = dy;
+  }
+  if (!btab.empty()) {
+    AA_CHECK(hipMemcpyAsync(dbt, btab.data(), btab.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    W.brdf_tab = dbt;
   }
   int* lists[2] = {l0, l1};
   W.item_list = d_items;
@@ -359,13 +373,19 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
                  const double* rot, const double* depol, const double* alpha, const double* tau, const double* ssa,
                  const double* pmom, const double* he, const double* albedo, const double* sza, const double* raa,
                  const double* vza, const double* flux_factor, double missing, int precision, double* radiance,
-                 double* reflectance, double* q, double* u, int* nfourier, int* rc) {
+                 double* reflectance, double* q, double* u, int* nfourier, int* rc,
+                 const double* kernel_wt = nullptr, const double* kparam = nullptr, double* br = nullptr) {
   if (!rc) return AA_EARG;
   *rc = 0;
+  const bool modis = kernel_wt != nullptr;
   if (km <= 0 || nch <= 0 || nobs < 0 || nmom <= 0 || npol <= 0) return AA_EARG;
-  if (!rot || !depol || !tau || !ssa || !pmom || !he || !albedo || !sza || !raa || !vza || !radiance ||
+  if (!rot || !depol || !tau || !ssa || !pmom || !he || (!albedo && !modis) || !sza || !raa || !vza || !radiance ||
       !reflectance)
     return AA_EARG;
+  if (modis && (vector || !kparam || !br)) return AA_EARG;
+  // VLIDORT_Init of the MODIS driver (V283/VLIDORT_Mod.F90:369) rejects nMom
+  // beyond the input maximum for the whole call
+  if (modis && nmom > MAX_MOMENTS_IN) { *rc = 5; return 0; }
   if (vector && (!q || !u || !flux_factor || npol < 6)) return AA_EARG;
   if (precision != 0 && precision != 1) return AA_EARG;
   const int S = vector ? 3 : 1;
@@ -388,7 +408,12 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
   W.pmom = B.up(pmom, nl * nmom * npol);
   W.optics_f32 = 0; W.opt_obs0 = 0; W.opt_nobs = nobs;
   W.he = B.up(he, (size_t)(km + 1) * nobs);
-  W.albedo = B.up(albedo, (size_t)nobs * nch);
+  W.albedo = modis ? nullptr : B.up(albedo, (size_t)nobs * nch);
+  if (modis) {
+    W.kernel_wt = B.up(kernel_wt, (size_t)nobs * nch * 3);
+    W.kparam = B.up(kparam, (size_t)nobs * nch * 2);
+    W.br_out = B.alloc<double>((size_t)nobs * nch);
+  }
   W.sza = B.up(sza, (size_t)nobs);
   W.raa = B.up(raa, (size_t)nobs);
   W.vza = B.up(vza, (size_t)nobs);
@@ -417,6 +442,7 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
     AA_CHECK(hipMemcpy(u, W.uout, no * sizeof(double), hipMemcpyDeviceToHost));
   }
   if (nfourier) AA_CHECK(hipMemcpy(nfourier, W.nfourier, no * sizeof(int), hipMemcpyDeviceToHost));
+  if (modis) AA_CHECK(hipMemcpy(br, W.br_out, no * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -451,19 +477,40 @@ int aa_vlidort_vector_lambert(int km, int nch, int nobs, const double* channels,
                           precision, radiance, reflectance, q, u, nfourier, rc);
 }
 
-int aa_column_radiance_dev(aa_mie* t, int km, int nobs, int nch, int nq, const int* tracer_idx, const int* chan_idx,
-                           const float* d_qm, const float* d_rh, int nmom, int npol, int nstreams, int vector,
-                           int precision, const int* d_nlayers, const int* d_nstreams_col, const double* d_rot,
-                           const double* d_depol, const double* d_pe, const double* d_he, const double* d_te,
-                           const double* d_albedo, const double* d_sza, const double* d_raa, const double* d_vza,
-                           const double* d_flux_factor, double missing, double* d_radiance, double* d_reflectance,
-                           double* d_q, double* d_u, int* d_nfourier, void* stream) {
-  (void)d_pe; (void)d_te;
-  using namespace aa;
+int aa_vlidort_scalar_landmodis(int km, int nch, int nobs, const double* channels, int nstreams, int plane_parallel,
+                                int nmom, int npol, const double* rot, const double* depol, const double* tau,
+                                const double* ssa, const double* g, const double* pmom, const double* pe,
+                                const double* he, const double* te, const double* kernel_wt, const double* param,
+                                const double* solar_zenith, const double* relat_azymuth, const double* sensor_zenith,
+                                double missing, int verbose, int precision, double* radiance, double* reflectance,
+                                double* br, int* nfourier, int* rc) {
+  (void)channels; (void)g; (void)pe; (void)te; (void)verbose;
+  if (!kernel_wt) return AA_EARG;
+  return aa::lambert_host(false, km, nch, nobs, nstreams, plane_parallel, nmom, npol, rot, depol, nullptr, tau, ssa,
+                          pmom, he, nullptr, solar_zenith, relat_azymuth, sensor_zenith, nullptr, missing, precision,
+                          radiance, reflectance, nullptr, nullptr, nfourier, rc, kernel_wt, param, br);
+}
+
+}  // extern "C"
+
+namespace aa {
+namespace {
+
+// Fused device path shared by the Lambertian and the MODIS land-surface entry
+// points (d_kernel_wt set: BRDF surface, d_albedo unused).
+int column_radiance(aa_mie* t, int km, int nobs, int nch, int nq, const int* tracer_idx, const int* chan_idx,
+                    const float* d_qm, const float* d_rh, int nmom, int npol, int nstreams, int vector, int precision,
+                    const int* d_nlayers, const int* d_nstreams_col, const double* d_rot, const double* d_depol,
+                    const double* d_he, const double* d_albedo, const double* d_kernel_wt, const double* d_param,
+                    double* d_br, const double* d_sza, const double* d_raa, const double* d_vza,
+                    const double* d_flux_factor, double missing, double* d_radiance, double* d_reflectance,
+                    double* d_q, double* d_u, int* d_nfourier, void* stream) {
+  const bool modis = d_kernel_wt != nullptr;
   if (!t || km <= 0 || km > MAX_LAYERS || nobs < 0 || nch <= 0 || nq <= 0 || nmom <= 0 || npol <= 0) return AA_EARG;
-  if (!tracer_idx || !chan_idx || !d_qm || !d_rh || !d_rot || !d_depol || !d_he || !d_albedo || !d_sza || !d_raa ||
-      !d_vza || !d_radiance || !d_reflectance || !d_nfourier)
+  if (!tracer_idx || !chan_idx || !d_qm || !d_rh || !d_rot || !d_depol || !d_he || (!d_albedo && !modis) || !d_sza ||
+      !d_raa || !d_vza || !d_radiance || !d_reflectance || !d_nfourier)
     return AA_EARG;
+  if (modis && (vector || !d_param || !d_br)) return AA_EARG;
   if (precision != 0 && precision != 1) return AA_EARG;
   const int S = vector ? 3 : 1;
   if (vector && (!d_q || !d_u || npol < 6)) return AA_EARG;
@@ -520,7 +567,8 @@ int aa_column_radiance_dev(aa_mie* t, int km, int nobs, int nch, int nq, const i
   W.N = nstreams; W.S = S; W.nmoments = std::min(2 * nstreams - 1, nmom);
   W.missing = missing;
   W.rot = d_rot; W.depol = d_depol; W.alpha = nullptr;
-  W.he = d_he; W.albedo = d_albedo; W.sza = d_sza; W.raa = d_raa; W.vza = d_vza;
+  W.he = d_he; W.albedo = modis ? nullptr : d_albedo; W.sza = d_sza; W.raa = d_raa; W.vza = d_vza;
+  W.kernel_wt = d_kernel_wt; W.kparam = d_param; W.br_out = d_br;
   W.flux = vector ? d_flux_factor : nullptr;
   W.nlayers = d_nlayers;
   W.radiance = d_radiance; W.reflectance = d_reflectance; W.qout = d_q; W.uout = d_u; W.nfourier = d_nfourier;
@@ -564,6 +612,41 @@ int aa_column_radiance_dev(aa_mie* t, int km, int nobs, int nch, int nq, const i
   }
   AA_CHECK(hipStreamSynchronize(st));
   return rcode;
+}
+
+}  // namespace
+}  // namespace aa
+
+extern "C" {
+
+int aa_column_radiance_dev(aa_mie* t, int km, int nobs, int nch, int nq, const int* tracer_idx, const int* chan_idx,
+                           const float* d_qm, const float* d_rh, int nmom, int npol, int nstreams, int vector,
+                           int precision, const int* d_nlayers, const int* d_nstreams_col, const double* d_rot,
+                           const double* d_depol, const double* d_pe, const double* d_he, const double* d_te,
+                           const double* d_albedo, const double* d_sza, const double* d_raa, const double* d_vza,
+                           const double* d_flux_factor, double missing, double* d_radiance, double* d_reflectance,
+                           double* d_q, double* d_u, int* d_nfourier, void* stream) {
+  (void)d_pe; (void)d_te;
+  if (!d_albedo) return AA_EARG;
+  return aa::column_radiance(t, km, nobs, nch, nq, tracer_idx, chan_idx, d_qm, d_rh, nmom, npol, nstreams, vector,
+                             precision, d_nlayers, d_nstreams_col, d_rot, d_depol, d_he, d_albedo, nullptr, nullptr,
+                             nullptr, d_sza, d_raa, d_vza, d_flux_factor, missing, d_radiance, d_reflectance, d_q,
+                             d_u, d_nfourier, stream);
+}
+
+int aa_column_radiance_modis_dev(aa_mie* t, int km, int nobs, int nch, int nq, const int* tracer_idx,
+                                 const int* chan_idx, const float* d_qm, const float* d_rh, int nmom, int nstreams,
+                                 int precision, const int* d_nlayers, const int* d_nstreams_col, const double* d_rot,
+                                 const double* d_depol, const double* d_pe, const double* d_he, const double* d_te,
+                                 const double* d_kernel_wt, const double* d_param, const double* d_sza,
+                                 const double* d_raa, const double* d_vza, double missing, double* d_radiance,
+                                 double* d_reflectance, double* d_br, int* d_nfourier, void* stream) {
+  (void)d_pe; (void)d_te;
+  if (!d_kernel_wt) return AA_EARG;
+  return aa::column_radiance(t, km, nobs, nch, nq, tracer_idx, chan_idx, d_qm, d_rh, nmom, 1, nstreams, 0, precision,
+                             d_nlayers, d_nstreams_col, d_rot, d_depol, d_he, nullptr, d_kernel_wt, d_param, d_br,
+                             d_sza, d_raa, d_vza, nullptr, missing, d_radiance, d_reflectance, nullptr, nullptr,
+                             d_nfourier, stream);
 }
 
 const char* aa_version(void) { return "aeroapps_amd 0.1 (gfx950)"; }

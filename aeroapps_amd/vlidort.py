@@ -73,6 +73,56 @@ def vector_lambert(channels, nstreams, plane_parallel, ROT, depol, alpha, tau, s
     return I, R, Q, U, int(rc[0])
 
 
+def scalar_landmodis(channels, nstreams, plane_parallel, ROT, depol, tau, ssa, g, pmom, pe, he, te, kernel_wt, param,
+                     solar_zenith, relat_azymuth, sensor_zenith, MISSING, verbose=0, precision=0,
+                     return_nfourier=False):
+    """VLIDORT_Scalar_LandMODIS (src/Components/rtms/geo_vlidort/VLIDORT_BRDF_MODIS.F90:35-202):
+    MODIS land-surface BRDF (kernel_wt[3, nch, nobs] = fiso, fgeo, fvol;
+    param[2, nch, nobs] = Li-sparse h/b, b/r).  Returns (radiance, reflectance,
+    BR, rc) with (nobs, nch) outputs; BR is the exact direct-bounce BRDF."""
+    channels, ROT, depol, tau, ssa, g, pmom, pe, he, te, kw, par, sza, raa, vza = map(
+        _f64, (channels, ROT, depol, tau, ssa, g, pmom, pe, he, te, kernel_wt, param, solar_zenith, relat_azymuth,
+               sensor_zenith))
+    km, nch, nobs = tau.shape
+    if pmom.ndim == 4:
+        pmom = _f64(pmom[..., None])
+    nmom, npol = pmom.shape[3], pmom.shape[4]
+    rad = np.zeros((nobs, nch), order="F")
+    refl = np.zeros((nobs, nch), order="F")
+    br = np.zeros((nobs, nch), order="F")
+    nf = np.zeros((nobs, nch), dtype=np.int32, order="F")
+    rc = np.zeros(1, dtype=np.int32)
+    e = lib().aa_vlidort_scalar_landmodis(km, nch, nobs, _ptr(channels), int(nstreams), int(bool(plane_parallel)),
+                                          nmom, npol, _ptr(ROT), _ptr(depol), _ptr(tau), _ptr(ssa), _ptr(g),
+                                          _ptr(pmom), _ptr(pe), _ptr(he), _ptr(te), _ptr(kw), _ptr(par), _ptr(sza),
+                                          _ptr(raa), _ptr(vza), float(MISSING), int(verbose), int(precision),
+                                          _ptr(rad), _ptr(refl), _ptr(br), _ptr(nf, _ip), _ptr(rc, _ip))
+    check(e, "aa_vlidort_scalar_landmodis")
+    if return_nfourier:
+        return rad, refl, br, int(rc[0]), nf
+    return rad, refl, br, int(rc[0])
+
+
+def vbrdf_modis(nstreams, sza, vza, raa, kernel_wt, param):
+    """The MODIS BRDF supplement alone for ngeom geometries (degrees), with
+    kernel_wt[3, ngeom] and param[2, ngeom]: returns a dict of F[g, M, i, j],
+    F0[g, M, i], UF[g, M, j], UF0[g, M], db[g], status[g]."""
+    sza, vza, raa = (np.ascontiguousarray(np.atleast_1d(a), dtype=np.float64) for a in (sza, vza, raa))
+    kw, par = _f64(kernel_wt), _f64(param)
+    ng, n, nm = sza.size, int(nstreams), 2 * int(nstreams)
+    F = np.zeros((nm, n, n, ng), order="F")
+    F0 = np.zeros((nm, n, ng), order="F")
+    UF = np.zeros((nm, n, ng), order="F")
+    UF0 = np.zeros((nm, ng), order="F")
+    db = np.zeros(ng)
+    st = np.zeros(ng, dtype=np.int32)
+    e = lib().aa_vbrdf_modis(n, ng, _ptr(sza), _ptr(vza), _ptr(raa), _ptr(kw), _ptr(par), _ptr(F), _ptr(F0),
+                             _ptr(UF), _ptr(UF0), _ptr(db), _ptr(st, _ip))
+    check(e, "aa_vbrdf_modis")
+    return dict(F=np.moveaxis(F, 3, 0), F0=np.moveaxis(F0, 2, 0), UF=np.moveaxis(UF, 2, 0), UF0=UF0.T.copy(), db=db,
+                status=st)
+
+
 def rot_calc(channels, pe, he, te, MISSING, verbose=0):
     """ROT_CALC (src/Components/rtms/leo_vlidort/VLIDORT_POLAR_py.F90:9-41 ->
     VLIDORT_ROT_CALC, src/Components/rtms/geo_vlidort/VLIDORT_ROT.F90:19-85):

@@ -129,6 +129,42 @@ int aa_vlidort_vector_lambert(int km, int nch, int nobs, const double* channels,
                               int verbose, int precision, double* radiance, double* reflectance, double* q,
                               double* u, int* nfourier, int* rc);
 
+/* ------------------------------------------------------------------ MODIS land surface (BRDF)
+ * Replaces VLIDORT_Scalar_LandMODIS (GEO/VLIDORT_BRDF_MODIS.F90:35-202): per
+ * (obs, channel) the BRDF supplement of VLIDORT_LANDMODIS_SingleGeom
+ * (V283/VLIDORT_SurfaceMod.F90:1376-1596; kernels Lambertian, Ross-thick,
+ * Li-sparse with weights kernel_wt(3,nch,nobs) = (fiso, fgeo, fvol) and
+ * Li-sparse parameters param(2,nch,nobs) = (h/b, b/r), 50 azimuth streams,
+ * white-sky-albedo check) followed by VLIDORT_Run_Scalar with that BRDF.
+ * Same arrays as aa_vlidort_scalar_lambert without albedo; BR(nobs,nch)
+ * receives the exact direct-bounce BRDF.  As the reference: BR < 0 gives
+ * radiance = reflectance = BR = -500 (VLIDORT not run), a missing geometry
+ * gives MISSING (BR = MISSING where the reference leaves it undefined), a
+ * MISSING kernel weight is not a skip (the reference's check only leaves its
+ * inner loop), nMom above 1000 gives rc = 5 with outputs untouched.  Scalar
+ * only (the reference's vector MODIS driver stops with rc = 9). */
+int aa_vlidort_scalar_landmodis(int km, int nch, int nobs, const double* channels, int nstreams,
+                                int plane_parallel, int nmom, int npol, const double* rot, const double* depol,
+                                const double* tau, const double* ssa, const double* g, const double* pmom,
+                                const double* pe, const double* he, const double* te, const double* kernel_wt,
+                                const double* param, const double* solar_zenith, const double* relat_azymuth,
+                                const double* sensor_zenith, double missing, int verbose, int precision,
+                                double* radiance, double* reflectance, double* br, int* nfourier, int* rc);
+
+/* The BRDF supplement alone (VBRDF_MAINMASTER under the
+ * VLIDORT_LANDMODIS_SingleGeom set-up, V283/vsup/vbrdf/vbrdf_sup_masters.f90:
+ * 1559-3293) for ngeom observation geometries (degrees) with their own
+ * kernel_wt(3,ngeom) and param(2,ngeom): the Fourier components the solver
+ * reads, in Fortran order per geometry g, brdf_f(0:2N-1,N,N,g) (incident
+ * stream j -> reflected i at (M,i,j)), brdf_f0(0:2N-1,N,g),
+ * user_brdf_f(0:2N-1,N,g), user_brdf_f0(0:2N-1,g), the exact direct bounce
+ * dbounce(g) and status(g) (4 = white-sky albedo outside (0,1]: the Li-sparse
+ * Fourier terms are left out, as the reference's supplement stops there).
+ * Host pointers, synchronous. */
+int aa_vbrdf_modis(int nstreams, int ngeom, const double* sza, const double* vza, const double* raa,
+                   const double* kernel_wt, const double* param, double* brdf_f, double* brdf_f0,
+                   double* user_brdf_f, double* user_brdf_f0, double* dbounce, int* status);
+
 /* ------------------------------------------------------------------ input preparation
  * The steps either side of the radiance path in a geo_vlidort run
  * (GEO/geo_vlidort_AI.F90:399-430): met fields -> edge variables and tracer
@@ -184,6 +220,19 @@ int aa_column_radiance_dev(aa_mie* t, int km, int nobs, int nch, int nq, const i
                            const double* d_vza, const double* d_flux_factor, double missing,
                            double* d_radiance, double* d_reflectance, double* d_q, double* d_u,
                            int* d_nfourier, void* stream);
+
+/* The fused device path over a MODIS land surface (the BRDF of
+ * aa_vlidort_scalar_landmodis computed on the device per item): d_kernel_wt
+ * (3,nch,nobs), d_param (2,nch,nobs) in place of albedo, d_br(nobs,nch) out.
+ * Scalar. */
+int aa_column_radiance_modis_dev(aa_mie* t, int km, int nobs, int nch, int nq, const int* tracer_idx,
+                                 const int* chan_idx, const float* d_qm, const float* d_rh, int nmom,
+                                 int nstreams, int precision, const int* d_nlayers, const int* d_nstreams_col,
+                                 const double* d_rot, const double* d_depol, const double* d_pe,
+                                 const double* d_he, const double* d_te, const double* d_kernel_wt,
+                                 const double* d_param, const double* d_sza, const double* d_raa,
+                                 const double* d_vza, double missing, double* d_radiance, double* d_reflectance,
+                                 double* d_br, int* d_nfourier, void* stream);
 
 /* Library version string. */
 const char* aa_version(void);

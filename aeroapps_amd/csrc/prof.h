@@ -11,6 +11,7 @@ enum ProfKind {
   PROF_ITEM,        // per-column recursions
   PROF_LAYER,       // per-layer eigensystem + particular solutions
   PROF_BVP,         // band LU + back-substitution + source + convergence
+  PROF_BRDF,        // BRDF supplement (MODIS land surface)
   PROF_KINDS
 };
 bool prof_on();

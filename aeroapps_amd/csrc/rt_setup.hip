@@ -703,7 +703,11 @@ int rt_setup(const RTWork& W, int nitems, int* d_active, int* d_nactive, hipStre
   prof_begin(PROF_CHECK, st);
   hipLaunchKernelGGL(k_check, dim3(gi), dim3(B), 0, st, W, nitems);
   prof_end(PROF_CHECK, st);
-  if (W.kernel_wt && rt_brdf(W, nitems, st)) return -2;
+  if (W.kernel_wt) {
+    prof_begin(PROF_BRDF, st);
+    if (rt_brdf(W, nitems, st)) return -2;
+    prof_end(PROF_BRDF, st);
+  }
   prof_begin(PROF_OPTICS, st);
   if (W.S == 1) {
     if (W.optics_f32) hipLaunchKernelGGL((k_optics<1, float>), dim3(gl), dim3(B), 0, st, W, nitems);

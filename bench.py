@@ -30,7 +30,7 @@ from aeroapps_amd import synthetic as S  # noqa: E402
 METRIC = "columns/sec at 72 layers/12 streams, mixed vs FP64 GPU; p99 granule latency"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
-PROF_NAMES = ["mie", "check", "optics", "geom", "item", "layer", "bvp"]
+PROF_NAMES = ["mie", "check", "optics", "geom", "item", "layer", "bvp", "brdf"]
 # HBM bytes per item-term (read, written) of the radiance kernels at N=6
 # scalar, from FETCH_SIZE/WRITE_SIZE counter runs scaled by a copy-kernel
 # calibration (profiles/hbm_traffic_40k_columns_4ch_12streams.md)
@@ -68,10 +68,10 @@ def bytes_per_column(km, nq, nch, vector):
 
 # ----------------------------------------------------------------- CPU baseline
 def _cpu_worker(args):
-    (c0, nc, wl, kind) = args
+    (c0, nc, wl, kind, modis) = args
     import oracle
     from oracle import chem_mie as CM
-    drv = oracle.ref() if kind == "reference" else oracle.port()
+    drv = (oracle.ref_modis() if modis else oracle.ref()) if kind == "reference" else oracle.port()
     ch = wl["channels"]
     t0 = time.perf_counter()
     cols = S.make_columns(nc, wl["km"], ch, seed=7, col0=c0)
@@ -81,18 +81,23 @@ def _cpu_worker(args):
     tau, ssa, g, pm, rc = CM.get_aop_vector(tabs, ch, tabs[0].lambda_m, S.TRACERS, cols.qm, cols.rh, wl["nmom"],
                                             wl["npol"])
     f = lambda a: np.asarray(a, dtype=np.float64)
-    drv.scalar_lambert(wl["km"], cols.channels, wl["nstreams"], wl["nmom"], cols.rot, cols.depol, f(tau), f(ssa),
-                       f(g), f(pm), cols.pe, cols.he, cols.te, cols.albedo, cols.sza, cols.raa, cols.vza)
+    if modis:
+        kw, par = S.make_modis_kernels(nc, len(ch), seed=7, col0=c0)
+        drv.scalar_landmodis(wl["km"], cols.channels, wl["nstreams"], wl["nmom"], cols.rot, cols.depol, f(tau),
+                             f(ssa), f(g), f(pm), cols.pe, cols.he, cols.te, kw, par, cols.sza, cols.raa, cols.vza)
+    else:
+        drv.scalar_lambert(wl["km"], cols.channels, wl["nstreams"], wl["nmom"], cols.rot, cols.depol, f(tau), f(ssa),
+                           f(g), f(pm), cols.pe, cols.he, cols.te, cols.albedo, cols.sza, cols.raa, cols.vza)
     return time.perf_counter() - t0, gen
 
 
-def cpu_baseline(wl, ncols, procs):
+def cpu_baseline(wl, ncols, procs, modis=False):
     import oracle
     kind = "reference" if os.path.exists(oracle.REF_SO) else "port"
-    if kind == "port" and not os.path.exists(oracle.PORT_SO):
+    if kind == "port" and (modis or not os.path.exists(oracle.PORT_SO)):
         return None
     per = max(1, ncols // procs)
-    jobs = [(i * per, per, wl, kind) for i in range(procs)]
+    jobs = [(i * per, per, wl, kind, modis) for i in range(procs)]
     ctx = mp.get_context("fork")
     t0 = time.perf_counter()
     with ctx.Pool(procs) as pool:
@@ -107,7 +112,8 @@ def cpu_baseline(wl, ncols, procs):
         "kind": kind,
         "sample": (f"{ncol} columns ({per} per process, {procs} processes, one core each) of the same workload: "
                    f"float32 Chem_Mie restatement + {'reference VLIDORT 2.8.3 (flang -O3)' if kind == 'reference' else 'C++ restatement'} "
-                   f"scalar Lambertian driver, {len(wl['channels'])} channels, N={wl['nstreams']}, nmom={wl['nmom']}; "
+                   f"scalar {'MODIS land-surface' if modis else 'Lambertian'} driver, {len(wl['channels'])} channels, "
+                   f"N={wl['nstreams']}, nmom={wl['nmom']}; "
                    f"slowest process {busy:.1f} s, pool wall {wall:.1f} s"),
     }
 
@@ -127,6 +133,8 @@ def main():
     ap.add_argument("--serving-granules", type=int, default=40,
                     help="granules of the serving-loop phase (0 skips it)")
     ap.add_argument("--serving-load", type=float, default=0.7, help="offered load as a fraction of peak")
+    ap.add_argument("--surface", choices=("lambert", "modis"), default="lambert",
+                    help="surface model: Lambertian albedo or the MODIS land BRDF (scalar workloads)")
     ap.add_argument("--host-cols", type=int, default=20000,
                     help="columns of the host-pointer (PCIe-inclusive) drop-in phase (0 skips it)")
     args = ap.parse_args()
@@ -147,7 +155,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         procs = args.cpu_procs or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(wl, args.cpu_cols, procs)
+        cpu = cpu_baseline(wl, args.cpu_cols, procs, modis=args.surface == "modis")
 
     import torch
     from aeroapps_amd import _lib
@@ -198,7 +206,24 @@ def main():
     stream = torch.cuda.current_stream(dev)
     ptr = lambda t: t.data_ptr() if t is not None else None
 
+    modis = args.surface == "modis"
+    if modis:
+        assert not vector, "the MODIS land surface is scalar"
+        kw_h, par_h = S.make_modis_kernels(ncol, nch, seed=7)
+        d_kw = torch.as_tensor(kw_h, dtype=torch.float64, device=dev).contiguous()    # (3,nch,nobs) Fortran
+        d_par = torch.as_tensor(par_h, dtype=torch.float64, device=dev).contiguous()  # (2,nch,nobs) Fortran
+        br = torch.empty_like(rad)
+        del kw_h, par_h
+
     def run_step(precision):
+        if modis:
+            e = L.aa_column_radiance_modis_dev(mie.handle, km, ncol, nch, nq, ptr(tr_idx), ptr(ch_idx), ptr(qm),
+                                               ptr(rh), nmom, N, precision, ptr(d_nl), ptr(d_nstr), ptr(rot),
+                                               ptr(depol), ptr(pe), ptr(he), ptr(te), ptr(d_kw), ptr(d_par), ptr(sza),
+                                               ptr(raa), ptr(vza), S.MISSING, ptr(rad), ptr(refl), ptr(br), ptr(nf),
+                                               stream.cuda_stream)
+            _lib.check(e, "aa_column_radiance_modis_dev")
+            return
         e = L.aa_column_radiance_dev(mie.handle, km, ncol, nch, nq, ptr(tr_idx), ptr(ch_idx), ptr(qm), ptr(rh), nmom,
                                      npol, N, int(vector), precision, ptr(d_nl), ptr(d_nstr), ptr(rot), ptr(depol),
                                      ptr(pe),
@@ -225,8 +250,8 @@ def main():
         msa = (ctypes.c_double * 8)()
         lna = (ctypes.c_longlong * 8)()
         _lib.check(L.aa_profile_read(msa, lna, 8), "aa_profile_read")
-        kms = {PROF_NAMES[k]: msa[k] for k in range(7)}
-        kln = {PROF_NAMES[k]: lna[k] for k in range(7)}
+        kms = {PROF_NAMES[k]: msa[k] for k in range(8) if k < 7 or modis}
+        kln = {PROF_NAMES[k]: lna[k] for k in range(8) if k < 7 or modis}
         ms_per_step = el * 1000.0 / args.steps
         nfh = nf.cpu().numpy().ravel().copy()
         radh = rad.cpu().numpy().ravel().copy()
@@ -261,7 +286,7 @@ def main():
     item_terms = float(nfh[ok].sum()) * args.steps
     tr_key = (dom, N, Sx)
     traffic = None
-    if tr_key in TRAFFIC_PER_ITEM_TERM and wl["nstreams"] is not None and not uneven:
+    if tr_key in TRAFFIC_PER_ITEM_TERM and wl["nstreams"] is not None and not uneven and not modis:
         rd, wr = TRAFFIC_PER_ITEM_TERM[tr_key]
         traffic = round((rd + wr) * item_terms / dom_launches, 0)
     el = main_run["el"]
@@ -285,10 +310,12 @@ def main():
                                f"{list(wl['channels'])} nm, "
                                f"{2 * N if wl['nstreams'] is not None else '/'.join(str(2 * c) for c in wl['stream_choices'])}"
                                f" streams, {nmom} moments, {nq} tracers, "
-                               f"{'vector' if vector else 'scalar'} Lambertian",
+                               f"{'vector' if vector else 'scalar'} "
+                               f"{'MODIS land BRDF (Lambertian + Ross-thick + Li-sparse)' if modis else 'Lambertian'}",
                    "precision": "mixed (FP32 optics/phase-matrix contraction on MFMA/source assembly; FP64 "
                                 "eigensystem and band elimination)" if rmx is not None else "fp64",
-                   "path": "aa_column_radiance_dev (Chem_Mie optics + VLIDORT radiance, device-resident)"},
+                   "path": ("aa_column_radiance_modis_dev" if modis else "aa_column_radiance_dev") +
+                           " (Chem_Mie optics + VLIDORT radiance, device-resident)"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                      "traffic_GBs": (round(traffic * dom_launches / (dom_ms / 1000.0) / 1e9, 1) if traffic else None),
@@ -319,7 +346,7 @@ def main():
         out["fp64_vs_cpu"] = round(r64["value"] / cpu["value"], 1)
 
     # serving loop: Poisson granules at a fraction of the measured peak
-    if args.serving_granules > 0 and not vector and wl["nstreams"] is not None and not uneven:
+    if args.serving_granules > 0 and not vector and wl["nstreams"] is not None and not uneven and not modis:
         from aeroapps_amd import serving as SV
         del qm, rh, rot, he, pe, te, albedo, sza, vza, raa, flux, rad, refl, nf
         torch.cuda.empty_cache()
@@ -338,7 +365,7 @@ def main():
                           "throughput_cols_per_s": round(st["throughput_cols_per_s"], 1), "slots": st["slots"],
                           "granule_columns": "log-uniform 1000-200000, 1-4 channels, 72 layers, 12 streams"}
     # drop-in host-pointer path (PCIe-inclusive): getAOPvector + VLIDORT_Scalar_Lambert
-    if args.host_cols > 0 and not vector and wl["nstreams"] is not None and not uneven:
+    if args.host_cols > 0 and not vector and wl["nstreams"] is not None and not uneven and not modis:
         from aeroapps_amd import vlidort as AV
         hc = min(args.host_cols, ncol)
         cols_h = S.make_columns(hc, km, wl["channels"], seed=7)

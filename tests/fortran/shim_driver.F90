@@ -1,18 +1,20 @@
 ! Test driver of the Fortran drop-in shim: reads a raw little-endian input
 ! file written by tests/test_gpu_fortran.py, calls the reference-named
 ! procedures (VLIDORT_Scalar_Lambert / VLIDORT_Vector_Lambert from module
-! VLIDORT_LAMBERT, VLIDORT_getAOPvector with a Chem_Mie object) and writes
+! VLIDORT_LAMBERT, VLIDORT_Scalar_LandMODIS from module VLIDORT_BRDF_MODIS,
+! VLIDORT_getAOPvector with a Chem_Mie object) and writes
 ! their outputs to a raw file.
 program shim_driver
   use Chem_MieMod
   use VLIDORT_LAMBERT
+  use VLIDORT_BRDF_MODIS
   implicit none
   character(len=512) :: fin, fout
   integer :: mode, km, nch, nobs, nstreams, nmom, npol, ipp, rc, u, i
   real*8 :: missing
   real*8, allocatable :: channels(:), rot(:,:,:), depol(:), alpha(:,:,:), tau(:,:,:), ssa(:,:,:), g(:,:,:)
   real*8, allocatable :: pmom(:,:,:,:,:), pe(:,:), he(:,:), te(:,:), albedo(:,:), sza(:), raa(:), vza(:)
-  real*8, allocatable :: ff(:,:), rad(:,:), refl(:,:), q(:,:), uu(:,:)
+  real*8, allocatable :: ff(:,:), rad(:,:), refl(:,:), q(:,:), uu(:,:), kw(:,:,:), kp(:,:,:), br(:,:)
   ! Mie mode
   integer :: nq, ntab, nrh, nbin, t, b
   type(Chem_Mie) :: mie
@@ -25,7 +27,22 @@ program shim_driver
   call get_command_argument(2, fout)
   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
   read(u) mode
-  if (mode == 1 .or. mode == 2) then
+  if (mode == 4) then
+    ! VLIDORT_Scalar_LandMODIS
+    read(u) km, nch, nobs, nstreams, nmom, npol, ipp
+    read(u) missing
+    allocate(channels(nch), rot(km,nobs,nch), depol(nch), tau(km,nch,nobs), ssa(km,nch,nobs), g(km,nch,nobs))
+    allocate(pmom(km,nch,nobs,nmom,npol), pe(km+1,nobs), he(km+1,nobs), te(km+1,nobs))
+    allocate(kw(3,nch,nobs), kp(2,nch,nobs), sza(nobs), raa(nobs), vza(nobs))
+    allocate(rad(nobs,nch), refl(nobs,nch), br(nobs,nch))
+    read(u) channels, rot, depol, tau, ssa, g, pmom, pe, he, te, kw, kp, sza, raa, vza
+    close(u)
+    call VLIDORT_Scalar_LandMODIS(km, nch, nobs, channels, nstreams, ipp /= 0, nmom, npol, rot, depol, tau, ssa, &
+                                  g, pmom, pe, he, te, kw, kp, sza, raa, vza, missing, 0, rad, refl, br, rc)
+    open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+    write(u) rc, rad, refl, br
+    close(u)
+  else if (mode == 1 .or. mode == 2) then
     read(u) km, nch, nobs, nstreams, nmom, npol, ipp
     read(u) missing
     allocate(channels(nch), rot(km,nobs,nch), depol(nch), alpha(km,nobs,nch), tau(km,nch,nobs), ssa(km,nch,nobs))

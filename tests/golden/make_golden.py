@@ -8,7 +8,7 @@ oracle.chem_mie; both inputs and reference outputs are stored, so the tests
 do not depend on regenerating them bit for bit.
 
 Run here (the reference is only available in this container):
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 """
 import os
 import sys
@@ -33,6 +33,9 @@ CASES = {
     "scalar_n4_pp": (4, (550.0,), 4, 32, False, True, False),
     "vector_n4": (4, (550.0, 870.0), 4, 32, True, False, False),
     "vector_n2": (4, (550.0,), 2, 32, True, False, False),
+    # GEOS optics-table moment counts (nmom up to 300, npol 6)
+    "scalar_n6_nmom300": (3, (550.0, 870.0), 6, 300, False, False, False),
+    "vector_n4_nmom300": (3, (550.0,), 4, 300, True, False, False),
 }
 
 
@@ -75,7 +78,10 @@ def main():
     ref = oracle.ref()
     if ref is None:
         raise SystemExit("reference library not built: make -C oracle -f Makefile.ref")
+    only = set(sys.argv[1:])
     for name, (ncol, ch, N, nmom, vector, pp, edge) in CASES.items():
+        if only and name not in only:
+            continue
         d = make_inputs(ncol, ch, nmom, vector, edge)
         out = run_driver(ref, d, N, vector, pp)
         meta = dict(nstreams=N, vector=int(vector), plane_parallel=int(pp))

@@ -24,6 +24,9 @@ _cp = ctypes.c_char_p
 SIGNATURES = {
     "aa_mie_create": (_ci, [ctypes.POINTER(_vp), _ci, _fp]),
     "aa_mie_add_table": (_ci, [_vp, _ci, _ci, _ci, _ci, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _cp]),
+    "aa_mie_add_table_native": (_ci, [_vp, _ci, _dp, _ci, _ci, _ci, _ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                      _dp, _dp, _dp, _dp, _dp, _cp, _ip]),
+    "aa_mie_query_extra": (_ci, [_vp, _ci, _cf, _ci, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp]),
     "aa_mie_query_idx": (_ci, [_vp, _cp]),
     "aa_mie_query": (_ci, [_vp, _ci, _cf, _ci, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _ci, _ci, _fp]),
     "aa_mie_dims": (_ci, [_vp, _ip, _ip, _ip, _ip]),

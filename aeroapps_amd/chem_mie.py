@@ -45,6 +45,42 @@ class MieTables:
         check(lib().aa_mie_add_table(self.handle, nrh, nbin, nmom, npol, _p(rh), _p(bext), _p(bsca), _p(bbck),
                                      _p(g), _p(refr), _p(refi), _p(pmom), names), "aa_mie_add_table")
 
+    def add_table_native(self, lambda_table, rh, reff, bext, bsca, bbck, g, bin_names: Sequence[str], nmom: int = 0,
+                         pback=None, pmom=None, gf=None, rhop=None, refr=None, refi=None) -> int:
+        """One species table from the variables of its optics-table file
+        (Chem_MieTableRead, src/Shared/GMAO_Shared/Chem_Base/Chem_MieTableMod.F90:
+        238-719): float64 arrays in the file's Fortran shapes, lambda_table[nch_table]
+        [m], rh[nrh], reff/gf/rhop[nrh, nbin], bext..refi[nch_table, nrh, nbin],
+        pback[nch_table, nrh, nbin, npol], pmom[nch_table, nrh, nbin, nmom_table,
+        npol]; absent variables as None.  Keeps ``nmom`` moments.  Returns rc (99:
+        the file has fewer moments than asked)."""
+        from ._lib import _dp
+        f64 = lambda a: None if a is None else np.asfortranarray(np.asarray(a, dtype=np.float64))
+        lam, rh, reff, bext, bsca, bbck, g, pback, pmom, gf, rhop, refr, refi = map(
+            f64, (lambda_table, rh, reff, bext, bsca, bbck, g, pback, pmom, gf, rhop, refr, refi))
+        nch_t, nrh, nbin = bext.shape
+        nmom_t = pmom.shape[3] if pmom is not None else 0
+        npol = pmom.shape[4] if pmom is not None else (pback.shape[3] if pback is not None else 6)
+        names = b"".join(n.encode().ljust(16)[:16] for n in bin_names)
+        assert len(bin_names) == nbin
+        pd = lambda a: None if a is None else a.ctypes.data_as(_dp)
+        rc = np.zeros(1, dtype=np.int32)
+        check(lib().aa_mie_add_table_native(self.handle, nch_t, pd(lam), nrh, nbin, nmom_t, npol, int(nmom), pd(rh),
+                                            pd(reff), pd(bext), pd(bsca), pd(bbck), pd(g), pd(pback), pd(pmom),
+                                            pd(gf), pd(rhop), pd(refr), pd(refi), names, _p(rc, _ip)),
+              "aa_mie_add_table_native")
+        return int(rc[0])
+
+    def query_extra(self, idx: int, channel: float, rh):
+        """rEff [um], p11, p22, gf, rhod, vol, area of Chem_MieQuery for arrays of
+        relative humidity (tables added with ``add_table_native``)."""
+        r = np.ascontiguousarray(rh, dtype=np.float32).ravel()
+        keys = ("reff", "p11", "p22", "gf", "rhod", "vol", "area")
+        outs = {k: np.zeros(len(r), dtype=np.float32) for k in keys}
+        check(lib().aa_mie_query_extra(self.handle, idx, float(channel), len(r), _p(r), *[_p(outs[k]) for k in keys]),
+              "aa_mie_query_extra")
+        return outs
+
     @classmethod
     def from_synthetic(cls, tables) -> "MieTables":
         """Tables built by aeroapps_amd.synthetic.make_mie_tables."""
@@ -171,3 +207,54 @@ def calc_qm(q, delp):
     qm = np.zeros((km, nq, nobs), dtype=np.float32, order="F")
     check(lib().aa_calc_qm(km, nq, nobs, _p(q), _p(dp), _p(qm)), "aa_calc_qm")
     return qm
+
+
+def parse_mie_rc(text: str) -> dict:
+    """The resource file Chem_MieCreateFromRC reads with i90
+    (src/Shared/GMAO_Shared/Chem_Base/Chem_MieMod.F90:200-300): ``label: value``
+    lines, ``#`` comments, tables of values after ``label::`` until ``::``.
+    Returns the labels with their value strings (tables as lists of rows)."""
+    out, table, rows = {}, None, []
+    for raw in text.splitlines():
+        line = raw.split("#", 1)[0].strip()
+        if table is not None:
+            if line == "::":
+                out[table] = rows
+                table, rows = None, []
+            elif line:
+                rows.append(line.split())
+            continue
+        if not line:
+            continue
+        if line.endswith("::"):
+            table = line[:-2].strip()
+            continue
+        if ":" in line:
+            k, v = line.split(":", 1)
+            out[k.strip()] = v.strip()
+    return out
+
+
+def mie_rc_settings(text: str) -> dict:
+    """The settings Chem_MieCreateFromRC takes from a Chem_Mie resource file
+    (src/Shared/GMAO_Shared/Chem_Base/Chem_MieMod.F90:200-300): ``n_channels``,
+    ``n_moments`` (0 when absent), ``r_channels`` [m] and the optics-table file
+    ``filename_optical_properties_XX`` of DU, SS, BC, OC, SU, NI, BRC."""
+    d = parse_mie_rc(text)
+    nch = int(d["n_channels"])
+    ch = [float(v) for v in d["r_channels"].replace(",", " ").split()][:nch]
+    files = {sp: d.get(f"filename_optical_properties_{sp}") for sp in ("DU", "SS", "BC", "OC", "SU", "NI", "BRC")}
+    return dict(n_channels=nch, channels_m=ch, nmom=int(d.get("n_moments", 0)), optics_files=files)
+
+
+def species_bin_names(species: str, tracer_names: Sequence[str]) -> list:
+    """Tracers a species table serves, bin by bin, as Chem_MieCreateFromRC
+    assigns them (src/Shared/GMAO_Shared/Chem_Base/Chem_MieMod.F90:338-400):
+    every tracer of the species in order, except that the sulfate table serves
+    only names starting so4/SO4/sul/SUL and the nitrate table only no3/NO3."""
+    names = [str(n) for n in tracer_names]
+    if species.upper() == "SU":
+        return [n for n in names if n[:3] in ("so4", "SO4", "sul", "SUL")]
+    if species.upper() == "NI":
+        return [n for n in names if n[:3] in ("no3", "NO3")]
+    return names

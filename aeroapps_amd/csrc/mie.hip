@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -233,6 +234,10 @@ struct aa_mie {
   std::vector<float> bext, bsca, g, bbck, refr, refi, pmom;   // [tr][ch][rh], pmom [tr][ch][rh][pol][mom]
   std::vector<int> rhi;
   std::vector<float> rha;
+  // per-tracer fields of tables built from the file arrays (aa_mie_add_table_native):
+  // reff/gf/rhop/rhod/vol/area [rh], pback [ch][rh][pol]; empty for pre-interpolated tables
+  std::vector<std::vector<float>> reff_t, gf_t, rhop_t, rhod_t, vol_t, area_t, pback_t;
+  std::vector<int> npback_t;
   // device copies
   void* dmem[12] = {nullptr};
   aa::MieDev dev;
@@ -399,8 +404,164 @@ int aa_mie_add_table(aa_mie* t, int nrh, int nbin, int nmom, int npol, const flo
     build_rh_map(rh, nrh, hi.data(), ha.data());
     t->rhi.insert(t->rhi.end(), hi.begin(), hi.end());
     t->rha.insert(t->rha.end(), ha.begin(), ha.end());
+    for (auto* v : {&t->reff_t, &t->gf_t, &t->rhop_t, &t->rhod_t, &t->vol_t, &t->area_t, &t->pback_t})
+      v->emplace_back();
+    t->npback_t.push_back(0);
   }
   t->dirty = true;
+  return 0;
+}
+
+// Linear interpolation of Chem_MieTableRead's polint (CB/Chem_MieTableMod.F90:
+// 671-717): float64 table abscissae x and values y, float32 target; slope
+// and dx are default reals (float32), the product slope*dx is float32 and the
+// sum with y(i) is float64, rounded to float32.  Below x(1) the first
+// interval's slope is used from x(1); above x(n) the value is y(n).
+static float polint_ref(const double* x, const double* y, int n, size_t ystride, float xw) {
+  int i = 0;
+  if ((double)xw < x[0]) i = 1;
+  if ((double)xw > x[n - 1]) i = n;
+  if (i == 0)
+    for (int j = 1; j <= n; ++j)
+      if ((double)xw >= x[j - 1]) i = j;
+  if (i == 0) i = 1;   // unordered target (NaN): the reference leaves i = 0; clamp to stay in bounds
+  float slope = 0.0f;
+  if (i != n) slope = (float)((y[(size_t)i * ystride] - y[(size_t)(i - 1) * ystride]) / (x[i] - x[i - 1]));
+  const float dx = (float)((double)xw - x[i - 1]);
+  const float sd = slope * dx;
+  return (float)(y[(size_t)(i - 1) * ystride] + (double)sd);
+}
+
+// Chem_MieTableRead (CB/Chem_MieTableMod.F90:238-719) from the arrays of an
+// optics-table file: fill values of absent fields, derived volume and area,
+// polint to the table set's channels, RH map; then the table is added as by
+// aa_mie_add_table and the per-bin fields Chem_MieQuery can return are kept.
+int aa_mie_add_table_native(aa_mie* t, int nch_table, const double* lambda_table, int nrh, int nbin, int nmom_table,
+                            int npol_table, int nmom, const double* rh, const double* reff, const double* bext,
+                            const double* bsca, const double* bbck, const double* g, const double* pback,
+                            const double* pmom, const double* gf, const double* rhop, const double* refr,
+                            const double* refi, const char* bin_names, int* rc) {
+  if (!t || !rc || nch_table <= 0 || nrh <= 0 || nbin <= 0 || npol_table <= 0 || npol_table > 6 || nmom < 0 ||
+      !lambda_table || !rh || !reff || !bext || !bsca || !bbck || !g || !bin_names)
+    return -1;
+  *rc = 0;
+  if (nmom > 0 && nmom > nmom_table) { *rc = 99; return 0; }   // :296-301
+  if (nmom > 0 && !pmom) return -1;
+  const int nch = t->nch;
+  const size_t nl = (size_t)nch_table, n3 = nl * nrh * nbin, n2 = (size_t)nrh * nbin;
+  // fields of the file, with the reference's fill values where absent (:405-470)
+  std::vector<double> pb(n3 * npol_table, 1.0), gfv(n2, -999.0), rhp(n2, -999.0), rhd(n2, -999.0);
+  std::vector<double> rr(n3, -999.0), ri(n3, -999.0);
+  if (pback) pb.assign(pback, pback + n3 * npol_table);
+  if (gf) gfv.assign(gf, gf + n2);
+  if (rhop) {
+    rhp.assign(rhop, rhop + n2);
+    // rhod is read from the same 'rhop' variable, first RH row broadcast (:433-444)
+    for (int j = 0; j < nbin; ++j)
+      for (int i = 0; i < nrh; ++i) rhd[(size_t)j * nrh + i] = rhop[(size_t)j * nrh];
+  }
+  if (refr) rr.assign(refr, refr + n3);
+  if (refi) for (size_t k = 0; k < n3; ++k) ri[k] = std::fabs(refi[k]);
+  // wet volume and cross-section area (:487-491)
+  std::vector<double> vol(n2), area(n2);
+  for (size_t k = 0; k < n2; ++k) {
+    vol[k] = gfv[k] * gfv[k] * gfv[k] / rhd[k];
+    area[k] = vol[k] / ((double)(4.0f / 3.0f) * reff[k]);
+  }
+  // polint to the channels (:560-589); outputs in aa_mie_add_table's layouts
+  const size_t o3 = (size_t)nrh * nch * nbin;
+  std::vector<float> ob(o3), os(o3), obb(o3), og(o3), orr(o3), ori(o3), opm(o3 * (size_t)std::max(nmom, 0) * npol_table);
+  std::vector<float> opb(o3 * npol_table), rhf(nrh);
+  for (int i = 0; i < nrh; ++i) rhf[i] = (float)rh[i];
+  for (int j = 0; j < nbin; ++j)
+    for (int i = 0; i < nrh; ++i)
+      for (int c = 0; c < nch; ++c) {
+        const float xw = t->channels_m[c];
+        const size_t src = ((size_t)j * nrh + i) * nl;        // (nch_table, nrh, nbin)
+        const size_t dst = ((size_t)j * nch + c) * nrh + i;   // (nrh, nch, nbin)
+        ob[dst] = polint_ref(lambda_table, bext + src, nch_table, 1, xw);
+        os[dst] = polint_ref(lambda_table, bsca + src, nch_table, 1, xw);
+        obb[dst] = polint_ref(lambda_table, bbck + src, nch_table, 1, xw);
+        og[dst] = polint_ref(lambda_table, g + src, nch_table, 1, xw);
+        orr[dst] = polint_ref(lambda_table, rr.data() + src, nch_table, 1, xw);
+        ori[dst] = polint_ref(lambda_table, ri.data() + src, nch_table, 1, xw);
+        for (int p = 0; p < npol_table; ++p)
+          opb[dst + o3 * p] = polint_ref(lambda_table, pb.data() + src + n3 * p, nch_table, 1, xw);
+        for (int m = 0; m < nmom; ++m)
+          for (int p = 0; p < npol_table; ++p)   // file pmom(nch_table, nrh, nbin, nmom_table, npol)
+            opm[dst + o3 * ((size_t)p * nmom + m)] =
+                polint_ref(lambda_table, pmom + src + n3 * ((size_t)p * nmom_table + m), nch_table, 1, xw);
+      }
+  const int old_tr = (int)t->names.size();
+  int e = aa_mie_add_table(t, nrh, nbin, nmom, npol_table, rhf.data(), ob.data(), os.data(), obb.data(), og.data(),
+                           orr.data(), ori.data(), nmom > 0 ? opm.data() : nullptr, bin_names);
+  if (e) return e;
+  for (int j = 0; j < nbin; ++j) {
+    const int tr = old_tr + j;
+    auto f32 = [&](const double* a) {
+      std::vector<float> v(nrh);
+      for (int i = 0; i < nrh; ++i) v[i] = (float)a[(size_t)j * nrh + i];
+      return v;
+    };
+    t->reff_t[tr] = f32(reff); t->gf_t[tr] = f32(gfv.data()); t->rhop_t[tr] = f32(rhp.data());
+    t->rhod_t[tr] = f32(rhd.data()); t->vol_t[tr] = f32(vol.data()); t->area_t[tr] = f32(area.data());
+    std::vector<float> pbt((size_t)nch * nrh * npol_table);
+    for (int c = 0; c < nch; ++c)
+      for (int i = 0; i < nrh; ++i)
+        for (int p = 0; p < npol_table; ++p)
+          pbt[((size_t)c * nrh + i) * npol_table + p] = opb[((size_t)j * nch + c) * nrh + i + o3 * p];
+    t->pback_t[tr] = std::move(pbt);
+    t->npback_t[tr] = npol_table;
+  }
+  return 0;
+}
+
+// The Chem_MieQuery outputs beyond the optics (CB/Chem_MieMod.F90:885-943):
+// rEff [um], p11, p22 (pback polarisations 1 and 5), growth factor, dry
+// density (first RH), wet volume and area, interpolated in RH with the same
+// float32 arithmetic as the optics.  Only for tables built by
+// aa_mie_add_table_native.  (The reference's rhop output is never assigned
+// there, so it is not offered.)
+int aa_mie_query_extra(const aa_mie* t, int idx, float channel, int n, const float* rh, float* reff, float* p11,
+                       float* p22, float* gf, float* rhod, float* vol, float* area) {
+  if (!t || idx < 0 || idx >= (int)t->names.size() || n < 0 || !rh) return -1;
+  if (t->reff_t[idx].empty()) return -1;
+  const int ich = (int)lrintf(channel) - 1;
+  if (ich < 0 || ich >= t->nch) return -1;
+  const int np = t->npback_t[idx], nrh = t->nrh_t[idx];
+  if ((p11 || p22) && np < 5) return -1;
+  const int* hi = t->rhi.data() + (size_t)idx * aa::NRH_MAP;
+  const float* ha = t->rha.data() + (size_t)idx * aa::NRH_MAP;
+  const std::vector<float>& pb = t->pback_t[idx];
+  for (int k = 0; k < n; ++k) {
+    float rhu = std::min(rh[k], 0.99f);
+    int isnap = (int)((rhu + 0.001f) * 1000.0f);
+    if (isnap < 1) isnap = 1;
+    const float arh = ha[isnap - 1];
+    const int irh = hi[isnap - 1] - 1;
+    const int irhp1 = std::min(irh + 1, nrh - 1);
+    const float w0 = 1.0f - arh;
+    auto lin = [&](const std::vector<float>& v) {
+      const float a = v[irh] * w0;
+      const float b = v[irhp1] * arh;
+      return a + b;
+    };
+    if (reff) { const float r = lin(t->reff_t[idx]); reff[k] = 1.0e6f * r; }
+    if (p11) {
+      const float a = pb[((size_t)ich * nrh + irh) * np + 0] * w0;
+      const float b = pb[((size_t)ich * nrh + irhp1) * np + 0] * arh;
+      p11[k] = a + b;
+    }
+    if (p22) {
+      const float a = pb[((size_t)ich * nrh + irh) * np + 4] * w0;
+      const float b = pb[((size_t)ich * nrh + irhp1) * np + 4] * arh;
+      p22[k] = a + b;
+    }
+    if (gf) gf[k] = lin(t->gf_t[idx]);
+    if (rhod) rhod[k] = t->rhod_t[idx][0];
+    if (vol) vol[k] = lin(t->vol_t[idx]);
+    if (area) area[k] = lin(t->area_t[idx]);
+  }
   return 0;
 }
 

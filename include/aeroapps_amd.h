@@ -59,6 +59,32 @@ int aa_mie_add_table(aa_mie* t, int nrh, int nbin, int nmom, int npol, const flo
                      const float* bext, const float* bsca, const float* bbck, const float* g,
                      const float* refr, const float* refi, const float* pmom, const char* bin_names);
 
+/* Adds one species table from the arrays of its optics-table file, doing
+ * what Chem_MieTableRead (CB/Chem_MieTableMod.F90:238-719) does after the
+ * netCDF reads: fill values of absent variables (pback 1, growth_factor /
+ * rhop / refreal / refimag -999; refimag made non-negative; rhod taken from
+ * the first RH row of 'rhop'), wet volume gf^3/rhod and area
+ * vol/(4/3 reff), the polint linear interpolation (float32 slope and offset)
+ * from the file wavelengths lambda_table [m] to the table set's channels, and
+ * the RH map.  File layouts (Fortran, float64): rh(nrh), reff/gf/rhop(nrh,nbin),
+ * bext/bsca/bbck/g/refr/refi(nch_table,nrh,nbin),
+ * pback(nch_table,nrh,nbin,npol_table), pmom(nch_table,nrh,nbin,nmom_table,
+ * npol_table); optional ones may be NULL (pmom only when nmom = 0).  nmom is
+ * the number of moments kept; rc = 99 when it exceeds nmom_table (as the
+ * reference, nothing is added). */
+int aa_mie_add_table_native(aa_mie* t, int nch_table, const double* lambda_table, int nrh, int nbin,
+                            int nmom_table, int npol_table, int nmom, const double* rh, const double* reff,
+                            const double* bext, const double* bsca, const double* bbck, const double* g,
+                            const double* pback, const double* pmom, const double* gf, const double* rhop,
+                            const double* refr, const double* refi, const char* bin_names, int* rc);
+
+/* The Chem_MieQuery outputs besides the optics (CB/Chem_MieMod.F90:885-943)
+ * for n relative humidities: rEff [um], p11, p22, growth factor, dry
+ * density, wet volume and area (any may be NULL), for tables added with
+ * aa_mie_add_table_native.  Host pointers, synchronous, no device work. */
+int aa_mie_query_extra(const aa_mie* t, int idx, float channel, int n, const float* rh, float* reff, float* p11,
+                       float* p22, float* gf, float* rhod, float* vol, float* area);
+
 /* Tracer index (0-based) of qname, or -1.  Replaces Chem_MieQueryIdx
  * (CB/Chem_MieMod.F90:720-777): case-insensitive, "XX::" qualifier stripped. */
 int aa_mie_query_idx(const aa_mie* t, const char* qname);

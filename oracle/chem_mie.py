@@ -11,6 +11,8 @@ reference's flang/x86-64 build does; there is no fused multiply-add).
 Reference (src/Shared/GMAO_Shared/Chem_Base = CB, src/Components/rtms/geo_vlidort = GEO):
   rh_map            CB/Chem_MieTableMod.F90:612-627   (991-entry RH hash rhi/rha)
   polint            CB/Chem_MieTableMod.F90:671-717   (linear wavelength interpolation)
+  table_read        CB/Chem_MieTableMod.F90:238-719   (Chem_MieTableRead after the netCDF reads)
+  query_extra       CB/Chem_MieMod.F90:885-943        (rEff, p11, p22, gf, rhod, vol, area)
   query             CB/Chem_MieMod.F90:789-965        (Chem_MieQueryByInt)
   query_idx         CB/Chem_MieMod.F90:720-777        (Chem_MieQueryIdx)
   get_aop_vector    GEO/VLIDORT_MieMod.F90:124-336    (VLIDORT_getAOPvector)
@@ -53,24 +55,97 @@ def rh_map(rh: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
 def polint(x: np.ndarray, y: np.ndarray, xwant: float) -> np.float32:
     """Linear interpolation of float64 table data y(x) at float32 xwant, result
     float32 (CB/Chem_MieTableMod.F90:671-717): out-of-range clamps to the end
-    node (slope 0 at the top node)."""
+    node (slope 0 at the top node).  Fortran kinds: slope and dx are default
+    reals assigned from float64 expressions (xWant is promoted in
+    ``xWant - x(i)``), slope*dx is a float32 product, and ``y(i) + slope*dx``
+    is a float64 sum rounded to float32 on assignment."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
     n = len(x)
     xw = f32(xwant)
+    xd = float(xw)
     i = 0
-    if xw < x[0]:
+    if xd < x[0]:
         i = 1
-    if xw > x[n - 1]:
+    if xd > x[n - 1]:
         i = n
     if i == 0:
         for j in range(n):
-            if xw >= x[j]:
+            if xd >= x[j]:
                 i = j + 1
     if i == n:
         slope = f32(0.0)
     else:
         slope = f32((y[i] - y[i - 1]) / (x[i] - x[i - 1]))
-    dx = f32(xw - f32(x[i - 1]))
-    return f32(f32(y[i - 1]) + f32(slope * dx))
+    dx = f32(xd - float(x[i - 1]))
+    sd = f32(slope * dx)
+    return f32(float(y[i - 1]) + float(sd))
+
+
+def table_read(lambda_table, channels_m, rh, reff, bext, bsca, bbck, g, nmom=0, pback=None, pmom=None, gf=None,
+               rhop=None, refr=None, refi=None):
+    """What Chem_MieTableRead (CB/Chem_MieTableMod.F90:238-719) does after its
+    netCDF reads.  Arrays in reversed-Fortran C order: rh[nrh], reff/gf/rhop
+    [nbin, nrh], bext..refi[nbin, nrh, nch_table], pback[npol, nbin, nrh,
+    nch_table], pmom[npol, nmom_table, nbin, nrh, nch_table]; absent ones None.
+    Returns (rc, dict) with float32 tables bext..refi[nbin, nch, nrh],
+    pback[npol, nbin, nch, nrh], pmom[npol, nmom, nbin, nch, nrh], rh,
+    reff/gf/rhop/rhod/vol/area[nbin, nrh] and the RH map."""
+    lam = np.asarray(lambda_table, dtype=np.float64)
+    bext = np.asarray(bext, dtype=np.float64)
+    nbin, nrh, nl = bext.shape
+    if nmom > 0 and (pmom is None or nmom > np.asarray(pmom).shape[1]):
+        return 99, None
+    npol = np.asarray(pmom).shape[0] if pmom is not None else (np.asarray(pback).shape[0] if pback is not None
+                                                                 else 6)
+    full = lambda v: np.full((nbin, nrh, nl), v)
+    pb = np.asarray(pback, dtype=np.float64) if pback is not None else np.ones((npol, nbin, nrh, nl))
+    gfv = np.asarray(gf, dtype=np.float64) if gf is not None else np.full((nbin, nrh), -999.0)
+    if rhop is not None:
+        rhp = np.asarray(rhop, dtype=np.float64)
+        rhd = np.repeat(rhp[:, :1], nrh, axis=1)          # rhod from 'rhop', first RH row
+    else:
+        rhp = np.full((nbin, nrh), -999.0)
+        rhd = np.full((nbin, nrh), -999.0)
+    rr = np.asarray(refr, dtype=np.float64) if refr is not None else full(-999.0)
+    ri = np.abs(np.asarray(refi, dtype=np.float64)) if refi is not None else full(-999.0)
+    reff = np.asarray(reff, dtype=np.float64)
+    vol = gfv * gfv * gfv / rhd
+    area = vol / (float(f32(4.0) / f32(3.0)) * reff)
+    nch = len(channels_m)
+    out = {k: np.zeros((nbin, nch, nrh), dtype=f32) for k in ("bext", "bsca", "bbck", "g", "refr", "refi")}
+    out["pback"] = np.zeros((npol, nbin, nch, nrh), dtype=f32)
+    out["pmom"] = np.zeros((npol, nmom, nbin, nch, nrh), dtype=f32)
+    src = dict(bext=bext, bsca=np.asarray(bsca, np.float64), bbck=np.asarray(bbck, np.float64),
+               g=np.asarray(g, np.float64), refr=rr, refi=ri)
+    pm = np.asarray(pmom, dtype=np.float64) if nmom > 0 else None
+    for j in range(nbin):
+        for i in range(nrh):
+            for c, cw in enumerate(channels_m):
+                for k, v in src.items():
+                    out[k][j, c, i] = polint(lam, v[j, i], cw)
+                for p in range(npol):
+                    out["pback"][p, j, c, i] = polint(lam, pb[p, j, i], cw)
+                for m in range(nmom):
+                    for p in range(npol):
+                        out["pmom"][p, m, j, c, i] = polint(lam, pm[p, m, j, i], cw)
+    out["rh"] = np.asarray(rh, dtype=np.float64).astype(f32)
+    for k, v in (("reff", reff), ("gf", gfv), ("rhop", rhp), ("rhod", rhd), ("vol", vol), ("area", area)):
+        out[k] = v.astype(f32)
+    out["rhi"], out["rha"] = rh_map(out["rh"])
+    return 0, out
+
+
+def query_extra(tab, ibin: int, ich: int, rh):
+    """rEff [um], p11, p22, gf, rhod, vol, area of Chem_MieQueryByInt
+    (CB/Chem_MieMod.F90:885-943) from a table_read result."""
+    nrh = len(tab["rh"])
+    irh, irhp1, arh = snap(rh, tab["rhi"], tab["rha"], nrh)
+    lin = lambda v: _lin(v[irh - 1], v[irhp1 - 1], arh)
+    return dict(reff=(f32(1.0e6) * lin(tab["reff"][ibin])).astype(f32),
+                p11=lin(tab["pback"][0, ibin, ich]), p22=lin(tab["pback"][4, ibin, ich]),
+                gf=lin(tab["gf"][ibin]), rhod=np.full(len(irh), tab["rhod"][ibin, 0], dtype=f32),
+                vol=lin(tab["vol"][ibin]), area=lin(tab["area"][ibin]))
 
 
 def query_idx(table_names: Sequence[str], qname: str) -> int:

@@ -36,12 +36,12 @@ namespace {
 // Table of one N (BrTab layout): azimuth quadrature, Fourier cosine factors,
 // white-sky quadrature, quadrature streams, and the white-sky albedo of
 // Ross-thick (parameter free, so evaluated once here with the reference's sums).
-std::vector<double> build_brdf_table(int N) {
-  const BrTab T(N);
+std::vector<double> build_brdf_table(int N, int A) {
+  const BrTab T(N, A);
   std::vector<double> t((size_t)T.size(), 0.0);
-  double xb[BR_NAZ], ab[BR_NAZ];
-  getquad2(0.0, 1.0, BR_NAZ, xb, ab);
-  for (int k = 0; k < BR_NAZ; ++k) {
+  std::vector<double> xb(A), ab(A);
+  getquad2(0.0, 1.0, A, xb.data(), ab.data());
+  for (int k = 0; k < A; ++k) {
     const double x = PIE * xb[k];
     t[T.x(k)] = x;
     t[T.cx(k)] = cos(x);
@@ -49,7 +49,7 @@ std::vector<double> build_brdf_table(int N) {
     t[T.a(k)] = ab[k];
   }
   for (int M = 0; M < 2 * N; ++M)
-    for (int k = 0; k < BR_NAZ; ++k)
+    for (int k = 0; k < A; ++k)
       t[T.caz(M, k)] = (M == 0) ? ab[k] : ab[k] * cos((double)M * t[T.x(k)]);
   double sq[BR_NSC], sw[BR_NSC];
   getquad2(0.0, 1.0, BR_NSC, sq, sw);
@@ -69,7 +69,7 @@ std::vector<double> build_brdf_table(int N) {
     double s = 0.0;
     for (int j = 0; j < BR_NSC; ++j) {
       double f = 0.0;
-      for (int k = 0; k < BR_NAZ; ++k)
+      for (int k = 0; k < A; ++k)
         f = f + ross_thick(t[T.sq(j)], t[T.ssin(j)], t[T.sq(i)], t[T.ssin(i)], t[T.cx(k)]) * t[T.a(k)];
       s = s + f * t[T.ssw(j)];
     }
@@ -196,6 +196,93 @@ __device__ void brdf_item(const double* __restrict__ tab, const double* kw, cons
   }
 }
 
+// One item's Cox-Munk supplement (VLIDORT_CoxMunk_SingleGeom: one kernel of
+// factor 1, 50 half-range azimuths, shadowing; the white-sky check drops the
+// kernel's Fourier terms, leaving them zero).  Same mapping as brdf_item; the
+// shadowing terms are computed once per direction.
+template <int N>
+__device__ void brdf_item_cx(const double* __restrict__ tab, double sigma2, double m2, double sza, double vza,
+                             double raa, double* __restrict__ rec, double* db_out, int* wsa_fail, bool skip) {
+#pragma clang fp contract(off)
+  constexpr int A = BR_NAZ_CM;
+  constexpr BrTab T(N, A);
+  constexpr BROff RO(N);
+  constexpr int NT = N * N + 2 * N + 1;
+  __shared__ double ssh[BR_NSC];          // shadowing terms of the scaling streams
+  __shared__ double dsh[N + 2];           // ... of the quadrature streams, sun, view
+  __shared__ double sbf[BR_NSC * BR_NSC];
+  __shared__ double rsum[BR_NSC];
+  const int t = threadIdx.x;
+  const double db = cm_db(sigma2, m2, sza, vza, raa);
+  if (t == 0) {
+    *db_out = db;
+    rec[RO.db()] = db;
+  }
+  if (skip) return;   // uniform over the wavefront
+  if (t < BR_NSC) ssh[t] = cm_shadow(sigma2, tab[T.sq(t)]);
+  const double mu0 = cos(sza * DEG_TO_RAD), s0 = sqrt(1.0 - mu0 * mu0);
+  const double muv = cos(vza * DEG_TO_RAD), sv = sqrt(1.0 - muv * muv);
+  if (t < N + 2) dsh[t] = cm_shadow(sigma2, (t < N) ? tab[T.qs(t)] : (t == N ? mu0 : muv));
+  __syncthreads();
+  for (int p = t; p < BR_NSC * BR_NSC; p += 64) {
+    const int i = p / BR_NSC, j = p - i * BR_NSC;     // incident j, reflected i
+    double f = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < A; ++k)
+      f = f + cox_munk(sigma2, m2, tab[T.sq(j)], tab[T.ssin(j)], tab[T.sq(i)], tab[T.ssin(i)], tab[T.cx(k)], ssh[j],
+                       ssh[i]) * tab[T.a(k)];
+    sbf[p] = f;
+  }
+  __syncthreads();
+  if (t < BR_NSC) {
+    double s = 0.0;
+    for (int j = 0; j < BR_NSC; ++j) s = s + sbf[t * BR_NSC + j] * tab[T.ssw(j)];
+    rsum[t] = s;
+  }
+  __syncthreads();
+  double help = 0.0;
+  for (int i = 0; i < BR_NSC; ++i) help = help + rsum[i] * tab[T.ssw(i)];
+  const double total = 0.0 + 1.0 * (help * 4.0 / PIE);
+  const bool with_k = !(total <= 0.0 || total > 1.0);
+  if (t == 0) *wsa_fail = with_k ? 0 : 1;
+  for (int p = t; p < NT; p += 64) {
+    double xi, sxi, xj, sxj, shi, shr;
+    int o0;
+    if (p < N * N) {
+      const int i = p / N, j = p - i * N;
+      xi = tab[T.qs(j)]; sxi = tab[T.qsin(j)]; xj = tab[T.qs(i)]; sxj = tab[T.qsin(i)];
+      shi = dsh[j]; shr = dsh[i]; o0 = RO.f(0, i, j);
+    } else if (p < N * N + N) {
+      const int i = p - N * N;
+      xi = mu0; sxi = s0; xj = tab[T.qs(i)]; sxj = tab[T.qsin(i)];
+      shi = dsh[N]; shr = dsh[i]; o0 = RO.f0(0, i);
+    } else if (p < N * N + 2 * N) {
+      const int j = p - N * N - N;
+      xi = tab[T.qs(j)]; sxi = tab[T.qsin(j)]; xj = muv; sxj = sv;
+      shi = dsh[j]; shr = dsh[N + 1]; o0 = RO.uf(0, j);
+    } else {
+      xi = mu0; sxi = s0; xj = muv; sxj = sv;
+      shi = dsh[N]; shr = dsh[N + 1]; o0 = RO.uf0(0);
+    }
+    // stride between Fourier terms of this output in the record
+    const int ms = (p < N * N) ? N * N : ((p < N * N + 2 * N) ? N : 1);
+    double dc[2 * N];
+#pragma unroll
+    for (int M = 0; M < 2 * N; ++M) dc[M] = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < A; ++k) {
+      const double kv = cox_munk(sigma2, m2, xi, sxi, xj, sxj, tab[T.cx(k)], shi, shr);
+#pragma unroll
+      for (int M = 0; M < 2 * N; ++M) dc[M] = dc[M] + kv * tab[T.caz(M, k)];
+    }
+#pragma unroll
+    for (int M = 0; M < 2 * N; ++M) {
+      const double delfac = (M == 0) ? 1.0 : 2.0;
+      rec[o0 + M * ms] = with_k ? 0.0 + 1.0 * (delfac * dc[M]) : 0.0;
+    }
+  }
+}
+
 template <int N>
 __global__ __launch_bounds__(64) void k_brdf(RTWork W, int nitems) {
   const int it = blockIdx.x;
@@ -211,9 +298,19 @@ __global__ __launch_bounds__(64) void k_brdf(RTWork W, int nitems) {
   const size_t kq = (size_t)obs * W.nch + ch;
   __shared__ double dbv;
   __shared__ int wf;
-  brdf_item<N>(W.brdf_tab, W.kernel_wt + kq * 3, W.kparam + kq * 2, W.sza[obs], W.vza[obs], W.raa[obs],
-               W.brdf + (size_t)it * W.brdf_stride, &dbv, &wf, st != 0, true);
+  if (W.kernel_wt) {
+    brdf_item<N>(W.brdf_tab, W.kernel_wt + kq * 3, W.kparam + kq * 2, W.sza[obs], W.vza[obs], W.raa[obs],
+                 W.brdf + (size_t)it * W.brdf_stride, &dbv, &wf, st != 0, true);
+  } else {
+    const double m = W.mr[ch];
+    brdf_item_cx<N>(W.brdf_tab, cm_sigma2(W.u10m[obs], W.v10m[obs]), m * m, W.sza[obs], W.vza[obs], W.raa[obs],
+                    W.brdf + (size_t)it * W.brdf_stride, &dbv, &wf, st != 0);
+  }
   __syncthreads();
+  if (!W.kernel_wt) {   // the Cox-Munk drivers keep any sign of the direct bounce
+    if (threadIdx.x == 0 && W.br_out) W.br_out[oi] = dbv;
+    return;
+  }
   if (threadIdx.x == 0) {
     // negative direct bounce: the driver writes -500 and skips VLIDORT (status 6)
     if (W.br_out) W.br_out[oi] = (dbv < 0.0) ? -500.0 : dbv;
@@ -238,18 +335,36 @@ __global__ __launch_bounds__(64) void k_brdf_only(int n, const double* tab, cons
   if (threadIdx.x == 0) { db[g] = dbv; ok[g] = wf ? 4 : 0; }
 }
 
+// Cox-Munk supplement alone for explicit geometries (aa_vbrdf_coxmunk).
+template <int N>
+__global__ __launch_bounds__(64) void k_brdf_cx_only(int n, const double* tab, const double* u10m,
+                                                     const double* v10m, const double* mr, const double* sza,
+                                                     const double* vza, const double* raa, double* rec, int stride,
+                                                     double* db, int* ok) {
+  const int g = blockIdx.x;
+  if (g >= n) return;
+  __shared__ double dbv;
+  __shared__ int wf;
+  brdf_item_cx<N>(tab, cm_sigma2(u10m[g], v10m[g]), mr[g] * mr[g], sza[g], vza[g], raa[g], rec + (size_t)g * stride,
+                  &dbv, &wf, false);
+  __syncthreads();
+  if (threadIdx.x == 0) { db[g] = dbv; ok[g] = wf ? 4 : 0; }
+}
+
 }  // namespace
 
-const std::vector<double>& brdf_table_cached(int N) {
+const std::vector<double>& brdf_table_cached(int N, int A) {
   static std::mutex mu;
-  static std::map<int, std::vector<double>> cache;
+  static std::map<std::pair<int, int>, std::vector<double>> cache;
   std::lock_guard<std::mutex> lk(mu);
-  auto it = cache.find(N);
-  if (it == cache.end()) it = cache.emplace(N, build_brdf_table(N)).first;
+  auto key = std::make_pair(N, A);
+  auto it = cache.find(key);
+  if (it == cache.end()) it = cache.emplace(key, build_brdf_table(N, A)).first;
   return it->second;
 }
 
 int rt_brdf(const RTWork& W, int nitems, hipStream_t st) {
+  // (the table in W.brdf_tab was built for W.N and the surface's azimuth count)
   if (nitems <= 0) return 0;
   switch (W.N) {
 #define AA_BR(NN) case NN: hipLaunchKernelGGL(k_brdf<NN>, dim3(nitems), dim3(64), 0, st, W, nitems); break;
@@ -263,18 +378,24 @@ int rt_brdf(const RTWork& W, int nitems, hipStream_t st) {
 
 }  // namespace aa
 
-extern "C" int aa_vbrdf_modis(int nstreams, int ngeom, const double* sza, const double* vza, const double* raa,
-                              const double* kernel_wt, const double* param, double* brdf_f, double* brdf_f0,
-                              double* user_brdf_f, double* user_brdf_f0, double* dbounce, int* status) {
+namespace {
+// Supplement alone over ngeom geometries: upload, one wavefront per geometry,
+// outputs in Fortran order.  cm: Cox-Munk (p1 = u10m, p2 = v10m, p3 = mr, one
+// value per geometry) else MODIS (p1 = kernel_wt(3,g), p2 = param(2,g)).
+int vbrdf_alone(bool cm, int nstreams, int ngeom, const double* sza, const double* vza, const double* raa,
+                const double* p1, const double* p2, const double* p3, double* brdf_f, double* brdf_f0,
+                double* user_brdf_f, double* user_brdf_f0, double* dbounce, int* status) {
   using namespace aa;
   if (nstreams < 1 || nstreams > MAX_STREAMS || ngeom < 0) return AA_EARG;
-  if (!sza || !vza || !raa || !kernel_wt || !param || !brdf_f || !brdf_f0 || !user_brdf_f || !user_brdf_f0 ||
+  if (!sza || !vza || !raa || !p1 || !p2 || (cm && !p3) || !brdf_f || !brdf_f0 || !user_brdf_f || !user_brdf_f0 ||
       !dbounce || !status)
     return AA_EARG;
   if (ngeom == 0) return 0;
   const int N = nstreams;
   const BROff RO(N);
-  const std::vector<double>& tab = brdf_table_cached(N);
+  const std::vector<double>& tab = brdf_table_cached(N, cm ? BR_NAZ_CM : BR_NAZ);
+  const double* kernel_wt = p1;
+  const double* param = p2;
   const size_t ng = (size_t)ngeom;
   double *d_tab = nullptr, *d_g = nullptr, *d_kw = nullptr, *d_kp = nullptr, *d_rec = nullptr, *d_db = nullptr;
   int* d_ok = nullptr;
@@ -282,8 +403,9 @@ extern "C" int aa_vbrdf_modis(int nstreams, int ngeom, const double* sza, const 
   auto fail = [&]() { rc = AA_EHIP; };
   if (hipMalloc(&d_tab, tab.size() * sizeof(double)) != hipSuccess) fail();
   if (!rc && hipMalloc(&d_g, 3 * ng * sizeof(double)) != hipSuccess) fail();
-  if (!rc && hipMalloc(&d_kw, 3 * ng * sizeof(double)) != hipSuccess) fail();
-  if (!rc && hipMalloc(&d_kp, 2 * ng * sizeof(double)) != hipSuccess) fail();
+  const size_t n1 = cm ? ng : 3 * ng, n2 = cm ? 2 * ng : 2 * ng;   // Cox-Munk: d_kw = u10m, d_kp = (v10m, mr)
+  if (!rc && hipMalloc(&d_kw, n1 * sizeof(double)) != hipSuccess) fail();
+  if (!rc && hipMalloc(&d_kp, n2 * sizeof(double)) != hipSuccess) fail();
   if (!rc && hipMalloc(&d_rec, ng * RO.size() * sizeof(double)) != hipSuccess) fail();
   if (!rc && hipMalloc(&d_db, ng * sizeof(double)) != hipSuccess) fail();
   if (!rc && hipMalloc(&d_ok, ng * sizeof(int)) != hipSuccess) fail();
@@ -292,8 +414,9 @@ extern "C" int aa_vbrdf_modis(int nstreams, int ngeom, const double* sza, const 
               hipMemcpy(d_g, sza, ng * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(d_g + ng, vza, ng * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(d_g + 2 * ng, raa, ng * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(d_kw, kernel_wt, 3 * ng * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(d_kp, param, 2 * ng * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d_kw, kernel_wt, n1 * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d_kp, param, (cm ? ng : 2 * ng) * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+              (!cm || hipMemcpy(d_kp + ng, p3, ng * sizeof(double), hipMemcpyHostToDevice) == hipSuccess) &&
               hipMemset(d_rec, 0, ng * RO.size() * sizeof(double)) == hipSuccess;
     if (!ok) fail();
   }
@@ -301,8 +424,12 @@ extern "C" int aa_vbrdf_modis(int nstreams, int ngeom, const double* sza, const 
     switch (N) {
 #define AA_BR(NN)                                                                                               \
   case NN:                                                                                                      \
-    hipLaunchKernelGGL(k_brdf_only<NN>, dim3(ngeom), dim3(64), 0, 0, ngeom, d_tab, d_kw, d_kp, d_g, d_g + ng,   \
-                       d_g + 2 * ng, d_rec, RO.size(), d_db, d_ok);                                             \
+    if (cm)                                                                                                     \
+      hipLaunchKernelGGL(k_brdf_cx_only<NN>, dim3(ngeom), dim3(64), 0, 0, ngeom, d_tab, d_kw, d_kp, d_kp + ng,  \
+                         d_g, d_g + ng, d_g + 2 * ng, d_rec, RO.size(), d_db, d_ok);                            \
+    else                                                                                                        \
+      hipLaunchKernelGGL(k_brdf_only<NN>, dim3(ngeom), dim3(64), 0, 0, ngeom, d_tab, d_kw, d_kp, d_g, d_g + ng, \
+                         d_g + 2 * ng, d_rec, RO.size(), d_db, d_ok);                                           \
     break;
       AA_BR(1) AA_BR(2) AA_BR(3) AA_BR(4) AA_BR(5) AA_BR(6) AA_BR(7) AA_BR(8)
 #undef AA_BR
@@ -336,4 +463,20 @@ extern "C" int aa_vbrdf_modis(int nstreams, int ngeom, const double* sza, const 
   (void)hipFree(d_tab); (void)hipFree(d_g); (void)hipFree(d_kw); (void)hipFree(d_kp);
   (void)hipFree(d_rec); (void)hipFree(d_db); (void)hipFree(d_ok);
   return rc;
+}
+}  // namespace
+
+extern "C" int aa_vbrdf_modis(int nstreams, int ngeom, const double* sza, const double* vza, const double* raa,
+                              const double* kernel_wt, const double* param, double* brdf_f, double* brdf_f0,
+                              double* user_brdf_f, double* user_brdf_f0, double* dbounce, int* status) {
+  return vbrdf_alone(false, nstreams, ngeom, sza, vza, raa, kernel_wt, param, nullptr, brdf_f, brdf_f0, user_brdf_f,
+                     user_brdf_f0, dbounce, status);
+}
+
+extern "C" int aa_vbrdf_coxmunk(int nstreams, int ngeom, const double* sza, const double* vza, const double* raa,
+                                const double* u10m, const double* v10m, const double* mr, double* brdf_f,
+                                double* brdf_f0, double* user_brdf_f, double* user_brdf_f0, double* dbounce,
+                                int* status) {
+  return vbrdf_alone(true, nstreams, ngeom, sza, vza, raa, u10m, v10m, mr, brdf_f, brdf_f0, user_brdf_f,
+                     user_brdf_f0, dbounce, status);
 }

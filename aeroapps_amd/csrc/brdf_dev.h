@@ -89,4 +89,86 @@ __host__ __device__ inline double modis_db(double fiso, double fgeo, double fvol
   return db;
 }
 
+// ---- Cox-Munk glint (COXMUNK_VFUNCTION, VB/vbrdf_sup_kernels.f90:1018-1141)
+// with the supplement's erfc (VB/vbrdf_sup_aux.f90:266-286).  pars: sigma^2,
+// m^2; the shadowing factor is split into its per-direction terms, which the
+// reference computes with the same operations in every call.
+__host__ __device__ inline double derfc_e(double x) {
+#pragma clang fp contract(off)
+  const double z = fabs(x);
+  const double t = 1.0 / (1.0 + 0.5 * z);
+  double r = t * exp(-z * z - 1.26551223 +
+                     t * (1.00002368 + t * (.37409196 + t * (.09678418 + t * (-.18628806 + t * (.27886807 + t *
+                     (-1.13520398 + t * (1.48851587 + t * (-.82215223 + t * .17087277)))))))));
+  if (x < 0.0) r = 2.0 - r;
+  return r;
+}
+
+// Shadowing term of one direction cosine x.
+__host__ __device__ inline double cm_shadow(double sigma2, double x) {
+#pragma clang fp contract(off)
+  const double s1 = sqrt(sigma2 / PIE);
+  const double s3 = 1.0 / (sqrt(sigma2));
+  const double s2 = s3 * s3;
+  const double xx = x * x;
+  const double dcot = x / sqrt(1.0 - xx);
+  const double t1 = exp(-dcot * dcot * s2);
+  const double t2 = derfc_e(dcot * s3);
+  return 0.5 * (s1 * t1 / dcot - t2);
+}
+
+// Kernel value with the shadowing terms shi (incident) and shr (reflected).
+__host__ __device__ inline double cox_munk(double sigma2, double m2, double xi, double sxi, double xj, double sxj,
+                                           double ckphi, double shi, double shr) {
+#pragma clang fp contract(off)
+  const double ckphi_neg = -ckphi;
+  double z = xi * xj + sxi * sxj * ckphi_neg;
+  if (z > 1.0) z = 1.0;
+  const double z1 = acos(z);
+  const double z2 = cos(z1 * 0.5);
+  const double z2_sq_m1 = z2 * z2 + -1.0;
+  const double h1 = m2 * z2;
+  const double h2 = sqrt(m2 + z2_sq_m1);
+  const double rp = (h1 - h2) / (h1 + h2);
+  const double rl = (z2 - h2) / (z2 + h2);
+  const double xmp = 0.5 * (rp * rp + rl * rl);
+  double a = 2.0 * z2;
+  double b = (xi + xj) / a;
+  if (b > 1.0) b = 1.0;
+  a = 0.5 * PIE - asin(b);
+  const double ta = tan(a);
+  const double argument = ta * ta / sigma2;
+  double k = 0.0;
+  if (argument < 88.0) {
+    const double prob = exp(-argument);
+    const double fac1 = prob / sigma2;
+    const double fac2 = 0.25 / xi / pow(b, 4.0);   // B ** FOUR with a real exponent
+    k = xmp * fac1 * fac2 / xj;
+  }
+  const double shadow = 1.0 / (1.0 + shi + shr);
+  return k * shadow;
+}
+
+// sigma^2 of VLIDORT_CoxMunk_SingleGeom (V283/VLIDORT_SurfaceMod.F90:880-886):
+// 0.003 + 0.00512 |w10| with default-real literals, |w10| capped at 50 m/s.
+__host__ __device__ inline double cm_sigma2(double u10m, double v10m) {
+#pragma clang fp contract(off)
+  double w = sqrt(u10m * u10m + v10m * v10m);
+  if (w < 0.0) w = 0.0;
+  if (w > 50.0) w = 50.0;
+  return (double)0.003f + (double)0.00512f * w;
+}
+
+// Exact direct bounce of the Cox-Munk surface (one kernel, factor 1).
+__host__ __device__ inline double cm_db(double sigma2, double m2, double sza, double vza, double raa) {
+#pragma clang fp contract(off)
+  const double mu0 = cos(sza * DEG_TO_RAD);
+  const double s0 = sqrt(1.0 - mu0 * mu0);
+  const double muv = cos(vza * DEG_TO_RAD);
+  const double sv = sqrt(1.0 - muv * muv);
+  const double phi = raa * DEG_TO_RAD;
+  const double k = cox_munk(sigma2, m2, mu0, s0, muv, sv, cos(phi), cm_shadow(sigma2, mu0), cm_shadow(sigma2, muv));
+  return 0.0 + 1.0 * k;
+}
+
 }  // namespace aa

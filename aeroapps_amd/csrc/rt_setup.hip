@@ -160,14 +160,18 @@ __device__ __forceinline__ ItemIdx item_of(const RTWork& W, int it) {
 // 160-170) runs VLIDORT only where the direct-bounce BRDF is not negative, so
 // only such items latch.
 __global__ void k_latch(int nitems, int nch, int nobs, const double* sza, const double* vza, const double* raa,
-                        const double* albedo, const double* kernel_wt, const double* kparam, double missing,
-                        int* latch) {
+                        const double* albedo, const double* kernel_wt, const double* kparam, const double* u10m,
+                        const double* v10m, const double* mr, double missing, int* latch) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nitems) return;
   int obs = g / nch, ch = g - obs * nch;
   if (sza[obs] != 0.0) return;
   if (is_missing(sza[obs], missing) || is_missing(vza[obs], missing) || is_missing(raa[obs], missing)) return;
   if (albedo && is_missing(albedo[(size_t)ch * nobs + obs], missing)) return;
+  // the Cox-Munk drivers skip elements with missing wind or refractive index
+  // (src/Components/rtms/geo_vlidort/VLIDORT_BRDF_CX.F90:506-512)
+  if (u10m && (is_missing(u10m[obs], missing) || is_missing(v10m[obs], missing) || is_missing(mr[ch], missing)))
+    return;
   if (kernel_wt) {
     const double* kw = kernel_wt + ((size_t)obs * nch + ch) * 3;
     const double* kp = kparam + ((size_t)obs * nch + ch) * 2;
@@ -187,6 +191,9 @@ __global__ void k_check(RTWork W, int nitems) {
   int st = 0;
   if (is_missing(sza, W.missing) || is_missing(vza, W.missing) || is_missing(raa, W.missing)) st = 1;
   else if (W.albedo && is_missing(W.albedo[(size_t)I.ch * W.nobs + I.obs], W.missing)) st = 1;
+  else if (W.u10m && (is_missing(W.u10m[I.obs], W.missing) || is_missing(W.v10m[I.obs], W.missing) ||
+                      is_missing(W.mr[I.ch], W.missing)))
+    st = 1;
   int L = W.nlayers ? W.nlayers[I.obs] : W.km;
   if (L < 2 || L > W.km) { if (!st) st = 4; L = 2; }
   if (!st) {
@@ -703,7 +710,7 @@ int rt_setup(const RTWork& W, int nitems, int* d_active, int* d_nactive, hipStre
   prof_begin(PROF_CHECK, st);
   hipLaunchKernelGGL(k_check, dim3(gi), dim3(B), 0, st, W, nitems);
   prof_end(PROF_CHECK, st);
-  if (W.kernel_wt) {
+  if (W.kernel_wt || W.u10m) {
     prof_begin(PROF_BRDF, st);
     if (rt_brdf(W, nitems, st)) return -2;
     prof_end(PROF_BRDF, st);
@@ -729,11 +736,11 @@ int rt_setup(const RTWork& W, int nitems, int* d_active, int* d_nactive, hipStre
 }
 
 int rt_latch(int nitems, int nch, int nobs, const double* sza, const double* vza, const double* raa,
-             const double* albedo, const double* kernel_wt, const double* kparam, double missing, int* d_latch,
-             hipStream_t st) {
+             const double* albedo, const double* kernel_wt, const double* kparam, const double* u10m,
+             const double* v10m, const double* mr, double missing, int* d_latch, hipStream_t st) {
   const int B = 256;
   hipLaunchKernelGGL(k_latch, dim3((nitems + B - 1) / B), dim3(B), 0, st, nitems, nch, nobs, sza, vza, raa, albedo,
-                     kernel_wt, kparam, missing, d_latch);
+                     kernel_wt, kparam, u10m, v10m, mr, missing, d_latch);
   AA_CHECK(hipGetLastError());
   return 0;
 }

@@ -51,6 +51,10 @@ struct RTWork {
   // when kernel_wt is set the surface is the BRDF, else Lambertian.
   const double* kernel_wt = nullptr;  // (3,nch,nobs): fiso, fgeo, fvol
   const double* kparam = nullptr;     // (2,nch,nobs): Li-sparse h/b, b/r
+  // Cox-Munk ocean surface (BRDF: one glint kernel); set instead of kernel_wt
+  const double* u10m = nullptr;       // (nobs) m/s
+  const double* v10m = nullptr;       // (nobs) m/s
+  const double* mr = nullptr;         // (nch) refractive index of water
   double* br_out = nullptr;           // (nobs,nch) exact direct-bounce BRDF
   const double* brdf_tab = nullptr;   // azimuth / scaling quadrature tables (BrTab)
   double* brdf = nullptr;             // [it][brdf_stride] Fourier components of the BRDF
@@ -150,25 +154,26 @@ struct BROff {
   __host__ __device__ constexpr int size() const { return (2 * N * N * N + 4 * N * N + 2 * N + 2) & ~1; }
 };
 
-// Host-built quadrature tables of the BRDF supplement for N streams
-// (BRDF_QUADRATURE_GAUSSIAN half range, 25 azimuth streams; the 24-stream
-// white-sky quadrature; the N quadrature streams with their sines).
-constexpr int BR_NAZ = 25, BR_NSC = 24;
+// Host-built quadrature tables of the BRDF supplement for N streams and NAZ
+// half-range azimuth streams (BRDF_QUADRATURE_GAUSSIAN; 25 for the MODIS
+// set-up, 50 for Cox-Munk), the 24-stream white-sky quadrature and the N
+// quadrature streams with their sines.
+constexpr int BR_NAZ = 25, BR_NAZ_CM = 50, BR_NSC = 24;
 struct BrTab {
-  int N;
-  __host__ __device__ constexpr BrTab(int N_) : N(N_) {}
+  int N, A;
+  __host__ __device__ constexpr BrTab(int N_, int A_ = BR_NAZ) : N(N_), A(A_) {}
   __host__ __device__ constexpr int x(int k) const { return k; }
-  __host__ __device__ constexpr int cx(int k) const { return BR_NAZ + k; }
-  __host__ __device__ constexpr int sx(int k) const { return 2 * BR_NAZ + k; }
-  __host__ __device__ constexpr int a(int k) const { return 3 * BR_NAZ + k; }
-  __host__ __device__ constexpr int caz(int M, int k) const { return (4 + M) * BR_NAZ + k; }   // M < 2N
-  __host__ __device__ constexpr int sq(int i) const { return (4 + 2 * N) * BR_NAZ + i; }
-  __host__ __device__ constexpr int ssin(int i) const { return (4 + 2 * N) * BR_NAZ + BR_NSC + i; }
-  __host__ __device__ constexpr int ssw(int i) const { return (4 + 2 * N) * BR_NAZ + 2 * BR_NSC + i; }
-  __host__ __device__ constexpr int qs(int i) const { return (4 + 2 * N) * BR_NAZ + 3 * BR_NSC + i; }
-  __host__ __device__ constexpr int qsin(int i) const { return (4 + 2 * N) * BR_NAZ + 3 * BR_NSC + N + i; }
-  __host__ __device__ constexpr int wsa_ross() const { return (4 + 2 * N) * BR_NAZ + 3 * BR_NSC + 2 * N; }
-  __host__ __device__ constexpr int size() const { return (4 + 2 * N) * BR_NAZ + 3 * BR_NSC + 2 * N + 1; }
+  __host__ __device__ constexpr int cx(int k) const { return A + k; }
+  __host__ __device__ constexpr int sx(int k) const { return 2 * A + k; }
+  __host__ __device__ constexpr int a(int k) const { return 3 * A + k; }
+  __host__ __device__ constexpr int caz(int M, int k) const { return (4 + M) * A + k; }   // M < 2N
+  __host__ __device__ constexpr int sq(int i) const { return (4 + 2 * N) * A + i; }
+  __host__ __device__ constexpr int ssin(int i) const { return (4 + 2 * N) * A + BR_NSC + i; }
+  __host__ __device__ constexpr int ssw(int i) const { return (4 + 2 * N) * A + 2 * BR_NSC + i; }
+  __host__ __device__ constexpr int qs(int i) const { return (4 + 2 * N) * A + 3 * BR_NSC + i; }
+  __host__ __device__ constexpr int qsin(int i) const { return (4 + 2 * N) * A + 3 * BR_NSC + N + i; }
+  __host__ __device__ constexpr int wsa_ross() const { return (4 + 2 * N) * A + 3 * BR_NSC + 2 * N; }
+  __host__ __device__ constexpr int size() const { return (4 + 2 * N) * A + 3 * BR_NSC + 2 * N + 1; }
 };
 
 // Rows/columns of the SAB/DAB contraction table for (N, S): rows r*n+c (SAB)

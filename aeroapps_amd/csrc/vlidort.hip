@@ -25,9 +25,9 @@ namespace aa {
 
 int rt_setup(const RTWork& W, int nitems, int* d_active, int* d_nactive, hipStream_t st);
 int rt_latch(int nitems, int nch, int nobs, const double* sza, const double* vza, const double* raa,
-             const double* albedo, const double* kernel_wt, const double* kparam, double missing, int* d_latch,
-             hipStream_t st);
-const std::vector<double>& brdf_table_cached(int N);
+             const double* albedo, const double* kernel_wt, const double* kparam, const double* u10m,
+             const double* v10m, const double* mr, double missing, int* d_latch, hipStream_t st);
+const std::vector<double>& brdf_table_cached(int N, int A);
 #define AA_DECL(N, S) \
   int rt_fourier_##N##_##S(const RTWork& W, int nitems, int* const lists[2], int* counts, hipStream_t st);
 AA_DECL(1, 1) AA_DECL(2, 1) AA_DECL(3, 1) AA_DECL(4, 1) AA_DECL(5, 1) AA_DECL(6, 1) AA_DECL(7, 1) AA_DECL(8, 1)
@@ -239,7 +239,7 @@ size_t carve_work(Carver& C, RTWork& W, int cap) {
   const int n = W.N * W.S, LM = 2 * W.N, km = W.km;
   W.lstride = LOff(W.N, W.S).size();
   W.bstride = BOff(W.N, n).size();
-  W.brdf_stride = W.kernel_wt ? BROff(W.N).size() : 0;
+  W.brdf_stride = (W.kernel_wt || W.u10m) ? BROff(W.N).size() : 0;
   W.ustride = 4 * n + 2;   // U row of a layer panel: 4n columns, transformed RHS, 1/pivot
   W.cap = cap;
   W.status = C.take<int>(cap);
@@ -264,7 +264,7 @@ size_t carve_work(Carver& C, RTWork& W, int cap) {
   W.lostrans = C.take<double>((size_t)cap * (km + 1));
   W.lbuf = C.take<double>((size_t)cap * km * W.lstride);
   W.bbuf = C.take<double>((size_t)cap * W.bstride);
-  W.brdf = W.kernel_wt ? C.take<double>((size_t)cap * W.brdf_stride) : nullptr;
+  W.brdf = (W.kernel_wt || W.u10m) ? C.take<double>((size_t)cap * W.brdf_stride) : nullptr;
   W.ubuf = C.take<double>((size_t)cap * (2 * n * km) * W.ustride);
   W.sab32 = (W.precision == 1) ? C.take<float>((size_t)cap * km * 2 * n * n) : nullptr;
   return C.off;
@@ -283,8 +283,8 @@ int compute_latch(RTWork& W, int* lp, hipStream_t st) {
   static const int big = INT_MAX;
   AA_CHECK(hipMemcpyAsync(lp, &big, sizeof(int), hipMemcpyHostToDevice, st));
   const int nitems = W.nobs * W.nch;
-  if (nitems > 0 && rt_latch(nitems, W.nch, W.nobs, W.sza, W.vza, W.raa, W.albedo, W.kernel_wt, W.kparam, W.missing,
-                             lp, st))
+  if (nitems > 0 && rt_latch(nitems, W.nch, W.nobs, W.sza, W.vza, W.raa, W.albedo, W.kernel_wt, W.kparam, W.u10m,
+                             W.v10m, W.mr, W.missing, lp, st))
     return AA_EHIP;
   W.latch = lp;
   return 0;
@@ -300,7 +300,8 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   static const std::vector<float> no_ytab;
   const std::vector<float>& ytab = (W.precision == 1) ? ytab_cached(W.N, W.S, W.nmoments) : no_ytab;
   static const std::vector<double> no_btab;
-  const std::vector<double>& btab = W.kernel_wt ? brdf_table_cached(W.N) : no_btab;
+  const std::vector<double>& btab =
+      W.kernel_wt ? brdf_table_cached(W.N, BR_NAZ) : (W.u10m ? brdf_table_cached(W.N, BR_NAZ_CM) : no_btab);
   double* dbt = nullptr;
   // chunk capacity from the work budget
   size_t freeb = 0, totalb = 0;
@@ -374,15 +375,18 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
                  const double* pmom, const double* he, const double* albedo, const double* sza, const double* raa,
                  const double* vza, const double* flux_factor, double missing, int precision, double* radiance,
                  double* reflectance, double* q, double* u, int* nfourier, int* rc,
-                 const double* kernel_wt = nullptr, const double* kparam = nullptr, double* br = nullptr) {
+                 const double* kernel_wt = nullptr, const double* kparam = nullptr, double* br = nullptr,
+                 const double* u10m = nullptr, const double* v10m = nullptr, const double* mr = nullptr) {
   if (!rc) return AA_EARG;
   *rc = 0;
-  const bool modis = kernel_wt != nullptr;
+  const bool cx = u10m != nullptr;
+  if (cx && (vector || !v10m || !mr || !br || kernel_wt)) return AA_EARG;
+  const bool modis = kernel_wt != nullptr || cx;   // a BRDF surface instead of the albedo
   if (km <= 0 || nch <= 0 || nobs < 0 || nmom <= 0 || npol <= 0) return AA_EARG;
   if (!rot || !depol || !tau || !ssa || !pmom || !he || (!albedo && !modis) || !sza || !raa || !vza || !radiance ||
       !reflectance)
     return AA_EARG;
-  if (modis && (vector || !kparam || !br)) return AA_EARG;
+  if (kernel_wt && (vector || !kparam || !br)) return AA_EARG;
   // VLIDORT_Init of the MODIS driver (V283/VLIDORT_Mod.F90:369) rejects nMom
   // beyond the input maximum for the whole call
   if (modis && nmom > MAX_MOMENTS_IN) { *rc = 5; return 0; }
@@ -409,11 +413,16 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
   W.optics_f32 = 0; W.opt_obs0 = 0; W.opt_nobs = nobs;
   W.he = B.up(he, (size_t)(km + 1) * nobs);
   W.albedo = modis ? nullptr : B.up(albedo, (size_t)nobs * nch);
-  if (modis) {
+  if (kernel_wt) {
     W.kernel_wt = B.up(kernel_wt, (size_t)nobs * nch * 3);
     W.kparam = B.up(kparam, (size_t)nobs * nch * 2);
-    W.br_out = B.alloc<double>((size_t)nobs * nch);
   }
+  if (cx) {
+    W.u10m = B.up(u10m, (size_t)nobs);
+    W.v10m = B.up(v10m, (size_t)nobs);
+    W.mr = B.up(mr, (size_t)nch);
+  }
+  if (modis) W.br_out = B.alloc<double>((size_t)nobs * nch);
   W.sza = B.up(sza, (size_t)nobs);
   W.raa = B.up(raa, (size_t)nobs);
   W.vza = B.up(vza, (size_t)nobs);
@@ -475,6 +484,21 @@ int aa_vlidort_vector_lambert(int km, int nch, int nobs, const double* channels,
   return aa::lambert_host(true, km, nch, nobs, nstreams, plane_parallel, nmom, npol, rot, depol, alpha, tau, ssa,
                           pmom, he, albedo, solar_zenith, relat_azymuth, sensor_zenith, flux_factor, missing,
                           precision, radiance, reflectance, q, u, nfourier, rc);
+}
+
+int aa_vlidort_scalar_cx(int km, int nch, int nobs, const double* channels, int nstreams, int plane_parallel,
+                         int nmom, int npol, const double* rot, const double* depol, const double* tau,
+                         const double* ssa, const double* g, const double* pmom, const double* pe, const double* he,
+                         const double* te, const double* u10m, const double* v10m, const double* mr,
+                         const double* solar_zenith, const double* relat_azymuth, const double* sensor_zenith,
+                         double missing, int verbose, int precision, double* radiance, double* reflectance,
+                         double* brdf, int* nfourier, int* rc) {
+  (void)channels; (void)g; (void)pe; (void)te; (void)verbose;
+  if (!u10m) return AA_EARG;
+  return aa::lambert_host(false, km, nch, nobs, nstreams, plane_parallel, nmom, npol, rot, depol, nullptr, tau, ssa,
+                          pmom, he, nullptr, solar_zenith, relat_azymuth, sensor_zenith, nullptr, missing, precision,
+                          radiance, reflectance, nullptr, nullptr, nfourier, rc, nullptr, nullptr, brdf, u10m, v10m,
+                          mr);
 }
 
 int aa_vlidort_scalar_landmodis(int km, int nch, int nobs, const double* channels, int nstreams, int plane_parallel,

@@ -82,6 +82,20 @@ module aeroapps_amd_capi
       integer(c_int), intent(out) :: rc
     end function
 
+    integer(c_int) function aa_vlidort_scalar_cx(km, nch, nobs, channels, nstreams, plane_parallel, nmom, &
+        npol, rot, depol, tau, ssa, g, pmom, pe, he, te, u10m, v10m, mr, sza, raa, vza, missing, verbose, &
+        precision, radiance, reflectance, brdf, nfourier, rc) bind(C, name="aa_vlidort_scalar_cx")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: km, nch, nobs, nstreams, plane_parallel, nmom, npol
+      real(c_double), intent(in) :: channels(*), rot(*), depol(*), tau(*), ssa(*), g(*), pmom(*)
+      real(c_double), intent(in) :: pe(*), he(*), te(*), u10m(*), v10m(*), mr(*), sza(*), raa(*), vza(*)
+      real(c_double), value :: missing
+      integer(c_int), value :: verbose, precision
+      real(c_double), intent(out) :: radiance(*), reflectance(*), brdf(*)
+      type(c_ptr), value :: nfourier
+      integer(c_int), intent(out) :: rc
+    end function
+
     integer(c_int) function aa_vlidort_vector_lambert(km, nch, nobs, channels, nstreams, plane_parallel, nmom, &
         npol, rot, depol, alpha, tau, ssa, pmom, pe, he, te, albedo, sza, raa, vza, flux_factor, missing, verbose, &
         precision, radiance, reflectance, q, u, nfourier, rc) bind(C, name="aa_vlidort_vector_lambert")

@@ -66,7 +66,7 @@ NRH = 36
 
 # field ids for the counter-based generator
 _F_DELP, _F_Q1, _F_Q2, _F_RH, _F_SZA, _F_VZA, _F_RAA, _F_NLAY, _F_NSTR = range(1, 10)
-_F_FISO, _F_FVOL, _F_FGEO = range(10, 13)
+_F_FISO, _F_FVOL, _F_FGEO, _F_U10, _F_V10 = range(10, 15)
 _F_TBEXT, _F_TSSA, _F_TG, _F_TC12, _F_TC33, _F_TC34, _F_TREFR, _F_TREFI, _F_TREFF = range(20, 29)
 
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
@@ -317,6 +317,17 @@ def make_modis_kernels(ncol: int, nch: int, seed: int = 7, col0: int = 0):
     par[..., 0] = 2.0
     par[..., 1] = 1.0
     return kw, par
+
+
+def make_ocean(ncol: int, channels_nm: Sequence[float], seed: int = 7, col0: int = 0):
+    """Ocean surface inputs of the Cox-Munk drivers: 10 m wind components
+    u10m, v10m [ncol] (m/s, |w| up to 14) and the water refractive index
+    mr [nch] (1.34 at 400 nm falling slowly with wavelength)."""
+    cols = np.arange(col0, col0 + ncol, dtype=np.int64)
+    u = -10.0 + 20.0 * uniform(seed, _F_U10, cols)
+    v = -10.0 + 20.0 * uniform(seed, _F_V10, cols)
+    mr = 1.34 - 1.0e-5 * (np.asarray(channels_nm, dtype=np.float64) - 400.0)
+    return u, v, mr
 
 
 # Named workloads.  nstreams is the per-hemisphere stream count NSTREAMS.

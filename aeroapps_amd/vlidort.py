@@ -103,6 +103,53 @@ def scalar_landmodis(channels, nstreams, plane_parallel, ROT, depol, tau, ssa, g
     return rad, refl, br, int(rc[0])
 
 
+def scalar_cx(channels, nstreams, plane_parallel, ROT, depol, tau, ssa, g, pmom, pe, he, te, U10m, V10m, mr,
+              solar_zenith, relat_azymuth, sensor_zenith, MISSING, verbose=0, precision=0, return_nfourier=False):
+    """VLIDORT_Scalar_CX (src/Components/rtms/geo_vlidort/VLIDORT_BRDF_CX.F90:420-586):
+    Cox-Munk ocean surface from the 10 m wind U10m, V10m [nobs] and the
+    refractive index mr [nch].  Returns (radiance, reflectance, BRDF[1, nobs,
+    nch], rc)."""
+    channels, ROT, depol, tau, ssa, g, pmom, pe, he, te, u, v, m, sza, raa, vza = map(
+        _f64, (channels, ROT, depol, tau, ssa, g, pmom, pe, he, te, U10m, V10m, mr, solar_zenith, relat_azymuth,
+               sensor_zenith))
+    km, nch, nobs = tau.shape
+    if pmom.ndim == 4:
+        pmom = _f64(pmom[..., None])
+    nmom, npol = pmom.shape[3], pmom.shape[4]
+    rad = np.zeros((nobs, nch), order="F")
+    refl = np.zeros((nobs, nch), order="F")
+    brdf = np.zeros((1, nobs, nch), order="F")
+    nf = np.zeros((nobs, nch), dtype=np.int32, order="F")
+    rc = np.zeros(1, dtype=np.int32)
+    e = lib().aa_vlidort_scalar_cx(km, nch, nobs, _ptr(channels), int(nstreams), int(bool(plane_parallel)), nmom,
+                                   npol, _ptr(ROT), _ptr(depol), _ptr(tau), _ptr(ssa), _ptr(g), _ptr(pmom), _ptr(pe),
+                                   _ptr(he), _ptr(te), _ptr(u), _ptr(v), _ptr(m), _ptr(sza), _ptr(raa), _ptr(vza),
+                                   float(MISSING), int(verbose), int(precision), _ptr(rad), _ptr(refl), _ptr(brdf),
+                                   _ptr(nf, _ip), _ptr(rc, _ip))
+    check(e, "aa_vlidort_scalar_cx")
+    if return_nfourier:
+        return rad, refl, brdf, int(rc[0]), nf
+    return rad, refl, brdf, int(rc[0])
+
+
+def vbrdf_coxmunk(nstreams, sza, vza, raa, u10m, v10m, mr):
+    """The Cox-Munk supplement alone for ngeom geometries (see vbrdf_modis)."""
+    sza, vza, raa, u, v, m = (np.ascontiguousarray(np.atleast_1d(a), dtype=np.float64)
+                              for a in (sza, vza, raa, u10m, v10m, mr))
+    ng, n, nm = sza.size, int(nstreams), 2 * int(nstreams)
+    F = np.zeros((nm, n, n, ng), order="F")
+    F0 = np.zeros((nm, n, ng), order="F")
+    UF = np.zeros((nm, n, ng), order="F")
+    UF0 = np.zeros((nm, ng), order="F")
+    db = np.zeros(ng)
+    st = np.zeros(ng, dtype=np.int32)
+    e = lib().aa_vbrdf_coxmunk(n, ng, _ptr(sza), _ptr(vza), _ptr(raa), _ptr(u), _ptr(v), _ptr(m), _ptr(F), _ptr(F0),
+                               _ptr(UF), _ptr(UF0), _ptr(db), _ptr(st, _ip))
+    check(e, "aa_vbrdf_coxmunk")
+    return dict(F=np.moveaxis(F, 3, 0), F0=np.moveaxis(F0, 2, 0), UF=np.moveaxis(UF, 2, 0), UF0=UF0.T.copy(), db=db,
+                status=st)
+
+
 def vbrdf_modis(nstreams, sza, vza, raa, kernel_wt, param):
     """The MODIS BRDF supplement alone for ngeom geometries (degrees), with
     kernel_wt[3, ngeom] and param[2, ngeom]: returns a dict of F[g, M, i, j],

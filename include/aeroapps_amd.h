@@ -191,6 +191,32 @@ int aa_vbrdf_modis(int nstreams, int ngeom, const double* sza, const double* vza
                    const double* kernel_wt, const double* param, double* brdf_f, double* brdf_f0,
                    double* user_brdf_f, double* user_brdf_f0, double* dbounce, int* status);
 
+/* ------------------------------------------------------------------ Cox-Munk ocean surface (BRDF)
+ * Replaces VLIDORT_Scalar_CX (GEO/VLIDORT_BRDF_CX.F90:420-586): per (obs,
+ * channel) the supplement of VLIDORT_CoxMunk_SingleGeom
+ * (V283/VLIDORT_SurfaceMod.F90:815-1042: one Cox-Munk glint kernel with
+ * shadowing, sigma^2 = 0.003 + 0.00512 |(u10m, v10m)| capped at 50 m/s,
+ * refractive index mr, 100 azimuth streams, white-sky check) followed by
+ * VLIDORT_Run_Scalar with that BRDF.  Arrays as aa_vlidort_scalar_lambert
+ * without albedo, plus u10m(nobs), v10m(nobs) [m/s], mr(nch); brdf(1,nobs,nch)
+ * receives the exact direct-bounce BRDF (MISSING where the reference leaves it
+ * undefined: missing geometry, wind or refractive index, which give MISSING
+ * radiances). */
+int aa_vlidort_scalar_cx(int km, int nch, int nobs, const double* channels, int nstreams, int plane_parallel,
+                         int nmom, int npol, const double* rot, const double* depol, const double* tau,
+                         const double* ssa, const double* g, const double* pmom, const double* pe,
+                         const double* he, const double* te, const double* u10m, const double* v10m,
+                         const double* mr, const double* solar_zenith, const double* relat_azymuth,
+                         const double* sensor_zenith, double missing, int verbose, int precision,
+                         double* radiance, double* reflectance, double* brdf, int* nfourier, int* rc);
+
+/* The Cox-Munk supplement alone, as aa_vbrdf_modis with u10m, v10m, mr per
+ * geometry (status 4: white-sky albedo outside (0,1], the kernel's Fourier
+ * terms are then zero). */
+int aa_vbrdf_coxmunk(int nstreams, int ngeom, const double* sza, const double* vza, const double* raa,
+                     const double* u10m, const double* v10m, const double* mr, double* brdf_f, double* brdf_f0,
+                     double* user_brdf_f, double* user_brdf_f0, double* dbounce, int* status);
+
 /* ------------------------------------------------------------------ input preparation
  * The steps either side of the radiance path in a geo_vlidort run
  * (GEO/geo_vlidort_AI.F90:399-430): met fields -> edge variables and tracer

@@ -131,6 +131,39 @@ class ModisRef:
         self.fl.restype = None
         self.fl.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                             _dp, _dp, _dp, _dp, _dp, cd, _dp, _dp, _dp, _ip, _ip]
+        self.fbc = lib.ref_vbrdf_coxmunk
+        self.fbc.restype = None
+        self.fbc.argtypes = [ci] + [cd] * 6 + [_dp] * 6 + [_ip]
+        self.fc = lib.ref_scalar_cx
+        self.fc.restype = None
+        self.fc.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                            _dp, _dp, _dp, _dp, _dp, _dp, cd, _dp, _dp, _dp, _ip, _ip]
+
+    def vbrdf_coxmunk(self, nstreams, sza, vza, raa, u10m, v10m, m):
+        n, nm = nstreams, 2 * nstreams
+        F, F0, UF, UF0 = np.zeros((n, n, nm)), np.zeros((n, nm)), np.zeros((n, nm)), np.zeros(nm)
+        db, wsa = np.zeros(1), np.zeros(1)
+        st = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fbc, n, sza, vza, raa, u10m, v10m, m, _p(F), _p(F0), _p(UF), _p(UF0), _p(db),
+                        _p(wsa), _p(st, np.int32))
+        return dict(F=F.transpose(2, 1, 0), F0=F0.T, UF=UF.T, UF0=UF0, db=float(db[0]), wsa=float(wsa[0]),
+                    status=int(st[0]))
+
+    def scalar_cx(self, km, channels, nstreams, nmom, rot, depol, tau, ssa, g, pmom, pe, he, te, u10m, v10m, mr,
+                  sza, raa, vza, missing=1e15, plane_parallel=False):
+        """VLIDORT_Scalar_CX: as scalar_landmodis with u10m, v10m [nobs], mr [nch]."""
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        channels, rot, depol, tau, ssa, g, pmom, pe, he, te, u10m, v10m, mr, sza, raa, vza = map(
+            f, (channels, rot, depol, tau, ssa, g, pmom, pe, he, te, u10m, v10m, mr, sza, raa, vza))
+        nch, nobs, npol = len(channels), len(sza), pmom.shape[0]
+        rad, refl, br = (np.zeros((nch, nobs)) for _ in range(3))
+        nf = np.zeros((nch, nobs), dtype=np.int32)
+        rc = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fc, km, nch, nobs, _p(channels), nstreams, int(plane_parallel), nmom, npol, _p(rot),
+                        _p(depol), _p(tau), _p(ssa), _p(g), _p(pmom), _p(pe), _p(he), _p(te), _p(u10m), _p(v10m),
+                        _p(mr), _p(sza), _p(raa), _p(vza), missing, _p(rad), _p(refl), _p(br), _p(nf, np.int32),
+                        _p(rc, np.int32))
+        return rad, refl, br, nf, int(rc[0])
 
     def vbrdf_modis(self, nstreams, sza, vza, raa, fiso, fgeo, fvol, p1, p2):
         """Dict F[M,i,j], F0[M,i], UF[M,j], UF0[M], db, wsa, status for one geometry."""
@@ -168,7 +201,7 @@ class ModisRef:
 
 def ref_modis() -> Optional[ModisRef]:
     lib = _load(REF_SO)
-    return None if lib is None or not hasattr(lib, "ref_vbrdf_modis") else ModisRef(lib)
+    return None if lib is None or not hasattr(lib, "ref_vbrdf_coxmunk") else ModisRef(lib)
 
 
 def ref() -> Optional[LambertDriver]:

@@ -3,8 +3,11 @@ for the kernels the AeroApps MODIS land surface uses, the checker for the GPU
 kernel k_brdf (aeroapps_amd/csrc/brdf.hip).  Nothing in the product imports it.
 
 * getquad2            <- VB/vbrdf_sup_aux.f90:180-262 (Gauss-Legendre on [a, b])
-* lambertian / rossthick / lisparse
-                      <- VB/vbrdf_sup_kernels.f90:201-233, 287-332, 478-582
+* lambertian / rossthick / lisparse / coxmunk, derfc_e
+                      <- VB/vbrdf_sup_kernels.f90:201-233, 287-332, 478-582,
+                         1018-1141; VB/vbrdf_sup_aux.f90:266-286
+* coxmunk_setup       <- V283/VLIDORT_SurfaceMod.F90:815-1042
+                         (VLIDORT_CoxMunk_SingleGeom: wind speed -> sigma^2)
 * vbrdf_master        <- VB/vbrdf_sup_masters.f90:1559-3293 (VBRDF_MAINMASTER)
                          with VB/vbrdf_sup_routines.f90:153-687 (VBRDF_MAKER),
                          :1962-2069 (SCALING_FOURIER_ZERO), :2073-2491
@@ -42,7 +45,7 @@ PIO2 = 0.5 * PIE
 PIO4 = 0.25 * PIE
 MAXSTREAMS_SCALING = 24          # vlidort_pars.f90:232
 
-LAMBERTIAN_IDX, ROSSTHICK_IDX, LISPARSE_IDX = 1, 3, 4
+LAMBERTIAN_IDX, ROSSTHICK_IDX, LISPARSE_IDX, COXMUNK_IDX = 1, 3, 4, 9
 
 
 def getquad2(a: float, b: float, n: int):
@@ -126,7 +129,61 @@ def lisparse(pars, xi, sxi, xj, sxj, phi, cphi, sphi):
     return 0.5 * p - q * r
 
 
-KERNELS = {LAMBERTIAN_IDX: lambertian, ROSSTHICK_IDX: rossthick, LISPARSE_IDX: lisparse}
+def derfc_e(x: float) -> float:
+    """The supplement's complementary error function (VB/vbrdf_sup_aux.f90:266-286),
+    a Chebyshev fit with fractional error below 1.2e-7."""
+    z = abs(x)
+    t = 1.0 / (1.0 + 0.5 * z)
+    r = t * math.exp(-z * z - 1.26551223 + t * (1.00002368 + t * (.37409196 + t * (.09678418 + t * (
+        -.18628806 + t * (.27886807 + t * (-1.13520398 + t * (1.48851587 + t * (-.82215223 + t * .17087277)))))))))
+    if x < 0.0:
+        r = 2.0 - r
+    return r
+
+
+def coxmunk(pars, xi, sxi, xj, sxj, phi, ckphi, skphi):
+    """COXMUNK_VFUNCTION (VB/vbrdf_sup_kernels.f90:1018-1141): pars = (sigma^2,
+    m^2, shadow flag)."""
+    ckphi_neg = -ckphi
+    z = xi * xj + sxi * sxj * ckphi_neg
+    if z > 1.0:
+        z = 1.0
+    z1 = math.acos(z)
+    z2 = math.cos(z1 * 0.5)
+    z2_sq_m1 = z2 * z2 + -1.0
+    h1 = pars[1] * z2
+    h2 = math.sqrt(pars[1] + z2_sq_m1)
+    rp = (h1 - h2) / (h1 + h2)
+    rl = (z2 - h2) / (z2 + h2)
+    xmp = 0.5 * (rp * rp + rl * rl)
+    a = 2.0 * z2
+    b = (xi + xj) / a
+    if b > 1.0:
+        b = 1.0
+    a = PIO2 - math.asin(b)
+    ta = math.tan(a)
+    argument = ta * ta / pars[0]
+    k = 0.0
+    if argument < 88.0:
+        prob = math.exp(-argument)
+        fac1 = prob / pars[0]
+        fac2 = 0.25 / xi / (b ** 4)
+        k = xmp * fac1 * fac2 / xj
+    if pars[2] == 0.0:
+        return k
+    s1 = math.sqrt(pars[0] / PIE)
+    s3 = 1.0 / math.sqrt(pars[0])
+    s2 = s3 * s3
+    shadow = []
+    for x in (xi, xj):
+        dcot = x / math.sqrt(1.0 - x * x) if x < 1.0 else math.inf
+        t1 = math.exp(-dcot * dcot * s2)
+        t2 = derfc_e(dcot * s3)
+        shadow.append(0.5 * (s1 * t1 / dcot - t2))
+    return k * (1.0 / (1.0 + shadow[0] + shadow[1]))
+
+
+KERNELS = {LAMBERTIAN_IDX: lambertian, ROSSTHICK_IDX: rossthick, LISPARSE_IDX: lisparse, COXMUNK_IDX: coxmunk}
 
 
 def _dot(a, b):
@@ -141,6 +198,21 @@ def landmodis_setup(fiso: float, fgeo: float, fvol: float, param):
     return [(LAMBERTIAN_IDX, fiso, (0.0, 0.0, 0.0)),
             (ROSSTHICK_IDX, fvol, (0.0, 0.0, 0.0)),
             (LISPARSE_IDX, fgeo, (float(param[0]), float(param[1]), 0.0))]
+
+
+def coxmunk_setup(u10m: float, v10m: float, m: float):
+    """Kernel list of VLIDORT_CoxMunk_SingleGeom (V283/VLIDORT_SurfaceMod.F90:
+    815-1042): one Cox-Munk kernel of factor 1 with sigma^2 = 0.003 + 0.00512 W
+    (default-real literals, W = |(u10m, v10m)| capped at 50 m/s), m^2, shadowing
+    on (the supplement sets the third parameter to 1 for Cox-Munk kernels);
+    100 azimuth streams."""
+    w10 = math.sqrt(u10m * u10m + v10m * v10m)
+    if w10 < 0:
+        w10 = 0.0
+    if w10 > 50:
+        w10 = 50.0
+    sigma2 = float(np.float32(0.003)) + float(np.float32(0.00512)) * w10
+    return [(COXMUNK_IDX, float(np.float32(1.0)), (sigma2, m * m, 1.0))]
 
 
 def vbrdf_master(nstreams: int, sza: float, vza: float, raa: float, kernels, nstreams_brdf_in: int = 50,

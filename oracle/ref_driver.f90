@@ -21,6 +21,9 @@
 !                          V283/VLIDORT_ScatMod.F90:975-1247 (VLIDORT_Run_Scalar, BRDF branch)
 !   ref_vbrdf_modis      <- the BRDF supplement alone for one geometry
 !                          (VLIDORT_LANDMODIS_SingleGeom + the reference's VBRDF_MAINMASTER)
+!   ref_scalar_cx        <- GEO/VLIDORT_BRDF_CX.F90:420-586 (VLIDORT_Scalar_CX)
+!                          V283/VLIDORT_SurfaceMod.F90:815-1042 (VLIDORT_CoxMunk_SingleGeom)
+!   ref_vbrdf_coxmunk    <- the Cox-Munk supplement alone for one geometry
 ! Where the reference leaves BR(j,:) undefined (missing geometry) the harness
 ! writes MISSING.
 ! (GEO = src/Components/rtms/geo_vlidort, V283 = src/Components/rtms/RTSI/VLIDORT2p8p3)
@@ -379,6 +382,62 @@ contains
     B%BS_BSA_VALUE = 0
   end subroutine landmodis_inputs
 
+  ! VBRDF_Sup_In as VLIDORT_CoxMunk_SingleGeom fills it (scalar)
+  ! (V283/VLIDORT_SurfaceMod.F90:880-1000): one Cox-Munk kernel, 100 azimuths.
+  subroutine coxmunk_inputs(B, nstreams, sza, vza, raa, u10m, v10m, m)
+    type(VBRDF_Sup_Inputs), intent(out) :: B
+    integer, intent(in) :: nstreams
+    real*8, intent(in) :: sza, vza, raa, u10m, v10m, m
+    real*8 :: w10m, sigma2
+    call landmodis_inputs(B, nstreams, sza, vza, raa, 0.0d0, 0.0d0, 0.0d0, 0.0d0, 0.0d0)
+    w10m = sqrt(u10m*u10m + v10m*v10m)
+    if (w10m < 0) w10m = 0
+    if (w10m > 50) w10m = 50
+    sigma2 = 0.003 + 0.00512 * w10m
+    B%BS_N_BRDF_KERNELS = 1
+    B%BS_BRDF_NAMES = '          '
+    B%BS_WHICH_BRDF = 0
+    B%BS_BRDF_FACTORS = 0.0d0
+    B%BS_N_BRDF_PARAMETERS = 0
+    B%BS_BRDF_PARAMETERS = 0.0d0
+    B%BS_LAMBERTIAN_KERNEL_FLAG = .false.
+    B%BS_BRDF_NAMES(1) = 'Cox-Munk  '
+    B%BS_WHICH_BRDF(1) = 9
+    B%BS_BRDF_FACTORS(1) = 1.
+    B%BS_N_BRDF_PARAMETERS(1) = 3
+    B%BS_BRDF_PARAMETERS(1,1) = sigma2
+    B%BS_BRDF_PARAMETERS(1,2) = m*m
+    B%BS_BRDF_PARAMETERS(1,3) = 0
+    B%BS_NSTREAMS_BRDF = 100
+    B%BS_DO_SHADOW_EFFECT = .true.
+  end subroutine coxmunk_inputs
+
+  subroutine ref_vbrdf_coxmunk(nstreams, sza, vza, raa, u10m, v10m, m, &
+       F, F0, UF, UF0, db, wsa, status) bind(C, name="ref_vbrdf_coxmunk")
+    integer(c_int), value, intent(in) :: nstreams
+    real(c_double), value, intent(in) :: sza, vza, raa, u10m, v10m, m
+    real(c_double), intent(out) :: F(0:2*nstreams-1,nstreams,nstreams), F0(0:2*nstreams-1,nstreams)
+    real(c_double), intent(out) :: UF(0:2*nstreams-1,nstreams), UF0(0:2*nstreams-1), db, wsa
+    integer(c_int), intent(out) :: status
+    type(VBRDF_Sup_Inputs), allocatable :: B
+    type(VBRDF_Sup_Outputs), allocatable :: O
+    type(VBRDF_Output_Exception_Handling) :: S
+    integer :: k
+    allocate(B, O)
+    call coxmunk_inputs(B, nstreams, sza, vza, raa, u10m, v10m, m)
+    call VBRDF_MAINMASTER(.false., 2*nstreams-1, B, O, S)
+    do k = 0, 2*nstreams-1
+      F(k,:,:) = O%BS_BRDF_F(k,1,1:nstreams,1:nstreams)
+      F0(k,:) = O%BS_BRDF_F_0(k,1,1:nstreams,1)
+      UF(k,:) = O%BS_USER_BRDF_F(k,1,1,1:nstreams)
+      UF0(k) = O%BS_USER_BRDF_F_0(k,1,1,1)
+    end do
+    db = O%BS_DBOUNCE_BRDFUNC(1,1,1,1)
+    wsa = O%BS_WSA_CALCULATED
+    status = S%BS_STATUS_OUTPUT
+    deallocate(B, O)
+  end subroutine ref_vbrdf_coxmunk
+
   ! The BRDF supplement for one geometry: F(0:2N-1,N,N) (incident j -> i),
   ! F0(0:2N-1,N), UF(0:2N-1,N), UF0(0:2N-1), exact direct bounce, total
   ! white-sky albedo and the supplement's output status.
@@ -539,5 +598,120 @@ contains
     deallocate(greekmat_total_input)
     deallocate(V, B)
   end subroutine ref_scalar_landmodis
+
+  subroutine ref_scalar_cx(km, nch, nobs, channels, nstreams, plane_parallel, nMom, &
+       nPol, ROT, depol, tau, ssa, g, pmom, pe, he, te, u10m, v10m, mr, &
+       solar_zenith, relat_azymuth, sensor_zenith, MISSING, &
+       radiance_VL, reflectance_VL, BR, nfourier, rc) bind(C, name="ref_scalar_cx")
+    integer(c_int), value, intent(in) :: km, nch, nobs, nstreams, plane_parallel, nMom, nPol
+    real(c_double), intent(in) :: channels(nch), ROT(km,nobs,nch), depol(nch)
+    real(c_double), intent(in) :: tau(km,nch,nobs), ssa(km,nch,nobs), g(km,nch,nobs)
+    real(c_double), intent(in) :: pmom(km,nch,nobs,nMom,nPol)
+    real(c_double), intent(in) :: pe(km+1,nobs), he(km+1,nobs), te(km+1,nobs)
+    real(c_double), intent(in) :: u10m(nobs), v10m(nobs), mr(nch)
+    real(c_double), intent(in) :: solar_zenith(nobs), relat_azymuth(nobs), sensor_zenith(nobs)
+    real(c_double), value, intent(in) :: MISSING
+    real(c_double), intent(out) :: radiance_VL(nobs,nch), reflectance_VL(nobs,nch), BR(nobs,nch)
+    integer(c_int), intent(out) :: nfourier(nobs,nch), rc
+
+    type(VLIDORT), allocatable :: V
+    type(VBRDF_Sup_Inputs), allocatable :: B
+    integer :: i, j, l, n, p, ier, irc
+    real*8 :: ray_l, alpha_l, tau_l, ssa_l, tau_ext, tau_scat, ssa_tot
+    real*8 :: raysmom2, aerswt, rayswt, depol_ratio
+    real*8, allocatable :: greekmat_total_input(:,:,:)
+    real*8 :: aersmom(0:MAXMOMENTS_INPUT)
+    real*8 :: deltau_vert_input(MAXLAYERS), omega_total_input(MAXLAYERS)
+    real*8, parameter :: pi = 4.*atan(1.0)
+
+    allocate(V, B)
+    allocate(greekmat_total_input(0:MAXMOMENTS_INPUT,MAXLAYERS,16))
+    greekmat_total_input = 0.0d0
+    rc = 0
+    nfourier = 0
+    V%NSTREAMS = nstreams
+    V%DO_PLANE_PARALLEL = (plane_parallel /= 0)
+    V%NGREEK_MOMENTS_INPUT = nMom
+    call VLIDORT_Init(V, km, irc)
+    rc = irc
+    if (rc /= 0) return
+
+    do j = 1, nobs
+      if (is_missing(solar_zenith(j),MISSING) .or. is_missing(sensor_zenith(j),MISSING) .or. &
+          is_missing(relat_azymuth(j),MISSING)) then
+        radiance_VL(j,:) = MISSING
+        reflectance_VL(j,:) = MISSING
+        BR(j,:) = MISSING
+        cycle
+      end if
+      do i = 1, nch
+        if (is_missing(u10m(j),MISSING) .or. is_missing(v10m(j),MISSING) .or. is_missing(mr(i),MISSING)) then
+          radiance_VL(j,i) = MISSING
+          reflectance_VL(j,i) = MISSING
+          BR(j,i) = MISSING
+          cycle
+        end if
+        call coxmunk_inputs(B, nstreams, solar_zenith(j), sensor_zenith(j), relat_azymuth(j), &
+                            u10m(j), v10m(j), mr(i))
+        call VBRDF_MAINMASTER(.false., 2*nstreams-1, B, V%VIO%VBRDF_Sup_Out, V%VIO%VBRDF_Sup_OutputStatus)
+        BR(j,i) = V%VIO%VBRDF_Sup_Out%BS_DBOUNCE_BRDFUNC(1,1,1,1)
+        ier = 0
+        call set_column(V, 1, nMom, 0.0d0, solar_zenith(j), sensor_zenith(j), &
+                        relat_azymuth(j), pe(:,j), he(:,j), te(:,j), km)
+        V%VIO%VLIDORT_FixIn%Bool%TS_DO_LAMBERTIAN_SURFACE = .false.
+        V%VIO%VLIDORT_FixIn%Optical%TS_LAMBERTIAN_ALBEDO = 0.0
+        V%VIO%VLIDORT_Sup%BRDF%TS_BRDF_F_0        = V%VIO%VBRDF_Sup_Out%BS_BRDF_F_0
+        V%VIO%VLIDORT_Sup%BRDF%TS_BRDF_F          = V%VIO%VBRDF_Sup_Out%BS_BRDF_F
+        V%VIO%VLIDORT_Sup%BRDF%TS_USER_BRDF_F_0   = V%VIO%VBRDF_Sup_Out%BS_USER_BRDF_F_0
+        V%VIO%VLIDORT_Sup%BRDF%TS_USER_BRDF_F     = V%VIO%VBRDF_Sup_Out%BS_USER_BRDF_F
+        V%VIO%VLIDORT_Sup%BRDF%TS_EXACTDB_BRDFUNC = V%VIO%VBRDF_Sup_Out%BS_DBOUNCE_BRDFUNC
+        V%VIO%VLIDORT_Sup%BRDF%TS_EMISSIVITY      = V%VIO%VBRDF_Sup_Out%BS_EMISSIVITY
+        V%VIO%VLIDORT_Sup%BRDF%TS_USER_EMISSIVITY = V%VIO%VBRDF_Sup_Out%BS_USER_EMISSIVITY
+        depol_ratio = depol(i)
+        aersmom = 0.0
+        raysmom2 = (1.0 - depol_ratio)/(2.0 + depol_ratio)
+        do l = 1, km
+          ray_l = ROT(l,j,i)
+          alpha_l = 0.0
+          tau_l = tau(l,i,j)
+          ssa_l = ssa(l,i,j)
+          tau_ext = alpha_l + ray_l + tau_l
+          tau_scat = ray_l + ssa_l * tau_l
+          ssa_tot = tau_scat / tau_ext
+          if (ssa_tot > 0.99999) then
+            ssa_tot = 0.99999
+          end if
+          deltau_vert_input(l) = tau_ext
+          omega_total_input(l) = ssa_tot
+          aerswt = ssa_l * tau_l / tau_scat
+          rayswt = ray_l / tau_scat
+          aersmom(0:nMom-1) = pmom(l,i,j,1:nMom,1)
+          greekmat_total_input(0,l,1) = 1.0
+          greekmat_total_input(1,l,1) = aersmom(1) * aerswt
+          greekmat_total_input(2,l,1) = raysmom2 * rayswt + aersmom(2) * aerswt
+          greekmat_total_input(3:nMom,l,1) = aersmom(3:nMom) * aerswt
+        end do
+        V%VIO%VLIDORT_FixIn%Optical%TS_DELTAU_VERT_INPUT = deltau_vert_input
+        V%VIO%VLIDORT_ModIn%MOptical%TS_OMEGA_TOTAL_INPUT = omega_total_input
+        V%VIO%VLIDORT_FixIn%Optical%TS_GREEKMAT_TOTAL_INPUT = greekmat_total_input
+        call VLIDORT_MASTER(V%DO_DEBUG_INPUT, V%VIO%VLIDORT_FixIn, V%VIO%VLIDORT_ModIn, &
+                            V%VIO%VLIDORT_Sup, V%VIO%VLIDORT_Out)
+        if (V%VIO%VLIDORT_Out%Status%TS_STATUS_INPUTCHECK /= 0) ier = 3
+        if (V%VIO%VLIDORT_Out%Status%TS_STATUS_CALCULATION /= 0) ier = 4
+        radiance_VL(j,i) = V%VIO%VLIDORT_Out%Main%TS_STOKES(1,1,1,1)
+        reflectance_VL(j,i) = (pi * radiance_VL(j,i)) / &
+             (cos(V%VIO%VLIDORT_ModIn%MSunRays%TS_SZANGLES(1)*pi/180.0) * &
+              V%VIO%VLIDORT_FixIn%SunRays%TS_FLUX_FACTOR)
+        if (ier /= 0) then
+          radiance_VL(j,i) = MISSING
+          reflectance_VL(j,i) = MISSING
+          cycle
+        end if
+        nfourier(j,i) = V%VIO%VLIDORT_Out%Main%TS_FOURIER_SAVED(1) + 1
+      end do
+    end do
+    deallocate(greekmat_total_input)
+    deallocate(V, B)
+  end subroutine ref_scalar_cx
 
 end module ref_driver_m

@@ -2,12 +2,14 @@
 ! file written by tests/test_gpu_fortran.py, calls the reference-named
 ! procedures (VLIDORT_Scalar_Lambert / VLIDORT_Vector_Lambert from module
 ! VLIDORT_LAMBERT, VLIDORT_Scalar_LandMODIS from module VLIDORT_BRDF_MODIS,
+! VLIDORT_Scalar_CX from module VLIDORT_BRDF_CX,
 ! VLIDORT_getAOPvector with a Chem_Mie object) and writes
 ! their outputs to a raw file.
 program shim_driver
   use Chem_MieMod
   use VLIDORT_LAMBERT
   use VLIDORT_BRDF_MODIS
+  use VLIDORT_BRDF_CX
   implicit none
   character(len=512) :: fin, fout
   integer :: mode, km, nch, nobs, nstreams, nmom, npol, ipp, rc, u, i
@@ -15,6 +17,7 @@ program shim_driver
   real*8, allocatable :: channels(:), rot(:,:,:), depol(:), alpha(:,:,:), tau(:,:,:), ssa(:,:,:), g(:,:,:)
   real*8, allocatable :: pmom(:,:,:,:,:), pe(:,:), he(:,:), te(:,:), albedo(:,:), sza(:), raa(:), vza(:)
   real*8, allocatable :: ff(:,:), rad(:,:), refl(:,:), q(:,:), uu(:,:), kw(:,:,:), kp(:,:,:), br(:,:)
+  real*8, allocatable :: u10(:), v10(:), mr(:), brx(:,:,:)
   ! Mie mode
   integer :: nq, ntab, nrh, nbin, t, b
   type(Chem_Mie) :: mie
@@ -27,7 +30,22 @@ program shim_driver
   call get_command_argument(2, fout)
   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
   read(u) mode
-  if (mode == 4) then
+  if (mode == 5) then
+    ! VLIDORT_Scalar_CX
+    read(u) km, nch, nobs, nstreams, nmom, npol, ipp
+    read(u) missing
+    allocate(channels(nch), rot(km,nobs,nch), depol(nch), tau(km,nch,nobs), ssa(km,nch,nobs), g(km,nch,nobs))
+    allocate(pmom(km,nch,nobs,nmom,npol), pe(km+1,nobs), he(km+1,nobs), te(km+1,nobs))
+    allocate(u10(nobs), v10(nobs), mr(nch), sza(nobs), raa(nobs), vza(nobs))
+    allocate(rad(nobs,nch), refl(nobs,nch), brx(1,nobs,nch))
+    read(u) channels, rot, depol, tau, ssa, g, pmom, pe, he, te, u10, v10, mr, sza, raa, vza
+    close(u)
+    call VLIDORT_Scalar_CX(km, nch, nobs, channels, nstreams, ipp /= 0, nmom, npol, rot, depol, tau, ssa, &
+                           g, pmom, pe, he, te, u10, v10, mr, sza, raa, vza, missing, 0, rad, refl, brx, rc)
+    open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+    write(u) rc, rad, refl, brx
+    close(u)
+  else if (mode == 4) then
     ! VLIDORT_Scalar_LandMODIS
     read(u) km, nch, nobs, nstreams, nmom, npol, ipp
     read(u) missing

@@ -13,6 +13,8 @@
   white-sky-albedo failures and a negative direct bounce.
 * modis_*.npz: VLIDORT_Scalar_LandMODIS radiances of the compiled reference on
   synthetic columns (inputs stored with the outputs).
+* supplement_cx.npz, cx_*.npz: the same for the Cox-Munk ocean surface
+  (VLIDORT_CoxMunk_SingleGeom supplement, VLIDORT_Scalar_CX radiances).
 
 Run here (the reference is only available in this container):
     python tests/golden/make_golden_brdf.py
@@ -126,6 +128,69 @@ def radiance_case(R, ncol, channels, nstreams, nmom, edge, seed=11):
                 out_radiance=rad, out_reflectance=refl, out_br=br, out_nfourier=nf, out_rc=rcr)
 
 
+CX_SUP_CASES = [
+    # (nstreams, sza, vza, raa, u10m, v10m, mr)
+    (6, 30.0, 20.0, 150.0, 3.0, 4.0, 1.34),
+    (6, 40.0, 35.0, 178.0, 7.0, 1.0, 1.33),     # near the specular direction
+    (6, 10.0, 5.0, 0.0, 0.5, 0.2, 1.34),        # calm sea
+    (6, 0.0, 30.0, 180.0, 9.0, -3.0, 1.34),     # overhead sun
+    (6, 50.0, 0.0, 90.0, -6.0, 8.0, 1.335),     # nadir view
+    (6, 30.0, 30.0, 180.0, 60.0, 0.0, 1.34),    # wind capped at 50 m/s
+    (4, 60.0, 50.0, 90.0, 15.0, 12.0, 1.34),
+    (8, 35.0, 40.0, 160.0, 2.0, 2.0, 1.34),
+    (2, 25.0, 15.0, 120.0, 5.0, 5.0, 1.34),
+]
+
+
+def supplement_cx(R):
+    recs = {k: [] for k in ("F", "F0", "UF", "UF0", "db", "wsa", "status")}
+    nmax = max(c[0] for c in CX_SUP_CASES)
+    for c in CX_SUP_CASES:
+        r = R.vbrdf_coxmunk(*c)
+        n = c[0]
+        F = np.zeros((2 * nmax, nmax, nmax)); F[:2 * n, :n, :n] = r["F"]
+        F0 = np.zeros((2 * nmax, nmax)); F0[:2 * n, :n] = r["F0"]
+        UF = np.zeros((2 * nmax, nmax)); UF[:2 * n, :n] = r["UF"]
+        UF0 = np.zeros(2 * nmax); UF0[:2 * n] = r["UF0"]
+        for k, v in (("F", F), ("F0", F0), ("UF", UF), ("UF0", UF0), ("db", r["db"]), ("wsa", r["wsa"]),
+                     ("status", r["status"])):
+            recs[k].append(v)
+    out = {k: np.asarray(v) for k, v in recs.items()}
+    out["cases"] = np.asarray(CX_SUP_CASES, dtype=np.float64)
+    return out
+
+
+CX_CASES = {
+    # name: (ncol, channels, nstreams, nmom, edge)
+    "cx_n6_2ch": (6, (470.0, 660.0), 6, 64, False),
+    "cx_n4_edge": (9, (550.0,), 4, 32, True),
+    "cx_n8": (3, (550.0,), 8, 32, False),
+}
+
+
+def cx_case(R, ncol, channels, nstreams, nmom, edge, seed=13):
+    cols = S.make_columns(ncol, 72, channels, seed=seed)
+    tabs = S.make_mie_tables(channels, nmom=nmom, npol=1)
+    tau, ssa, g, pm, rc = CM.get_aop_vector(tabs, channels, tabs[0].lambda_m, S.TRACERS, cols.qm, cols.rh, nmom, 1)
+    assert rc == 0
+    u, v, mr = S.make_ocean(ncol, channels, seed=seed)
+    sza, vza, raa = cols.sza.copy(), cols.vza.copy(), cols.raa.copy()
+    if edge:
+        vza[1] = 0.0                 # nadir view
+        sza[2] = S.MISSING           # missing geometry
+        u[3] = S.MISSING             # missing wind: element MISSING
+        sza[4], vza[4], raa[4] = 35.0, 35.0, 180.0   # specular glint
+        sza[5] = 0.0                 # overhead sun latches nadir FO for later columns
+        u[7], v[7] = 60.0, 10.0      # wind capped at 50 m/s
+    f = lambda a: np.asarray(a, dtype=np.float64)
+    rad, refl, br, nf, rcr = R.scalar_cx(72, cols.channels, nstreams, nmom, cols.rot, cols.depol, f(tau), f(ssa),
+                                         f(g), f(pm), cols.pe, cols.he, cols.te, u, v, mr, sza, raa, vza)
+    return dict(km=72, channels=np.asarray(channels, float), nmom=nmom, npol=1, meta_nstreams=nstreams,
+                tau=tau, ssa=ssa, g=g, pmom=pm, rot=cols.rot, depol=cols.depol, pe=cols.pe, he=cols.he,
+                te=cols.te, u10m=u, v10m=v, mr=mr, sza=sza, vza=vza, raa=raa,
+                out_radiance=rad, out_reflectance=refl, out_br=br, out_nfourier=nf, out_rc=rcr)
+
+
 def main():
     os.makedirs(OUT, exist_ok=True)
     with open(os.path.join(OUT, "selftest.json"), "w") as fh:
@@ -135,6 +200,11 @@ def main():
     np.savez_compressed(os.path.join(OUT, "supplement.npz"), **supplement(R))
     for name, c in RAD_CASES.items():
         d = radiance_case(R, *c)
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), **d)
+        print(name, d["out_rc"], d["out_radiance"].ravel()[:4], d["out_nfourier"].ravel()[:6])
+    np.savez_compressed(os.path.join(OUT, "supplement_cx.npz"), **supplement_cx(R))
+    for name, c in CX_CASES.items():
+        d = cx_case(R, *c)
         np.savez_compressed(os.path.join(OUT, name + ".npz"), **d)
         print(name, d["out_rc"], d["out_radiance"].ravel()[:4], d["out_nfourier"].ravel()[:6])
 

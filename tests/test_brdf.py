@@ -113,3 +113,42 @@ def test_fourier_series_reconstructs_kernel():
         exact = B.rossthick(None, mu0, math.sqrt(1 - mu0 ** 2), muv, math.sqrt(1 - muv ** 2), phi, math.cos(phi),
                             math.sin(phi))
         assert abs(series - exact) < 2e-3
+
+
+# ---------------------------------------------------------------- Cox-Munk
+def test_coxmunk_vs_compiled_reference_supplement():
+    z = np.load(os.path.join(BDIR, "supplement_cx.npz"), allow_pickle=False)
+    for c, case in enumerate(z["cases"]):
+        n = int(case[0])
+        sza, vza, raa, u, v, m = (float(x) for x in case[1:])
+        r = B.vbrdf_master(n, sza, vza, raa, B.coxmunk_setup(u, v, m), 100)
+        assert r["ok"] == (int(z["status"][c]) == 0), c
+        assert abs(r["db"] - z["db"][c]) <= 1e-15 * abs(z["db"][c]) + 1e-300, c
+        for key, got in (("F", r["F"]), ("F0", r["F0"]), ("UF", r["UF"]), ("UF0", r["UF0"])):
+            want = z[key][c][tuple(slice(0, 2 * n) if i == 0 else slice(0, n) for i in range(got.ndim))]
+            assert np.max(np.abs(got - want)) <= 1e-14 * np.max(np.abs(want)), (c, key)
+
+
+def test_coxmunk_closed_forms():
+    # nadir sun and view: Fresnel reflectance at normal incidence ((m-1)/(m+1))^2
+    # over 4 sigma^2, no shadowing (the shadow terms vanish at nadir)
+    for m, (u, v) in ((1.34, (3.0, 4.0)), (1.33, (10.0, 0.0))):
+        kern = B.coxmunk_setup(u, v, m)[0]
+        s2 = kern[2][0]
+        k = B.coxmunk((s2, m * m, 1.0), 1.0, 0.0, 1.0, 0.0, 0.0, 1.0, 0.0)
+        assert abs(k - ((m - 1) / (m + 1)) ** 2 / (4 * s2)) <= 1e-15
+    # sigma^2 = 0.003 + 0.00512 |w| with the reference's float32 literals, |w| capped at 50
+    assert B.coxmunk_setup(3.0, 4.0, 1.34)[0][2][0] == float(np.float32(0.003)) + float(np.float32(0.00512)) * 5.0
+    assert B.coxmunk_setup(60.0, 0.0, 1.34)[0][2][0] == B.coxmunk_setup(50.0, 0.0, 1.34)[0][2][0]
+    # the supplement's erfc fit: fractional error below its published 1.2e-7
+    for x in np.linspace(-2.0, 4.0, 61):
+        assert abs(B.derfc_e(x) - math.erfc(x)) <= 1.2e-7 * math.erfc(x)
+    # reciprocity of the glint kernel with shadowing
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        a, b = np.radians(rng.uniform(1, 80, 2))
+        ph = rng.uniform(0, math.pi)
+        p = (0.03, 1.34 ** 2, 1.0)
+        k1 = B.coxmunk(p, math.cos(a), math.sin(a), math.cos(b), math.sin(b), ph, math.cos(ph), math.sin(ph))
+        k2 = B.coxmunk(p, math.cos(b), math.sin(b), math.cos(a), math.sin(a), ph, math.cos(ph), math.sin(ph))
+        assert abs(k1 - k2) <= 1e-13 * max(abs(k1), 1e-300)

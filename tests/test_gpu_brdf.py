@@ -121,3 +121,52 @@ def test_fused_modis_equals_host_path(gpu):
     assert np.array_equal(nf.cpu().numpy(), nf1.T)
     assert I[3, 1] == -500.0
     mie.close()
+
+
+# ---------------------------------------------------------------- Cox-Munk ocean
+CX_CASES = ["cx_n6_2ch", "cx_n4_edge", "cx_n8"]
+
+
+def test_coxmunk_supplement_vs_reference(gpu):
+    z = _load("supplement_cx")
+    for c, case in enumerate(z["cases"]):
+        n = int(case[0])
+        sza, vza, raa, u, v, m = (float(x) for x in case[1:])
+        print(f"case cx supplement {c} N={n}", flush=True)
+        r = V.vbrdf_coxmunk(n, [sza], [vza], [raa], [u], [v], [m])
+        assert abs(r["db"][0] - z["db"][c]) <= 1e-13 * abs(z["db"][c]), c
+        assert r["status"][0] == int(z["status"][c]), c
+        for key in ("F", "F0", "UF", "UF0"):
+            got = r[key][0]
+            want = z[key][c][tuple(slice(0, 2 * n) if i == 0 else slice(0, n) for i in range(got.ndim))]
+            assert np.max(np.abs(got - want)) <= 1e-13 * np.max(np.abs(want)), (c, key)
+
+
+def _run_cx(d, precision):
+    f = lambda a: np.asarray(a, dtype=np.float64)
+    return V.scalar_cx(d["channels"], int(d["meta_nstreams"]), False, d["rot"].T, d["depol"], f(d["tau"]).T,
+                       f(d["ssa"]).T, f(d["g"]).T, f(d["pmom"]).T, d["pe"].T, d["he"].T, d["te"].T, d["u10m"],
+                       d["v10m"], d["mr"], d["sza"], d["raa"], d["vza"], S.MISSING, precision=precision,
+                       return_nfourier=True)
+
+
+@pytest.mark.parametrize("name", CX_CASES)
+def test_scalar_cx_fp64_vs_reference(gpu, name):
+    print(f"case cx fp64 {name}", flush=True)
+    d = _load(name)
+    rad, refl, brdf, rc, nf = _run_cx(d, 0)
+    assert rc == int(d["out_rc"]) == 0
+    assert max_rel(rad, d["out_radiance"].T) < RTOL_FP64
+    assert max_rel(refl, d["out_reflectance"].T) < RTOL_FP64
+    assert max_rel(brdf[0], d["out_br"].T) < 1e-13
+    assert np.array_equal(nf, d["out_nfourier"].T)
+
+
+def test_scalar_cx_mixed_vs_reference(gpu):
+    print("case cx mixed", flush=True)
+    d = _load("cx_n6_2ch")
+    rad, refl, brdf, rc, nf = _run_cx(d, 1)
+    want, nf0 = d["out_radiance"].T, d["out_nfourier"].T
+    same = nf == nf0
+    assert same.mean() >= 0.9
+    assert np.max(np.abs(rad[same] - want[same]) / np.abs(want[same])) < 1e-5

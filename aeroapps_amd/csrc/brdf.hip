@@ -79,49 +79,23 @@ std::vector<double> build_brdf_table(int N, int A) {
   return t;
 }
 
-// One item's supplement.  kw = (fiso, fgeo, fvol), kp = (h/b, b/r).  Lane 0
-// writes the direct bounce to *db_out and, when the Fourier components were
-// computed, whether the white-sky-albedo check dropped Li-sparse to
-// *wsa_fail.  The Fourier components are skipped when `skip` is set or, with
-// `stop_negative`, for a negative direct bounce (the MODIS driver does not
-// run VLIDORT then).
+// White-sky albedo of Li-sparse for (h/b, b/r) (SCALING_FOURIER_ZERO, half
+// range, then the WSA sums of VBRDF_MAINMASTER), computed by the 64 lanes of
+// one wavefront; every lane returns the value.
 template <int N>
-__device__ void brdf_item(const double* __restrict__ tab, const double* kw, const double* kp, double sza, double vza,
-                          double raa, double* __restrict__ rec, double* db_out, int* wsa_fail, bool skip,
-                          bool stop_negative) {
+__device__ double li_wsa(const double* __restrict__ tab, double hb, double br) {
 #pragma clang fp contract(off)
   constexpr BrTab T(N);
-  constexpr BROff RO(N);
-  constexpr int NT = N * N + 2 * N + 1;     // direction pairs of the Fourier components
   __shared__ double sang[BR_NSC][4];        // Li-sparse transforms of the scaling streams
-  __shared__ double dang[N + 2][4];         // ... of the quadrature streams, sun, view
   __shared__ double sbf[BR_NSC * BR_NSC];
   __shared__ double rsum[BR_NSC];
   const int t = threadIdx.x;
-  const double fiso = kw[0], fgeo = kw[1], fvol = kw[2], hb = kp[0], br = kp[1];
-  const double db = modis_db(fiso, fgeo, fvol, hb, br, sza, vza, raa);
-  if (t == 0) {
-    *db_out = db;
-    rec[RO.db()] = db;
-  }
-  if (skip || (stop_negative && db < 0.0)) return;   // uniform over the wavefront
-
-  // ---- white-sky albedo of Li-sparse (SCALING_FOURIER_ZERO, half range)
   if (t < BR_NSC) {
     const LiAng a = li_angle(br, tab[T.sq(t)], tab[T.ssin(t)]);
     sang[t][0] = a.t; sang[t][1] = a.t2; sang[t][2] = a.x; sang[t][3] = a.sx;
   }
-  const double mu0 = cos(sza * DEG_TO_RAD), s0 = sqrt(1.0 - mu0 * mu0);
-  const double muv = cos(vza * DEG_TO_RAD), sv = sqrt(1.0 - muv * muv);
-  if (t < N + 2) {
-    const double x = (t < N) ? tab[T.qs(t)] : (t == N ? mu0 : muv);
-    const double sx = (t < N) ? tab[T.qsin(t)] : (t == N ? s0 : sv);
-    const LiAng a = li_angle(br, x, sx);
-    dang[t][0] = a.t; dang[t][1] = a.t2; dang[t][2] = a.x; dang[t][3] = a.sx;
-  }
   __syncthreads();
   auto sa = [&](int i) { LiAng a; a.t = sang[i][0]; a.t2 = sang[i][1]; a.x = sang[i][2]; a.sx = sang[i][3]; return a; };
-  auto da = [&](int i) { LiAng a; a.t = dang[i][0]; a.t2 = dang[i][1]; a.x = dang[i][2]; a.sx = dang[i][3]; return a; };
   for (int p = t; p < BR_NSC * BR_NSC; p += 64) {
     const int i = p / BR_NSC, j = p - i * BR_NSC;     // incident j, reflected i
     const LiAng in = sa(j), rf = sa(i);
@@ -139,7 +113,48 @@ __device__ void brdf_item(const double* __restrict__ tab, const double* kw, cons
   __syncthreads();
   double help = 0.0;
   for (int i = 0; i < BR_NSC; ++i) help = help + rsum[i] * tab[T.ssw(i)];
-  const double wsa_li = help * 4.0 / PIE;
+  return help * 4.0 / PIE;
+}
+
+// One item's supplement.  kw = (fiso, fgeo, fvol), kp = (h/b, b/r).  Lane 0
+// writes the direct bounce to *db_out and, when the Fourier components were
+// computed, whether the white-sky-albedo check dropped Li-sparse to
+// *wsa_fail.  The Fourier components are skipped when `skip` is set or, with
+// `stop_negative`, for a negative direct bounce (the MODIS driver does not
+// run VLIDORT then).
+// wcache (may be null) holds (h/b, b/r, white-sky albedo) of one parameter
+// pair; items with bit-identical parameters take the albedo from it.
+template <int N>
+__device__ void brdf_item(const double* __restrict__ tab, const double* kw, const double* kp, double sza, double vza,
+                          double raa, double* __restrict__ rec, double* db_out, int* wsa_fail, bool skip,
+                          bool stop_negative, const double* wcache = nullptr) {
+#pragma clang fp contract(off)
+  constexpr BrTab T(N);
+  constexpr BROff RO(N);
+  constexpr int NT = N * N + 2 * N + 1;     // direction pairs of the Fourier components
+  __shared__ double dang[N + 2][4];         // Li-sparse transforms of the quadrature streams, sun, view
+  const int t = threadIdx.x;
+  const double fiso = kw[0], fgeo = kw[1], fvol = kw[2], hb = kp[0], br = kp[1];
+  const double db = modis_db(fiso, fgeo, fvol, hb, br, sza, vza, raa);
+  if (t == 0) {
+    *db_out = db;
+    rec[RO.db()] = db;
+  }
+  if (skip || (stop_negative && db < 0.0)) return;   // uniform over the wavefront
+
+  // ---- white-sky albedo of Li-sparse
+  const bool cached = wcache && hb == wcache[0] && br == wcache[1];   // uniform over the wavefront
+  const double wsa_li = cached ? wcache[2] : li_wsa<N>(tab, hb, br);
+  const double mu0 = cos(sza * DEG_TO_RAD), s0 = sqrt(1.0 - mu0 * mu0);
+  const double muv = cos(vza * DEG_TO_RAD), sv = sqrt(1.0 - muv * muv);
+  if (t < N + 2) {
+    const double x = (t < N) ? tab[T.qs(t)] : (t == N ? mu0 : muv);
+    const double sx = (t < N) ? tab[T.qsin(t)] : (t == N ? s0 : sv);
+    const LiAng a = li_angle(br, x, sx);
+    dang[t][0] = a.t; dang[t][1] = a.t2; dang[t][2] = a.x; dang[t][3] = a.sx;
+  }
+  __syncthreads();
+  auto da = [&](int i) { LiAng a; a.t = dang[i][0]; a.t2 = dang[i][1]; a.x = dang[i][2]; a.sx = dang[i][3]; return a; };
   double total = fiso * 1.0;
   total = total + fvol * tab[T.wsa_ross()];
   total = total + fgeo * wsa_li;
@@ -283,8 +298,8 @@ __device__ void brdf_item_cx(const double* __restrict__ tab, double sigma2, doub
   }
 }
 
-template <int N>
-__global__ __launch_bounds__(64) void k_brdf(RTWork W, int nitems) {
+template <int N, bool CX>
+__global__ __launch_bounds__(64, 2) void k_brdf(RTWork W, int nitems) {
   const int it = blockIdx.x;
   if (it >= nitems) return;
   const int g = global_item(W, it);
@@ -298,16 +313,16 @@ __global__ __launch_bounds__(64) void k_brdf(RTWork W, int nitems) {
   const size_t kq = (size_t)obs * W.nch + ch;
   __shared__ double dbv;
   __shared__ int wf;
-  if (W.kernel_wt) {
+  if constexpr (!CX) {
     brdf_item<N>(W.brdf_tab, W.kernel_wt + kq * 3, W.kparam + kq * 2, W.sza[obs], W.vza[obs], W.raa[obs],
-                 W.brdf + (size_t)it * W.brdf_stride, &dbv, &wf, st != 0, true);
+                 W.brdf + (size_t)it * W.brdf_stride, &dbv, &wf, st != 0, true, W.wsa_cache);
   } else {
     const double m = W.mr[ch];
     brdf_item_cx<N>(W.brdf_tab, cm_sigma2(W.u10m[obs], W.v10m[obs]), m * m, W.sza[obs], W.vza[obs], W.raa[obs],
                     W.brdf + (size_t)it * W.brdf_stride, &dbv, &wf, st != 0);
   }
   __syncthreads();
-  if (!W.kernel_wt) {   // the Cox-Munk drivers keep any sign of the direct bounce
+  if constexpr (CX) {   // the Cox-Munk drivers keep any sign of the direct bounce
     if (threadIdx.x == 0 && W.br_out) W.br_out[oi] = dbv;
     return;
   }
@@ -316,6 +331,18 @@ __global__ __launch_bounds__(64) void k_brdf(RTWork W, int nitems) {
     if (W.br_out) W.br_out[oi] = (dbv < 0.0) ? -500.0 : dbv;
     if (dbv < 0.0) W.status[it] = 6;
   }
+}
+
+// White-sky albedo of the first item's Li-sparse parameters into W.wsa_cache
+// (one wavefront), reused by k_brdf for every item with the same parameters.
+template <int N>
+__global__ __launch_bounds__(64) void k_li_wsa_cache(RTWork W) {
+  const int g = global_item(W, 0);
+  const int obs = g / W.nch, ch = g - obs * W.nch;
+  const double* kp = W.kparam + ((size_t)obs * W.nch + ch) * 2;
+  const double hb = kp[0], br = kp[1];
+  const double w = li_wsa<N>(W.brdf_tab, hb, br);
+  if (threadIdx.x == 0) { W.wsa_cache[0] = hb; W.wsa_cache[1] = br; W.wsa_cache[2] = w; }
 }
 
 // Supplement alone for explicit geometries (aa_vbrdf_modis): item g uses
@@ -367,7 +394,12 @@ int rt_brdf(const RTWork& W, int nitems, hipStream_t st) {
   // (the table in W.brdf_tab was built for W.N and the surface's azimuth count)
   if (nitems <= 0) return 0;
   switch (W.N) {
-#define AA_BR(NN) case NN: hipLaunchKernelGGL(k_brdf<NN>, dim3(nitems), dim3(64), 0, st, W, nitems); break;
+#define AA_BR(NN)                                                                                    \
+  case NN:                                                                                           \
+    if (W.kernel_wt && W.wsa_cache) hipLaunchKernelGGL(k_li_wsa_cache<NN>, dim3(1), dim3(64), 0, st, W); \
+    if (W.kernel_wt) hipLaunchKernelGGL((k_brdf<NN, false>), dim3(nitems), dim3(64), 0, st, W, nitems);  \
+    else hipLaunchKernelGGL((k_brdf<NN, true>), dim3(nitems), dim3(64), 0, st, W, nitems);             \
+    break;
     AA_BR(1) AA_BR(2) AA_BR(3) AA_BR(4) AA_BR(5) AA_BR(6) AA_BR(7) AA_BR(8)
 #undef AA_BR
     default: return -4;

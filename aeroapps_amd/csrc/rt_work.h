@@ -59,6 +59,7 @@ struct RTWork {
   const double* brdf_tab = nullptr;   // azimuth / scaling quadrature tables (BrTab)
   double* brdf = nullptr;             // [it][brdf_stride] Fourier components of the BRDF
   int brdf_stride = 0;
+  double* wsa_cache = nullptr;        // [4]: h/b, b/r and Li-sparse white-sky albedo of the chunk's first item
   const double* sza = nullptr;     // (nobs)
   const double* raa = nullptr;
   const double* vza = nullptr;

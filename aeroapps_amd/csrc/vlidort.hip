@@ -265,6 +265,7 @@ size_t carve_work(Carver& C, RTWork& W, int cap) {
   W.lbuf = C.take<double>((size_t)cap * km * W.lstride);
   W.bbuf = C.take<double>((size_t)cap * W.bstride);
   W.brdf = (W.kernel_wt || W.u10m) ? C.take<double>((size_t)cap * W.brdf_stride) : nullptr;
+  W.wsa_cache = W.kernel_wt ? C.take<double>(4) : nullptr;
   W.ubuf = C.take<double>((size_t)cap * (2 * n * km) * W.ustride);
   W.sab32 = (W.precision == 1) ? C.take<float>((size_t)cap * km * 2 * n * n) : nullptr;
   return C.off;

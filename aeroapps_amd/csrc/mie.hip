@@ -78,6 +78,7 @@ __global__ void __launch_bounds__(64) get_aop_kernel(MieDev T, int km, int nobs,
                                                      float* __restrict__ tau_o, float* __restrict__ ssa_o,
                                                      float* __restrict__ g_o, float* __restrict__ pmom_o) {
   __shared__ float4 info[64];          // (node, arh, ssa_i, tau_i) of lane (layer, tracer)
+  __shared__ int4 rows[64];            // (lower row, upper row, P22/P44 tied, tracer found) of lane (layer, tracer)
   __shared__ float gin[64];            // g_i of lane (layer, tracer)
   __shared__ float stl[MIE_KC_MAX];    // ssa*tau of the chunk's layers (normalisation)
   const int lane = threadIdx.x;
@@ -105,7 +106,10 @@ __global__ void __launch_bounds__(64) get_aop_kernel(MieDev T, int km, int nobs,
           const float g_in = lin(T.g[tb + s.irh], T.g[tb + s.irhp1], s.arh);
           const float q = qm[((long)i * nq + iq) * km + k];
           info[lane] = make_float4(__int_as_float(s.irh), s.arh, __fdiv_rn(bsca_in, bext_in), __fmul_rn(bext_in, q));
+          rows[lane] = make_int4((int)tb + s.irh, (int)tb + s.irhp1, (npol == 6 && T.npol_t[tr] == 4) ? 1 : 0, 1);
           gin[lane] = g_in;
+        } else {
+          rows[lane] = make_int4(0, 0, 0, 0);
         }
       }
     }
@@ -115,7 +119,7 @@ __global__ void __launch_bounds__(64) get_aop_kernel(MieDev T, int km, int nobs,
     if (lane < kc) {
       float tau = 0.0f, ssa = 0.0f, g = 0.0f;
       for (int iq = 0; iq < ntr_eff; ++iq) {
-        if (tracer_idx[iq] < 0) continue;
+        if (rows[lane * ntr + iq].w == 0) continue;
         const float4 inf = info[lane * ntr + iq];
         const float ssa_ = inf.z, tau_ = inf.w;
         tau = __fadd_rn(tau, tau_);
@@ -150,19 +154,17 @@ __global__ void __launch_bounds__(64) get_aop_kernel(MieDev T, int km, int nobs,
 #pragma unroll
           for (int kk = 0; kk < MIE_KC_MAX; ++kk) { acc_a[kk] = 0.0f; acc_b[kk] = 0.0f; }
           for (int iq = 0; iq < ntr_eff; ++iq) {
-            const int tr = tracer_idx[iq];
-            if (tr < 0) continue;
-            const bool sph = (npol == 6 && T.npol_t[tr] == 4);
-            const int nrh = T.nrh_t[tr];
-            const long pb0 = ((long)tr * T.nch + ich) * T.nrh;
+            // tracer found and tied polarisations: the same for every layer
+            const int4 r0 = rows[iq];
+            if (r0.w == 0) continue;
+            const bool sph = r0.z != 0;
 #pragma unroll
             for (int kk = 0; kk < MIE_KC_MAX; ++kk) {
               if (kk >= kc) break;
               const float4 inf = info[kk * ntr + iq];
-              const int irh = __float_as_int(inf.x);
-              const int irhp1 = (irh + 1 > nrh - 1) ? nrh - 1 : irh + 1;
-              const float* plo = T.pmom + (pb0 + irh) * rowstride + mr;
-              const float* phi = T.pmom + (pb0 + irhp1) * rowstride + mr;
+              const int4 rr = rows[kk * ntr + iq];
+              const float* plo = T.pmom + (long)rr.x * rowstride + mr;
+              const float* phi = T.pmom + (long)rr.y * rowstride + mr;
               const float pma = lin(plo[pa * T.nmom], phi[pa * T.nmom], inf.y);
               acc_a[kk] = __fadd_rn(acc_a[kk], __fmul_rn(__fmul_rn(pma, inf.z), inf.w));
               if (pb >= 0) {
@@ -182,8 +184,9 @@ __global__ void __launch_bounds__(64) get_aop_kernel(MieDev T, int km, int nobs,
               const float st = stl[kk];
               float va = acc_a[kk], vb = acc_b[kk];
               if (normalize && st > 0.0f) { va = __fdiv_rn(va, st); vb = __fdiv_rn(vb, st); }
-              pmom_o[((long)pa * nmom + m) * ostride + obase + k0 + kk] = va;
-              if (pb >= 0) pmom_o[((long)pb * nmom + m) * ostride + obase + k0 + kk] = vb;
+              const long e = obase + k0 + kk;
+              pmom_o[((long)pa * nmom + m) * ostride + e] = va;
+              if (pb >= 0) pmom_o[((long)pb * nmom + m) * ostride + e] = vb;
             }
           }
         }
@@ -331,6 +334,8 @@ int launch_get_aop(const aa_mie* t, int km, int nobs, int nch, int nq, const int
   if (ne == 0) return 0;
   const long blocks = (long)nobs * nch;
   if (ntr_eff > MAXTR || blocks > 0x7fffffffL) return -1;
+  // table rows are addressed with 32-bit row indices
+  if ((long)t->dev.ntracer * t->dev.nch * t->dev.nrh > 0x7fffffffL) return -1;
   prof_begin(PROF_MIE, st);
   hipLaunchKernelGGL(get_aop_kernel, dim3((unsigned)blocks), dim3(64), 0, st, t->dev, km, nobs, nch, nq,
                      d_tracer_idx, d_chan_idx, d_qm, d_rh, nmom, npol, ntr_eff, normalize, d_tau, d_ssa, d_g,

@@ -73,43 +73,55 @@ __device__ inline void find_sun(bool donadir, bool overhead, double radius, cons
   theta = acos(ctheta);
 }
 
-// sum_k ext(k) * sunpath(k) for a direct sun (FOC/FO_geometry_Generic.f90:164-213).
-template <class Ext>
-__device__ inline double sunpaths_d_sum(bool zerosunboa, double radstart, const ColumnGeo& G, double thstart,
-                                        double sthstart, int n, const Ext& ext) {
+// For each channel c in the mask cm: s[c] = sum_k ext(k, c) * sunpath(k) for a
+// direct sun (FOC/FO_geometry_Generic.f90:164-213).  The path lengths depend on
+// the column only and are computed once for all channels; each channel's sum
+// keeps the reference's term order.
+#define AA_FOR_CH(c) _Pragma("unroll") for (int c = 0; c < CH; ++c) if ((cm >> c) & 1u)
+template <int CH, class Ext>
+__device__ inline void sunpaths_d_sum(bool zerosunboa, double radstart, const ColumnGeo& G, double thstart,
+                                      double sthstart, int n, const Ext& ext, unsigned cm, double (&s)[CH]) {
   const int n1 = n - 1;
   if (zerosunboa) {
-    double s = ext(n) * (G.radius(n1) - radstart);
-    for (int k = n1; k >= 1; --k) s += ext(k) * (G.radius(k - 1) - G.radius(k));
-    return s;
+    const double p = G.radius(n1) - radstart;
+    AA_FOR_CH(c) s[c] = ext(n, c) * p;
+    for (int k = n1; k >= 1; --k) {
+      const double q = G.radius(k - 1) - G.radius(k);
+      AA_FOR_CH(c) s[c] += ext(k, c) * q;
+    }
+    return;
   }
   double sth0 = sthstart, th0 = thstart;
   double sth1 = sth0 * radstart / G.radius(n1);
   double th1 = asin(sth1);
   double ks1 = th0 - th1;
-  double s = ext(n) * (sin(ks1) * radstart / sth1);
+  {
+    const double p = sin(ks1) * radstart / sth1;
+    AA_FOR_CH(c) s[c] = ext(n, c) * p;
+  }
   sth0 = sth1; th0 = th1;
   for (int k = n1; k >= 1; --k) {
     sth1 = sth0 * G.radius(k) / G.radius(k - 1);
     th1 = asin(sth1);
     ks1 = th0 - th1;
-    s += ext(k) * (sin(ks1) * G.radius(k) / sth1);
+    const double p = sin(ks1) * G.radius(k) / sth1;
+    AA_FOR_CH(c) s[c] += ext(k, c) * p;
     sth0 = sth1; th0 = th1;
   }
-  return s;
 }
 
 // Same for a sun behind the tangent point (FOC/FO_geometry_Generic.f90:217-290);
 // the path sum covers levels 1..nt.
-template <class Ext>
-__device__ inline double sunpaths_t_sum(double radstart, const ColumnGeo& G, double thstart, double sthstart, int n,
-                                        int L, const Ext& ext) {
+template <int CH, class Ext>
+__device__ inline void sunpaths_t_sum(double radstart, const ColumnGeo& G, double thstart, double sthstart, int n,
+                                      int L, const Ext& ext, unsigned cm, double (&s)[CH]) {
   const int n1 = n - 1;
   const double tanr = sthstart * radstart;
   int k = n1; bool trawl = true;
   while (k >= n1 && trawl && k <= L + 1) { trawl = (G.radius(k) > tanr); k = k + 1; }
   const int nt = k - 1;
-  double s = 0.0, spn = 0.0;
+  AA_FOR_CH(c) s[c] = 0.0;
+  double spn = 0.0;
   double sth0, th0, sth1, th1, ks1;
   if (nt > n) {
     th0 = PIE - thstart; sth0 = sthstart;
@@ -120,11 +132,13 @@ __device__ inline double sunpaths_t_sum(double radstart, const ColumnGeo& G, dou
     for (int kk = n + 1; kk <= nt - 1; ++kk) {
       sth1 = sth0 * G.radius(kk - 1) / G.radius(kk);
       th1 = asin(sth1); ks1 = th0 - th1;
-      s += ext(kk) * (2.0 * sin(ks1) * G.radius(kk) / sth0);
+      const double p = 2.0 * sin(ks1) * G.radius(kk) / sth0;
+      AA_FOR_CH(c) s[c] += ext(kk, c) * p;
       sth0 = sth1; th0 = th1;
     }
     ks1 = 0.5 * PIE - th0;
-    s += ext(nt) * (2.0 * sin(ks1) * G.radius(nt - 1));
+    const double p = 2.0 * sin(ks1) * G.radius(nt - 1);
+    AA_FOR_CH(c) s[c] += ext(nt, c) * p;
   } else if (nt == n) {
     spn = -2.0 * radstart * cos(thstart);
   }
@@ -133,14 +147,16 @@ __device__ inline double sunpaths_t_sum(double radstart, const ColumnGeo& G, dou
   th1 = asin(sth1); ks1 = th0 - th1;
   spn = spn + sin(ks1) * radstart / sth1;
   sth0 = sth1; th0 = th1;
-  if (n <= nt) s += ext(n) * spn;
+  if (n <= nt) { AA_FOR_CH(c) s[c] += ext(n, c) * spn; }
   for (int kk = n1; kk >= 1; --kk) {
     sth1 = sth0 * G.radius(kk) / G.radius(kk - 1);
     th1 = asin(sth1); ks1 = th0 - th1;
-    if (kk <= nt) s += ext(kk) * (sin(ks1) * G.radius(kk) / sth1);
+    if (kk <= nt) {
+      const double p = sin(ks1) * G.radius(kk) / sth1;
+      AA_FOR_CH(c) s[c] += ext(kk, c) * p;
+    }
     sth0 = sth1; th0 = th1;
   }
-  return s;
 }
 
 struct ItemIdx {
@@ -416,37 +432,42 @@ __global__ void k_optics(RTWork W, int nitems) {
   }
 }
 
-// One thread per (item, level n in 1..L): Chapman slant optical depth to the
-// level and the first-order sun paths (outgoing-sphericity quadrature).
-__global__ void k_geom(RTWork W, int nitems) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int it = t / W.km, n = t - it * W.km + 1;
-  if (it >= nitems) return;
-  if (W.status[it] != 0) return;
-  const int L = W.L_it[it];
-  if (n > L) return;
-  ItemIdx I = item_of(W, it);
+// Chapman slant optical depth to level n and the first-order sun paths
+// (outgoing-sphericity quadrature) of the items ita + c, c in the mask cm: the
+// channels of one column (same geometry and L), so the slant-path lengths are
+// computed once and summed with each channel's extinction.
+template <int CH>
+__device__ void geom_level(const RTWork& W, int ita, unsigned cm, int n) {
+  const int itf = ita + (__ffs(cm) - 1);
+  const ItemIdx I = item_of(W, itf);
   const int km = W.km;
+  const int L = W.L_it[itf];
   ColumnGeo G{W.he + (size_t)I.obs * (km + 1), L};
-  const double* deltau = W.deltau + (size_t)it * km;
-  const double* fac1 = W.fac1 + (size_t)it * km;
-  const double* trr = W.tr + (size_t)it * km * TR_NFIELDS;
-  auto dts = [&](int k) { return trr[(size_t)k * TR_NFIELDS + TR_DTAU]; };
+  const double *deltau[CH], *fac1[CH], *trr[CH];
+  double* lev[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) {
+    const size_t it = (size_t)(ita + c);
+    deltau[c] = W.deltau + it * km;
+    fac1[c] = W.fac1 + it * km;
+    trr[c] = W.tr + it * km * TR_NFIELDS;
+    lev[c] = W.lev + (it * (km + 1) + n) * 4;
+  }
   const double sza = W.sza[I.obs], vza = W.vza[I.obs];
   double raa = W.raa[I.obs];
   if (raa < 0.0) raa = -raa;
   if (raa > 360.0) raa = 360.0 - raa - 360.0;
   const double dtr = DEG_TO_RAD;
-  double* lev = W.lev + ((size_t)it * (km + 1) + n) * 4;
 
   // ---- Chapman factors (REG/vlidort_geometry.f90:119-318, non-refractive)
   {
     const double th_toa = sza * DEG_TO_RAD, mu_toa = cos(th_toa), sn_toa = sqrt(1.0 - mu_toa * mu_toa);
-    double tau = 0.0;
+    double tau[CH];
+    AA_FOR_CH(c) tau[c] = 0.0;
     if (W.plane_parallel) {
-      for (int k = 1; k <= n; ++k) tau += deltau[k - 1] * (1.0 / mu_toa) * fac1[k - 1];
+      for (int k = 1; k <= n; ++k) { AA_FOR_CH(c) tau[c] += deltau[c][k - 1] * (1.0 / mu_toa) * fac1[c][k - 1]; }
     } else if (sn_toa == 0.0) {
-      for (int k = 1; k <= n; ++k) tau += deltau[k - 1] * 1.0 * fac1[k - 1];
+      for (int k = 1; k <= n; ++k) { AA_FOR_CH(c) tau[c] += deltau[c][k - 1] * 1.0 * fac1[c][k - 1]; }
     } else {
       const double r0 = G.h(0) + EARTH_RADIUS;
       double sinth1 = sn_toa * (G.h(n) + EARTH_RADIUS) / r0;
@@ -461,34 +482,48 @@ __global__ void k_geom(RTWork W, int nitems) {
         double sinphi = sin(phi);
         double dist = re_upper * sinphi / sinth2;
         double chap = dist / delz;
-        tau += deltau[k - 1] * chap * fac1[k - 1];
+        AA_FOR_CH(c) tau[c] += deltau[c][k - 1] * chap * fac1[c][k - 1];
         re_upper = re_lower; sinth1 = sinth2; sth1 = sth2;
       }
     }
-    lev[0] = tau;
+    AA_FOR_CH(c) lev[c][0] = tau[c];
   }
 
   // ---- first-order geometry and sun-path optical depths
   // layer extinction per km; levels below the surface (a ray tangent under
   // the ground) contribute nothing
-  auto ext = [&](int k) { return k <= L ? dts(k - 1) / (G.h(k - 1) - G.h(k)) : 0.0; };
-  const bool latched = global_item(W, it) >= *W.latch;
+  auto ext = [&](int k, int c) {
+    return k <= L ? trr[c][(size_t)(k - 1) * TR_NFIELDS + TR_DTAU] / (G.h(k - 1) - G.h(k)) : 0.0;
+  };
   const bool planpar = W.plane_parallel != 0;
-  const bool enhanced = !planpar && !latched;
-  double suntau = 0.0, multenh = 0.0, lostrans = 0.0;
+  // enhanced (outgoing-sphericity) mode unless plane-parallel or past the nadir latch
+  unsigned em = 0u;
+  if (!planpar) {
+    const int latch = *W.latch;
+    AA_FOR_CH(c) if (global_item(W, ita + c) < latch) em |= 1u << c;
+  }
+  double suntau[CH], multenh[CH], lostrans[CH];
+#pragma unroll
+  for (int c = 0; c < CH; ++c) { suntau[c] = 0.0; multenh[c] = 0.0; lostrans[c] = 0.0; }
   if (planpar) {
     // ObsGeom_PlanPar_WP (FOC/FO_WPgeometry_Routines.f90:126-311)
     const double utheta = 1.0 / cos(sza * dtr);
-    for (int k = 1; k <= n; ++k) suntau += ext(k) * ((G.h(k - 1) - G.h(k)) * utheta);
-  } else if (!enhanced) {
+    for (int k = 1; k <= n; ++k) {
+      const double q = (G.h(k - 1) - G.h(k)) * utheta;
+      AA_FOR_CH(c) suntau[c] += ext(k, c) * q;
+    }
+  }
+  if (!planpar && (cm & ~em)) {
     // Obsgeom_RegularPS_WP (FOC/FO_WPgeometry_Routines.f90:776-991)
     const double theta_r = sza * dtr;
     double stheta = sin(theta_r);
     const bool overhead = (sza == 0.0);
-    suntau = sunpaths_d_sum(overhead, G.radius(n), G, theta_r, stheta, n, ext);
-  } else {
+    sunpaths_d_sum<CH>(overhead, G.radius(n), G, theta_r, stheta, n, ext, cm & ~em, suntau);
+  }
+  if (em) {
     // LosOut_EnhancedPS_Initial/Quadrature + SolarIn_EnhancedPS_ObsGeom_SunPaths
     // (FOC/FO_WPgeometry_Routines.f90:1511-1971, 2409-2859)
+    const unsigned cm = em;
     const bool donadir = (vza == 0.0);
     double salpha_boa, calpha1;
     if (vza == 90.0) { salpha_boa = 1.0; calpha1 = 0.0; }
@@ -537,28 +572,67 @@ __global__ void k_geom(RTWork W, int nitems) {
     double theta_n, stheta, ctheta; bool directsun;
     if (n == L) { theta_n = theta_boa_r; stheta = stheta_boa; ctheta = ctheta_boa; directsun = true; }
     else find_sun(donadir, overhead, G.radius(n), sdir, alphaL - alpha_at(n), theta_boa_r, theta_n, stheta, ctheta, directsun);
-    if (directsun) suntau = sunpaths_d_sum(zerosunboa, radstart, G, theta_n, stheta, n, ext);
-    else suntau = sunpaths_t_sum(radstart, G, theta_n, stheta, n, L, ext);
-    const double kn = ext(n);
-    double sum = 0.0;
+    if (directsun) sunpaths_d_sum<CH>(zerosunboa, radstart, G, theta_n, stheta, n, ext, cm, suntau);
+    else sunpaths_t_sum<CH>(radstart, G, theta_n, stheta, n, L, ext, cm, suntau);
+    double kn[CH], sum[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) { kn[c] = 0.0; sum[c] = 0.0; }
+    AA_FOR_CH(c) kn[c] = ext(n, c);
     for (int j = 0; j < NFINE; ++j) {
       double thf, sthf, cthf; bool dsf;
       find_sun(donadir, overhead, radiif[j], sdir, alphaL - alphaf[j], theta_boa_r, thf, sthf, cthf, dsf);
-      double sf = dsf ? sunpaths_d_sum(zerosunboa, radiif[j], G, thf, sthf, n, ext)
-                      : sunpaths_t_sum(radiif[j], G, thf, sthf, n, L, ext);
-      double attenf = (sf < FO_CUTOFF) ? exp(-sf) : 0.0;
-      double dj = losw - tfine[j];
-      double tran = exp(-kn * dj);
-      sum = sum + attenf * tran * afine[j];
+      double sf[CH];
+      if (dsf) sunpaths_d_sum<CH>(zerosunboa, radiif[j], G, thf, sthf, n, ext, cm, sf);
+      else sunpaths_t_sum<CH>(radiif[j], G, thf, sthf, n, L, ext, cm, sf);
+      const double dj = losw - tfine[j];
+      AA_FOR_CH(c) {
+        double attenf = (sf[c] < FO_CUTOFF) ? exp(-sf[c]) : 0.0;
+        double tran = exp(-kn[c] * dj);
+        sum[c] = sum[c] + attenf * tran * afine[j];
+      }
     }
-    multenh = sum * kn;
-    const double lostau = kn * losw;
-    if (lostau < FO_CUTOFF) lostrans = exp(-lostau);
+    AA_FOR_CH(c) {
+      multenh[c] = sum[c] * kn[c];
+      const double lostau = kn[c] * losw;
+      if (lostau < FO_CUTOFF) lostrans[c] = exp(-lostau);
+    }
   }
-  lev[1] = suntau;
-  lev[2] = (suntau < FO_CUTOFF) ? exp(-suntau) : 0.0;
-  lev[3] = multenh;
-  W.lostrans[(size_t)it * (km + 1) + n] = lostrans;
+  AA_FOR_CH(c) {
+    lev[c][1] = suntau[c];
+    lev[c][2] = (suntau[c] < FO_CUTOFF) ? exp(-suntau[c]) : 0.0;
+    lev[c][3] = multenh[c];
+    W.lostrans[(size_t)(ita + c) * (km + 1) + n] = lostrans[c];
+  }
+}
+
+// One thread per (column, level n in 1..L) over the column's items in the
+// chunk, GEO_CH channels at a time; with an item list (stream-count buckets)
+// one thread per (item, level).  The path loops of level n run over the n
+// layers above it, so a wavefront holds 8 columns x 8 adjacent levels: its
+// lanes' trip counts differ by at most 7.
+constexpr int GEO_LV = 8, GEO_CH = 4;
+__global__ __launch_bounds__(64, 2) void k_geom(RTWork W, int nitems, int obs_first, int nruns) {
+  const int nlg = (W.km + GEO_LV - 1) / GEO_LV;
+  const int grp = blockIdx.x / nlg, lg = blockIdx.x - grp * nlg;
+  const int r = grp * (64 / GEO_LV) + threadIdx.x / GEO_LV, n = lg * GEO_LV + threadIdx.x % GEO_LV + 1;
+  if (r >= nruns) return;
+  int ita, itb;
+  if (W.item_list) {
+    ita = r; itb = r + 1;
+  } else {
+    const long g0 = (long)(obs_first + r) * W.nch - W.item0;
+    ita = (int)(g0 > 0 ? g0 : 0L);
+    itb = (int)(g0 + W.nch < nitems ? g0 + W.nch : (long)nitems);
+  }
+  for (int a = ita; a < itb; a += GEO_CH) {
+    unsigned cm = 0u;
+#pragma unroll
+    for (int c = 0; c < GEO_CH; ++c) {
+      const int it = a + c;
+      if (it < itb && W.status[it] == 0 && n <= W.L_it[it]) cm |= 1u << c;
+    }
+    if (cm) geom_level<GEO_CH>(W, a, cm, n);
+  }
 }
 
 // One thread per item: beam cutoff and average secants (QSPREP), layer
@@ -725,7 +799,16 @@ int rt_setup(const RTWork& W, int nitems, int* d_active, int* d_nactive, hipStre
   }
   prof_end(PROF_OPTICS, st);
   prof_begin(PROF_GEOM, st);
-  hipLaunchKernelGGL(k_geom, dim3(gl), dim3(B), 0, st, W, nitems);
+  {
+    // runs: the chunk's columns (items are (obs, channel) in order), or single items
+    int obs_first = 0, nruns = nitems;
+    if (!W.item_list && nitems > 0) {
+      obs_first = W.item0 / W.nch;
+      nruns = (W.item0 + nitems - 1) / W.nch - obs_first + 1;
+    }
+    const size_t gg = (size_t)((nruns + 64 / GEO_LV - 1) / (64 / GEO_LV)) * ((W.km + GEO_LV - 1) / GEO_LV);
+    if (gg > 0) hipLaunchKernelGGL(k_geom, dim3((unsigned)gg), dim3(64), 0, st, W, nitems, obs_first, nruns);
+  }
   prof_end(PROF_GEOM, st);
   prof_begin(PROF_ITEM, st);
   if (W.S == 1) hipLaunchKernelGGL((k_item<1>), dim3(gi), dim3(B), 0, st, W, nitems, d_active, d_nactive);

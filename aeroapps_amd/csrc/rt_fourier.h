@@ -275,17 +275,6 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
     const double help = kv * dtau;
     tdel[k] = (help > MAX_TAU_PATH) ? 0.0 : exp(-help);
   }
-  // X for the lower half-space streams (i + N)
-  auto XPF = [&](int i, int o, int k) -> double {
-    if (i < N) return xp[i * S + o][k];
-    const int ii = i - N;
-    return (o < 2) ? xn[ii * S + o][k] : -xn[ii * S + o][k];
-  };
-  auto XNF = [&](int i, int o, int k) -> double {
-    if (i < N) return xn[i * S + o][k];
-    const int ii = i - N;
-    return (o < 2) ? xp[ii * S + o][k] : -xp[ii * S + o][k];
-  };
 
   // ---- PI matrices at the user stream and the solar cosine
   const int obs = global_item(W, it) / W.nch;
@@ -469,17 +458,17 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
   // ---- write the layer record
   double* rec = W.lbuf + il * (size_t)W.lstride;
 #pragma unroll
-  for (int i = 0; i < 2 * N; ++i)
+  for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int o = 0; o < S; ++o)
 #pragma unroll
-      for (int k = 0; k < n; ++k) rec[O.xp(i, o, k)] = XPF(i, o, k);
+      for (int k = 0; k < n; ++k) rec[O.xp(i, o, k)] = xp[i * S + o][k];
 #pragma unroll
-  for (int i = 0; i < 2 * N; ++i)
+  for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int o = 0; o < S; ++o)
 #pragma unroll
-      for (int k = 0; k < n; ++k) rec[O.xn(i, o, k)] = XNF(i, o, k);
+      for (int k = 0; k < n; ++k) rec[O.xn(i, o, k)] = xn[i * S + o][k];
 #pragma unroll
   for (int k = 0; k < n; ++k) rec[O.tdel(k)] = tdel[k];
 #pragma unroll
@@ -631,6 +620,16 @@ __device__ __forceinline__ unsigned grp_max_u(unsigned x) {
 __host__ __device__ constexpr int bvp_group(int n) { return 2 * n <= 16 ? 16 : (2 * n <= 32 ? 32 : 64); }
 __host__ __device__ constexpr int bvp_slots(int n) { return (3 * n + bvp_group(n) - 1) / bvp_group(n); }
 
+// U rows of a layer panel are kept in compact form: row i from column
+// (i + 1) & ~1 (the columns right of its pivot, from an even column so the
+// pairs stay 16-byte aligned) through the right-hand side and 1/pivot.
+__host__ __device__ constexpr int urow_start(int i) { return (i + 1) & ~1; }
+__host__ __device__ constexpr int urow_off(int uw, int i) {
+  int o = 0;
+  for (int r = 0; r < i; ++r) o += uw - urow_start(r);
+  return o;
+}
+
 // Pivot of each group from per-slot candidate flags: lq[q]/sq[q] are the lane
 // and slot of group q's candidate (lowest lane, slot 0 first; -1 if none);
 // `multi` reports a group with more than one candidate.  Returns the lane of
@@ -662,11 +661,12 @@ __device__ __forceinline__ int grp_pivot(const bool (&eq)[SL], int g, bool& mult
 }
 
 template <int N, int S>
-__global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restrict__ active,
+__global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int M, const int* __restrict__ active,
                                             const int* __restrict__ nactive, int* __restrict__ next,
                                             int* __restrict__ nnext) {
   constexpr int n = N * S, n2 = 2 * n, NC = 4 * n, RH = NC, UW = NC + 2;
   constexpr int G = bvp_group(n), IPW = 64 / G, SL = bvp_slots(n);
+  constexpr int UC = urow_off(UW, n2);   // doubles per layer panel of compact U rows
   static_assert(2 * n <= 64 && SL <= 2, "panel rows exceed the wavefront");
   constexpr LOff O(N, S);
   constexpr BOff BO(N, n);
@@ -837,9 +837,13 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
       sing = sing || (pv == 0.0);
     }
     __syncthreads();
-    double2* dst = reinterpret_cast<double2*>(ub + (size_t)j0 * UW);
-    const double2* srcu = reinterpret_cast<const double2*>(&us[g][0][0]);
-    for (int e = t; e < n2 * UW / 2; e += G) dst[e] = srcu[e];
+#pragma unroll
+    for (int r = 0; r < n2; ++r) {
+      const int s0 = urow_start(r);
+      double2* dst = reinterpret_cast<double2*>(ub + (size_t)k * UC + urow_off(UW, r));
+      const double2* srcu = reinterpret_cast<const double2*>(&us[g][r][s0]);
+      for (int e = t; e < (UW - s0) / 2; e += G) dst[e] = srcu[e];
+    }
     if (!LAST) {
 #pragma unroll
       for (int sl = 0; sl < SL; ++sl)
@@ -853,28 +857,35 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
   auto load_row = [&](double (&r)[NC + 1], int k, bool lastp, int rank) __attribute__((always_inline)) {
     const double* lu = lb + (size_t)k * LS;
     const int i = rank / S, o1 = rank - (rank / S) * S;
+    double sp, sm;
     if (!lastp) {
       const double* ll = lu + LS;
+      const double* xpr = lu + O.xrow(true, i, o1, sp);
+      const double* xnr = lu + O.xrow(false, i, o1, sm);
+      const double* xpl = ll + O.xrow(true, i, o1, sp);
+      const double* xnl = ll + O.xrow(false, i, o1, sm);
 #pragma unroll
-      for (int q = 0; q < n; ++q) r[q] = lu[O.tdel(q)] * lu[O.xp(i, o1, q)];
+      for (int q = 0; q < n; ++q) r[q] = lu[O.tdel(q)] * (sp * xpr[q]);
 #pragma unroll
-      for (int q = 0; q < n; ++q) r[n + q] = lu[O.xn(i, o1, q)];
+      for (int q = 0; q < n; ++q) r[n + q] = sm * xnr[q];
 #pragma unroll
-      for (int q = 0; q < n; ++q) r[n2 + q] = -ll[O.xp(i, o1, q)];
+      for (int q = 0; q < n; ++q) r[n2 + q] = -(sp * xpl[q]);
 #pragma unroll
-      for (int q = 0; q < n; ++q) r[3 * n + q] = -ll[O.tdel(q)] * ll[O.xn(i, o1, q)];
+      for (int q = 0; q < n; ++q) r[3 * n + q] = -ll[O.tdel(q)] * (sm * xnl[q]);
       r[RH] = ll[O.wu(i, o1)] - lu[O.wl(i, o1)];
     } else {
       const bool sr = surf && o1 == 0;
+      const double* xpr = lu + O.xrow(true, i + N, o1, sp);
+      const double* xnr = lu + O.xrow(false, i + N, o1, sm);
 #pragma unroll
       for (int q = 0; q < n; ++q) {
-        double xpn = lu[O.xp(i + N, o1, q)];
+        double xpn = sp * xpr[q];
         if (sr) xpn = xpn - bb[BO.r2p(i, q)];
         r[q] = lu[O.tdel(q)] * xpn;
       }
 #pragma unroll
       for (int q = 0; q < n; ++q) {
-        double xmn = lu[O.xn(i + N, o1, q)];
+        double xmn = sm * xnr[q];
         if (sr) xmn = xmn - bb[BO.r2m(i, q)];
         r[n + q] = xmn;
       }
@@ -923,13 +934,17 @@ __global__ __launch_bounds__(64) void k_bvp(RTWork W, int M, const int* __restri
   for (int o = 0; o < S; ++o) cum[o] = 0.0;
   const double* trr = W.tr + (size_t)it * km * TR_NFIELDS;
   double u[UW], cf[S], cs[S], tdu = 0.0;
+  // lane t's compact U row: offset in the panel and first stored column
+  const int ut0 = urow_start(t);
+  int uoff = 0;
+  for (int r = 0; r < t && r < n2; ++r) uoff += UW - urow_start(r);
 #define AA_BS_LOAD(K)                                                                        \
   {                                                                                          \
     const int k_ = (K);                                                                      \
     if (valid && k_ >= 0 && k_ < L && t < n2) {                                              \
-      const double2* src_ = reinterpret_cast<const double2*>(ub + ((size_t)k_ * n2 + t) * UW); \
+      const double2* src_ = reinterpret_cast<const double2*>(ub + (size_t)k_ * UC + uoff) - ut0 / 2; \
       _Pragma("unroll") for (int e = 0; e < UW / 2; ++e) {                                   \
-        const double2 v_ = src_[e];                                                          \
+        const double2 v_ = (2 * e >= ut0) ? src_[e] : make_double2(0.0, 0.0);               \
         u[2 * e] = v_.x;                                                                     \
         u[2 * e + 1] = v_.y;                                                                 \
       }                                                                                      \

@@ -111,19 +111,29 @@ __device__ __forceinline__ int global_item(const RTWork& W, int it) {
   return W.item_list ? W.item_list[W.item0 + it] : W.item0 + it;
 }
 
-// Offsets inside one lbuf record (per item and layer) for n = N*S.
+// Offsets inside one lbuf record (per item and layer) for n = N*S.  The
+// eigenvectors X+/X- are stored for the upper-hemisphere streams i < N only:
+// those of stream i + N are X-/X+ of stream i, negated for Stokes components
+// o >= 2 (REG/vlidort_solutions.f90:680-700), which xrow() applies.
 struct LOff {
   int n, S, N;
   __host__ __device__ constexpr LOff(int N_, int S_) : n(N_ * S_), S(S_), N(N_) {}
-  __host__ __device__ constexpr int xp(int i, int o, int k) const { return (i * S + o) * n + k; }
-  __host__ __device__ constexpr int xn(int i, int o, int k) const { return 2 * n * n + (i * S + o) * n + k; }
-  __host__ __device__ constexpr int tdel(int k) const { return 4 * n * n + k; }
-  __host__ __device__ constexpr int wu(int i, int o) const { return 4 * n * n + n + i * S + o; }
-  __host__ __device__ constexpr int wl(int i, int o) const { return 4 * n * n + 3 * n + i * S + o; }
+  __host__ __device__ constexpr int xp(int i, int o, int k) const { return (i * S + o) * n + k; }   // i < N
+  __host__ __device__ constexpr int xn(int i, int o, int k) const { return n * n + (i * S + o) * n + k; }
+  // X+ (plus = true) or X- of stream i in [0, 2N), Stokes o: offset of its n
+  // values and their sign
+  __host__ __device__ constexpr int xrow(bool plus, int i, int o, double& sgn) const {
+    const bool lo = i >= N;
+    sgn = (lo && o >= 2) ? -1.0 : 1.0;
+    return (plus != lo) ? xp(lo ? i - N : i, o, 0) : xn(lo ? i - N : i, o, 0);
+  }
+  __host__ __device__ constexpr int tdel(int k) const { return 2 * n * n + k; }
+  __host__ __device__ constexpr int wu(int i, int o) const { return 2 * n * n + n + i * S + o; }
+  __host__ __device__ constexpr int wl(int i, int o) const { return 2 * n * n + 3 * n + i * S + o; }
   // source coefficients: src[o] = sum_q x(q) * coef(o, q) + csrc(o), q < 2n (lcon then mcon)
-  __host__ __device__ constexpr int coef(int o, int q) const { return 4 * n * n + 5 * n + o * 2 * n + q; }
-  __host__ __device__ constexpr int csrc(int o) const { return 4 * n * n + 5 * n + 2 * n * S + o; }
-  __host__ __device__ constexpr int size() const { return (4 * n * n + 5 * n + 2 * n * S + S + 1) & ~1; }
+  __host__ __device__ constexpr int coef(int o, int q) const { return 2 * n * n + 5 * n + o * 2 * n + q; }
+  __host__ __device__ constexpr int csrc(int o) const { return 2 * n * n + 5 * n + 2 * n * S + o; }
+  __host__ __device__ constexpr int size() const { return (2 * n * n + 5 * n + 2 * n * S + S + 1) & ~1; }
 };
 
 // Offsets inside one bbuf record: surface reflection terms of the bottom

@@ -358,6 +358,11 @@ __global__ void k_optics(RTWork W, int nitems) {
   W.deltau[(size_t)it * km + n] = tau_ext;
   W.fac1[(size_t)it * km + n] = fac1;
   W.tr[((size_t)it * km + n) * TR_NFIELDS + TR_DTAU] = dtau;
+  {
+    // layer extinction per km for the first-order sun paths (n < L here)
+    const ColumnGeo Gc{W.he + (size_t)I.obs * (km + 1), W.L_it[it]};
+    W.ext[(size_t)it * km + n] = dtau / (Gc.h(n) - Gc.h(n + 1));
+  }
 
   // first-order phase function at the scattering angle, all nmom moments
   // (FOC/FO_VectorSS_Spherfuncs.f90:93-300, FOC/FO_VectorSS_RTCalcs_I.f90)
@@ -443,14 +448,14 @@ __device__ void geom_level(const RTWork& W, int ita, unsigned cm, int n) {
   const int km = W.km;
   const int L = W.L_it[itf];
   ColumnGeo G{W.he + (size_t)I.obs * (km + 1), L};
-  const double *deltau[CH], *fac1[CH], *trr[CH];
+  const double *deltau[CH], *fac1[CH], *extc[CH];
   double* lev[CH];
 #pragma unroll
   for (int c = 0; c < CH; ++c) {
     const size_t it = (size_t)(ita + c);
     deltau[c] = W.deltau + it * km;
     fac1[c] = W.fac1 + it * km;
-    trr[c] = W.tr + it * km * TR_NFIELDS;
+    extc[c] = W.ext + it * km;
     lev[c] = W.lev + (it * (km + 1) + n) * 4;
   }
   const double sza = W.sza[I.obs], vza = W.vza[I.obs];
@@ -490,11 +495,10 @@ __device__ void geom_level(const RTWork& W, int ita, unsigned cm, int n) {
   }
 
   // ---- first-order geometry and sun-path optical depths
-  // layer extinction per km; levels below the surface (a ray tangent under
-  // the ground) contribute nothing
-  auto ext = [&](int k, int c) {
-    return k <= L ? trr[c][(size_t)(k - 1) * TR_NFIELDS + TR_DTAU] / (G.h(k - 1) - G.h(k)) : 0.0;
-  };
+  // layer extinction per km (k_optics: scaled optical thickness over the
+  // layer depth); levels below the surface (a ray tangent under the ground)
+  // contribute nothing
+  auto ext = [&](int k, int c) { return k <= L ? extc[c][k - 1] : 0.0; };
   const bool planpar = W.plane_parallel != 0;
   // enhanced (outgoing-sphericity) mode unless plane-parallel or past the nadir latch
   unsigned em = 0u;

@@ -82,6 +82,7 @@ struct RTWork {
   // ---- per (item, layer)
   double* deltau = nullptr;        // [it][km] unscaled
   double* fac1 = nullptr;          // [it][km] 1 - f*omega
+  double* ext = nullptr;           // [it][km] scaled layer optical thickness per km of depth
   double* exact = nullptr;         // [it][km][3] FO phase function x TMS factor
   double* og = nullptr;            // [it][km][LM][S][S] omega * scaled Greek
   unsigned* lsc = nullptr;         // [it][km] bit m: layer scatters at Fourier m

@@ -256,6 +256,7 @@ size_t carve_work(Carver& C, RTWork& W, int cap) {
   W.stokes = C.take<double>((size_t)cap * 3);
   W.deltau = C.take<double>((size_t)cap * km);
   W.fac1 = C.take<double>((size_t)cap * km);
+  W.ext = C.take<double>((size_t)cap * km);
   W.exact = C.take<double>((size_t)cap * km * 3);
   W.og = C.take<double>((size_t)cap * km * LM * W.S * W.S);
   W.lsc = C.take<unsigned>((size_t)cap * km);

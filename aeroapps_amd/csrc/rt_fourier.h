@@ -630,34 +630,24 @@ __host__ __device__ constexpr int urow_off(int uw, int i) {
   return o;
 }
 
-// Pivot of each group from per-slot candidate flags: lq[q]/sq[q] are the lane
-// and slot of group q's candidate (lowest lane, slot 0 first; -1 if none);
-// `multi` reports a group with more than one candidate.  Returns the lane of
-// the lane's own group and sets `slot`.
+// Pivot of each group from per-slot candidate flags: the candidate of the
+// group is the one in the lowest slot, and in it the lowest lane (a group
+// reduction over the code slot * 64 + lane); -1 if the group has none.
+// `multi` (wave-uniform) reports a group with more than one candidate.
+// Returns the lane of the lane's own group and sets `slot`.
 template <int G, int SL>
-__device__ __forceinline__ int grp_pivot(const bool (&eq)[SL], int g, bool& multi, int& slot) {
-  constexpr int IPW = 64 / G;
-  constexpr unsigned long long GM = (G == 64) ? ~0ull : ((1ull << (G & 63)) - 1ull);
-  unsigned long long B[SL];
+__device__ __forceinline__ int grp_pivot(const bool (&eq)[SL], int g, int t, bool& multi, int& slot) {
+  constexpr int NONE = 0x7fff;
+  int code = NONE;
 #pragma unroll
-  for (int s = 0; s < SL; ++s) B[s] = __ballot(eq[s]);
-  int sel = -1;
-  slot = 0;
-  multi = false;
+  for (int s = SL - 1; s >= 0; --s) code = eq[s] ? s * 64 + t : code;
+  const int cmin = grp_min_i<G>(code);
+  bool other = false;
 #pragma unroll
-  for (int q = 0; q < IPW; ++q) {
-    int lq = -1, sq = 0, cnt = 0;
-#pragma unroll
-    for (int s = SL - 1; s >= 0; --s) {
-      const unsigned long long bq = (B[s] >> (q * G)) & GM;
-      cnt += __builtin_popcountll(bq);
-      if (bq) { lq = (int)__builtin_ctzll(bq) + q * G; sq = s; }
-    }
-    multi = multi || cnt > 1;
-    sel = (g == q) ? lq : sel;
-    slot = (g == q) ? sq : slot;
-  }
-  return sel;
+  for (int s = 0; s < SL; ++s) other = other || (eq[s] && s * 64 + t != cmin);
+  multi = __ballot(other) != 0ull;
+  slot = (cmin == NONE) ? 0 : (cmin >> 6);
+  return (cmin == NONE) ? -1 : g * G + (cmin & 63);
 }
 
 template <int N, int S>
@@ -766,7 +756,7 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
       for (int sl = 0; sl < SL; ++sl) eq[sl] = act[sl] && hs[sl] == mh;
       bool multi;
       int psl;
-      int sel = grp_pivot<G, SL>(eq, g, multi, psl);
+      int sel = grp_pivot<G, SL>(eq, g, t, multi, psl);
       if (multi) {   // wave-uniform: some group has equal high words
         unsigned ls[SL], k2 = 0u;
 #pragma unroll
@@ -777,7 +767,7 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
         const unsigned ml = grp_max_u<G>(k2);
 #pragma unroll
         for (int sl = 0; sl < SL; ++sl) eq[sl] = eq[sl] && ls[sl] == ml;
-        sel = grp_pivot<G, SL>(eq, g, multi, psl);
+        sel = grp_pivot<G, SL>(eq, g, t, multi, psl);
         if (multi) {
           int k3 = BIG;
 #pragma unroll
@@ -785,7 +775,7 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
           const int kp = grp_min_i<G>(k3);
 #pragma unroll
           for (int sl = 0; sl < SL; ++sl) eq[sl] = eq[sl] && posl[sl] == kp;
-          sel = grp_pivot<G, SL>(eq, g, multi, psl);
+          sel = grp_pivot<G, SL>(eq, g, t, multi, psl);
         }
       }
       bool ispiv[SL];

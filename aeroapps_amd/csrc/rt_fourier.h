@@ -585,12 +585,14 @@ __device__ __forceinline__ double grp_sum(double x) {
   if (G == 64) x = readlane_d(x, 0) + readlane_d(x, 32);
   return x;
 }
+// (the DPP moves take INT_MAX, the identity, as the old value so they fold
+// into v_min_i32's DPP form)
 template <int G>
 __device__ __forceinline__ int grp_min_i(int x) {
-  x = min(x, __builtin_amdgcn_update_dpp(x, x, 0xB1, 0xF, 0xF, false));
-  x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x4E, 0xF, 0xF, false));
-  x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x141, 0xF, 0xF, false));
-  x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x140, 0xF, 0xF, false));
+  x = min(x, __builtin_amdgcn_update_dpp(0x7fffffff, x, 0xB1, 0xF, 0xF, false));
+  x = min(x, __builtin_amdgcn_update_dpp(0x7fffffff, x, 0x4E, 0xF, 0xF, false));
+  x = min(x, __builtin_amdgcn_update_dpp(0x7fffffff, x, 0x141, 0xF, 0xF, false));
+  x = min(x, __builtin_amdgcn_update_dpp(0x7fffffff, x, 0x140, 0xF, 0xF, false));
   if (G >= 32) {
     const auto r = __builtin_amdgcn_permlane16_swap((unsigned)x, (unsigned)x, false, false);
     x = min((int)r[0], (int)r[1]);

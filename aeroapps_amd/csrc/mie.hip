@@ -182,11 +182,29 @@ __global__ void __launch_bounds__(64) get_aop_kernel(MieDev T, int km, int nobs,
             for (int kk = 0; kk < MIE_KC_MAX; ++kk) {
               if (kk >= kc) break;
               const float st = stl[kk];
-              float va = acc_a[kk], vb = acc_b[kk];
-              if (normalize && st > 0.0f) { va = __fdiv_rn(va, st); vb = __fdiv_rn(vb, st); }
-              const long e = obase + k0 + kk;
-              pmom_o[((long)pa * nmom + m) * ostride + e] = va;
-              if (pb >= 0) pmom_o[((long)pb * nmom + m) * ostride + e] = vb;
+              if (normalize && st > 0.0f) { acc_a[kk] = __fdiv_rn(acc_a[kk], st); acc_b[kk] = __fdiv_rn(acc_b[kk], st); }
+            }
+            float* oa = pmom_o + ((long)pa * nmom + m) * ostride + obase + k0;
+            float* ob = pb >= 0 ? pmom_o + ((long)pb * nmom + m) * ostride + obase + k0 : nullptr;
+            // a moment's layers are contiguous: 16-byte stores when the chunk
+            // holds whole aligned groups of 4 layers
+            if (kc % 4 == 0 && (((size_t)oa | (size_t)ob) & 15) == 0) {
+#pragma unroll
+              for (int q = 0; q < MIE_KC_MAX / 4; ++q) {
+                if (4 * q >= kc) break;
+                *reinterpret_cast<float4*>(oa + 4 * q) =
+                    make_float4(acc_a[4 * q], acc_a[4 * q + 1], acc_a[4 * q + 2], acc_a[4 * q + 3]);
+                if (ob)
+                  *reinterpret_cast<float4*>(ob + 4 * q) =
+                      make_float4(acc_b[4 * q], acc_b[4 * q + 1], acc_b[4 * q + 2], acc_b[4 * q + 3]);
+              }
+            } else {
+#pragma unroll
+              for (int kk = 0; kk < MIE_KC_MAX; ++kk) {
+                if (kk >= kc) break;
+                oa[kk] = acc_a[kk];
+                if (ob) ob[kk] = acc_b[kk];
+              }
             }
           }
         }

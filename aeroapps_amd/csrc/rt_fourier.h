@@ -1101,12 +1101,16 @@ __global__ __launch_bounds__(64) void k_sabdab(RTWork W, int M, const int* __res
       const float av = (row < ROWS && k < KK) ? Y[YO.at(M, row, k)] : 0.0f;
       c = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bk[ks], c, 0, 0, 0);
     }
-    if (valid) {
+    // the lane's 4 accumulator rows are consecutive in its pair's record: one
+    // b128 store when records hold whole 4-row groups (and so stay 16-byte
+    // aligned), else row by row
+    const int r0 = rt * 16 + 4 * kq;
+    if constexpr (ROWS % 4 == 0) {
+      if (valid && r0 < ROWS) *reinterpret_cast<f4v*>(out + r0) = c;
+    } else if (valid) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = rt * 16 + 4 * kq + v;
-        if (r < ROWS) out[r] = c[v];
-      }
+      for (int v = 0; v < 4; ++v)
+        if (r0 + v < ROWS) out[r0 + v] = c[v];
     }
   }
 }

@@ -59,9 +59,9 @@ constexpr int MAXTR = 32;
 // when the reference would have returned early with rc 666/777, in which case
 // the partial, un-normalised sums are what the reference leaves behind).
 //
-// One wavefront per (column, channel), KC layers at a time:
-//  1. lane = (layer, tracer): RH snap and the tracer's tau_i, ssa_i, g_i
-//     (independent per tracer, so all of them at once), into LDS;
+// One wavefront per (column, channel), KC layers at a time (KC * nq <= 128):
+//  1. lane = (layer, tracer), in rounds of 64: RH snap and the tracer's tau_i,
+//     ssa_i, g_i (independent per tracer, so all of them at once), into LDS;
 //  2. lane = layer: the tracer-order sums tau, ssa*tau, g*ssa*tau and the
 //     normalisation, exactly in the reference's order;
 //  3. lane = moment: the phase-moment sums.  The two table rows of a (tracer,
@@ -69,7 +69,7 @@ constexpr int MAXTR = 32;
 //     is a 256-byte coalesced load from L2; the KC layers give KC independent
 //     accumulations per tracer.  Per output element the order is the
 //     reference's (tracer order; (pm*ssa_i)*tau_i).
-constexpr int MIE_KC_MAX = 8;
+constexpr int MIE_KC_MAX = 8, MIE_CAP = 128;   // layers per chunk, (layer, tracer) slots
 __global__ void __launch_bounds__(64) get_aop_kernel(MieDev T, int km, int nobs, int nch, int nq,
                                                      const int* __restrict__ tracer_idx,
                                                      const int* __restrict__ chan_idx,
@@ -77,25 +77,25 @@ __global__ void __launch_bounds__(64) get_aop_kernel(MieDev T, int km, int nobs,
                                                      int nmom, int npol, int ntr_eff, int normalize,
                                                      float* __restrict__ tau_o, float* __restrict__ ssa_o,
                                                      float* __restrict__ g_o, float* __restrict__ pmom_o) {
-  __shared__ float4 info[64];          // (node, arh, ssa_i, tau_i) of lane (layer, tracer)
-  __shared__ int4 rows[64];            // (lower row, upper row, P22/P44 tied, tracer found) of lane (layer, tracer)
-  __shared__ float gin[64];            // g_i of lane (layer, tracer)
+  __shared__ float4 info[MIE_CAP];     // (node, arh, ssa_i, tau_i) of slot (layer, tracer)
+  __shared__ int4 rows[MIE_CAP];       // (lower row, upper row, P22/P44 tied, tracer found) of slot (layer, tracer)
+  __shared__ float gin[MIE_CAP];       // g_i of slot (layer, tracer)
   __shared__ float stl[MIE_KC_MAX];    // ssa*tau of the chunk's layers (normalisation)
   const int lane = threadIdx.x;
   const int i = blockIdx.x / nch, n = blockIdx.x - (blockIdx.x / nch) * nch;
   const int ich = chan_idx[n];
   const int ntr = ntr_eff > 0 ? ntr_eff : 1;
-  const int KC = min(MIE_KC_MAX, 64 / ntr);
+  const int KC = max(1, min(MIE_KC_MAX, MIE_CAP / ntr));
   const long ostride = (long)nobs * nch * km;   // stride of one moment
   const long obase = ((long)i * nch + n) * km;
   const long rowstride = (long)T.npol * T.nmom;
   const int npass = (npol == 6) ? 4 : npol;
   for (int k0 = 0; k0 < km; k0 += KC) {
     const int kc = min(KC, km - k0);
-    // ---- 1. lane = (layer, tracer)
-    {
-      const int kk = lane / ntr, iq = lane - (lane / ntr) * ntr;
-      if (kk < kc && iq < ntr_eff) {
+    // ---- 1. slot = (layer, tracer)
+    for (int e = lane; e < kc * ntr; e += 64) {
+      const int kk = e / ntr, iq = e - (e / ntr) * ntr;
+      if (iq < ntr_eff) {
         const int k = k0 + kk;
         const int tr = tracer_idx[iq];
         if (tr >= 0) {
@@ -105,11 +105,11 @@ __global__ void __launch_bounds__(64) get_aop_kernel(MieDev T, int km, int nobs,
           const float bsca_in = lin(T.bsca[tb + s.irh], T.bsca[tb + s.irhp1], s.arh);
           const float g_in = lin(T.g[tb + s.irh], T.g[tb + s.irhp1], s.arh);
           const float q = qm[((long)i * nq + iq) * km + k];
-          info[lane] = make_float4(__int_as_float(s.irh), s.arh, __fdiv_rn(bsca_in, bext_in), __fmul_rn(bext_in, q));
-          rows[lane] = make_int4((int)tb + s.irh, (int)tb + s.irhp1, (npol == 6 && T.npol_t[tr] == 4) ? 1 : 0, 1);
-          gin[lane] = g_in;
+          info[e] = make_float4(__int_as_float(s.irh), s.arh, __fdiv_rn(bsca_in, bext_in), __fmul_rn(bext_in, q));
+          rows[e] = make_int4((int)tb + s.irh, (int)tb + s.irhp1, (npol == 6 && T.npol_t[tr] == 4) ? 1 : 0, 1);
+          gin[e] = g_in;
         } else {
-          rows[lane] = make_int4(0, 0, 0, 0);
+          rows[e] = make_int4(0, 0, 0, 0);
         }
       }
     }

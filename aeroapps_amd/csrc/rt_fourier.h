@@ -1062,8 +1062,15 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
 //   C[row][pair] = sum_k Y_M[row][k] * og[pair][k],
 // rows = 2n^2 (SAB then DAB entries), k = (l, o, o3) over the omega-Greek
 // coefficients (REG/vlidort_solutions.f90:439-475).  One wavefront computes
-// 16 pairs x all rows with v_mfma_f32_16x16x4f32 tiles.
+// sab_groups(N, S) groups of 16 pairs x all rows with v_mfma_f32_16x16x4f32
+// tiles.  When the A tiles (Y_M) are few they are loaded once per wavefront,
+// and the pairs' B columns of all groups are loaded up front; large n (the
+// vector path) keeps one group and loads the A tiles as it goes.
 typedef float f4v __attribute__((ext_vector_type(4)));
+__host__ __device__ constexpr int sab_tiles(int N, int S) {
+  return ((2 * N * S * N * S + 15) / 16) * ((2 * N * S * S + 3) / 4);
+}
+__host__ __device__ constexpr int sab_groups(int N, int S) { return sab_tiles(N, S) <= 32 ? 4 : 1; }
 template <int N, int S>
 __global__ __launch_bounds__(64) void k_sabdab(RTWork W, int M, const int* __restrict__ active,
                                                const int* __restrict__ nactive) {
@@ -1071,54 +1078,82 @@ __global__ __launch_bounds__(64) void k_sabdab(RTWork W, int M, const int* __res
   constexpr YOff YO(N, S);
   constexpr int ROWS = YO.rows(), KK = YO.K;
   constexpr int RT = (ROWS + 15) / 16, KT = (KK + 3) / 4;
+  constexpr int SAB_PG = sab_groups(N, S);
+  constexpr bool PRE = sab_tiles(N, S) <= 32;   // A tiles held in registers
   const int lane = threadIdx.x;
   const int km = W.km;
-  const long p = (long)blockIdx.x * 16 + (lane & 15);   // pair of this lane's B column / C column
-  const int a = (int)(p / km), layer = (int)(p - (long)a * km);
   const int na = *nactive;
-  bool valid = a < na;
-  int it = 0;
-  if (valid) { it = active[a]; valid = layer < W.L_it[it]; }
-  if (__ballot(valid) == 0ull) return;
-  const size_t il = (size_t)it * km + layer;
-  const double* ogp = W.og + il * (size_t)(LM * S * S);
-  const float* Y = W.ytab;
   const int kq = lane >> 4;
-  float bk[KT];
+  bool valid[SAB_PG];
+  size_t il[SAB_PG];
+  bool any = false;
 #pragma unroll
-  for (int ks = 0; ks < KT; ++ks) {
-    const int k = 4 * ks + kq;
-    bk[ks] = (valid && k < KK) ? (float)ogp[k] : 0.0f;
+  for (int q = 0; q < SAB_PG; ++q) {
+    // pair of this lane's B column / C column in group q
+    const long p = ((long)blockIdx.x * SAB_PG + q) * 16 + (lane & 15);
+    const int a = (int)(p / km), layer = (int)(p - (long)a * km);
+    bool v = a < na;
+    int it = 0;
+    if (v) { it = active[a]; v = layer < W.L_it[it]; }
+    valid[q] = v;
+    il[q] = (size_t)it * km + layer;
+    any = any || v;
   }
-  float* out = W.sab32 + il * (size_t)ROWS;
+  if (__ballot(any) == 0ull) return;
+  float bk[SAB_PG][KT];
 #pragma unroll
-  for (int rt = 0; rt < RT; ++rt) {
-    f4v c = {0.0f, 0.0f, 0.0f, 0.0f};
-    const int row = rt * 16 + (lane & 15);
+  for (int q = 0; q < SAB_PG; ++q) {
+    const double* ogp = W.og + il[q] * (size_t)(LM * S * S);
 #pragma unroll
     for (int ks = 0; ks < KT; ++ks) {
       const int k = 4 * ks + kq;
-      const float av = (row < ROWS && k < KK) ? Y[YO.at(M, row, k)] : 0.0f;
-      c = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bk[ks], c, 0, 0, 0);
+      bk[q][ks] = (valid[q] && k < KK) ? (float)ogp[k] : 0.0f;
+    }
+  }
+  const float* Y = W.ytab;
+  auto atile = [&](int rt, int ks) -> float {
+    const int row = rt * 16 + (lane & 15), k = 4 * ks + kq;
+    return (row < ROWS && k < KK) ? Y[YO.at(M, row, k)] : 0.0f;
+  };
+  float at[PRE ? RT : 1][PRE ? KT : 1];
+  if constexpr (PRE) {
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int ks = 0; ks < KT; ++ks) at[rt][ks] = atile(rt, ks);
+  }
+#pragma unroll
+  for (int q = 0; q < SAB_PG; ++q) {
+  float* out = W.sab32 + il[q] * (size_t)ROWS;
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt) {
+    f4v c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < KT; ++ks) {
+      float av;
+      if constexpr (PRE) av = at[rt][ks];
+      else av = atile(rt, ks);
+      c = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bk[q][ks], c, 0, 0, 0);
     }
     // the lane's 4 accumulator rows are consecutive in its pair's record: one
     // b128 store when records hold whole 4-row groups (and so stay 16-byte
     // aligned), else row by row
     const int r0 = rt * 16 + 4 * kq;
     if constexpr (ROWS % 4 == 0) {
-      if (valid && r0 < ROWS) *reinterpret_cast<f4v*>(out + r0) = c;
-    } else if (valid) {
+      if (valid[q] && r0 < ROWS) *reinterpret_cast<f4v*>(out + r0) = c;
+    } else if (valid[q]) {
 #pragma unroll
       for (int v = 0; v < 4; ++v)
         if (r0 + v < ROWS) out[r0 + v] = c[v];
     }
+  }
   }
 }
 
 template <int N, int S>
 int fourier_loop(const RTWork& W, int nitems, int* const lists[2], int* counts, hipStream_t st) {
   const int nl = (int)(((size_t)nitems * W.km + 63) / 64);
-  const int ns = (int)(((size_t)nitems * W.km + 15) / 16);
+  const int ns = (int)(((size_t)nitems * W.km + 16 * sab_groups(N, S) - 1) / (16 * sab_groups(N, S)));
   const int nb = (nitems + 64 / bvp_group(N * S) - 1) / (64 / bvp_group(N * S));
   for (int M = 0; M <= W.nmoments; ++M) {
     const int* in = lists[M & 1];

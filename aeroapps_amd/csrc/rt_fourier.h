@@ -564,8 +564,8 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
 }
 
 // ------------------------------------------------------------------ k_bvp
-// Group reductions over the G lanes that hold one item's rows (G = 16, 32 or
-// 64; every lane of the group ends with the same value).
+// Group reductions over the G lanes that hold one item's rows (G = 8, 16, 32
+// or 64; every lane of the group ends with the same value).
 // Values of the two 16-lane rows of each 32-lane half, in both rows
 // (v_permlane16_swap on x against itself): for every lane, {r.x, r.y} holds
 // the value of its own row and of the partner row at the same position.
@@ -580,7 +580,7 @@ __device__ __forceinline__ double grp_sum(double x) {
   x += dpp_d<0xB1>(x);
   x += dpp_d<0x4E>(x);
   x += dpp_d<0x141>(x);
-  x += dpp_d<0x140>(x);
+  if (G >= 16) x += dpp_d<0x140>(x);
   if (G >= 32) { double p, q; xrow16_d(x, p, q); x = p + q; }
   if (G == 64) x = readlane_d(x, 0) + readlane_d(x, 32);
   return x;
@@ -592,7 +592,7 @@ __device__ __forceinline__ int grp_min_i(int x) {
   x = min(x, __builtin_amdgcn_update_dpp(0x7fffffff, x, 0xB1, 0xF, 0xF, false));
   x = min(x, __builtin_amdgcn_update_dpp(0x7fffffff, x, 0x4E, 0xF, 0xF, false));
   x = min(x, __builtin_amdgcn_update_dpp(0x7fffffff, x, 0x141, 0xF, 0xF, false));
-  x = min(x, __builtin_amdgcn_update_dpp(0x7fffffff, x, 0x140, 0xF, 0xF, false));
+  if (G >= 16) x = min(x, __builtin_amdgcn_update_dpp(0x7fffffff, x, 0x140, 0xF, 0xF, false));
   if (G >= 32) {
     const auto r = __builtin_amdgcn_permlane16_swap((unsigned)x, (unsigned)x, false, false);
     x = min((int)r[0], (int)r[1]);
@@ -607,7 +607,7 @@ __device__ __forceinline__ unsigned grp_max_u(unsigned x) {
   x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false));
   x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false));
   x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false));
-  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false));
+  if (G >= 16) x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false));
   if (G >= 32) {
     const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
     x = max((unsigned)r[0], (unsigned)r[1]);
@@ -616,21 +616,22 @@ __device__ __forceinline__ unsigned grp_max_u(unsigned x) {
   return x;
 }
 
-// Lanes per item in k_bvp (G) and panel rows per lane (SL): the back-
-// substitution puts one unknown of a layer on each lane (2n <= G) and the 3n
-// rows of a panel fill SL row slots per lane.
-__host__ __device__ constexpr int bvp_group(int n) { return 2 * n <= 16 ? 16 : (2 * n <= 32 ? 32 : 64); }
+// Lanes per item in k_bvp (G) and panel rows per lane (SL): the 3n rows of a
+// panel fill SL row slots per lane (8 lanes and up to 3 slots for n <= 6, so
+// a wavefront carries 8 items), and the back-substitution puts ceil(2n / G)
+// unknowns of a layer on each lane.
+__host__ __device__ constexpr int bvp_group(int n) {
+  return n <= 6 ? 8 : (2 * n <= 16 ? 16 : (2 * n <= 32 ? 32 : 64));
+}
 __host__ __device__ constexpr int bvp_slots(int n) { return (3 * n + bvp_group(n) - 1) / bvp_group(n); }
 
 // U rows of a layer panel are kept in compact form: row i from column
 // (i + 1) & ~1 (the columns right of its pivot, from an even column so the
 // pairs stay 16-byte aligned) through the right-hand side and 1/pivot.
 __host__ __device__ constexpr int urow_start(int i) { return (i + 1) & ~1; }
-__host__ __device__ constexpr int urow_off(int uw, int i) {
-  int o = 0;
-  for (int r = 0; r < i; ++r) o += uw - urow_start(r);
-  return o;
-}
+// sum over r < i of (uw - urow_start(r)), in closed form
+__host__ __device__ constexpr int urow_off(int uw, int i) { return i * uw - 2 * (i / 2) * ((i + 1) / 2); }
+static_assert(urow_off(26, 12) == 240 && urow_off(26, 1) == 26 && urow_off(26, 3) == 74, "compact U row offsets");
 
 // Pivot of each group from per-slot candidate flags: the candidate of the
 // group is the one in the lowest slot, and in it the lowest lane (a group
@@ -659,11 +660,14 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
   constexpr int n = N * S, n2 = 2 * n, NC = 4 * n, RH = NC, UW = NC + 2;
   constexpr int G = bvp_group(n), IPW = 64 / G, SL = bvp_slots(n);
   constexpr int UC = urow_off(UW, n2);   // doubles per layer panel of compact U rows
-  static_assert(2 * n <= 64 && SL <= 2, "panel rows exceed the wavefront");
+  constexpr int BR = (n2 + G - 1) / G;   // U rows per lane in the back-substitution
+  static_assert(2 * n <= 64 && SL <= 3 && n <= G, "panel rows exceed the wavefront");
   constexpr LOff O(N, S);
   constexpr BOff BO(N, n);
   constexpr int BIG = 0x3fffffff;
-  __shared__ __align__(16) double us[IPW][n2][UW];   // U rows of the current panel (cols, RHS, 1/pivot)
+  // compact U rows of the current panel (columns right of the pivot, RHS,
+  // 1/pivot); column c of row i is at us[g][urow_off(UW, i) - urow_start(i) + c]
+  __shared__ __align__(16) double us[IPW][UC];
   __shared__ double xs[IPW][n2];       // integration constants of the current/next layer
   const int lane = threadIdx.x, g = lane / G, t = lane - g * G;
   const unsigned long long gm = (G == 64) ? ~0ull : (((1ull << (G & 63)) - 1ull) << (g * G));
@@ -788,31 +792,31 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
       const int src = 4 * (sel < 0 ? lane : sel);
       double myv = a[0][i];
       int myp = posl[0];
-      if (SL > 1) {
-        const double v1 = a[SL - 1][i];
-        const int p1 = posl[SL - 1];
-        myv = (psl == 1) ? v1 : myv;
-        myp = (psl == 1) ? p1 : myp;
+#pragma unroll
+      for (int sl = 1; sl < SL; ++sl) {
+        myv = (psl == sl) ? a[sl][i] : myv;
+        myp = (psl == sl) ? posl[sl] : myp;
       }
       const double pv = __hiloint2double(__builtin_amdgcn_ds_bpermute(src, __double2hiint(myv)),
                                          __builtin_amdgcn_ds_bpermute(src, __double2loint(myv)));
       const int ppos = __builtin_amdgcn_ds_bpermute(src, myp);
       const double rp = 1.0 / pv;
+      double* __restrict__ urow = &us[g][urow_off(UW, i) - urow_start(i)];   // column-indexed
 #pragma unroll
       for (int sl = 0; sl < SL; ++sl) {
-        if (ispiv[sl]) {   // aligned pairs from the even column at or before i
+        if (ispiv[sl]) {   // aligned pairs from the row's first stored column
 #pragma unroll
           for (int c = 0; c < NCOL; c += 2)
-            if (c >= (i & ~1)) *reinterpret_cast<double2*>(&us[g][i][c]) = make_double2(a[sl][c], a[sl][c + 1]);
-          *reinterpret_cast<double2*>(&us[g][i][RH]) = make_double2(a[sl][RH], rp);
+            if (c >= urow_start(i)) *reinterpret_cast<double2*>(&urow[c]) = make_double2(a[sl][c], a[sl][c + 1]);
+          *reinterpret_cast<double2*>(&urow[RH]) = make_double2(a[sl][RH], rp);
         }
       }
       __syncthreads();
       double u[NC + 1];
 #pragma unroll
       for (int c = 0; c < NCOL; ++c)
-        if (c > i) u[c] = us[g][i][c];
-      u[RH] = us[g][i][RH];
+        if (c > i) u[c] = urow[c];
+      u[RH] = urow[RH];
 #pragma unroll
       for (int sl = 0; sl < SL; ++sl) {
         const double l = (act[sl] && !ispiv[sl]) ? a[sl][i] * rp : 0.0;
@@ -829,12 +833,10 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
       sing = sing || (pv == 0.0);
     }
     __syncthreads();
-#pragma unroll
-    for (int r = 0; r < n2; ++r) {
-      const int s0 = urow_start(r);
-      double2* dst = reinterpret_cast<double2*>(ub + (size_t)k * UC + urow_off(UW, r));
-      const double2* srcu = reinterpret_cast<const double2*>(&us[g][r][s0]);
-      for (int e = t; e < (UW - s0) / 2; e += G) dst[e] = srcu[e];
+    {
+      double2* dst = reinterpret_cast<double2*>(ub + (size_t)k * UC);
+      const double2* srcu = reinterpret_cast<const double2*>(&us[g][0]);
+      for (int e = t; e < UC / 2; e += G) dst[e] = srcu[e];
     }
     if (!LAST) {
 #pragma unroll
@@ -915,86 +917,130 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
   }
 
   // ---------------- back-substitution (DTBSV column order) fused with the
-  // TOA source recursion (REG/vlidort_intensity.f90:145-643): lane t < 2n
-  // holds U row t of the panel and ends with x_t, the t-th integration
-  // constant of the layer (LCON for t < n, MCON after).  The U rows and source
-  // coefficients of the next panel up are loaded one panel ahead; x_i is
-  // broadcast within the group by lane reads.
+  // TOA source recursion (REG/vlidort_intensity.f90:145-643): lane t holds U
+  // rows t + G*b of the panel (b < BR) and ends with their x, the integration
+  // constants of the layer (LCON for rows < n, MCON after).  With one row per
+  // lane the U rows and source coefficients of the next panel up are loaded
+  // one panel ahead; x_i is broadcast within the group by lane reads.
   const double kmult = brdf ? ((M == 0) ? 2.0 : 1.0) : 2.0 * albedo;
+  constexpr bool PF = (BR == 1);
   double cum[S];
 #pragma unroll
   for (int o = 0; o < S; ++o) cum[o] = 0.0;
   const double* trr = W.tr + (size_t)it * km * TR_NFIELDS;
-  double u[UW], cf[S], cs[S], tdu = 0.0;
-  // lane t's compact U row: offset in the panel and first stored column
-  const int ut0 = urow_start(t);
-  int uoff = 0;
-  for (int r = 0; r < t && r < n2; ++r) uoff += UW - urow_start(r);
-#define AA_BS_LOAD(K)                                                                        \
-  {                                                                                          \
-    const int k_ = (K);                                                                      \
-    if (valid && k_ >= 0 && k_ < L && t < n2) {                                              \
-      const double2* src_ = reinterpret_cast<const double2*>(ub + (size_t)k_ * UC + uoff) - ut0 / 2; \
-      _Pragma("unroll") for (int e = 0; e < UW / 2; ++e) {                                   \
-        const double2 v_ = (2 * e >= ut0) ? src_[e] : make_double2(0.0, 0.0);               \
-        u[2 * e] = v_.x;                                                                     \
-        u[2 * e + 1] = v_.y;                                                                 \
-      }                                                                                      \
-      const double* lr_ = lb + (size_t)k_ * LS;                                              \
-      _Pragma("unroll") for (int o = 0; o < S; ++o) {                                        \
-        cf[o] = lr_[O.coef(o, t)];                                                           \
-        cs[o] = lr_[O.csrc(o)];                                                              \
-      }                                                                                      \
-      tdu = trr[(size_t)k_ * TR_NFIELDS + TR_TDUSER];                                        \
-    } else {                                                                                 \
-      _Pragma("unroll") for (int e = 0; e < UW; ++e) u[e] = 0.0;                             \
-      _Pragma("unroll") for (int o = 0; o < S; ++o) { cf[o] = 0.0; cs[o] = 0.0; }            \
-      tdu = 0.0;                                                                             \
-    }                                                                                        \
+  // the lane's compact U rows: offset in the panel and first stored column
+  int ut0[BR], uoff[BR];
+#pragma unroll
+  for (int b = 0; b < BR; ++b) {
+    const int r = min(t + G * b, n2);
+    ut0[b] = urow_start(r);
+    uoff[b] = urow_off(UW, r);
   }
-  AA_BS_LOAD(Lmax - 1);
+  // U rows, source coefficients and user transmittance of layer K into U, CF,
+  // CS, TDU (zeros outside the item's layers)
+#define AA_BS_LOAD(K, U, CF, CS, TDU)                                                             \
+  {                                                                                               \
+    const int k_ = (K);                                                                           \
+    const bool ok_ = valid && k_ >= 0 && k_ < L;                                                  \
+    const double* lr_ = lb + (size_t)(ok_ ? k_ : 0) * LS;                                         \
+    _Pragma("unroll") for (int b = 0; b < BR; ++b) {                                              \
+      const int r_ = t + G * b;                                                                   \
+      if (ok_ && r_ < n2) {                                                                       \
+        const double2* src_ = reinterpret_cast<const double2*>(ub + (size_t)k_ * UC + uoff[b]) - ut0[b] / 2; \
+        _Pragma("unroll") for (int e = 0; e < UW / 2; ++e) {                                      \
+          const double2 v_ = (2 * e >= ut0[b]) ? src_[e] : make_double2(0.0, 0.0);                \
+          U[b][2 * e] = v_.x;                                                                     \
+          U[b][2 * e + 1] = v_.y;                                                                 \
+        }                                                                                         \
+        _Pragma("unroll") for (int o = 0; o < S; ++o) CF[b][o] = lr_[O.coef(o, r_)];              \
+      } else {                                                                                    \
+        _Pragma("unroll") for (int e = 0; e < UW; ++e) U[b][e] = 0.0;                             \
+        _Pragma("unroll") for (int o = 0; o < S; ++o) CF[b][o] = 0.0;                             \
+      }                                                                                           \
+    }                                                                                             \
+    if (ok_ && t < n2) {                                                                          \
+      _Pragma("unroll") for (int o = 0; o < S; ++o) CS[o] = lr_[O.csrc(o)];                       \
+      TDU = trr[(size_t)k_ * TR_NFIELDS + TR_TDUSER];                                             \
+    } else {                                                                                      \
+      _Pragma("unroll") for (int o = 0; o < S; ++o) CS[o] = 0.0;                                  \
+      TDU = 0.0;                                                                                  \
+    }                                                                                             \
+  }
+  double u[BR][UW], cf[BR][S], cs[S], tdu = 0.0;
+  if (PF) AA_BS_LOAD(Lmax - 1, u, cf, cs, tdu);
   for (int k = Lmax - 1; k >= 0; --k) {
-    double uc[UW], cfc[S], csc[S];
+    double uc[BR][UW], cfc[BR][S], csc[S], tduc;
+    if (PF) {
 #pragma unroll
-    for (int e = 0; e < UW; ++e) uc[e] = u[e];
+      for (int b = 0; b < BR; ++b) {
 #pragma unroll
-    for (int o = 0; o < S; ++o) { cfc[o] = cf[o]; csc[o] = cs[o]; }
-    const double tduc = tdu;
-    AA_BS_LOAD(k - 1);
+        for (int e = 0; e < UW; ++e) uc[b][e] = u[b][e];
+#pragma unroll
+        for (int o = 0; o < S; ++o) cfc[b][o] = cf[b][o];
+      }
+#pragma unroll
+      for (int o = 0; o < S; ++o) csc[o] = cs[o];
+      tduc = tdu;
+      AA_BS_LOAD(k - 1, u, cf, cs, tdu);
+    } else {
+      AA_BS_LOAD(k, uc, cfc, csc, tduc);
+    }
     const bool here = valid && k < L;
     if (__ballot(here) == 0ull) continue;
     const bool lastp = k == L - 1;
-    const bool own = here && t < n2;
-    double tv = uc[RH];
+    bool own[BR];
+    double tv[BR];
+#pragma unroll
+    for (int b = 0; b < BR; ++b) {
+      own[b] = here && t + G * b < n2;
+      tv[b] = uc[b][RH];
+    }
     if (here && !lastp) {
 #pragma unroll
-      for (int c = n2 - 1; c >= 0; --c) tv = fma(-uc[n2 + c], xs[g][c], tv);
+      for (int b = 0; b < BR; ++b)
+#pragma unroll
+        for (int c = n2 - 1; c >= 0; --c) tv[b] = fma(-uc[b][n2 + c], xs[g][c], tv[b]);
     }
-    double xt = 0.0;
+    double xt[BR];
+#pragma unroll
+    for (int b = 0; b < BR; ++b) xt[b] = 0.0;
 #pragma unroll
     for (int i = n2 - 1; i >= 0; --i) {
-      const double cand = tv * uc[RH + 1];
+      const int bi = i / G, ti = i - bi * G;   // lane and slot of row i
+      const double cand = tv[bi] * uc[bi][RH + 1];
       double xv = 0.0;
 #pragma unroll
       for (int q = 0; q < IPW; ++q) {
-        const double v = readlane_d(cand, q * G + i);
+        const double v = readlane_d(cand, q * G + ti);
         xv = (g == q) ? v : xv;
       }
-      if (t == i) xt = xv;
-      if (t < i) tv = fma(-uc[i], xv, tv);
+      if (t == ti) xt[bi] = xv;
+#pragma unroll
+      for (int b = 0; b < BR; ++b)
+        if (t + G * b < i) tv[b] = fma(-uc[b][i], xv, tv[b]);
     }
     __syncthreads();
-    if (own) xs[g][t] = xt;
+#pragma unroll
+    for (int b = 0; b < BR; ++b)
+      if (own[b]) xs[g][t + G * b] = xt[b];
     __syncthreads();
     if (here) {
       double sv[S];
 #pragma unroll
-      for (int o = 0; o < S; ++o) sv[o] = grp_sum<G>(own ? xt * cfc[o] : 0.0);
+      for (int o = 0; o < S; ++o) {
+        double p = own[0] ? xt[0] * cfc[0][o] : 0.0;
+#pragma unroll
+        for (int b = 1; b < BR; ++b) p = own[b] ? fma(xt[b], cfc[b][o], p) : p;
+        sv[o] = grp_sum<G>(p);
+      }
       if (lastp) {
 #pragma unroll
         for (int o = 0; o < S; ++o) cum[o] = 0.0;
         if (surf) {
-          const double bv = grp_sum<G>(own ? xt * bb[BO.bcoef(t)] : 0.0);
+          double p = own[0] ? xt[0] * bb[BO.bcoef(t)] : 0.0;
+#pragma unroll
+          for (int b = 1; b < BR; ++b) p = own[b] ? fma(xt[b], bb[BO.bcoef(t + G * b)], p) : p;
+          const double bv = grp_sum<G>(p);
           cum[0] = kmult * (bv + bb[BO.bc()]);
         }
       }

@@ -617,11 +617,11 @@ __device__ __forceinline__ unsigned grp_max_u(unsigned x) {
 }
 
 // Lanes per item in k_bvp (G) and panel rows per lane (SL): the 3n rows of a
-// panel fill SL row slots per lane (8 lanes and up to 3 slots for n <= 6, so
+// panel fill SL row slots per lane (8 lanes and up to 3 slots for n <= 8, so
 // a wavefront carries 8 items), and the back-substitution puts ceil(2n / G)
 // unknowns of a layer on each lane.
 __host__ __device__ constexpr int bvp_group(int n) {
-  return n <= 6 ? 8 : (2 * n <= 16 ? 16 : (2 * n <= 32 ? 32 : 64));
+  return n <= 8 ? 8 : (2 * n <= 16 ? 16 : (2 * n <= 32 ? 32 : 64));
 }
 __host__ __device__ constexpr int bvp_slots(int n) { return (3 * n + bvp_group(n) - 1) / bvp_group(n); }
 

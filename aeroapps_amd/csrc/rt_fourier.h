@@ -117,6 +117,11 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
   const int NMOM = W.nmoments;
   const double* __restrict__ Qt = W.quad;
   auto XQ = [&](int l, int i, int s1, int s2) -> double { return Qt[QO.xq(M, l, i, s1, s2)]; };
+  // the same entry in the contraction type (FP32 table in the mixed path)
+  auto XQT = [&](int l, int i, int s1, int s2) -> TR {
+    if constexpr (MIX) return W.quad32[QO.xq(M, l, i, s1, s2)];
+    else return Qt[QO.xq(M, l, i, s1, s2)];
+  };
   auto pisign = [&](int l) -> double { return (((l - M) & 1) == 0) ? 1.0 : -1.0; };
   auto inl = [&](int l) -> bool { return l >= M && l <= NMOM; };
   double qs[N], qw[N], qhw[N], qsw[N];
@@ -309,7 +314,7 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
               TR sp = 0.0, sm = 0.0;
 #pragma unroll
               for (int o = 0; o < S; ++o) {
-                const TR xqv = (TR)XQ(l, j, o1, o);
+                const TR xqv = XQT(l, j, o1, o);
                 sp += xqv * a5 * (TR)xp[j * S + o][k];
                 sm += ((TR)DMS[o1] * xqv * (TR)pisign(l)) * a5 * (TR)xn[j * S + o][k];
               }
@@ -410,7 +415,7 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
 #pragma unroll
           for (int l = 0; l < LM; ++l) {
             if (!inl(l)) continue;
-            const TR xqv = (TR)XQ(l, i, o1, o);
+            const TR xqv = XQT(l, i, o1, o);
             sp += xqv * hq[l][o];
             sm += (xqv * (TR)DMS[o] * (TR)pisign(l)) * hq[l][o];
           }

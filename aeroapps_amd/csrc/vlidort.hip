@@ -178,6 +178,19 @@ const std::vector<double>& quad_table_cached(int N, int S, int nmoments) {
   return it->second;
 }
 
+// FP32 copy of the quadrature table for the FP32 contractions of the mixed
+// path (the same values as the FP64 table rounded once).
+const std::vector<float>& quad32_cached(int N, int S, int nmoments) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int>, std::vector<float>> cache;
+  const std::vector<double>& q = quad_table_cached(N, S, nmoments);
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_tuple(N, S, nmoments);
+  auto it = cache.find(key);
+  if (it == cache.end()) it = cache.emplace(key, std::vector<float>(q.begin(), q.end())).first;
+  return it->second;
+}
+
 // Contraction weights of the mixed-precision SAB/DAB GEMM for every Fourier
 // term: SAB(r,c) = sum_k Y[r*n+c][k] og[k], DAB(r,c) = sum_k Y[n^2+r*n+c][k] og[k]
 // with k = (l,o,o3) (restates the SAB/DAB sums of REG/vlidort_solutions.f90:
@@ -301,6 +314,7 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   const std::vector<double>& qtab = quad_table_cached(W.N, W.S, W.nmoments);
   static const std::vector<float> no_ytab;
   const std::vector<float>& ytab = (W.precision == 1) ? ytab_cached(W.N, W.S, W.nmoments) : no_ytab;
+  const std::vector<float>& qtab32 = (W.precision == 1) ? quad32_cached(W.N, W.S, W.nmoments) : no_ytab;
   static const std::vector<double> no_btab;
   const std::vector<double>& btab =
       W.kernel_wt ? brdf_table_cached(W.N, BR_NAZ) : (W.u10m ? brdf_table_cached(W.N, BR_NAZ_CM) : no_btab);
@@ -316,11 +330,13 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   if (envcap) cap = (int)std::min<size_t>((size_t)cap, envcap);
   // arena: work + quad table + active lists + counters
   float* dy = nullptr;
+  float* dq32 = nullptr;
   auto carve_all = [&](char* base, RTWork& w, double** dq, int** l0, int** l1, int** cnt) {
     Carver C(base);
     carve_work(C, w, cap);
     *dq = C.take<double>(qtab.size());
     dy = C.take<float>(std::max<size_t>(1, ytab.size()));
+    dq32 = C.take<float>(std::max<size_t>(1, qtab32.size()));
     dbt = C.take<double>(std::max<size_t>(1, btab.size()));
     *l0 = C.take<int>(cap);
     *l1 = C.take<int>(cap);
@@ -338,6 +354,10 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   if (!ytab.empty()) {
     AA_CHECK(hipMemcpyAsync(dy, ytab.data(), ytab.size() * sizeof(float), hipMemcpyHostToDevice, st));
     W.ytab = dy;
+  }
+  if (!qtab32.empty()) {
+    AA_CHECK(hipMemcpyAsync(dq32, qtab32.data(), qtab32.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    W.quad32 = dq32;
   }
   if (!btab.empty()) {
     AA_CHECK(hipMemcpyAsync(dbt, btab.data(), btab.size() * sizeof(double), hipMemcpyHostToDevice, st));

@@ -671,8 +671,10 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
   constexpr BOff BO(N, n);
   constexpr int BIG = 0x3fffffff;
   // compact U rows of the current panel (columns right of the pivot, RHS,
-  // 1/pivot); column c of row i is at us[g][urow_off(UW, i) - urow_start(i) + c]
-  __shared__ __align__(16) double us[IPW][UC];
+  // 1/pivot); column c of row i is at us[g][urow_off(UW, i) - urow_start(i) + c].
+  // The groups' blocks are padded by two doubles so that the groups' broadcast
+  // reads of the same row fall in different LDS banks.
+  __shared__ __align__(16) double us[IPW][UC + 2];
   __shared__ double xs[IPW][n2];       // integration constants of the current/next layer
   const int lane = threadIdx.x, g = lane / G, t = lane - g * G;
   const unsigned long long gm = (G == 64) ? ~0ull : (((1ull << (G & 63)) - 1ull) << (g * G));

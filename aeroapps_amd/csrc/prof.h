@@ -12,6 +12,7 @@ enum ProfKind {
   PROF_LAYER,       // per-layer eigensystem + particular solutions
   PROF_BVP,         // band LU + back-substitution + source + convergence
   PROF_BRDF,        // BRDF supplement (MODIS land surface)
+  PROF_SABDAB,      // FP32 SAB/DAB contraction of the mixed path (matrix cores)
   PROF_KINDS
 };
 bool prof_on();

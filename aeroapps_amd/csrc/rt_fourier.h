@@ -1211,14 +1211,18 @@ int fourier_loop(const RTWork& W, int nitems, int* const lists[2], int* counts, 
   for (int M = 0; M <= W.nmoments; ++M) {
     const int* in = lists[M & 1];
     int* out = lists[(M + 1) & 1];
-    prof_begin(PROF_LAYER, st);
     if (W.precision == 1) {
+      prof_begin(PROF_SABDAB, st);
       hipLaunchKernelGGL((k_sabdab<N, S>), dim3(ns), dim3(64), 0, st, W, M, in, counts + M);
+      prof_end(PROF_SABDAB, st);
+      prof_begin(PROF_LAYER, st);
       hipLaunchKernelGGL((k_layer<N, S, true>), dim3(nl), dim3(64), 0, st, W, M, in, counts + M);
+      prof_end(PROF_LAYER, st);
     } else {
+      prof_begin(PROF_LAYER, st);
       hipLaunchKernelGGL((k_layer<N, S, false>), dim3(nl), dim3(64), 0, st, W, M, in, counts + M);
+      prof_end(PROF_LAYER, st);
     }
-    prof_end(PROF_LAYER, st);
     prof_begin(PROF_BVP, st);
     hipLaunchKernelGGL((k_bvp<N, S>), dim3(nb), dim3(64), 0, st, W, M, in, counts + M, out, counts + M + 1);
     prof_end(PROF_BVP, st);

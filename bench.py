@@ -30,7 +30,7 @@ from aeroapps_amd import synthetic as S  # noqa: E402
 METRIC = "columns/sec at 72 layers/12 streams, mixed vs FP64 GPU; p99 granule latency"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
-PROF_NAMES = ["mie", "check", "optics", "geom", "item", "layer", "bvp", "brdf"]
+PROF_NAMES = ["mie", "check", "optics", "geom", "item", "layer", "bvp", "brdf", "sabdab"]
 # HBM bytes per item-term (read, written) of the radiance kernels at N=6
 # scalar, from FETCH_SIZE/WRITE_SIZE counter runs scaled by a copy-kernel
 # calibration (profiles/hbm_traffic_40k_columns_4ch_12streams.md)
@@ -247,11 +247,13 @@ def main():
             run_step(precision)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        msa = (ctypes.c_double * 8)()
-        lna = (ctypes.c_longlong * 8)()
-        _lib.check(L.aa_profile_read(msa, lna, 8), "aa_profile_read")
-        kms = {PROF_NAMES[k]: msa[k] for k in range(8) if k < 7 or modis}
-        kln = {PROF_NAMES[k]: lna[k] for k in range(8) if k < 7 or modis}
+        nk = len(PROF_NAMES)
+        msa = (ctypes.c_double * nk)()
+        lna = (ctypes.c_longlong * nk)()
+        _lib.check(L.aa_profile_read(msa, lna, nk), "aa_profile_read")
+        keep = [k for k in range(nk) if k < 7 or (k == 7 and modis) or (k == 8 and lna[k] > 0)]
+        kms = {PROF_NAMES[k]: msa[k] for k in keep}
+        kln = {PROF_NAMES[k]: lna[k] for k in keep}
         ms_per_step = el * 1000.0 / args.steps
         nfh = nf.cpu().numpy().ravel().copy()
         radh = rad.cpu().numpy().ravel().copy()

@@ -296,7 +296,8 @@ const char* aa_version(void);
  * milliseconds and launch counts for nkinds classes (0 Chem_Mie optics,
  * 1 input checks, 2 layer optics/delta-M, 3 Chapman + first-order geometry,
  * 4 per-column setup, 5 per-layer eigensystems, 6 boundary-value problem +
- * source + convergence, 7 BRDF supplement) and switches recording off. */
+ * source + convergence, 7 BRDF supplement, 8 the mixed path's SAB/DAB
+ * contraction on the matrix cores) and switches recording off. */
 int aa_profile_start(void);
 int aa_profile_read(double* ms, long long* launches, int nkinds);
 

@@ -52,6 +52,10 @@ def test_fp64_matches_reference_golden(gpu, name):
     (24, (354.0, 388.0, 550.0, 870.0), 6, 64),
     (40, (550.0,), 4, 32),
     (16, (550.0, 870.0), 3, 16),
+    # stream counts whose band-solver layout no golden case reaches: two
+    # back-substitution rows per lane with two (N=5) or three (N=7) row slots
+    (12, (470.0,), 5, 32),
+    (12, (660.0,), 7, 32),
 ])
 def test_fp64_matches_oracle_seeded(gpu, ncol, channels, nstreams, nmom):
     print(f"case seeded ncol={ncol} N={nstreams} nmom={nmom}", flush=True)
@@ -69,6 +73,34 @@ def test_fp64_matches_oracle_seeded(gpu, ncol, channels, nstreams, nmom):
     assert rc1 == rc0
     assert max_rel(I, I0.T) < RTOL_FP64
     assert max_rel(R, R0.T) < RTOL_FP64
+    assert (nf == nf0.T).all()
+
+
+@pytest.mark.parametrize("ncol,channels,nstreams,nmom", [
+    (8, (550.0,), 3, 16),   # n = 9: one row slot per lane in 32-lane groups
+])
+def test_vector_fp64_matches_oracle_seeded(gpu, ncol, channels, nstreams, nmom):
+    print(f"case vector seeded ncol={ncol} N={nstreams} nmom={nmom}", flush=True)
+    cols = S.make_columns(ncol, 72, channels, seed=321, col0=7000)
+    tabs = S.make_mie_tables(channels, nmom=nmom, npol=6)
+    tau, ssa, g, pm, rc = CM.get_aop_vector(tabs, channels, tabs[0].lambda_m, S.TRACERS, cols.qm, cols.rh, nmom, 6)
+    f = lambda a: np.asarray(a, dtype=np.float64)
+    alpha = np.zeros_like(cols.rot)
+    P = oracle.port()
+    I0, R0, Q0, U0, nf0, rc0 = P.vector_lambert(72, cols.channels, nstreams, nmom, cols.rot, cols.depol, alpha,
+                                                f(tau), f(ssa), f(pm), cols.pe, cols.he, cols.te, cols.albedo,
+                                                cols.sza, cols.raa, cols.vza, cols.flux_factor)
+    I, R, Q, U, rc1, nf = V.vector_lambert(cols.channels, nstreams, False, cols.rot.T, cols.depol, alpha.T, f(tau).T,
+                                           f(ssa).T, f(pm).T, cols.pe.T, cols.he.T, cols.te.T, cols.albedo.T,
+                                           cols.sza, cols.raa, cols.vza, cols.flux_factor.T, S.MISSING,
+                                           return_nfourier=True)
+    assert rc1 == rc0
+    assert max_rel(I, I0.T) < RTOL_FP64
+    assert max_rel(R, R0.T) < RTOL_FP64
+    ok = np.abs(I0.T / S.MISSING - 1) > 1e-3
+    for got, want in ((Q, Q0), (U, U0)):
+        # Q and U are small differences of large terms: bound them against |I|
+        assert (np.abs(got - want.T)[ok] / np.abs(I0.T)[ok]).max() < RTOL_FP64
     assert (nf == nf0.T).all()
 
 

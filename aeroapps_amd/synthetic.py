@@ -332,12 +332,12 @@ def make_ocean(ncol: int, channels_nm: Sequence[float], seed: int = 7, col0: int
 
 # Named workloads.  nstreams is the per-hemisphere stream count NSTREAMS.
 WORKLOADS: Dict[str, dict] = {
-    "cfg0_scalar_1024": dict(ncol=1024, km=72, channels=(550.0,), nstreams=4, nmom=32, npol=1, vector=False),
-    "cfg1_scalar_500k": dict(ncol=500_000, km=72, channels=(354.0, 388.0, 550.0, 870.0), nstreams=6, nmom=64,
+    "scalar_1k_1ch_n4": dict(ncol=1024, km=72, channels=(550.0,), nstreams=4, nmom=32, npol=1, vector=False),
+    "scalar_500k_4ch": dict(ncol=500_000, km=72, channels=(354.0, 388.0, 550.0, 870.0), nstreams=6, nmom=64,
                              npol=1, vector=False),
-    "cfg2_vector_200k": dict(ncol=200_000, km=72, channels=(550.0, 870.0), nstreams=4, nmom=32, npol=6,
+    "vector_200k_2ch": dict(ncol=200_000, km=72, channels=(550.0, 870.0), nstreams=4, nmom=32, npol=6,
                              vector=True),
-    "cfg3_uneven_300k": dict(ncol=300_000, km=72, channels=(550.0,), nstreams=None, nmom=32, npol=1, vector=False,
+    "uneven_300k_1ch": dict(ncol=300_000, km=72, channels=(550.0,), nstreams=None, nmom=32, npol=1, vector=False,
                              uneven_layers=(30, 72), stream_choices=(2, 4, 6, 8)),
 }
 

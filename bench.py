@@ -124,7 +124,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg1_scalar_500k")
+    ap.add_argument("--workload", default="scalar_500k_4ch")
     ap.add_argument("--ncol", type=int, default=0, help="override the column count")
     ap.add_argument("--fp64-only", action="store_true", help="skip the mixed-precision run")
     ap.add_argument("--cpu-cols", type=int, default=512)

@@ -24,13 +24,14 @@
 // and the n BOA rows touch the bottom layer.  DGBTF2's band window at a column
 // of layer k therefore holds, besides structural zeros, exactly the n rows left
 // over from layer k-1 and the 2n rows of boundary k: one panel of 3n rows over
-// the 4n columns of layers k, k+1 and the right-hand side.  One lane holds one
-// row in registers; the pivot search is a lane reduction, the pivot row is
-// broadcast through LDS, and the rows of the next boundary move into the lanes
-// whose rows became pivots.  Row positions are tracked as DGBTF2 numbers them
-// (ties go to the smallest position), so pivots, interchanges and updated
-// values are those of the band factorisation without its arithmetic on
-// structural zeros.  U rows (with 1/pivot) go to a per-item buffer that the
+// the 4n columns of layers k, k+1 and the right-hand side.  Each lane of an
+// item's group holds up to three of these rows in registers (row slots); the
+// pivot search is a group reduction, the pivot row is broadcast through LDS,
+// and the rows of the next boundary move into the slots whose rows became
+// pivots.  Row positions are tracked as DGBTF2 numbers them (ties go to the
+// smallest position), so pivots, interchanges and updated values are those of
+// the band factorisation without its arithmetic on structural zeros.  U rows
+// (with 1/pivot) go, in compact form, to a per-item buffer that the
 // back-substitution reads layer by layer.
 #pragma once
 #include <type_traits>

@@ -102,10 +102,8 @@ template <int N, int S, bool MIX>
 __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __restrict__ active,
                                               const int* __restrict__ nactive) {
   constexpr int n = N * S, LM = 2 * N;
-  // MIX: SAB/DAB come from the FP32 MFMA contraction, the source-assembly
-  // contractions accumulate in FP32; eigensystem, exponentials and
-  // normalisations stay FP64.
-  using TR = typename std::conditional<MIX, float, double>::type;
+  // SAB/DAB come from the MFMA contraction (FP32 when MIX, else FP64); all
+  // other arithmetic of the layer is FP64.
   constexpr LOff O(N, S);
   constexpr QOff QO(N, S);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -118,11 +116,6 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
   const int NMOM = W.nmoments;
   const double* __restrict__ Qt = W.quad;
   auto XQ = [&](int l, int i, int s1, int s2) -> double { return Qt[QO.xq(M, l, i, s1, s2)]; };
-  // the same entry in the contraction type (FP32 table in the mixed path)
-  auto XQT = [&](int l, int i, int s1, int s2) -> TR {
-    if constexpr (MIX) return W.quad32[QO.xq(M, l, i, s1, s2)];
-    else return Qt[QO.xq(M, l, i, s1, s2)];
-  };
   auto pisign = [&](int l) -> double { return (((l - M) & 1) == 0) ? 1.0 : -1.0; };
   auto inl = [&](int l) -> bool { return l >= M && l <= NMOM; };
   double qs[N], qw[N], qhw[N], qsw[N];
@@ -149,49 +142,24 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
         for (int o2 = 0; o2 < S; ++o2) og[l][o1][o2] = ogp[(l * S + o1) * S + o2];
   }
 
-  // ---- SAB / DAB (REG/vlidort_solutions.f90:439-475)
-  // (dp, dm) of stream pair (i,o1), (j,o2)
-  auto dpdm = [&](int i, int o1, int j, int o2, double& dp, double& dm) {
-    dp = 0.0; dm = 0.0;
-#pragma unroll
-    for (int l = 0; l < LM; ++l) {
-      if (!inl(l)) continue;
-      double sp = 0.0, sm = 0.0;
-#pragma unroll
-      for (int o = 0; o < S; ++o) {
-        double h2p = 0.0, h2m = 0.0;
-#pragma unroll
-        for (int o3 = 0; o3 < S; ++o3) {
-          const double xqv = XQ(l, j, o3, o2);
-          h2p += og[l][o][o3] * xqv;
-          h2m += og[l][o][o3] * (DMS[o3] * xqv * pisign(l));
-        }
-        const double xi = XQ(l, i, o1, o);
-        sp += xi * h2p; sm += xi * h2m;
-      }
-      dp += sp; dm += sm;
-    }
-  };
-  const float* s32 = MIX ? W.sab32 + il * (size_t)(2 * n * n) : nullptr;
+  // ---- SAB / DAB (REG/vlidort_solutions.f90:439-475): the omega-Greek
+  // contraction of every (item, layer) comes from the matrix-core kernel
+  // k_sabdab (FP32 in the mixed path, FP64 otherwise); the diagonal term is
+  // added here
+  using TSD = typename std::conditional<MIX, float, double>::type;
+  const TSD* srec;
+  if constexpr (MIX) srec = W.sab32 + il * (size_t)(2 * n * n);
+  else srec = W.sab64 + il * (size_t)(2 * n * n);
   double sab[n][n];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     const double xinv = -1.0 / qs[i];
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-      const double fac = xinv * qhw[j];
+    for (int j = 0; j < N; ++j)
 #pragma unroll
       for (int o1 = 0; o1 < S; ++o1)
 #pragma unroll
-        for (int o2 = 0; o2 < S; ++o2) {
-          if (MIX) {
-            sab[i * S + o1][j * S + o2] = (double)s32[(i * S + o1) * n + j * S + o2];
-          } else {
-            double dp, dm; dpdm(i, o1, j, o2, dp, dm);
-            sab[i * S + o1][j * S + o2] = fac * (dp + dm);
-          }
-        }
-    }
+        for (int o2 = 0; o2 < S; ++o2) sab[i * S + o1][j * S + o2] = (double)srec[(i * S + o1) * n + j * S + o2];
 #pragma unroll
     for (int o1 = 0; o1 < S; ++o1) sab[i * S + o1][i * S + o1] -= xinv;
   }
@@ -210,18 +178,7 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
         const int r = i * S + o1;
         double drow[n];
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-          const double fac = xinv * qhw[j];
-#pragma unroll
-          for (int o2 = 0; o2 < S; ++o2) {
-            if (MIX) {
-              drow[j * S + o2] = (double)s32[n * n + r * n + j * S + o2];
-            } else {
-              double dp, dm; dpdm(i, o1, j, o2, dp, dm);
-              drow[j * S + o2] = fac * (dp - dm);
-            }
-          }
-        }
+        for (int c = 0; c < n; ++c) drow[c] = (double)srec[n * n + r * n + c];
         drow[r] -= xinv;
 #pragma unroll
         for (int c = 0; c < n; ++c) {
@@ -301,23 +258,23 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
 #pragma unroll
       for (int o = 0; o < S; ++o) { updn[k][o] = 0.0; upup[k][o] = 0.0; }
     } else {
-      TR gaux[LM][S];
+      double gaux[LM][S];
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
-        TR hs[S];
+        double hs[S];
 #pragma unroll
         for (int o1 = 0; o1 < S; ++o1) {
-          TR sps = 0.0;
+          double sps = 0.0;
           if (inl(l)) {
 #pragma unroll
             for (int j = 0; j < N; ++j) {
-              const TR a5 = (TR)qhw[j];
-              TR sp = 0.0, sm = 0.0;
+              const double a5 = qhw[j];
+              double sp = 0.0, sm = 0.0;
 #pragma unroll
               for (int o = 0; o < S; ++o) {
-                const TR xqv = XQT(l, j, o1, o);
-                sp += xqv * a5 * (TR)xp[j * S + o][k];
-                sm += ((TR)DMS[o1] * xqv * (TR)pisign(l)) * a5 * (TR)xn[j * S + o][k];
+                const double xqv = XQ(l, j, o1, o);
+                sp += xqv * a5 * xp[j * S + o][k];
+                sm += (DMS[o1] * xqv * pisign(l)) * a5 * xn[j * S + o][k];
               }
               sps = sps + sp + sm;
             }
@@ -326,22 +283,22 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
         }
 #pragma unroll
         for (int o1 = 0; o1 < S; ++o1) {
-          TR s = 0.0;
+          double s = 0.0;
 #pragma unroll
-          for (int o = 0; o < S; ++o) s += (TR)og[l][o1][o] * hs[o];
+          for (int o = 0; o < S; ++o) s += og[l][o1][o] * hs[o];
           gaux[l][o1] = s;
         }
       }
 #pragma unroll
       for (int o1 = 0; o1 < S; ++o1) {
-        TR sdn = 0.0, sup = 0.0;
+        double sdn = 0.0, sup = 0.0;
 #pragma unroll
         for (int o = 0; o < S; ++o)
 #pragma unroll
           for (int l = 0; l < LM; ++l) {
             if (!inl(l)) continue;
-            const TR xum = (TR)(DMS[o1] * xup[l][o1][o] * DMS[o] * pisign(l));
-            const TR xupre = (TR)(DMS[o1] * xup[l][o1][o]);
+            const double xum = (DMS[o1] * xup[l][o1][o] * DMS[o] * pisign(l));
+            const double xupre = (DMS[o1] * xup[l][o1][o]);
             sdn += xum * gaux[l][o];
             sup += xupre * gaux[l][o];
           }
@@ -395,51 +352,51 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
       else cfunc[k] = cst * (zdel - wdel) / gamm[k];
       dfunc[k] = cst * (1.0 - zwdel) / gamp[k];
     }
-    TR hq[LM][S];
+    double hq[LM][S];
 #pragma unroll
     for (int o1 = 0; o1 < S; ++o1)
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
-        TR gsum = 0.0;
+        double gsum = 0.0;
 #pragma unroll
-        for (int o2 = 0; o2 < S; ++o2) gsum = gsum + (TR)og[l][o1][o2] * (TR)x0p[l][o2][0];
+        for (int o2 = 0; o2 < S; ++o2) gsum = gsum + og[l][o1][o2] * x0p[l][o2][0];
         hq[l][o1] = gsum;
       }
-    TR dpi[N][S], dmi[N][S];
+    double dpi[N][S], dmi[N][S];
 #pragma unroll
     for (int o1 = 0; o1 < S; ++o1)
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        TR sp = 0.0, sm = 0.0;
+        double sp = 0.0, sm = 0.0;
 #pragma unroll
         for (int o = 0; o < S; ++o)
 #pragma unroll
           for (int l = 0; l < LM; ++l) {
             if (!inl(l)) continue;
-            const TR xqv = XQT(l, i, o1, o);
+            const double xqv = XQ(l, i, o1, o);
             sp += xqv * hq[l][o];
-            sm += (xqv * (TR)DMS[o] * (TR)pisign(l)) * hq[l][o];
+            sm += (xqv * DMS[o] * pisign(l)) * hq[l][o];
           }
-        dpi[i][o1] = (TR)F1 * sp; dmi[i][o1] = (TR)F1 * sm;
+        dpi[i][o1] = F1 * sp; dmi[i][o1] = F1 * sm;
       }
     double gup[n], gdn[n];
 #pragma unroll
     for (int k = 0; k < n; ++k) {
-      TR av = 0.0, bv = 0.0;
+      double av = 0.0, bv = 0.0;
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        TR h1 = 0.0, h2 = 0.0, h1b = 0.0, h2b = 0.0;
+        double h1 = 0.0, h2 = 0.0, h1b = 0.0, h2b = 0.0;
 #pragma unroll
-        for (int o = 0; o < S; ++o) h1 += dpi[i][o] * (TR)xp[i * S + o][k];
+        for (int o = 0; o < S; ++o) h1 += dpi[i][o] * xp[i * S + o][k];
 #pragma unroll
-        for (int o = 0; o < S; ++o) h1b += dmi[i][o] * (TR)xn[i * S + o][k];
+        for (int o = 0; o < S; ++o) h1b += dmi[i][o] * xn[i * S + o][k];
         h1 = h1 + h1b;
 #pragma unroll
-        for (int o = 0; o < S; ++o) h2 += dmi[i][o] * (TR)xp[i * S + o][k];
+        for (int o = 0; o < S; ++o) h2 += dmi[i][o] * xp[i * S + o][k];
 #pragma unroll
-        for (int o = 0; o < S; ++o) h2b += dpi[i][o] * (TR)xn[i * S + o][k];
+        for (int o = 0; o < S; ++o) h2b += dpi[i][o] * xn[i * S + o][k];
         h2 = h2 + h2b;
-        av += (TR)qw[i] * h1; bv += (TR)qw[i] * h2;
+        av += qw[i] * h1; bv += qw[i] * h2;
       }
       aterm[k] = (double)av / normv[k];
       bterm[k] = (double)bv / normv[k];
@@ -1121,13 +1078,20 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
 // and the pairs' B columns of all groups are loaded up front; large n (the
 // vector path) keeps one group and loads the A tiles as it goes.
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d4v __attribute__((ext_vector_type(4)));
 __host__ __device__ constexpr int sab_tiles(int N, int S) {
   return ((2 * N * S * N * S + 15) / 16) * ((2 * N * S * S + 3) / 4);
 }
 __host__ __device__ constexpr int sab_groups(int N, int S) { return sab_tiles(N, S) <= 32 ? 4 : 1; }
-template <int N, int S>
+// T = float: the mixed path (v_mfma_f32_16x16x4f32); T = double: the FP64
+// path (v_mfma_f64_16x16x4f64).  Both take A and B in the same lane layout;
+// the accumulator rows of lane l are 4*(l/16) + v for f32 but l/16 + 4*v for
+// f64 (v = 0..3; checked on the device with a probe of known products).
+template <int N, int S, typename T>
 __global__ __launch_bounds__(64) void k_sabdab(RTWork W, int M, const int* __restrict__ active,
                                                const int* __restrict__ nactive) {
+  constexpr bool F64 = std::is_same<T, double>::value;
+  using V4 = typename std::conditional<F64, d4v, f4v>::type;
   constexpr int n = N * S, LM = 2 * N;
   constexpr YOff YO(N, S);
   constexpr int ROWS = YO.rows(), KK = YO.K;
@@ -1154,22 +1118,24 @@ __global__ __launch_bounds__(64) void k_sabdab(RTWork W, int M, const int* __res
     any = any || v;
   }
   if (__ballot(any) == 0ull) return;
-  float bk[SAB_PG][KT];
+  T bk[SAB_PG][KT];
 #pragma unroll
   for (int q = 0; q < SAB_PG; ++q) {
     const double* ogp = W.og + il[q] * (size_t)(LM * S * S);
 #pragma unroll
     for (int ks = 0; ks < KT; ++ks) {
       const int k = 4 * ks + kq;
-      bk[q][ks] = (valid[q] && k < KK) ? (float)ogp[k] : 0.0f;
+      bk[q][ks] = (valid[q] && k < KK) ? (T)ogp[k] : (T)0;
     }
   }
-  const float* Y = W.ytab;
-  auto atile = [&](int rt, int ks) -> float {
+  const T* Y;
+  if constexpr (F64) Y = W.ytab64;
+  else Y = W.ytab;
+  auto atile = [&](int rt, int ks) -> T {
     const int row = rt * 16 + (lane & 15), k = 4 * ks + kq;
-    return (row < ROWS && k < KK) ? Y[YO.at(M, row, k)] : 0.0f;
+    return (row < ROWS && k < KK) ? Y[YO.at(M, row, k)] : (T)0;
   };
-  float at[PRE ? RT : 1][PRE ? KT : 1];
+  T at[PRE ? RT : 1][PRE ? KT : 1];
   if constexpr (PRE) {
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
@@ -1178,27 +1144,41 @@ __global__ __launch_bounds__(64) void k_sabdab(RTWork W, int M, const int* __res
   }
 #pragma unroll
   for (int q = 0; q < SAB_PG; ++q) {
-  float* out = W.sab32 + il[q] * (size_t)ROWS;
+  T* out;
+  if constexpr (F64) out = W.sab64 + il[q] * (size_t)ROWS;
+  else out = W.sab32 + il[q] * (size_t)ROWS;
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt) {
-    f4v c = {0.0f, 0.0f, 0.0f, 0.0f};
+    V4 c = {(T)0, (T)0, (T)0, (T)0};
 #pragma unroll
     for (int ks = 0; ks < KT; ++ks) {
-      float av;
+      T av;
       if constexpr (PRE) av = at[rt][ks];
       else av = atile(rt, ks);
-      c = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bk[q][ks], c, 0, 0, 0);
+      if constexpr (F64) c = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bk[q][ks], c, 0, 0, 0);
+      else c = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bk[q][ks], c, 0, 0, 0);
     }
-    // the lane's 4 accumulator rows are consecutive in its pair's record: one
-    // b128 store when records hold whole 4-row groups (and so stay 16-byte
-    // aligned), else row by row
-    const int r0 = rt * 16 + 4 * kq;
-    if constexpr (ROWS % 4 == 0) {
-      if (valid[q] && r0 < ROWS) *reinterpret_cast<f4v*>(out + r0) = c;
-    } else if (valid[q]) {
+    if constexpr (F64) {
+      // rows rt*16 + kq + 4v: one store each
+      if (valid[q]) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
-        if (r0 + v < ROWS) out[r0 + v] = c[v];
+        for (int v = 0; v < 4; ++v) {
+          const int r = rt * 16 + kq + 4 * v;
+          if (r < ROWS) out[r] = c[v];
+        }
+      }
+    } else {
+      // the lane's 4 accumulator rows are consecutive in its pair's record:
+      // one b128 store when records hold whole 4-row groups (and so stay
+      // 16-byte aligned), else row by row
+      const int r0 = rt * 16 + 4 * kq;
+      if constexpr (ROWS % 4 == 0) {
+        if (valid[q] && r0 < ROWS) *reinterpret_cast<V4*>(out + r0) = c;
+      } else if (valid[q]) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (r0 + v < ROWS) out[r0 + v] = c[v];
+      }
     }
   }
   }
@@ -1212,18 +1192,14 @@ int fourier_loop(const RTWork& W, int nitems, int* const lists[2], int* counts, 
   for (int M = 0; M <= W.nmoments; ++M) {
     const int* in = lists[M & 1];
     int* out = lists[(M + 1) & 1];
-    if (W.precision == 1) {
-      prof_begin(PROF_SABDAB, st);
-      hipLaunchKernelGGL((k_sabdab<N, S>), dim3(ns), dim3(64), 0, st, W, M, in, counts + M);
-      prof_end(PROF_SABDAB, st);
-      prof_begin(PROF_LAYER, st);
-      hipLaunchKernelGGL((k_layer<N, S, true>), dim3(nl), dim3(64), 0, st, W, M, in, counts + M);
-      prof_end(PROF_LAYER, st);
-    } else {
-      prof_begin(PROF_LAYER, st);
-      hipLaunchKernelGGL((k_layer<N, S, false>), dim3(nl), dim3(64), 0, st, W, M, in, counts + M);
-      prof_end(PROF_LAYER, st);
-    }
+    prof_begin(PROF_SABDAB, st);
+    if (W.precision == 1) hipLaunchKernelGGL((k_sabdab<N, S, float>), dim3(ns), dim3(64), 0, st, W, M, in, counts + M);
+    else hipLaunchKernelGGL((k_sabdab<N, S, double>), dim3(ns), dim3(64), 0, st, W, M, in, counts + M);
+    prof_end(PROF_SABDAB, st);
+    prof_begin(PROF_LAYER, st);
+    if (W.precision == 1) hipLaunchKernelGGL((k_layer<N, S, true>), dim3(nl), dim3(64), 0, st, W, M, in, counts + M);
+    else hipLaunchKernelGGL((k_layer<N, S, false>), dim3(nl), dim3(64), 0, st, W, M, in, counts + M);
+    prof_end(PROF_LAYER, st);
     prof_begin(PROF_BVP, st);
     hipLaunchKernelGGL((k_bvp<N, S>), dim3(nb), dim3(64), 0, st, W, M, in, counts + M, out, counts + M + 1);
     prof_end(PROF_BVP, st);

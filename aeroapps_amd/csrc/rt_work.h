@@ -98,8 +98,9 @@ struct RTWork {
   // ---- mixed precision: FP32 phase-matrix contraction (SAB/DAB) by MFMA
   int precision = 0;               // 0 FP64, 1 mixed
   const float* ytab = nullptr;     // [M][2n^2 rows][2N*S*S] contraction weights
-  const float* quad32 = nullptr;   // FP32 copy of quad (mixed precision)
+  const double* ytab64 = nullptr;  // ytab in FP64 (FP64 path)
   float* sab32 = nullptr;          // [it][km][2n^2] SAB rows then DAB rows (FP32)
+  double* sab64 = nullptr;         // the same in FP64 (FP64 path)
   // ---- outputs (device, (nobs,nch) Fortran)
   double* radiance = nullptr;
   double* reflectance = nullptr;

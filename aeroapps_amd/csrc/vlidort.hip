@@ -178,26 +178,13 @@ const std::vector<double>& quad_table_cached(int N, int S, int nmoments) {
   return it->second;
 }
 
-// FP32 copy of the quadrature table for the FP32 contractions of the mixed
-// path (the same values as the FP64 table rounded once).
-const std::vector<float>& quad32_cached(int N, int S, int nmoments) {
-  static std::mutex mu;
-  static std::map<std::tuple<int, int, int>, std::vector<float>> cache;
-  const std::vector<double>& q = quad_table_cached(N, S, nmoments);
-  std::lock_guard<std::mutex> lk(mu);
-  auto key = std::make_tuple(N, S, nmoments);
-  auto it = cache.find(key);
-  if (it == cache.end()) it = cache.emplace(key, std::vector<float>(q.begin(), q.end())).first;
-  return it->second;
-}
-
 // Contraction weights of the mixed-precision SAB/DAB GEMM for every Fourier
 // term: SAB(r,c) = sum_k Y[r*n+c][k] og[k], DAB(r,c) = sum_k Y[n^2+r*n+c][k] og[k]
 // with k = (l,o,o3) (restates the SAB/DAB sums of REG/vlidort_solutions.f90:
 // 439-475 with the quadrature PI matrices folded in).
-const std::vector<float>& ytab_cached(int N, int S, int nmoments) {
+const std::vector<double>& ytab64_cached(int N, int S, int nmoments) {
   static std::mutex mu;
-  static std::map<std::tuple<int, int, int>, std::vector<float>> cache;
+  static std::map<std::tuple<int, int, int>, std::vector<double>> cache;
   const std::vector<double>& q = quad_table_cached(N, S, nmoments);
   std::lock_guard<std::mutex> lk(mu);
   auto key = std::make_tuple(N, S, nmoments);
@@ -206,7 +193,7 @@ const std::vector<float>& ytab_cached(int N, int S, int nmoments) {
   const int n = N * S, LM = 2 * N;
   QOff QO(N, S);
   YOff YO(N, S);
-  std::vector<float> y((size_t)YO.size(LM), 0.0f);
+  std::vector<double> y((size_t)YO.size(LM), 0.0);
   const double DM[4] = {1.0, 1.0, -1.0, -1.0};
   for (int M = 0; M < LM && M <= nmoments; ++M)
     for (int i = 0; i < N; ++i)
@@ -221,12 +208,24 @@ const std::vector<float>& ytab_cached(int N, int S, int nmoments) {
                 for (int o3 = 0; o3 < S; ++o3) {
                   const int k = (l * S + o) * S + o3;
                   const double xx = q[QO.xq(M, l, i, o1, o)] * q[QO.xq(M, l, j, o3, o2)];
-                  y[YO.at(M, r * n + c, k)] = (float)(fac * xx * (1.0 + DM[o3] * ps));
-                  y[YO.at(M, n * n + r * n + c, k)] = (float)(fac * xx * (1.0 - DM[o3] * ps));
+                  y[YO.at(M, r * n + c, k)] = fac * xx * (1.0 + DM[o3] * ps);
+                  y[YO.at(M, n * n + r * n + c, k)] = fac * xx * (1.0 - DM[o3] * ps);
                 }
             }
           }
   return cache.emplace(key, std::move(y)).first->second;
+}
+
+// The same weights rounded to FP32 for the mixed path.
+const std::vector<float>& ytab_cached(int N, int S, int nmoments) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int>, std::vector<float>> cache;
+  const std::vector<double>& y = ytab64_cached(N, S, nmoments);
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_tuple(N, S, nmoments);
+  auto it = cache.find(key);
+  if (it == cache.end()) it = cache.emplace(key, std::vector<float>(y.begin(), y.end())).first;
+  return it->second;
 }
 
 // Library streams (default priority) used to run stream-count buckets of one
@@ -282,6 +281,7 @@ size_t carve_work(Carver& C, RTWork& W, int cap) {
   W.wsa_cache = W.kernel_wt ? C.take<double>(4) : nullptr;
   W.ubuf = C.take<double>((size_t)cap * (2 * n * km) * W.ustride);
   W.sab32 = (W.precision == 1) ? C.take<float>((size_t)cap * km * 2 * n * n) : nullptr;
+  W.sab64 = (W.precision == 0) ? C.take<double>((size_t)cap * km * 2 * n * n) : nullptr;
   return C.off;
 }
 
@@ -314,7 +314,8 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   const std::vector<double>& qtab = quad_table_cached(W.N, W.S, W.nmoments);
   static const std::vector<float> no_ytab;
   const std::vector<float>& ytab = (W.precision == 1) ? ytab_cached(W.N, W.S, W.nmoments) : no_ytab;
-  const std::vector<float>& qtab32 = (W.precision == 1) ? quad32_cached(W.N, W.S, W.nmoments) : no_ytab;
+  static const std::vector<double> no_ytab64;
+  const std::vector<double>& ytab64 = (W.precision == 0) ? ytab64_cached(W.N, W.S, W.nmoments) : no_ytab64;
   static const std::vector<double> no_btab;
   const std::vector<double>& btab =
       W.kernel_wt ? brdf_table_cached(W.N, BR_NAZ) : (W.u10m ? brdf_table_cached(W.N, BR_NAZ_CM) : no_btab);
@@ -330,13 +331,13 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   if (envcap) cap = (int)std::min<size_t>((size_t)cap, envcap);
   // arena: work + quad table + active lists + counters
   float* dy = nullptr;
-  float* dq32 = nullptr;
+  double* dy64 = nullptr;
   auto carve_all = [&](char* base, RTWork& w, double** dq, int** l0, int** l1, int** cnt) {
     Carver C(base);
     carve_work(C, w, cap);
     *dq = C.take<double>(qtab.size());
     dy = C.take<float>(std::max<size_t>(1, ytab.size()));
-    dq32 = C.take<float>(std::max<size_t>(1, qtab32.size()));
+    dy64 = C.take<double>(std::max<size_t>(1, ytab64.size()));
     dbt = C.take<double>(std::max<size_t>(1, btab.size()));
     *l0 = C.take<int>(cap);
     *l1 = C.take<int>(cap);
@@ -355,9 +356,9 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
     AA_CHECK(hipMemcpyAsync(dy, ytab.data(), ytab.size() * sizeof(float), hipMemcpyHostToDevice, st));
     W.ytab = dy;
   }
-  if (!qtab32.empty()) {
-    AA_CHECK(hipMemcpyAsync(dq32, qtab32.data(), qtab32.size() * sizeof(float), hipMemcpyHostToDevice, st));
-    W.quad32 = dq32;
+  if (!ytab64.empty()) {
+    AA_CHECK(hipMemcpyAsync(dy64, ytab64.data(), ytab64.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    W.ytab64 = dy64;
   }
   if (!btab.empty()) {
     AA_CHECK(hipMemcpyAsync(dbt, btab.data(), btab.size() * sizeof(double), hipMemcpyHostToDevice, st));

@@ -34,7 +34,7 @@ PROF_NAMES = ["mie", "check", "optics", "geom", "item", "layer", "bvp", "brdf", 
 # HBM bytes per item-term (read, written) of the radiance kernels at N=6
 # scalar, from FETCH_SIZE/WRITE_SIZE counter runs scaled by a copy-kernel
 # calibration (profiles/hbm_traffic_40k_columns_4ch_12streams.md)
-TRAFFIC_PER_ITEM_TERM = {("bvp", 6, 1): (285.8e3, 138.9e3), ("layer", 6, 1): (17.4e3, 75.7e3)}
+TRAFFIC_PER_ITEM_TERM = {("bvp", 6, 1): (285.5e3, 138.9e3), ("layer", 6, 1): (56.3e3, 70.9e3)}
 
 
 # ----------------------------------------------------------------- algorithmic work

@@ -314,8 +314,9 @@ def main():
                                f" streams, {nmom} moments, {nq} tracers, "
                                f"{'vector' if vector else 'scalar'} "
                                f"{'MODIS land BRDF (Lambertian + Ross-thick + Li-sparse)' if modis else 'Lambertian'}",
-                   "precision": "mixed (FP32 optics/phase-matrix contraction on MFMA/source assembly; FP64 "
-                                "eigensystem and band elimination)" if rmx is not None else "fp64",
+                   "precision": "mixed (FP32 Chem_Mie optics and FP32 phase-matrix contraction on MFMA; FP64 "
+                                "everything else, the FP64 line with an FP64 MFMA contraction)"
+                                if rmx is not None else "fp64",
                    "path": ("aa_column_radiance_modis_dev" if modis else "aa_column_radiance_dev") +
                            " (Chem_Mie optics + VLIDORT radiance, device-resident)"},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",

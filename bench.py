@@ -29,6 +29,7 @@ from aeroapps_amd import synthetic as S  # noqa: E402
 
 METRIC = "columns/sec at 72 layers/12 streams, mixed vs FP64 GPU; p99 granule latency"
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 PROF_NAMES = ["mie", "check", "optics", "geom", "item", "layer", "bvp", "brdf", "sabdab"]
 # HBM bytes per item-term (read, written) of the radiance kernels at N=6
@@ -55,6 +56,17 @@ def flops_per_term(N, S_, L, nmoments, M):
     Fc = (L - 1) * (mid + bs_mid) + last + bs_last
     Fd = L * S_ * (2 * n2 + 3)
     return Fb, Fc, Fd
+
+
+def sabdab_per_term(N, S_, L, nmoments, M, elem):
+    """Algorithmic flops and HBM bytes of the SAB/DAB contraction of one
+    (column, channel) at Fourier term M: per layer the 2n^2 SAB and DAB entries
+    are each a dot product over the Lm * S^2 non-zero omega-Greek coefficients
+    (l >= M); bytes are those coefficients read plus the entries written, in
+    elements of `elem` bytes."""
+    n = S_ * N
+    k = (nmoments - M + 1) * S_ * S_
+    return L * 2 * (2 * n * n) * k, L * elem * (2 * n * n + k)
 
 
 def bytes_per_column(km, nq, nch, vector):
@@ -270,6 +282,7 @@ def main():
     nfh = main_run["nf"]
     ok = nfh > 0
     fb = fc = fd = 0.0
+    sab_f = sab_b4 = sab_b8 = 0.0
     # items are (channel, column) in nf's order; each with its column's N and L
     n_it = np.tile(nstr_h, nch)[ok]
     l_it = np.tile(nl_h, nch)[ok]
@@ -278,7 +291,23 @@ def main():
         for M in range(int(terms)):
             b_, c_, d_ = flops_per_term(int(n_c), Sx, int(l_c), min(2 * int(n_c) - 1, nmom), M)
             fb += cnt * b_; fc += cnt * c_; fd += cnt * d_
+            f_, b4_ = sabdab_per_term(int(n_c), Sx, int(l_c), min(2 * int(n_c) - 1, nmom), M, 4)
+            sab_f += cnt * f_; sab_b4 += cnt * b4_
+            sab_b8 += cnt * sabdab_per_term(int(n_c), Sx, int(l_c), min(2 * int(n_c) - 1, nmom), M, 8)[1]
     fb *= args.steps; fc *= args.steps; fd *= args.steps
+    sab_f *= args.steps; sab_b4 *= args.steps; sab_b8 *= args.steps
+
+    def mfma_use(run, f32):
+        """Matrix-core use of the SAB/DAB contraction in one timed run."""
+        ms = run["kms"].get("sabdab", 0.0)
+        if ms <= 0.0:
+            return None
+        peak = PEAK_FP32_MFMA_TFLOPS if f32 else PEAK_FP64_TFLOPS
+        tf = sab_f / (ms / 1000.0) / 1e12
+        gbs = (sab_b4 if f32 else sab_b8) / (ms / 1000.0) / 1e9
+        return {"kernel": f"k_sabdab<{N},{Sx},{'float' if f32 else 'double'}>", "ms": round(ms, 3),
+                "achieved": round(tf, 3), "peak": peak, "unit": "TFLOP/s", "frac": round(tf / peak, 4),
+                "algorithmic_GBs": round(gbs, 1), "hbm_frac": round(gbs / PEAK_HBM_GBS, 4)}
     kms, kln = main_run["kms"], main_run["kln"]
     dom = max(("layer", "bvp"), key=lambda k: kms[k])
     dom_flops = fc + fd if dom == "bvp" else fb
@@ -331,6 +360,7 @@ def main():
                      "achieved_GBs": round(path_bytes / el / 1e9, 3),
                      "frac": round(path_bytes / el / 1e9 / PEAK_HBM_GBS, 6)},
         "kernel_ms": {k: round(v, 3) for k, v in kms.items()},
+        "sabdab_mfma": {"mixed": mfma_use(rmx, True) if rmx is not None else None, "fp64": mfma_use(r64, False)},
         "fourier_terms_mean": round(float(nfh[ok].mean()), 3),
         "fp64": {"value": round(r64["value"], 1), "ms_per_step": round(r64["ms_per_step"], 3),
                  "kernel_ms": {k: round(v, 3) for k, v in r64["kms"].items()}},

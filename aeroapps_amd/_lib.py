@@ -58,6 +58,8 @@ SIGNATURES = {
     "aa_edge_vars_dev": (_ci, [_ci, _ci, _vp, _vp, _cf, _vp, _vp, _vp, _vp]),
     "aa_calc_qm": (_ci, [_ci, _ci, _ci, _fp, _fp, _fp]),
     "aa_calc_qm_dev": (_ci, [_ci, _ci, _ci, _vp, _vp, _vp, _vp]),
+    "aa_set_arena_bytes": (_ci, [ctypes.c_size_t, ctypes.c_size_t]),
+    "aa_release_arenas": (_ci, []),
     "aa_version": (ctypes.c_char_p, []),
     "aa_profile_start": (_ci, []),
     "aa_profile_read": (_ci, [_dp, ctypes.POINTER(ctypes.c_longlong), _ci]),

@@ -7,6 +7,7 @@
 // quadrature streams (identical for every column of a call), and the chunked
 // pipeline setup -> Fourier terms.
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -94,6 +95,20 @@ int arena_reserve(hipStream_t st, int purpose, size_t bytes, char** out) {
   return 0;
 }
 enum ArenaPurpose { AR_WORK = 0, AR_OPTICS = 1, AR_CALL = 2 };
+
+// Budgets of the work and optics arenas of later calls (0: AA_WORK_BYTES /
+// AA_OPTICS_BYTES, else 24 GiB / 4 GiB, which leaves room for several
+// streams with an arena each).
+std::atomic<size_t> g_work_bytes{0}, g_optics_bytes{0};
+size_t env_bytes(const char* name, size_t dflt);
+size_t work_budget() {
+  const size_t v = g_work_bytes.load();
+  return v ? v : env_bytes("AA_WORK_BYTES", (size_t)24 << 30);
+}
+size_t optics_budget() {
+  const size_t v = g_optics_bytes.load();
+  return v ? v : env_bytes("AA_OPTICS_BYTES", (size_t)4 << 30);
+}
 
 // Sequential carving of one arena block (a first pass with base == nullptr
 // measures the block).
@@ -292,6 +307,18 @@ size_t bytes_per_item(const RTWork& W0) {
   return s2 - s1 + 256 * 24;
 }
 
+// Items of one work chunk of W's shape on the current GPU: the work budget
+// (capped at 60% of the free device memory) over the bytes of one item.
+int work_cap_items(const RTWork& W, size_t* cap) {
+  size_t freeb = 0, totalb = 0;
+  AA_CHECK(hipMemGetInfo(&freeb, &totalb));
+  const size_t budget = std::min(work_budget(), (size_t)(0.6 * (double)freeb));
+  *cap = std::max<size_t>(1, budget / bytes_per_item(W));
+  const size_t envcap = env_bytes("AA_CHUNK_ITEMS", 0);
+  if (envcap) *cap = std::min(*cap, envcap);
+  return 0;
+}
+
 // First item of the SZA==0 nadir latch for the current call, in the call
 // arena of the stream.
 int compute_latch(RTWork& W, int* lp, hipStream_t st) {
@@ -321,14 +348,12 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
       W.kernel_wt ? brdf_table_cached(W.N, BR_NAZ) : (W.u10m ? brdf_table_cached(W.N, BR_NAZ_CM) : no_btab);
   double* dbt = nullptr;
   // chunk capacity from the work budget
-  size_t freeb = 0, totalb = 0;
-  AA_CHECK(hipMemGetInfo(&freeb, &totalb));
-  const size_t per = bytes_per_item(W);
-  size_t budget = env_bytes("AA_WORK_BYTES", (size_t)24 << 30);
-  budget = std::min(budget, (size_t)(0.6 * (double)freeb));
-  int cap = (int)std::max<size_t>(1, std::min<size_t>((size_t)nitems, budget / per));
-  const size_t envcap = env_bytes("AA_CHUNK_ITEMS", 0);
-  if (envcap) cap = (int)std::min<size_t>((size_t)cap, envcap);
+  size_t capi = 1;
+  if (work_cap_items(W, &capi)) return AA_EHIP;
+  // chunks of equal size: a small remainder chunk would run every Fourier
+  // term on a nearly idle GPU
+  const size_t nchunk = ((size_t)nitems + capi - 1) / capi;
+  const int cap = (int)(((size_t)nitems + nchunk - 1) / nchunk);
   // arena: work + quad table + active lists + counters
   float* dy = nullptr;
   double* dy64 = nullptr;
@@ -572,10 +597,33 @@ int column_radiance(aa_mie* t, int km, int nobs, int nch, int nq, const int* tra
     AA_CHECK(hipStreamSynchronize(st));
     for (int v : nstr) if (v < 1 || v > max_streams(S)) return AA_EDIMS;
   }
+  RTWork W;
+  W.km = km; W.nch = nch; W.nobs = nobs; W.nmom = nmom; W.npol = npol;
+  W.N = nstreams; W.S = S; W.nmoments = std::min(2 * nstreams - 1, nmom);
+  W.missing = missing;
+  W.rot = d_rot; W.depol = d_depol; W.alpha = nullptr;
+  W.he = d_he; W.albedo = modis ? nullptr : d_albedo; W.sza = d_sza; W.raa = d_raa; W.vza = d_vza;
+  W.kernel_wt = d_kernel_wt; W.kparam = d_param; W.br_out = d_br;
+  W.flux = vector ? d_flux_factor : nullptr;
+  W.nlayers = d_nlayers;
+  W.radiance = d_radiance; W.reflectance = d_reflectance; W.qout = d_q; W.uout = d_u; W.nfourier = d_nfourier;
+  W.optics_f32 = 1;
+  W.precision = precision;
+  // optics blocks that fit one work chunk of the largest stream count, split
+  // evenly, so that each block's items run as one chunk
+  int maxN = 1;
+  for (int v : nstr) maxN = std::max(maxN, v);
+  RTWork Wp = W;
+  Wp.N = maxN; Wp.nmoments = std::min(2 * maxN - 1, nmom);
+  size_t wcap = 1;
+  if (work_cap_items(Wp, &wcap)) return AA_EHIP;
   const size_t per_obs = (size_t)km * nch * (3 + (size_t)nmom * npol) * sizeof(float);
-  size_t ob = std::max<size_t>(1, env_bytes("AA_OPTICS_BYTES", (size_t)4 << 30) / per_obs);
+  size_t ob = std::max<size_t>(1, optics_budget() / per_obs);
+  ob = std::min<size_t>(ob, std::max<size_t>(1, wcap / (size_t)nch));
   ob = std::min<size_t>(ob, (size_t)nobs);
-  const int nblk = (int)((nobs + ob - 1) / ob);
+  int nblk = (int)((nobs + ob - 1) / ob);
+  ob = ((size_t)nobs + nblk - 1) / nblk;
+  nblk = (int)((nobs + ob - 1) / ob);
   // host item lists: for block b and stream count N, items [off, off+cnt)
   struct Bucket { int blk, N, off, cnt; };
   std::vector<Bucket> buckets;
@@ -609,18 +657,6 @@ int column_radiance(aa_mie* t, int km, int nobs, int nch, int nq, const int* tra
   float* ds = dt + n3;
   float* dg = ds + n3;
   float* dp = dg + n3;
-  RTWork W;
-  W.km = km; W.nch = nch; W.nobs = nobs; W.nmom = nmom; W.npol = npol;
-  W.N = nstreams; W.S = S; W.nmoments = std::min(2 * nstreams - 1, nmom);
-  W.missing = missing;
-  W.rot = d_rot; W.depol = d_depol; W.alpha = nullptr;
-  W.he = d_he; W.albedo = modis ? nullptr : d_albedo; W.sza = d_sza; W.raa = d_raa; W.vza = d_vza;
-  W.kernel_wt = d_kernel_wt; W.kparam = d_param; W.br_out = d_br;
-  W.flux = vector ? d_flux_factor : nullptr;
-  W.nlayers = d_nlayers;
-  W.radiance = d_radiance; W.reflectance = d_reflectance; W.qout = d_q; W.uout = d_u; W.nfourier = d_nfourier;
-  W.optics_f32 = 1;
-  W.precision = precision;
   W.tau = dt; W.ssa = ds; W.pmom = dp;
   int rcode = compute_latch(W, lp, st);
   for (int b = 0; b < nblk && !rcode; ++b) {
@@ -665,6 +701,28 @@ int column_radiance(aa_mie* t, int km, int nobs, int nch, int nq, const int* tra
 }  // namespace aa
 
 extern "C" {
+
+int aa_set_arena_bytes(size_t work_bytes, size_t optics_bytes) {
+  aa::g_work_bytes.store(work_bytes);
+  aa::g_optics_bytes.store(optics_bytes);
+  return 0;
+}
+
+int aa_release_arenas(void) {
+  int dev = 0;
+  AA_CHECK(hipGetDevice(&dev));
+  AA_CHECK(hipDeviceSynchronize());
+  std::lock_guard<std::mutex> lk(aa::g_arena_mu);
+  for (auto it = aa::g_arenas.begin(); it != aa::g_arenas.end();) {
+    if (std::get<0>(it->first) == dev) {
+      if (it->second.base) AA_CHECK(hipFree(it->second.base));
+      it = aa::g_arenas.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  return 0;
+}
 
 int aa_column_radiance_dev(aa_mie* t, int km, int nobs, int nch, int nq, const int* tracer_idx, const int* chan_idx,
                            const float* d_qm, const float* d_rh, int nmom, int npol, int nstreams, int vector,

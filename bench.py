@@ -33,9 +33,11 @@ PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 PROF_NAMES = ["mie", "check", "optics", "geom", "item", "layer", "bvp", "brdf", "sabdab"]
 # HBM bytes per item-term (read, written) of the radiance kernels at N=6
-# scalar, from FETCH_SIZE/WRITE_SIZE counter runs scaled by a copy-kernel
-# calibration (profiles/hbm_traffic_40k_columns_4ch_12streams.md)
-TRAFFIC_PER_ITEM_TERM = {("bvp", 6, 1): (285.5e3, 138.9e3), ("layer", 6, 1): (56.3e3, 70.9e3)}
+# scalar, keyed (kernel, N, S, mixed), from FETCH_SIZE/WRITE_SIZE counter runs
+# scaled by a copy-kernel calibration
+# (profiles/hbm_traffic_40k_columns_4ch_12streams.md)
+TRAFFIC_PER_ITEM_TERM = {("bvp", 6, 1, False): (287.6e3, 138.9e3), ("bvp", 6, 1, True): (287.6e3, 138.9e3),
+                         ("layer", 6, 1, False): (56.1e3, 70.8e3), ("layer", 6, 1, True): (35.7e3, 74.0e3)}
 
 
 # ----------------------------------------------------------------- algorithmic work
@@ -149,6 +151,8 @@ def main():
                     help="surface model: Lambertian albedo or the MODIS land BRDF (scalar workloads)")
     ap.add_argument("--host-cols", type=int, default=20000,
                     help="columns of the host-pointer (PCIe-inclusive) drop-in phase (0 skips it)")
+    ap.add_argument("--work-gib", type=float, default=64.0,
+                    help="work-arena budget of the throughput runs in GiB (one stream; 0 keeps the library default)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the path has no cross-GPU step (every column is independent); "
@@ -274,9 +278,15 @@ def main():
         return dict(el=el, kms=kms, kln=kln, ms_per_step=ms_per_step, value=ncol / (ms_per_step / 1000.0), nf=nfh,
                     rad=radh)
 
+    if args.work_gib > 0:
+        _lib.check(L.aa_set_arena_bytes(int(args.work_gib * (1 << 30)), 0), "aa_set_arena_bytes")
     r64 = timed(0)
     rmx = timed(1) if not args.fp64_only else None
     main_run = rmx if rmx is not None else r64
+    if args.work_gib > 0:
+        # the serving slots and the host-pointer phase run with the default budgets
+        _lib.check(L.aa_release_arenas(), "aa_release_arenas")
+        _lib.check(L.aa_set_arena_bytes(0, 0), "aa_set_arena_bytes")
 
     # algorithmic work of the timed region from the Fourier-term counts
     nfh = main_run["nf"]
@@ -315,7 +325,7 @@ def main():
     dom_launches = max(1, kln[dom])
     achieved = dom_flops / (dom_ms / 1000.0) / 1e12
     item_terms = float(nfh[ok].sum()) * args.steps
-    tr_key = (dom, N, Sx)
+    tr_key = (dom, N, Sx, rmx is not None)
     traffic = None
     if tr_key in TRAFFIC_PER_ITEM_TERM and wl["nstreams"] is not None and not uneven and not modis:
         rd, wr = TRAFFIC_PER_ITEM_TERM[tr_key]
@@ -347,7 +357,8 @@ def main():
                                 "everything else, the FP64 line with an FP64 MFMA contraction)"
                                 if rmx is not None else "fp64",
                    "path": ("aa_column_radiance_modis_dev" if modis else "aa_column_radiance_dev") +
-                           " (Chem_Mie optics + VLIDORT radiance, device-resident)"},
+                           " (Chem_Mie optics + VLIDORT radiance, device-resident)",
+                   "work_arena_gib": args.work_gib if args.work_gib > 0 else 24.0},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                      "traffic_GBs": (round(traffic * dom_launches / (dom_ms / 1000.0) / 1e9, 1) if traffic else None),

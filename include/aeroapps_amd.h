@@ -20,6 +20,8 @@
 #ifndef AEROAPPS_AMD_H
 #define AEROAPPS_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -285,6 +287,20 @@ int aa_column_radiance_modis_dev(aa_mie* t, int km, int nobs, int nch, int nq, c
                                  const double* d_param, const double* d_sza, const double* d_raa,
                                  const double* d_vza, double missing, double* d_radiance, double* d_reflectance,
                                  double* d_br, int* d_nfourier, void* stream);
+
+/* ------------------------------------------------------------------ work memory
+ * No counterpart in the reference (its work arrays are Fortran locals).
+ * aa_set_arena_bytes() sets the byte budgets of the per-(GPU, stream) work
+ * arena and Chem_Mie optics block of later calls, process-wide; 0 restores the
+ * default (AA_WORK_BYTES / AA_OPTICS_BYTES from the environment, else 24 GiB
+ * and 4 GiB, sized so that several streams can each hold an arena).  A
+ * one-stream bulk caller on a 288 GB GPU gains from 64 GiB / 4 GiB: fewer,
+ * larger chunks keep the Fourier terms' shrinking active lists filling the
+ * GPU.  The work budget is further capped at 60% of the free device memory.
+ * aa_release_arenas() synchronises the current GPU and frees every arena the
+ * library holds on it (they are grown again on the next call). */
+int aa_set_arena_bytes(size_t work_bytes, size_t optics_bytes);
+int aa_release_arenas(void);
 
 /* Library version string. */
 const char* aa_version(void);

@@ -69,4 +69,16 @@ def test_fused_equals_host_path(gpu, monkeypatch, ncol, channels, N, nmom):
     monkeypatch.setenv("AA_CHUNK_ITEMS", "7")
     rad2, refl2, nf2 = fused(torch, mie, cols, channels, N, nmom, 1)
     assert np.array_equal(rad, rad2) and np.array_equal(nf, nf2)
+    # the same through budgets set by the API (they override the environment),
+    # after the arenas were released
+    monkeypatch.delenv("AA_OPTICS_BYTES")
+    monkeypatch.delenv("AA_CHUNK_ITEMS")
+    L = _lib.lib()
+    _lib.check(L.aa_release_arenas(), "aa_release_arenas")
+    _lib.check(L.aa_set_arena_bytes(1 << 20, 72 * len(channels) * (3 + nmom) * 4 * 3), "aa_set_arena_bytes")
+    try:
+        rad3, refl3, nf3 = fused(torch, mie, cols, channels, N, nmom, 1)
+    finally:
+        _lib.check(L.aa_set_arena_bytes(0, 0), "aa_set_arena_bytes")
+    assert np.array_equal(rad, rad3) and np.array_equal(refl, refl3) and np.array_equal(nf, nf3)
     mie.close()

@@ -609,17 +609,16 @@ int column_radiance(aa_mie* t, int km, int nobs, int nch, int nq, const int* tra
   W.radiance = d_radiance; W.reflectance = d_reflectance; W.qout = d_q; W.uout = d_u; W.nfourier = d_nfourier;
   W.optics_f32 = 1;
   W.precision = precision;
-  // optics blocks that fit one work chunk of the largest stream count, split
-  // evenly, so that each block's items run as one chunk
-  int maxN = 1;
-  for (int v : nstr) maxN = std::max(maxN, v);
-  RTWork Wp = W;
-  Wp.N = maxN; Wp.nmoments = std::min(2 * maxN - 1, nmom);
-  size_t wcap = 1;
-  if (work_cap_items(Wp, &wcap)) return AA_EHIP;
+  // optics blocks split evenly; with one stream count they also fit one work
+  // chunk, so that each block's items run as one chunk (per-column stream
+  // counts split a block into buckets, each chunked on its own)
   const size_t per_obs = (size_t)km * nch * (3 + (size_t)nmom * npol) * sizeof(float);
   size_t ob = std::max<size_t>(1, optics_budget() / per_obs);
-  ob = std::min<size_t>(ob, std::max<size_t>(1, wcap / (size_t)nch));
+  if (!d_nstreams_col) {
+    size_t wcap = 1;
+    if (work_cap_items(W, &wcap)) return AA_EHIP;
+    ob = std::min<size_t>(ob, std::max<size_t>(1, wcap / (size_t)nch));
+  }
   ob = std::min<size_t>(ob, (size_t)nobs);
   int nblk = (int)((nobs + ob - 1) / ob);
   ob = ((size_t)nobs + nblk - 1) / nblk;

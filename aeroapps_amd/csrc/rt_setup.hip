@@ -91,22 +91,25 @@ __device__ inline void sunpaths_d_sum(bool zerosunboa, double radstart, const Co
     }
     return;
   }
-  double sth0 = sthstart, th0 = thstart;
-  double sth1 = sth0 * radstart / G.radius(n1);
-  double th1 = asin(sth1);
-  double ks1 = th0 - th1;
+  // The reference's layer path sin(th0 - th1) r0 / sin(th1) (law of sines in
+  // the triangle of the Earth's centre and the ray's crossings of radii r0 <
+  // r1) equals c1 - c0 with c = sqrt(r^2 - b^2), b = r sin(th) the ray's
+  // impact parameter, constant along it; written (r1 - r0)(r1 + r0)/(c1 + c0)
+  // it needs one sqrt and one division per layer and has no cancellation.
+  (void)thstart;
+  const double b = sthstart * radstart;
+  double r0 = radstart, c0 = radstart * sqrt((1.0 - sthstart) * (1.0 + sthstart));
   {
-    const double p = sin(ks1) * radstart / sth1;
+    const double r1 = G.radius(n1), c1 = sqrt((r1 - b) * (r1 + b));
+    const double p = (r1 - r0) * (r1 + r0) / (c1 + c0);
     AA_FOR_CH(c) s[c] = ext(n, c) * p;
+    r0 = r1; c0 = c1;
   }
-  sth0 = sth1; th0 = th1;
   for (int k = n1; k >= 1; --k) {
-    sth1 = sth0 * G.radius(k) / G.radius(k - 1);
-    th1 = asin(sth1);
-    ks1 = th0 - th1;
-    const double p = sin(ks1) * G.radius(k) / sth1;
+    const double r1 = G.radius(k - 1), c1 = sqrt((r1 - b) * (r1 + b));
+    const double p = (r1 - r0) * (r1 + r0) / (c1 + c0);
     AA_FOR_CH(c) s[c] += ext(k, c) * p;
-    sth0 = sth1; th0 = th1;
+    r0 = r1; c0 = c1;
   }
 }
 

@@ -39,6 +39,12 @@ SIGNATURES = {
     "aa_vlidort_vector_lambert": (_ci, [_ci, _ci, _ci, _dp, _ci, _ci, _ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                         _dp, _dp, _dp, _dp, _dp, _dp, _dp, _cd, _ci, _ci, _dp, _dp, _dp, _dp, _ip,
                                         _ip]),
+    "aa_vlidort_vector_lambert_opt": (_ci, [_ci, _ci, _ci, _dp, _ci, _ci, _ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp,
+                                            _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _cd, _ci, _ci, _ci, _ci, _dp,
+                                            _dp, _dp, _dp, _ip, _ip]),
+    "aa_vlidort_vector_lambert_multigeom": (_ci, [_ci, _ci, _ci, _ci, _dp, _ci, _ci, _ci, _ci, _dp, _dp, _dp, _dp,
+                                                  _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _cd, _ci, _ci,
+                                                  _ci, _ci, _dp, _dp, _dp, _dp, _ip, _ip]),
     "aa_column_radiance_dev": (_ci, [_vp, _ci, _ci, _ci, _ci, _vp, _vp, _vp, _vp, _ci, _ci, _ci, _ci, _ci, _vp, _vp,
                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _cd, _vp, _vp, _vp, _vp, _vp,
                                      _vp]),
@@ -66,7 +72,7 @@ SIGNATURES = {
 }
 
 ERRORS = {-1: "AA_EARG (bad argument)", -2: "AA_EHIP (HIP runtime error)", -3: "AA_ENOTABLE (no table)",
-          -4: "AA_EDIMS (beyond compiled maxima)"}
+          -4: "AA_EDIMS (beyond compiled maxima)", -5: "AA_EUNSUP (option not implemented)"}
 
 _LIB = None
 

@@ -21,6 +21,7 @@
 #include <cctype>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -259,10 +260,13 @@ struct aa_mie {
   // reff/gf/rhop/rhod/vol/area [rh], pback [ch][rh][pol]; empty for pre-interpolated tables
   std::vector<std::vector<float>> reff_t, gf_t, rhop_t, rhod_t, vol_t, area_t, pback_t;
   std::vector<int> npback_t;
-  // device copies
+  // device copies, uploaded on first use after a change; `mu` serialises the
+  // upload (and the table edits that mark it dirty) between host threads
+  // that share one table set, e.g. the slots of a serving loop
   void* dmem[12] = {nullptr};
   aa::MieDev dev;
   bool dirty = true;
+  std::mutex mu;
 };
 
 namespace {
@@ -306,7 +310,13 @@ void free_dev(aa_mie* t) {
 }
 
 int upload(aa_mie* t) {
+  std::lock_guard<std::mutex> lk(t->mu);
   if (!t->dirty) return 0;
+  // tables changed after an earlier upload: work of earlier calls may still
+  // read the old copies on some stream
+  bool had = false;
+  for (void* p : t->dmem) had = had || p != nullptr;
+  if (had) AA_CHECK(hipDeviceSynchronize());
   free_dev(t);
   auto put = [&](int slot, const void* src, size_t bytes) -> int {
     if (bytes == 0) bytes = 4;
@@ -375,9 +385,22 @@ int aa_mie_create(aa_mie** out, int nch, const float* channels_m) {
   return 0;
 }
 
+static int add_table_locked(aa_mie* t, int nrh, int nbin, int nmom, int npol, const float* rh, const float* bext,
+                            const float* bsca, const float* bbck, const float* g, const float* refr,
+                            const float* refi, const float* pmom, const char* bin_names);
+
 int aa_mie_add_table(aa_mie* t, int nrh, int nbin, int nmom, int npol, const float* rh, const float* bext,
                      const float* bsca, const float* bbck, const float* g, const float* refr, const float* refi,
                      const float* pmom, const char* bin_names) {
+  if (!t) return -1;
+  std::lock_guard<std::mutex> lk(t->mu);
+  return add_table_locked(t, nrh, nbin, nmom, npol, rh, bext, bsca, bbck, g, refr, refi, pmom, bin_names);
+}
+
+// Body of aa_mie_add_table; the caller holds t->mu.
+static int add_table_locked(aa_mie* t, int nrh, int nbin, int nmom, int npol, const float* rh, const float* bext,
+                            const float* bsca, const float* bbck, const float* g, const float* refr,
+                            const float* refi, const float* pmom, const char* bin_names) {
   if (!t || nrh <= 0 || nbin <= 0 || nmom < 0 || npol <= 0 || npol > 6 || !rh || !bext || !bsca || !g ||
       !bin_names)
     return -1;
@@ -469,6 +492,7 @@ int aa_mie_add_table_native(aa_mie* t, int nch_table, const double* lambda_table
     return -1;
   *rc = 0;
   if (nmom > 0 && nmom > nmom_table) { *rc = 99; return 0; }   // :296-301
+  std::lock_guard<std::mutex> lk(t->mu);
   if (nmom > 0 && !pmom) return -1;
   const int nch = t->nch;
   const size_t nl = (size_t)nch_table, n3 = nl * nrh * nbin, n2 = (size_t)nrh * nbin;
@@ -516,7 +540,7 @@ int aa_mie_add_table_native(aa_mie* t, int nch_table, const double* lambda_table
                 polint_ref(lambda_table, pmom + src + n3 * ((size_t)p * nmom_table + m), nch_table, 1, xw);
       }
   const int old_tr = (int)t->names.size();
-  int e = aa_mie_add_table(t, nrh, nbin, nmom, npol_table, rhf.data(), ob.data(), os.data(), obb.data(), og.data(),
+  int e = add_table_locked(t, nrh, nbin, nmom, npol_table, rhf.data(), ob.data(), os.data(), obb.data(), og.data(),
                            orr.data(), ori.data(), nmom > 0 ? opm.data() : nullptr, bin_names);
   if (e) return e;
   for (int j = 0; j < nbin; ++j) {

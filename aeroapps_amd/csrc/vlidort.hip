@@ -424,7 +424,8 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
                  const double* vza, const double* flux_factor, double missing, int precision, double* radiance,
                  double* reflectance, double* q, double* u, int* nfourier, int* rc,
                  const double* kernel_wt = nullptr, const double* kparam = nullptr, double* br = nullptr,
-                 const double* u10m = nullptr, const double* v10m = nullptr, const double* mr = nullptr) {
+                 const double* u10m = nullptr, const double* v10m = nullptr, const double* mr = nullptr,
+                 bool stokes1 = false, const int* host_latch = nullptr) {
   if (!rc) return AA_EARG;
   *rc = 0;
   const bool cx = u10m != nullptr;
@@ -440,7 +441,10 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
   if (modis && nmom > MAX_MOMENTS_IN) { *rc = 5; return 0; }
   if (vector && (!q || !u || !flux_factor || npol < 6)) return AA_EARG;
   if (precision != 0 && precision != 1) return AA_EARG;
-  const int S = vector ? 3 : 1;
+  // stokes1: the vector driver run with NSTOKES = 1 (its DO_2OS_CORRECTION
+  // option): Stokes-1 arithmetic on the vector driver's inputs (gas
+  // absorption, flux factor); Q and U are 0
+  const int S = (vector && !stokes1) ? 3 : 1;
   if (nstreams < 1 || nstreams > max_streams(S)) { *rc = 2; return AA_EDIMS; }
   if (km > MAX_LAYERS) { *rc = 3; return AA_EDIMS; }
   if (nobs == 0) return 0;
@@ -478,8 +482,8 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
   const size_t no = (size_t)nobs * nch;
   W.radiance = B.alloc<double>(no);
   W.reflectance = B.alloc<double>(no);
-  W.qout = vector ? B.alloc<double>(no) : nullptr;
-  W.uout = vector ? B.alloc<double>(no) : nullptr;
+  W.qout = S == 3 ? B.alloc<double>(no) : nullptr;
+  W.uout = S == 3 ? B.alloc<double>(no) : nullptr;
   W.nfourier = B.alloc<int>(no);
   for (void* p : B.ptrs) if (!p) return AA_EHIP;
   int* lp = nullptr;
@@ -488,18 +492,126 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
     if (arena_reserve(0, AR_CALL, 256, &cb)) return AA_EHIP;
     lp = (int*)cb;
   }
-  int e = compute_latch(W, lp, 0);
+  int e = 0;
+  if (host_latch) {
+    AA_CHECK(hipMemcpy(lp, host_latch, sizeof(int), hipMemcpyHostToDevice));
+    W.latch = lp;
+  } else {
+    e = compute_latch(W, lp, 0);
+  }
   if (!e) e = run_items(W, 0, nobs * nch, 0);
   if (e) return e;
   AA_CHECK(hipDeviceSynchronize());
   AA_CHECK(hipMemcpy(radiance, W.radiance, no * sizeof(double), hipMemcpyDeviceToHost));
   AA_CHECK(hipMemcpy(reflectance, W.reflectance, no * sizeof(double), hipMemcpyDeviceToHost));
-  if (vector) {
+  if (S == 3) {
     AA_CHECK(hipMemcpy(q, W.qout, no * sizeof(double), hipMemcpyDeviceToHost));
     AA_CHECK(hipMemcpy(u, W.uout, no * sizeof(double), hipMemcpyDeviceToHost));
+  } else if (vector) {
+    // NSTOKES = 1: the reference returns STOKES(..,2..3,..) = 0, MISSING where
+    // the element is MISSING
+    for (size_t i = 0; i < no; ++i) {
+      const bool miss = std::fabs(radiance[i] / missing - 1.0) < 0.001;
+      q[i] = u[i] = miss ? missing : 0.0;
+    }
   }
   if (nfourier) AA_CHECK(hipMemcpy(nfourier, W.nfourier, no * sizeof(int), hipMemcpyDeviceToHost));
   if (modis) AA_CHECK(hipMemcpy(br, W.br_out, no * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// VLIDORT_Vector_Lambert_MultiGeom (GEO/VLIDORT_LAMBERT.F90:346-520 over
+// VLIDORT_Run_Vector_MultiGeom, V283/VLIDORT_ScatMod.F90:621-972): ngeom
+// (sza, vza, raa) triplets per column in one VLIDORT_MASTER call.  Every
+// geometry has its own beam solution, convergence test and Fourier count, so a
+// geometry's result equals that of a single-geometry run of the same column,
+// except for what the driver shares across geometries:
+//   * the missing-geometry and missing-albedo checks look at geometry 1 only
+//     (IS_MISSING(solar_zenith(j,1)), albedo(j,i,1)) and the surface albedo is
+//     albedo(j,i,1) for every geometry (VLIDORT_SurfaceLamb_MultiGeom);
+//   * the overhead-sun switch to nadir first-order mode fires when any
+//     geometry of the column has SZA == 0 and stays on for the rest of the call;
+//   * a failed MASTER call makes every geometry of that (column, channel)
+//     MISSING.
+// The columns are expanded to nobs*ngeom single-geometry observations
+// (observation o = j*ngeom + g) and run through the single-geometry path
+// with the latch item computed here.
+int lambert_multigeom_host(int km, int nch, int nobs, int ngeom, int nstreams, int plane_parallel, int nmom,
+                           int npol, const double* rot, const double* depol, const double* alpha, const double* tau,
+                           const double* ssa, const double* pmom, const double* he, const double* albedo,
+                           const double* sza, const double* raa, const double* vza, const double* flux_factor,
+                           double missing, int precision, bool stokes1, double* radiance, double* reflectance,
+                           double* q, double* u, int* nfourier, int* rc) {
+  if (!rc) return AA_EARG;
+  *rc = 0;
+  if (km <= 0 || nch <= 0 || nobs < 0 || ngeom <= 0 || nmom <= 0 || npol < 6) return AA_EARG;
+  if (!rot || !depol || !tau || !ssa || !pmom || !he || !albedo || !sza || !raa || !vza || !flux_factor ||
+      !radiance || !reflectance || !q || !u)
+    return AA_EARG;
+  // VLIDORT_Init rejects more beams than MAX_SZANGLES (V283/VLIDORT_Mod.F90:370)
+  if (ngeom > MAX_OBSGEOMS) { *rc = 6; return 0; }
+  if (nobs == 0) return 0;
+  const int no = nobs * ngeom;
+  const size_t L = (size_t)km;
+  auto miss = [&](double x) { return std::fabs(x / missing - 1.0) < 0.001; };
+  std::vector<double> e_rot((size_t)km * no * nch), e_alpha(alpha ? e_rot.size() : 0);
+  std::vector<double> e_tau(L * nch * no), e_ssa(e_tau.size()), e_pmom(e_tau.size() * nmom * npol);
+  std::vector<double> e_he((L + 1) * no), e_alb((size_t)no * nch), e_sza(no), e_raa(no), e_vza(no);
+  std::vector<double> e_ff((size_t)nch * no);
+  int latch = INT_MAX;
+  for (int j = 0; j < nobs; ++j) {
+    const bool gmiss = miss(sza[j]) || miss(vza[j]) || miss(raa[j]);
+    bool any_zero = false;
+    for (int g = 0; g < ngeom; ++g) any_zero = any_zero || sza[j + (size_t)g * nobs] == 0.0;
+    for (int g = 0; g < ngeom; ++g) {
+      const int o = j * ngeom + g;
+      const size_t gj = j + (size_t)g * nobs;
+      e_sza[o] = gmiss ? missing : sza[gj];
+      e_vza[o] = gmiss ? missing : vza[gj];
+      e_raa[o] = gmiss ? missing : raa[gj];
+      std::memcpy(&e_he[(size_t)o * (L + 1)], &he[(size_t)j * (L + 1)], (L + 1) * sizeof(double));
+      for (int c = 0; c < nch; ++c) {
+        e_alb[o + (size_t)c * no] = albedo[j + (size_t)c * nobs];            // albedo(j,c,1)
+        e_ff[c + (size_t)o * nch] = flux_factor[c + (size_t)j * nch];
+        std::memcpy(&e_rot[((size_t)c * no + o) * L], &rot[((size_t)c * nobs + j) * L], L * sizeof(double));
+        if (alpha)
+          std::memcpy(&e_alpha[((size_t)c * no + o) * L], &alpha[((size_t)c * nobs + j) * L], L * sizeof(double));
+        std::memcpy(&e_tau[((size_t)o * nch + c) * L], &tau[((size_t)j * nch + c) * L], L * sizeof(double));
+        std::memcpy(&e_ssa[((size_t)o * nch + c) * L], &ssa[((size_t)j * nch + c) * L], L * sizeof(double));
+        for (int pm = 0; pm < nmom * npol; ++pm)
+          std::memcpy(&e_pmom[((size_t)pm * no * nch + (size_t)o * nch + c) * L],
+                      &pmom[((size_t)pm * nobs * nch + (size_t)j * nch + c) * L], L * sizeof(double));
+      }
+    }
+    // the latch fires on the first (column, channel) that reaches the run
+    if (latch == INT_MAX && any_zero && !gmiss)
+      for (int c = 0; c < nch; ++c)
+        if (!miss(albedo[j + (size_t)c * nobs])) { latch = (j * ngeom) * nch + c; break; }
+  }
+  const size_t nout = (size_t)no * nch;
+  std::vector<double> r(nout), f(nout), qq(nout), uu(nout);
+  std::vector<int> nf(nout);
+  int e = lambert_host(true, km, nch, no, nstreams, plane_parallel, nmom, npol, e_rot.data(), depol,
+                       alpha ? e_alpha.data() : nullptr, e_tau.data(), e_ssa.data(), e_pmom.data(), e_he.data(),
+                       e_alb.data(), e_sza.data(), e_raa.data(), e_vza.data(), e_ff.data(), missing, precision,
+                       r.data(), f.data(), qq.data(), uu.data(), nf.data(), rc, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, stokes1, &latch);
+  if (e) return e;
+  // outputs (nobs, nch, ngeom); one failed geometry fails the (column, channel)
+  for (int j = 0; j < nobs; ++j)
+    for (int c = 0; c < nch; ++c) {
+      bool failed = false;
+      for (int g = 0; g < ngeom; ++g) failed = failed || miss(r[(size_t)c * no + j * ngeom + g]);
+      for (int g = 0; g < ngeom; ++g) {
+        const size_t src = (size_t)c * no + j * ngeom + g;
+        const size_t dst = j + (size_t)c * nobs + (size_t)g * nobs * nch;
+        radiance[dst] = failed ? missing : r[src];
+        reflectance[dst] = failed ? missing : f[src];
+        q[dst] = failed ? missing : qq[src];
+        u[dst] = failed ? missing : uu[src];
+        if (nfourier) nfourier[dst] = failed ? 0 : nf[src];
+      }
+    }
   return 0;
 }
 
@@ -528,10 +640,43 @@ int aa_vlidort_vector_lambert(int km, int nch, int nobs, const double* channels,
                               const double* solar_zenith, const double* relat_azymuth, const double* sensor_zenith,
                               const double* flux_factor, double missing, int verbose, int precision,
                               double* radiance, double* reflectance, double* q, double* u, int* nfourier, int* rc) {
+  return aa_vlidort_vector_lambert_opt(km, nch, nobs, channels, nstreams, plane_parallel, nmom, npol, rot, depol,
+                                       alpha, tau, ssa, pmom, pe, he, te, albedo, solar_zenith, relat_azymuth,
+                                       sensor_zenith, flux_factor, missing, verbose, precision, 0, 0, radiance,
+                                       reflectance, q, u, nfourier, rc);
+}
+
+int aa_vlidort_vector_lambert_opt(int km, int nch, int nobs, const double* channels, int nstreams,
+                                  int plane_parallel, int nmom, int npol, const double* rot, const double* depol,
+                                  const double* alpha, const double* tau, const double* ssa, const double* pmom,
+                                  const double* pe, const double* he, const double* te, const double* albedo,
+                                  const double* solar_zenith, const double* relat_azymuth,
+                                  const double* sensor_zenith, const double* flux_factor, double missing,
+                                  int verbose, int precision, int do_2os_correction, int do_boa, double* radiance,
+                                  double* reflectance, double* q, double* u, int* nfourier, int* rc) {
   (void)channels; (void)pe; (void)te; (void)verbose;
+  if (do_boa) return AA_EUNSUP;
   return aa::lambert_host(true, km, nch, nobs, nstreams, plane_parallel, nmom, npol, rot, depol, alpha, tau, ssa,
                           pmom, he, albedo, solar_zenith, relat_azymuth, sensor_zenith, flux_factor, missing,
-                          precision, radiance, reflectance, q, u, nfourier, rc);
+                          precision, radiance, reflectance, q, u, nfourier, rc, nullptr, nullptr, nullptr, nullptr,
+                          nullptr, nullptr, do_2os_correction != 0);
+}
+
+int aa_vlidort_vector_lambert_multigeom(int km, int nch, int nobs, int ngeom, const double* channels, int nstreams,
+                                        int plane_parallel, int nmom, int npol, const double* rot,
+                                        const double* depol, const double* alpha, const double* tau,
+                                        const double* ssa, const double* pmom, const double* pe, const double* he,
+                                        const double* te, const double* albedo, const double* solar_zenith,
+                                        const double* relat_azymuth, const double* sensor_zenith,
+                                        const double* flux_factor, double missing, int verbose, int precision,
+                                        int do_2os_correction, int do_boa, double* radiance, double* reflectance,
+                                        double* q, double* u, int* nfourier, int* rc) {
+  (void)channels; (void)pe; (void)te; (void)verbose;
+  if (do_boa) return AA_EUNSUP;
+  return aa::lambert_multigeom_host(km, nch, nobs, ngeom, nstreams, plane_parallel, nmom, npol, rot, depol, alpha,
+                                    tau, ssa, pmom, he, albedo, solar_zenith, relat_azymuth, sensor_zenith,
+                                    flux_factor, missing, precision, do_2os_correction != 0, radiance, reflectance,
+                                    q, u, nfourier, rc);
 }
 
 int aa_vlidort_scalar_cx(int km, int nch, int nobs, const double* channels, int nstreams, int plane_parallel,

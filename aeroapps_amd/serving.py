@@ -40,6 +40,16 @@ class Granule:
     finish: float = 0.0
     radiance: Optional[np.ndarray] = None
     nfourier: Optional[np.ndarray] = None
+    error: Optional[str] = None   # set when the granule failed; its slot keeps serving
+
+
+class ServingError(RuntimeError):
+    """One or more granules failed; ``failed`` lists them."""
+
+    def __init__(self, failed: List[Granule]):
+        super().__init__(f"{len(failed)} granule(s) failed: " +
+                         "; ".join(f"granule {g.gid}: {g.error}" for g in failed[:5]))
+        self.failed = failed
 
 
 def granule_bytes(ncol: int, km: int, nch: int) -> int:
@@ -136,6 +146,9 @@ class Server:
             g, nbytes = item
             try:
                 self._run(g, slot)
+            except Exception as e:  # recorded on the granule; the slot keeps serving
+                g.error = f"{type(e).__name__}: {e}"
+                g.finish = time.perf_counter()
             finally:
                 with self.cv:
                     self.in_use -= nbytes
@@ -143,8 +156,14 @@ class Server:
                     self.done.append(g)
                     self.cv.notify_all()
 
-    def serve(self, granules: List[Granule], arrivals: Sequence[float]) -> Dict[str, float]:
-        """Serves granules arriving at the given offsets (seconds from start)."""
+    def serve(self, granules: List[Granule], arrivals: Sequence[float], raise_on_error: bool = True) -> Dict[str, float]:
+        """Serves granules arriving at the given offsets (seconds from start).
+
+        A granule whose run raises gets ``error`` set and does not stop the
+        loop; at the end serve() raises ServingError listing the failed
+        granules (or, with raise_on_error=False, reports their count)."""
+        with self.cv:
+            self.done = []
         queues = [queue.Queue() for _ in range(self.slots)]
         threads = [threading.Thread(target=self._worker, args=(k, queues[k]), daemon=True) for k in range(self.slots)]
         for t in threads:
@@ -184,12 +203,16 @@ class Server:
         for t in threads:
             t.join()
         t1 = time.perf_counter()
+        failed = [g for g in granules if g.error is not None]
+        if failed and raise_on_error:
+            raise ServingError(failed)
         lat = np.array([g.finish - g.arrival for g in granules])
         cols = sum(g.ncol for g in granules)
         span = max(g.finish for g in granules) - min(g.arrival for g in granules)
         return {"granules": n, "columns": int(cols), "p50_s": float(np.percentile(lat, 50)),
                 "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
-                "throughput_cols_per_s": cols / span, "wall_s": t1 - t0, "slots": self.slots}
+                "throughput_cols_per_s": cols / span, "wall_s": t1 - t0, "slots": self.slots,
+                "failed": len(failed)}
 
 
 def make_granules(n: int, pool_ncol: int, channels: Sequence[float], seed: int = 11, lo: int = 1000,

@@ -51,22 +51,38 @@ def scalar_lambert(channels, nstreams, plane_parallel, ROT, depol, tau, ssa, g, 
 
 def vector_lambert(channels, nstreams, plane_parallel, ROT, depol, alpha, tau, ssa, pmom, pe, he, te, albedo,
                    solar_zenith, relat_azymuth, sensor_zenith, flux_factor, MISSING, verbose=0, precision=0,
-                   return_nfourier=False):
-    """VLIDORT_Vector_Lambert (src/Components/rtms/geo_vlidort/VLIDORT_LAMBERT.F90:172-342)."""
+                   return_nfourier=False, DO_2OS_CORRECTION=False, DO_BOA=False):
+    """VLIDORT_Vector_Lambert (src/Components/rtms/geo_vlidort/VLIDORT_LAMBERT.F90:172-520).
+
+    Single geometry when the angles are 1-D (nobs); MultiGeom when they are
+    2-D (nobs, ngeom) with albedo (nobs, nch, ngeom): outputs are then
+    (nobs, nch, ngeom).  DO_2OS_CORRECTION=True runs the vector driver with
+    NSTOKES = 1 (Q = U = 0); DO_BOA is not implemented (AA_EUNSUP)."""
     (channels, ROT, depol, alpha, tau, ssa, pmom, pe, he, te, albedo, sza, raa, vza, ff) = map(
         _f64, (channels, ROT, depol, alpha, tau, ssa, pmom, pe, he, te, albedo, solar_zenith, relat_azymuth,
                sensor_zenith, flux_factor))
     km, nch, nobs = tau.shape
     nmom, npol = pmom.shape[3], pmom.shape[4]
-    out = [np.zeros((nobs, nch), order="F") for _ in range(4)]
-    nf = np.zeros((nobs, nch), dtype=np.int32, order="F")
+    multi = sza.ndim == 2
+    ngeom = sza.shape[1] if multi else 1
+    shape = (nobs, nch, ngeom) if multi else (nobs, nch)
+    out = [np.zeros(shape, order="F") for _ in range(4)]
+    nf = np.zeros(shape, dtype=np.int32, order="F")
     rc = np.zeros(1, dtype=np.int32)
-    e = lib().aa_vlidort_vector_lambert(km, nch, nobs, _ptr(channels), int(nstreams), int(bool(plane_parallel)),
-                                        nmom, npol, _ptr(ROT), _ptr(depol), _ptr(alpha), _ptr(tau), _ptr(ssa),
-                                        _ptr(pmom), _ptr(pe), _ptr(he), _ptr(te), _ptr(albedo), _ptr(sza),
-                                        _ptr(raa), _ptr(vza), _ptr(ff), float(MISSING), int(verbose),
-                                        int(precision), *[_ptr(o) for o in out], _ptr(nf, _ip), _ptr(rc, _ip))
-    check(e, "aa_vlidort_vector_lambert")
+    if multi:
+        e = lib().aa_vlidort_vector_lambert_multigeom(
+            km, nch, nobs, ngeom, _ptr(channels), int(nstreams), int(bool(plane_parallel)), nmom, npol, _ptr(ROT),
+            _ptr(depol), _ptr(alpha), _ptr(tau), _ptr(ssa), _ptr(pmom), _ptr(pe), _ptr(he), _ptr(te), _ptr(albedo),
+            _ptr(sza), _ptr(raa), _ptr(vza), _ptr(ff), float(MISSING), int(verbose), int(precision),
+            int(bool(DO_2OS_CORRECTION)), int(bool(DO_BOA)), *[_ptr(o) for o in out], _ptr(nf, _ip), _ptr(rc, _ip))
+        check(e, "aa_vlidort_vector_lambert_multigeom")
+    else:
+        e = lib().aa_vlidort_vector_lambert_opt(
+            km, nch, nobs, _ptr(channels), int(nstreams), int(bool(plane_parallel)), nmom, npol, _ptr(ROT),
+            _ptr(depol), _ptr(alpha), _ptr(tau), _ptr(ssa), _ptr(pmom), _ptr(pe), _ptr(he), _ptr(te), _ptr(albedo),
+            _ptr(sza), _ptr(raa), _ptr(vza), _ptr(ff), float(MISSING), int(verbose), int(precision),
+            int(bool(DO_2OS_CORRECTION)), int(bool(DO_BOA)), *[_ptr(o) for o in out], _ptr(nf, _ip), _ptr(rc, _ip))
+        check(e, "aa_vlidort_vector_lambert_opt")
     I, R, Q, U = out
     if return_nfourier:
         return I, R, Q, U, int(rc[0]), nf

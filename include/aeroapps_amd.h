@@ -31,7 +31,8 @@ enum {
   AA_EARG = -1,        /* bad size / null pointer / unsupported option */
   AA_EHIP = -2,        /* HIP runtime error (allocation, launch) */
   AA_ENOTABLE = -3,    /* table handle has no species table */
-  AA_EDIMS = -4        /* dimension beyond the compiled maxima (streams, layers, moments) */
+  AA_EDIMS = -4,       /* dimension beyond the compiled maxima (streams, layers, moments) */
+  AA_EUNSUP = -5       /* a reference option this library does not implement (DO_BOA) */
 };
 
 /* Reference return codes reproduced in the rc out-arguments. */
@@ -138,8 +139,10 @@ int aa_get_aop_vector_dev(aa_mie* t, int km, int nobs, int nch, int nq, const in
  *   [flux_factor(nch,nobs)] -> radiance,reflectance,[Q,U](nobs,nch).
  * nfourier (may be NULL) receives the number of Fourier terms used per
  * (obs, channel), 0 where MISSING.  precision: 0 = FP64 throughout (the
- * drop-in), 1 = mixed (FP32 optics/phase-matrix/source assembly, FP64
- * eigensystem and boundary-value solve).  Host pointers, synchronous. */
+ * drop-in), 1 = mixed (FP32 Chem_Mie optics and FP32 phase-matrix (SAB/DAB)
+ * contraction on the matrix cores; FP64 eigensystem, beam and user-angle
+ * solutions, source terms and boundary-value solve).  Host pointers,
+ * synchronous. */
 int aa_vlidort_scalar_lambert(int km, int nch, int nobs, const double* channels, int nstreams,
                               int plane_parallel, int nmom, int npol, const double* rot, const double* depol,
                               const double* tau, const double* ssa, const double* g, const double* pmom,
@@ -156,6 +159,41 @@ int aa_vlidort_vector_lambert(int km, int nch, int nobs, const double* channels,
                               const double* sensor_zenith, const double* flux_factor, double missing,
                               int verbose, int precision, double* radiance, double* reflectance, double* q,
                               double* u, int* nfourier, int* rc);
+
+/* VLIDORT_Vector_Lambert_SingleGeom with its optional arguments
+ * (GEO/VLIDORT_LAMBERT.F90:172-342): do_2os_correction != 0 runs the vector
+ * driver with NSTOKES = 1 (what DO_2OS_CORRECTION = .true. selects with the
+ * VLIDORT 2.8.3 library, GEO/VLIDORT_LAMBERT.F90:256-265): the Stokes-1
+ * radiance of the vector driver's inputs (gas absorption, flux factor), Q and
+ * U = 0.  do_boa != 0 (downwelling output at user level 1) is not implemented
+ * and returns AA_EUNSUP.  aa_vlidort_vector_lambert is this call with both 0. */
+int aa_vlidort_vector_lambert_opt(int km, int nch, int nobs, const double* channels, int nstreams,
+                                  int plane_parallel, int nmom, int npol, const double* rot, const double* depol,
+                                  const double* alpha, const double* tau, const double* ssa, const double* pmom,
+                                  const double* pe, const double* he, const double* te, const double* albedo,
+                                  const double* solar_zenith, const double* relat_azymuth,
+                                  const double* sensor_zenith, const double* flux_factor, double missing,
+                                  int verbose, int precision, int do_2os_correction, int do_boa, double* radiance,
+                                  double* reflectance, double* q, double* u, int* nfourier, int* rc);
+
+/* Replaces VLIDORT_Vector_Lambert_MultiGeom (GEO/VLIDORT_LAMBERT.F90:346-520,
+ * called by missions/ACCP/accp_polar_vlidort.F90:988): ngeom geometries per
+ * observation, solar_zenith/relat_azymuth/sensor_zenith(nobs,ngeom),
+ * albedo(nobs,nch,ngeom) -> radiance,reflectance,Q,U,nfourier(nobs,nch,ngeom).
+ * As the reference: the missing-geometry and missing-albedo checks use
+ * geometry 1, the surface albedo is albedo(:,:,1) for every geometry, an
+ * overhead sun in any geometry switches first-order nadir mode on for that
+ * and every later observation, and a failed element fails all its
+ * geometries.  ngeom above the reference's MAX_SZANGLES (2) gives rc = 6. */
+int aa_vlidort_vector_lambert_multigeom(int km, int nch, int nobs, int ngeom, const double* channels, int nstreams,
+                                        int plane_parallel, int nmom, int npol, const double* rot,
+                                        const double* depol, const double* alpha, const double* tau,
+                                        const double* ssa, const double* pmom, const double* pe, const double* he,
+                                        const double* te, const double* albedo, const double* solar_zenith,
+                                        const double* relat_azymuth, const double* sensor_zenith,
+                                        const double* flux_factor, double missing, int verbose, int precision,
+                                        int do_2os_correction, int do_boa, double* radiance, double* reflectance,
+                                        double* q, double* u, int* nfourier, int* rc);
 
 /* ------------------------------------------------------------------ MODIS land surface (BRDF)
  * Replaces VLIDORT_Scalar_LandMODIS (GEO/VLIDORT_BRDF_MODIS.F90:35-202): per
@@ -298,7 +336,10 @@ int aa_column_radiance_modis_dev(aa_mie* t, int km, int nobs, int nch, int nq, c
  * larger chunks keep the Fourier terms' shrinking active lists filling the
  * GPU.  The work budget is further capped at 60% of the free device memory.
  * aa_release_arenas() synchronises the current GPU and frees every arena the
- * library holds on it (they are grown again on the next call). */
+ * library holds on it (they are grown again on the next call).  It must not
+ * be called while another host thread is inside a library call on that GPU:
+ * that call keeps the arena pointers it carved and would launch into freed
+ * memory. */
 int aa_set_arena_bytes(size_t work_bytes, size_t optics_bytes);
 int aa_release_arenas(void);
 

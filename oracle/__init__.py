@@ -6,7 +6,13 @@ CPU checkers for the aerosol-optics + discrete-ordinate radiance path:
   per-tracer optics summation;
 * ``oracle.ref``        the reference VLIDORT 2.8.3 core compiled from its
   unmodified sources by ``oracle/Makefile.ref`` (into ``oracle/_ref/``) and
-  driven through ``oracle/ref_driver.f90``;
+  driven through ``oracle/ref_driver.f90`` (a statement-by-statement harness
+  of the AeroApps drivers that also returns the Fourier-term counts);
+* ``oracle.drivers``    the reference's own AeroApps driver layer
+  (VLIDORT_LAMBERT, VLIDORT_ROT, VLIDORT_BRDF_MODIS, VLIDORT_BRDF_CX over
+  VLIDORT_ScatMod / VLIDORT_SurfaceMod / VLIDORT_Mod) compiled into
+  ``oracle/_ref/libvlidort_drv.so`` and called through ``oracle/ref_calls.f90``;
+  the harness in the same library must reproduce it bit for bit;
 * ``oracle.port``       the C++ restatement of the same path
   (``oracle/vlidort_oracle.cpp``, built by ``oracle/Makefile``).
 
@@ -23,6 +29,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_SO = os.path.join(HERE, "_ref", "libvlidort_ref.so")
+DRV_SO = os.path.join(HERE, "_ref", "libvlidort_drv.so")
 PORT_SO = os.path.join(HERE, "liboracle_vlidort.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -197,6 +204,152 @@ class ModisRef:
                         _p(param), _p(sza), _p(raa), _p(vza), missing, _p(rad), _p(refl), _p(br),
                         _p(nf, np.int32), _p(rc, np.int32))
         return rad, refl, br, nf, int(rc[0])
+
+
+class RefDrivers:
+    """ctypes binding of ``oracle/ref_calls.f90``: the reference's own driver
+    routines.  Arrays in reversed-Fortran C order as for LambertDriver.
+    Returns outputs in [nch, nobs] (MultiGeom: [ngeom, nch, nobs]) order."""
+
+    def __init__(self, lib: ctypes.CDLL):
+        ci, cd = ctypes.c_int, ctypes.c_double
+        self.lib = lib
+        self.fs = lib.drv_scalar_lambert
+        self.fs.restype = None
+        self.fs.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci] + [_dp] * 13 + [cd, _dp, _dp, _ip]
+        self.fv = lib.drv_vector_lambert
+        self.fv.restype = None
+        self.fv.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci] + [_dp] * 14 + [cd, ci, ci] + [_dp] * 4 + [_ip]
+        self.fm = lib.drv_vector_lambert_mg
+        self.fm.restype = None
+        self.fm.argtypes = [ci, ci, ci, ci, _dp, ci, ci, ci, ci] + [_dp] * 14 + [cd, ci, ci] + [_dp] * 4 + [_ip]
+        self.fr = lib.drv_rot_calc
+        self.fr.restype = None
+        self.fr.argtypes = [ci, ci, ci, _dp, _dp, _dp, _dp, cd, _dp, _dp, _ip]
+        self.fd = lib.drv_coef_depol
+        self.fd.restype = cd
+        self.fd.argtypes = [cd]
+        self.fl = lib.drv_scalar_landmodis
+        self.fl.restype = None
+        self.fl.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci] + [_dp] * 14 + [cd, _dp, _dp, _dp, _ip]
+        self.fc = lib.drv_scalar_cx
+        self.fc.restype = None
+        self.fc.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci] + [_dp] * 15 + [cd, _dp, _dp, _dp, _ip]
+
+    @staticmethod
+    def _f(*arrs):
+        return [np.ascontiguousarray(a, dtype=np.float64) for a in arrs]
+
+    def scalar_lambert(self, km, channels, nstreams, nmom, rot, depol, tau, ssa, g, pmom, pe, he, te,
+                       albedo, sza, raa, vza, missing=1e15, plane_parallel=False):
+        channels, rot, depol, tau, ssa, g, pmom, pe, he, te, albedo, sza, raa, vza = self._f(
+            channels, rot, depol, tau, ssa, g, pmom, pe, he, te, albedo, sza, raa, vza)
+        nch, nobs, npol = len(channels), len(sza), pmom.shape[0]
+        rad, refl = np.zeros((nch, nobs)), np.zeros((nch, nobs))
+        rc = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fs, km, nch, nobs, _p(channels), nstreams, int(plane_parallel), nmom, npol,
+                        _p(rot), _p(depol), _p(tau), _p(ssa), _p(g), _p(pmom), _p(pe), _p(he), _p(te),
+                        _p(albedo), _p(sza), _p(raa), _p(vza), missing, _p(rad), _p(refl), _p(rc, np.int32))
+        return rad, refl, int(rc[0])
+
+    def vector_lambert(self, km, channels, nstreams, nmom, rot, depol, alpha, tau, ssa, pmom, pe, he, te,
+                       albedo, sza, raa, vza, flux_factor, missing=1e15, plane_parallel=False,
+                       do_2os=False, do_boa=False):
+        (channels, rot, depol, alpha, tau, ssa, pmom, pe, he, te, albedo, sza, raa, vza,
+         flux_factor) = self._f(channels, rot, depol, alpha, tau, ssa, pmom, pe, he, te, albedo, sza, raa,
+                                vza, flux_factor)
+        nch, nobs, npol = len(channels), len(sza), pmom.shape[0]
+        out = [np.zeros((nch, nobs)) for _ in range(4)]
+        rc = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fv, km, nch, nobs, _p(channels), nstreams, int(plane_parallel), nmom, npol,
+                        _p(rot), _p(depol), _p(alpha), _p(tau), _p(ssa), _p(pmom), _p(pe), _p(he), _p(te),
+                        _p(albedo), _p(sza), _p(raa), _p(vza), _p(flux_factor), missing, int(do_2os),
+                        int(do_boa), *[_p(o) for o in out], _p(rc, np.int32))
+        return out[0], out[1], out[2], out[3], int(rc[0])
+
+    def vector_lambert_mg(self, km, channels, nstreams, nmom, rot, depol, alpha, tau, ssa, pmom, pe, he, te,
+                          albedo, sza, raa, vza, flux_factor, missing=1e15, plane_parallel=False,
+                          do_2os=False, do_boa=False):
+        """MultiGeom: albedo[ngeom, nch, nobs], sza/raa/vza[ngeom, nobs]."""
+        (channels, rot, depol, alpha, tau, ssa, pmom, pe, he, te, albedo, sza, raa, vza,
+         flux_factor) = self._f(channels, rot, depol, alpha, tau, ssa, pmom, pe, he, te, albedo, sza, raa,
+                                vza, flux_factor)
+        nch, npol = len(channels), pmom.shape[0]
+        ngeom, nobs = sza.shape
+        out = [np.zeros((ngeom, nch, nobs)) for _ in range(4)]
+        rc = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fm, km, nch, nobs, ngeom, _p(channels), nstreams, int(plane_parallel), nmom, npol,
+                        _p(rot), _p(depol), _p(alpha), _p(tau), _p(ssa), _p(pmom), _p(pe), _p(he), _p(te),
+                        _p(albedo), _p(sza), _p(raa), _p(vza), _p(flux_factor), missing, int(do_2os),
+                        int(do_boa), *[_p(o) for o in out], _p(rc, np.int32))
+        return out[0], out[1], out[2], out[3], int(rc[0])
+
+    def rot_calc(self, km, channels, pe, he, te, missing=1e15):
+        """VLIDORT_ROT_CALC: returns ROT[nch, nobs, km], depol[nch], rc."""
+        channels, pe, he, te = self._f(channels, pe, he, te)
+        nch, nobs = len(channels), pe.shape[0]
+        rot = np.zeros((nch, nobs, km))
+        depol = np.zeros(nch)
+        rc = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fr, km, nch, nobs, _p(channels), _p(pe), _p(he), _p(te), missing, _p(rot),
+                        _p(depol), _p(rc, np.int32))
+        return rot, depol, int(rc[0])
+
+    def coef_depol(self, wmicron: float) -> float:
+        return float(self.fd(float(wmicron)))
+
+    def scalar_landmodis(self, km, channels, nstreams, nmom, rot, depol, tau, ssa, g, pmom, pe, he, te,
+                         kernel_wt, param, sza, raa, vza, missing=1e15, plane_parallel=False):
+        channels, rot, depol, tau, ssa, g, pmom, pe, he, te, kernel_wt, param, sza, raa, vza = self._f(
+            channels, rot, depol, tau, ssa, g, pmom, pe, he, te, kernel_wt, param, sza, raa, vza)
+        nch, nobs, npol = len(channels), len(sza), pmom.shape[0]
+        rad, refl, br = (np.zeros((nch, nobs)) for _ in range(3))
+        rc = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fl, km, nch, nobs, _p(channels), nstreams, int(plane_parallel), nmom, npol,
+                        _p(rot), _p(depol), _p(tau), _p(ssa), _p(g), _p(pmom), _p(pe), _p(he), _p(te),
+                        _p(kernel_wt), _p(param), _p(sza), _p(raa), _p(vza), missing, _p(rad), _p(refl),
+                        _p(br), _p(rc, np.int32))
+        return rad, refl, br, int(rc[0])
+
+    def scalar_cx(self, km, channels, nstreams, nmom, rot, depol, tau, ssa, g, pmom, pe, he, te, u10m, v10m, mr,
+                  sza, raa, vza, missing=1e15, plane_parallel=False):
+        channels, rot, depol, tau, ssa, g, pmom, pe, he, te, u10m, v10m, mr, sza, raa, vza = self._f(
+            channels, rot, depol, tau, ssa, g, pmom, pe, he, te, u10m, v10m, mr, sza, raa, vza)
+        nch, nobs, npol = len(channels), len(sza), pmom.shape[0]
+        rad, refl, br = (np.zeros((nch, nobs)) for _ in range(3))
+        rc = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fc, km, nch, nobs, _p(channels), nstreams, int(plane_parallel), nmom, npol,
+                        _p(rot), _p(depol), _p(tau), _p(ssa), _p(g), _p(pmom), _p(pe), _p(he), _p(te),
+                        _p(u10m), _p(v10m), _p(mr), _p(sza), _p(raa), _p(vza), missing, _p(rad), _p(refl),
+                        _p(br), _p(rc, np.int32))
+        return rad, refl, br, int(rc[0])
+
+
+_DRV_LIB = None
+
+
+def _drv_lib() -> Optional[ctypes.CDLL]:
+    global _DRV_LIB
+    if _DRV_LIB is None:
+        _DRV_LIB = _load(DRV_SO)
+    return _DRV_LIB
+
+
+def drivers() -> Optional[RefDrivers]:
+    """The reference's own driver layer (built here only; absent on a GPU box)."""
+    lib = _drv_lib()
+    return None if lib is None else RefDrivers(lib)
+
+
+def drv_harness() -> Optional[LambertDriver]:
+    """The statement-by-statement harness linked into the driver-layer library."""
+    lib = _drv_lib()
+    return None if lib is None else LambertDriver(lib, "ref")
+
+
+def drv_harness_modis() -> Optional["ModisRef"]:
+    lib = _drv_lib()
+    return None if lib is None else ModisRef(lib)
 
 
 def ref_modis() -> Optional[ModisRef]:

@@ -5,7 +5,8 @@ from aeroapps_amd import _lib
 from aeroapps_amd import synthetic as S
 
 
-def fused(torch, mie, cols, channels, N, nmom, npol, nlayers=None, nstreams_col=None, vector=False, stream=None):
+def fused(torch, mie, cols, channels, N, nmom, npol, nlayers=None, nstreams_col=None, vector=False, stream=None,
+          precision=0):
     """Runs aa_column_radiance_dev on host Columns; returns (rad, refl, nf[, q, u]) as
     reversed-Fortran [nch, nobs] arrays."""
     dev = torch.device("cuda:0")
@@ -30,7 +31,7 @@ def fused(torch, mie, cols, channels, N, nmom, npol, nlayers=None, nstreams_col=
     st = stream if stream is not None else torch.cuda.current_stream(dev)
     with torch.cuda.stream(st):
         e = _lib.lib().aa_column_radiance_dev(mie.handle, km, ncol, nch, len(S.TRACERS), p(tr), p(ci), p(qm), p(rh),
-                                              nmom, npol, N, int(vector), 0, p(nl), p(ns), p(rot), p(depol), p(pe),
+                                              nmom, npol, N, int(vector), int(precision), p(nl), p(ns), p(rot), p(depol), p(pe),
                                               p(he), p(te), p(alb), p(sza), p(raa), p(vza), p(ff), S.MISSING, p(rad),
                                               p(refl), p(q), p(u), p(nf), st.cuda_stream)
     _lib.check(e, "aa_column_radiance_dev")

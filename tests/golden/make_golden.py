@@ -1,11 +1,16 @@
-"""Writes the radiance golden fixtures from the reference VLIDORT 2.8.3 core.
+"""Writes the radiance golden fixtures (inputs stored) from the reference's own
+VLIDORT_Scalar_Lambert / VLIDORT_Vector_Lambert drivers.
 
-The reference is built from its own sources by oracle/Makefile.ref into
-oracle/_ref/libvlidort_ref.so and driven through oracle/ref_driver.f90 (the
-VLIDORT_Scalar/Vector_Lambert driver set-up).  Inputs come from the seeded
-generator aeroapps_amd.synthetic and the float32 Chem_Mie restatement
-oracle.chem_mie; both inputs and reference outputs are stored, so the tests
-do not depend on regenerating them bit for bit.
+The reference's driver layer and VLIDORT 2.8.3 core are built from their
+sources by oracle/Makefile.ref into oracle/_ref/libvlidort_drv.so and called
+through oracle/ref_calls.f90.  The drivers do not return the number of
+Fourier terms; the oracle's statement-by-statement harness
+(oracle/ref_driver.f90, linked into the same library) does, and this script
+requires the harness's outputs to equal the drivers' bit for bit before it
+stores that count.  Inputs come from the seeded generator
+aeroapps_amd.synthetic and the float32 Chem_Mie restatement oracle.chem_mie;
+both inputs and reference outputs are stored, so the tests do not depend on
+regenerating them bit for bit.
 
 Run here (the reference is only available in this container):
     python tests/golden/make_golden.py [case ...]
@@ -74,8 +79,24 @@ def run_driver(drv, d, nstreams, vector, plane_parallel):
     return dict(radiance=I, reflectance=R, nfourier=nf, rc=rc)
 
 
+def driver_outputs(D, d, nstreams, vector, plane_parallel):
+    f = lambda a: np.asarray(a, dtype=np.float64)
+    km = d["km"]
+    if vector:
+        alpha = np.zeros_like(d["rot"])
+        I, R, Q, U, rc = D.vector_lambert(km, d["channels"], nstreams, d["nmom"], d["rot"], d["depol"], alpha,
+                                          f(d["tau"]), f(d["ssa"]), f(d["pmom"]), d["pe"], d["he"], d["te"],
+                                          d["albedo"], d["sza"], d["raa"], d["vza"], d["flux_factor"],
+                                          plane_parallel=plane_parallel)
+        return dict(radiance=I, reflectance=R, q=Q, u=U, rc=rc)
+    I, R, rc = D.scalar_lambert(km, d["channels"], nstreams, d["nmom"], d["rot"], d["depol"], f(d["tau"]),
+                                f(d["ssa"]), f(d["g"]), f(d["pmom"]), d["pe"], d["he"], d["te"], d["albedo"],
+                                d["sza"], d["raa"], d["vza"], plane_parallel=plane_parallel)
+    return dict(radiance=I, reflectance=R, rc=rc)
+
+
 def main():
-    ref = oracle.ref()
+    ref, D = oracle.drv_harness(), oracle.drivers()
     if ref is None:
         raise SystemExit("reference library not built: make -C oracle -f Makefile.ref")
     only = set(sys.argv[1:])
@@ -84,6 +105,9 @@ def main():
             continue
         d = make_inputs(ncol, ch, nmom, vector, edge)
         out = run_driver(ref, d, N, vector, pp)
+        want = driver_outputs(D, d, N, vector, pp)
+        for k, v in want.items():
+            assert np.array_equal(np.asarray(out[k]), np.asarray(v)), (name, k)
         meta = dict(nstreams=N, vector=int(vector), plane_parallel=int(pp))
         np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **d, **{"out_" + k: v for k, v in out.items()},
                             **{"meta_" + k: v for k, v in meta.items()})

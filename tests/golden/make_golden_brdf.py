@@ -11,8 +11,11 @@
   (oracle/_ref, VBRDF_MAINMASTER under the VLIDORT_LANDMODIS_SingleGeom set-up)
   for a set of geometries, kernel weights and stream counts, including
   white-sky-albedo failures and a negative direct bounce.
-* modis_*.npz: VLIDORT_Scalar_LandMODIS radiances of the compiled reference on
-  synthetic columns (inputs stored with the outputs).
+* modis_*.npz: radiances of the reference's own VLIDORT_Scalar_LandMODIS
+  driver (oracle/_ref/libvlidort_drv.so via oracle/ref_calls.f90) on
+  synthetic columns (inputs stored with the outputs); the oracle's
+  statement-by-statement harness must reproduce them bit for bit, and its
+  Fourier-term counts are stored with them.
 * supplement_cx.npz, cx_*.npz: the same for the Cox-Munk ocean surface
   (VLIDORT_CoxMunk_SingleGeom supplement, VLIDORT_Scalar_CX radiances).
 
@@ -103,6 +106,15 @@ RAD_CASES = {
 }
 
 
+def _same_as_driver(got, want):
+    """Harness output equals the reference driver's bit for bit (the driver
+    wrapper fills the BR it leaves undefined for a missing geometry with
+    MISSING, as the harness does)."""
+    for a, b in zip(got, want):
+        a, b = np.array(a), np.array(b)
+        assert np.array_equal(a, b), np.max(np.abs(a - b))
+
+
 def radiance_case(R, ncol, channels, nstreams, nmom, edge, seed=11):
     cols = S.make_columns(ncol, 72, channels, seed=seed)
     tabs = S.make_mie_tables(channels, nmom=nmom, npol=1)
@@ -119,9 +131,12 @@ def radiance_case(R, ncol, channels, nstreams, nmom, edge, seed=11):
         sza[7] = 0.0                       # overhead sun latches nadir FO for later columns
         raa[8] = 0.0
     f = lambda a: np.asarray(a, dtype=np.float64)
-    rad, refl, br, nf, rcr = R.scalar_landmodis(72, cols.channels, nstreams, nmom, cols.rot, cols.depol, f(tau),
-                                                f(ssa), f(g), f(pm), cols.pe, cols.he, cols.te, kw, par, sza, raa,
-                                                vza)
+    args = (72, cols.channels, nstreams, nmom, cols.rot, cols.depol, f(tau), f(ssa), f(g), f(pm), cols.pe, cols.he,
+            cols.te, kw, par, sza, raa, vza)
+    rad, refl, br, nf, rcr = R.scalar_landmodis(*args)
+    drad, drefl, dbr, drc = oracle.drivers().scalar_landmodis(*args)
+    assert drc == rcr
+    _same_as_driver((rad, refl, br), (drad, drefl, dbr))
     return dict(km=72, channels=np.asarray(channels, float), nmom=nmom, npol=1, meta_nstreams=nstreams,
                 tau=tau, ssa=ssa, g=g, pmom=pm, rot=cols.rot, depol=cols.depol, pe=cols.pe, he=cols.he,
                 te=cols.te, kernel_wt=kw, param=par, sza=sza, vza=vza, raa=raa,
@@ -183,8 +198,12 @@ def cx_case(R, ncol, channels, nstreams, nmom, edge, seed=13):
         sza[5] = 0.0                 # overhead sun latches nadir FO for later columns
         u[7], v[7] = 60.0, 10.0      # wind capped at 50 m/s
     f = lambda a: np.asarray(a, dtype=np.float64)
-    rad, refl, br, nf, rcr = R.scalar_cx(72, cols.channels, nstreams, nmom, cols.rot, cols.depol, f(tau), f(ssa),
-                                         f(g), f(pm), cols.pe, cols.he, cols.te, u, v, mr, sza, raa, vza)
+    args = (72, cols.channels, nstreams, nmom, cols.rot, cols.depol, f(tau), f(ssa), f(g), f(pm), cols.pe, cols.he,
+            cols.te, u, v, mr, sza, raa, vza)
+    rad, refl, br, nf, rcr = R.scalar_cx(*args)
+    drad, drefl, dbr, drc = oracle.drivers().scalar_cx(*args)
+    assert drc == rcr
+    _same_as_driver((rad, refl, br), (drad, drefl, dbr))
     return dict(km=72, channels=np.asarray(channels, float), nmom=nmom, npol=1, meta_nstreams=nstreams,
                 tau=tau, ssa=ssa, g=g, pmom=pm, rot=cols.rot, depol=cols.depol, pe=cols.pe, he=cols.he,
                 te=cols.te, u10m=u, v10m=v, mr=mr, sza=sza, vza=vza, raa=raa,
@@ -195,7 +214,7 @@ def main():
     os.makedirs(OUT, exist_ok=True)
     with open(os.path.join(OUT, "selftest.json"), "w") as fh:
         json.dump(selftest_vectors(), fh, indent=1)
-    R = oracle.ref_modis()
+    R = oracle.drv_harness_modis()
     assert R is not None, "build oracle/_ref first (make -C oracle -f Makefile.ref)"
     np.savez_compressed(os.path.join(OUT, "supplement.npz"), **supplement(R))
     for name, c in RAD_CASES.items():

@@ -1,0 +1,156 @@
+"""GPU parity against the reference drivers' outputs on the sampled fixtures
+(tests/golden/sampled/*.npz, written by tests/golden/make_golden_sampled.py
+from the reference's own VLIDORT_Scalar_Lambert / VLIDORT_Vector_Lambert):
+
+* FP64 (the drop-in): radiance and reflectance within 1e-9 relative, Q and U
+  within 1e-9 of |I|, identical Fourier-term counts, identical MISSING
+  pattern, on every column: all 1,024 columns of the CPU-runnable workload
+  (also through the fused device path, which must give the same bits as the
+  host-pointer path), 256 columns x 4 channels of the 500,000-column
+  workload, 64 x 2 vector columns, 64 uneven columns (per-column layers and
+  streams in one fused call), sun zenith 75-89.5 degrees (tangent-point sun
+  paths on the line of sight), MultiGeom and NSTOKES = 1 vector calls.
+* Mixed precision (precision = 1): within 1e-5 of the reference where the
+  Fourier-term counts agree; the fraction of elements whose count differs is
+  printed and must stay below 1%.
+"""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from aeroapps_amd import chem_mie as AM
+from aeroapps_amd import synthetic as S
+from aeroapps_amd import vlidort as V
+from _fixtures import GOLDEN, RTOL_FP64, max_rel
+from _gpu_helpers import fused
+
+sys.path.insert(0, GOLDEN)
+from golden_inputs import input_checksum, load_sampled, sampled_inputs, sampled_names  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+RTOL_MIXED = 1e-5
+f64 = lambda a: np.asarray(a, dtype=np.float64)  # noqa: E731
+
+
+def case(name):
+    z = load_sampled(name)
+    d = sampled_inputs(z["spec"])
+    assert input_checksum(d) == str(z["checksum"]), "generator drift: inputs differ from the fixture's"
+    return z, z["spec"], d
+
+
+def host_run(spec, d, precision=0):
+    N = spec["nstreams"]
+    common = (d["channels"], N, False, d["rot"].T, d["depol"])
+    geo = (d["pe"].T, d["he"].T, d["te"].T, d["albedo"].T, d["sza"].T, d["raa"].T, d["vza"].T)
+    if spec["vector"]:
+        alpha = np.zeros_like(d["rot"]).T
+        I, R, Q, U, rc, nf = V.vector_lambert(*common, alpha, f64(d["tau"]).T, f64(d["ssa"]).T, f64(d["pmom"]).T,
+                                              *geo, d["flux_factor"].T, S.MISSING, precision=precision,
+                                              return_nfourier=True, DO_2OS_CORRECTION=spec.get("do_2os", False))
+        return dict(radiance=I, reflectance=R, q=Q, u=U, rc=rc, nfourier=nf)
+    I, R, rc, nf = V.scalar_lambert(*common, f64(d["tau"]).T, f64(d["ssa"]).T, f64(d["g"]).T, f64(d["pmom"]).T,
+                                    *geo, S.MISSING, precision=precision, return_nfourier=True)
+    return dict(radiance=I, reflectance=R, rc=rc, nfourier=nf)
+
+
+def to_ref_order(a):
+    """(nobs, nch[, ngeom]) product outputs -> the fixtures' reversed order."""
+    return np.asarray(a).T
+
+
+def check_fp64(out, z, vector):
+    assert out["rc"] == int(z["out_rc"])
+    assert max_rel(to_ref_order(out["radiance"]), z["out_radiance"]) < RTOL_FP64
+    assert max_rel(to_ref_order(out["reflectance"]), z["out_reflectance"]) < RTOL_FP64
+    if vector:
+        I = np.abs(z["out_radiance"])
+        ok = np.abs(z["out_radiance"] / S.MISSING - 1) > 1e-3
+        for k in ("q", "u"):
+            err = np.abs(to_ref_order(out[k]) - z["out_" + k])[ok] / I[ok]
+            assert err.max() < RTOL_FP64, (k, err.max())
+    want = z["out_nfourier"]
+    known = want >= 0
+    assert (to_ref_order(out["nfourier"])[known] == want[known]).all()
+
+
+@pytest.mark.parametrize("name", [n for n in sampled_names() if n != "cfg3_uneven_64"])
+def test_sampled_fp64_host_path(gpu, name):
+    print(f"case sampled {name} fp64", flush=True)
+    z, spec, d = case(name)
+    out = host_run(spec, d)
+    check_fp64(out, z, spec["vector"])
+
+
+def columns_of(spec, d):
+    cols = S.make_columns(spec["ncol"], 72, tuple(spec["channels"]), seed=spec["seed"], col0=spec["col0"],
+                          uneven_layers=(30, 72) if spec["kind"] == "uneven" else None)
+    return dataclasses.replace(cols, sza=d["sza"], vza=d["vza"], raa=d["raa"])
+
+
+def test_cfg0_fused_equals_host_path(gpu):
+    """All 1,024 columns through the fused device path (Chem_Mie on the GPU ->
+    radiance): the same bits as the host path fed the oracle's float32 optics,
+    hence the same parity with the reference."""
+    import torch
+    print("case sampled cfg0_scalar_1024 fused", flush=True)
+    z, spec, d = case("cfg0_scalar_1024")
+    out = host_run(spec, d)
+    tabs = S.make_mie_tables(tuple(spec["channels"]), nmom=spec["nmom"], npol=1)
+    mie = AM.MieTables.from_synthetic(tabs)
+    try:
+        rad, refl, nf = fused(torch, mie, columns_of(spec, d), spec["channels"], spec["nstreams"], spec["nmom"], 1)
+    finally:
+        mie.close()
+    assert np.array_equal(rad, out["radiance"].T) and np.array_equal(nf, out["nfourier"].T)
+    assert max_rel(rad, z["out_radiance"]) < RTOL_FP64
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_cfg3_uneven_fused(gpu, precision):
+    """64 columns with their own layer counts (30-72) and stream counts
+    (4/8/12/16) in one fused call, FP64 and mixed."""
+    import torch
+    print(f"case sampled cfg3_uneven_64 precision={precision}", flush=True)
+    z, spec, d = case("cfg3_uneven_64")
+    tabs = S.make_mie_tables(tuple(spec["channels"]), nmom=spec["nmom"], npol=1)
+    mie = AM.MieTables.from_synthetic(tabs)
+    try:
+        rad, refl, nf = fused(torch, mie, columns_of(spec, d), spec["channels"], 0, spec["nmom"], 1,
+                              nlayers=d["nlayers"], nstreams_col=d["nstreams_col"], precision=precision)
+    finally:
+        mie.close()
+    if precision == 0:
+        assert max_rel(rad, z["out_radiance"]) < RTOL_FP64
+        assert max_rel(refl, z["out_reflectance"]) < RTOL_FP64
+        assert (nf == z["out_nfourier"]).all()
+        return
+    same = nf == z["out_nfourier"]
+    frac = 1.0 - same.mean()
+    err = np.abs(rad - z["out_radiance"])[same] / np.abs(z["out_radiance"])[same]
+    print(f"mixed uneven: max rel {err.max():.3e}, Fourier-count mismatch fraction {frac:.4f}", flush=True)
+    assert err.max() < RTOL_MIXED
+    assert frac < 0.01
+
+
+@pytest.mark.parametrize("name", ["cfg1_scalar_256x4", "cfg2_vector_64x2", "high_sza_scalar"])
+def test_sampled_mixed_host_path(gpu, name):
+    print(f"case sampled {name} mixed", flush=True)
+    z, spec, d = case(name)
+    out = host_run(spec, d, precision=1)
+    got = to_ref_order(out["radiance"])
+    same = to_ref_order(out["nfourier"]) == z["out_nfourier"]
+    frac = 1.0 - same.mean()
+    err = np.abs(got - z["out_radiance"])[same] / np.abs(z["out_radiance"])[same]
+    print(f"mixed {name}: I max rel {err.max():.3e}, Fourier-count mismatch fraction {frac:.4f}", flush=True)
+    assert err.max() < RTOL_MIXED
+    assert frac < 0.01
+    if spec["vector"]:
+        I = np.abs(z["out_radiance"])[same]
+        for k in ("q", "u"):
+            e = np.abs(to_ref_order(out[k]) - z["out_" + k])[same] / I
+            print(f"mixed {name}: {k} max err / |I| {e.max():.3e}", flush=True)
+            assert e.max() < RTOL_MIXED

@@ -22,9 +22,14 @@ COMMON = [
     "-Wno-pass-failed",
 ]
 # Chem_Mie arithmetic and the input preparation must not contract (the
-# reference's operation order, bit-exactness); the radiance kernels keep the
-# compiler default.
-NO_CONTRACT = {"mie.hip", "prep.hip", "brdf.hip"}
+# reference's operation order, bit-exactness).  Neither must the column setup
+# (rt_setup.hip): its first-order geometry takes sines as sqrt(1 - c*c) of
+# cosines near 1 (sun positions along the line of sight, the Stokes rotation
+# angles), where a fused multiply-add changes the result by up to ~1e-8
+# relative; near an overhead sun or in the principal plane that reaches the
+# TOA radiance (3.7e-9 at SZA 0.002 deg) and U.  The Fourier-term kernels keep
+# the compiler default.
+NO_CONTRACT = {"mie.hip", "prep.hip", "brdf.hip", "rt_setup.hip"}
 
 FOURIER_PARTS = [(n, 1) for n in range(1, 9)] + [(n, 3) for n in range(1, 5)]
 

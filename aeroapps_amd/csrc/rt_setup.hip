@@ -91,19 +91,26 @@ __device__ inline void sunpaths_d_sum(bool zerosunboa, double radstart, const Co
     }
     return;
   }
-  // The reference's layer path sin(th0 - th1) r0 / sin(th1) (law of sines in
-  // the triangle of the Earth's centre and the ray's crossings of radii r0 <
-  // r1) equals c1 - c0 with c = sqrt(r^2 - b^2), b = r sin(th) the ray's
-  // impact parameter, constant along it; written (r1 - r0)(r1 + r0)/(c1 + c0)
+  // First segment (radstart up to the top of layer n) exactly as the
+  // reference: sin(th0 - th1) r0 / sin(th1) with th1 = asin(sin(th0) r0/r1).
+  // th0 and sin(th0) come from acos(c) and sqrt(1 - c*c) of one cosine c, and
+  // near an overhead sun their rounding differs by ~1e-8 relative, which
+  // th0 - th1 ~ th0 (r1 - r0)/r1 magnifies to ~1e-3 of this segment; the
+  // radiance follows the reference only if this segment keeps its arithmetic.
+  // Above it the ray is parametrised consistently and the reference's layer
+  // path equals c1 - c0 with c = sqrt(r^2 - b^2), b = r sin(th) the ray's
+  // impact parameter (law of sines in the triangle of the Earth's centre and
+  // the ray's crossings of radii r0 < r1); written (r1 - r0)(r1 + r0)/(c1 + c0)
   // it needs one sqrt and one division per layer and has no cancellation.
-  (void)thstart;
   const double b = sthstart * radstart;
-  double r0 = radstart, c0 = radstart * sqrt((1.0 - sthstart) * (1.0 + sthstart));
+  double r0, c0;
   {
-    const double r1 = G.radius(n1), c1 = sqrt((r1 - b) * (r1 + b));
-    const double p = (r1 - r0) * (r1 + r0) / (c1 + c0);
+    const double r1 = G.radius(n1);
+    const double sth1 = sthstart * radstart / r1;
+    const double th1 = asin(sth1);
+    const double p = sin(thstart - th1) * radstart / sth1;
     AA_FOR_CH(c) s[c] = ext(n, c) * p;
-    r0 = r1; c0 = c1;
+    r0 = r1; c0 = sqrt((r1 - b) * (r1 + b));
   }
   for (int k = n1; k >= 1; --k) {
     const double r1 = G.radius(k - 1), c1 = sqrt((r1 - b) * (r1 + b));

@@ -12,9 +12,12 @@ Case kinds:
   uneven     per-column layer counts 30..72 and stream counts {2,4,6,8}: one
              reference driver call per column with its own km and nstreams
   high_sza   as lambert with the sun at 75..89.5 degrees, views up to 70
-             degrees and relative azimuth 0 or 180 degrees, so that points on
-             the outgoing line of sight see the sun at or below the horizon
-             (the tangent-point sun paths of the first-order geometry)
+             degrees and relative azimuth 0 or 180 degrees
+  grazing    the sun at 88..89.9 degrees, views 50..80 degrees, relative
+             azimuth 0 degrees for three columns in four (180 for the rest):
+             points high on the outgoing line of sight then see the sun below
+             the horizon, so the first-order geometry takes the sun paths
+             through a tangent point
   multigeom  vector MultiGeom driver: two geometries per column
 """
 from __future__ import annotations
@@ -37,6 +40,7 @@ SAMPLED_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "sampled"
 
 # field ids of the overrides (disjoint from aeroapps_amd.synthetic's)
 _F_HSZA, _F_HVZA, _F_HRAA, _F_G2SZA, _F_G2VZA, _F_G2RAA, _F_G2ALB = range(60, 67)
+_F_GSZA, _F_GVZA, _F_GRAA = range(67, 70)
 
 SAMPLED = {
     # the CPU-runnable workload, all of it: 1,024 columns, 550 nm, 8 streams
@@ -55,6 +59,10 @@ SAMPLED = {
                             nmom=64, vector=False),
     "high_sza_vector": dict(kind="high_sza", ncol=16, col0=950, seed=11, channels=(550.0,), nstreams=4,
                             nmom=32, vector=True),
+    "grazing_scalar": dict(kind="grazing", ncol=32, col0=1200, seed=13, channels=(550.0, 870.0), nstreams=6,
+                           nmom=64, vector=False),
+    "grazing_vector": dict(kind="grazing", ncol=16, col0=1300, seed=13, channels=(550.0,), nstreams=4,
+                           nmom=32, vector=True),
     "vector_multigeom": dict(kind="multigeom", ncol=12, col0=40, seed=5, channels=(550.0, 870.0), nstreams=4,
                              nmom=32, vector=True, ngeom=2),
     "vector_2os": dict(kind="lambert", ncol=12, col0=70, seed=5, channels=(550.0, 870.0), nstreams=6,
@@ -83,6 +91,10 @@ def sampled_inputs(spec: dict) -> dict:
         sza = 75.0 + 14.5 * S.uniform(spec["seed"], _F_HSZA, idx)
         vza = 70.0 * S.uniform(spec["seed"], _F_HVZA, idx)
         raa = np.where(S.uniform(spec["seed"], _F_HRAA, idx) < 0.5, 0.0, 180.0)
+    if kind == "grazing":
+        sza = 88.0 + 1.9 * S.uniform(spec["seed"], _F_GSZA, idx)
+        vza = 50.0 + 30.0 * S.uniform(spec["seed"], _F_GVZA, idx)
+        raa = np.where(S.uniform(spec["seed"], _F_GRAA, idx) < 0.75, 0.0, 180.0)
     d = dict(km=72, channels=np.asarray(ch, float), nmom=nmom, npol=npol, tau=tau, ssa=ssa, g=g, pmom=pm,
              rot=cols.rot, depol=cols.depol, pe=cols.pe, he=cols.he, te=cols.te, albedo=cols.albedo.copy(),
              sza=sza, vza=vza, raa=raa, flux_factor=cols.flux_factor, qm=cols.qm, rh=cols.rh)

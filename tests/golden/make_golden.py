@@ -38,6 +38,8 @@ CASES = {
     "scalar_n4_pp": (4, (550.0,), 4, 32, False, True, False),
     "vector_n4": (4, (550.0, 870.0), 4, 32, True, False, False),
     "vector_n2": (4, (550.0,), 2, 32, True, False, False),
+    # nadir view, missing geometry, overhead sun (latch), zero and missing albedo, RAA 0
+    "vector_n4_edge": (12, (550.0, 870.0), 4, 32, True, False, True),
     # GEOS optics-table moment counts (nmom up to 300, npol 6)
     "scalar_n6_nmom300": (3, (550.0, 870.0), 6, 300, False, False, False),
     "vector_n4_nmom300": (3, (550.0,), 4, 300, True, False, False),

@@ -75,7 +75,7 @@ def lambert(D, H, d, km, N, vector, do_2os=False, sl=slice(None)):
 
 def run_case(D, H, spec, d):
     kind = spec["kind"]
-    if kind in ("lambert", "high_sza"):
+    if kind in ("lambert", "high_sza", "grazing"):
         return lambert(D, H, d, 72, spec["nstreams"], spec["vector"], spec.get("do_2os", False))
     if kind == "uneven":
         nch, n = len(spec["channels"]), spec["ncol"]

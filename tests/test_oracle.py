@@ -29,8 +29,12 @@ def test_oracle_matches_reference_fixture(name):
                                               f(d["tau"]), f(d["ssa"]), f(d["pmom"]), d["pe"], d["he"], d["te"],
                                               d["albedo"], d["sza"], d["raa"], d["vza"], d["flux_factor"],
                                               plane_parallel=pp)
-        assert max_rel(Q, d["out_q"]) < RTOL_FP64 * 10
-        assert max_rel(U, d["out_u"]) < RTOL_FP64 * 10
+        # Q and U are small differences of large terms (and 0 up to rounding in
+        # the principal plane or under an overhead sun): bound them against |I|
+        ok = np.abs(d["out_radiance"] / 1e15 - 1) > 1e-3
+        for got, want in ((Q, d["out_q"]), (U, d["out_u"])):
+            assert ((np.abs(got - want) / np.abs(d["out_radiance"]))[ok]).max() < RTOL_FP64
+            assert (got[~ok] == want[~ok]).all()
     else:
         I, R, nf, rc = P.scalar_lambert(d["km"], d["channels"], N, d["nmom"], d["rot"], d["depol"], f(d["tau"]),
                                         f(d["ssa"]), f(d["g"]), f(d["pmom"]), d["pe"], d["he"], d["te"],

@@ -48,6 +48,9 @@ SIGNATURES = {
     "aa_column_radiance_dev": (_ci, [_vp, _ci, _ci, _ci, _ci, _vp, _vp, _vp, _vp, _ci, _ci, _ci, _ci, _ci, _vp, _vp,
                                      _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _cd, _vp, _vp, _vp, _vp, _vp,
                                      _vp]),
+    "aa_column_radiance_dev_noalloc": (_ci, [_vp, _ci, _ci, _ci, _ci, _vp, _vp, _vp, _vp, _ci, _ci, _ci, _ci, _ci,
+                                             _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _cd, _vp, _vp,
+                                             _vp, _vp, _vp, _vp]),
     "aa_vlidort_scalar_landmodis": (_ci, [_ci, _ci, _ci, _dp, _ci, _ci, _ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp,
                                           _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _cd, _ci, _ci, _dp, _dp, _dp, _ip,
                                           _ip]),
@@ -72,7 +75,8 @@ SIGNATURES = {
 }
 
 ERRORS = {-1: "AA_EARG (bad argument)", -2: "AA_EHIP (HIP runtime error)", -3: "AA_ENOTABLE (no table)",
-          -4: "AA_EDIMS (beyond compiled maxima)", -5: "AA_EUNSUP (option not implemented)"}
+          -4: "AA_EDIMS (beyond compiled maxima)", -5: "AA_EUNSUP (option not implemented)",
+          -6: "AA_ENOTRES (tables not resident)"}
 
 _LIB = None
 

@@ -353,6 +353,12 @@ int upload(aa_mie* t) {
 namespace aa {
 // Accessors used by the fused radiance path.
 int mie_upload(aa_mie* t) { return upload(t); }
+// True when the device copies are current (no upload, hence no allocation,
+// is needed before the tables are read).
+bool mie_resident(aa_mie* t) {
+  std::lock_guard<std::mutex> lk(t->mu);
+  return !t->dirty;
+}
 const MieDev& mie_dev(const aa_mie* t) { return t->dev; }
 
 int launch_get_aop(const aa_mie* t, int km, int nobs, int nch, int nq, const int* d_tracer_idx,

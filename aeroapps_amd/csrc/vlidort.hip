@@ -60,6 +60,7 @@ int rt_fourier(const RTWork& W, int nitems, int* const lists[2], int* counts, hi
   }
 }
 int mie_upload(aa_mie* t);
+bool mie_resident(aa_mie* t);
 int launch_get_aop(const aa_mie* t, int km, int nobs, int nch, int nq, const int* d_tracer_idx,
                    const int* d_chan_idx, const float* d_qm, const float* d_rh, int nmom, int npol, int ntr_eff,
                    int normalize, float* d_tau, float* d_ssa, float* d_g, float* d_pmom, hipStream_t st);
@@ -334,8 +335,11 @@ int compute_latch(RTWork& W, int* lp, hipStream_t st) {
 
 // Runs the radiance pipeline over items [g0, g1) (item g = obs*nch + ch), or
 // over d_items[g0..g1) when d_items (device) is given.  W carries the inputs
-// and outputs (device pointers) and the latch.
-int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nullptr) {
+// and outputs (device pointers) and the latch.  Work memory comes from the
+// stream's work arena, or, when fixed_base is given, from that caller-owned
+// device block of fixed_bytes (chunks are then sized to fit it).
+int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nullptr, char* fixed_base = nullptr,
+              size_t fixed_bytes = 0) {
   const int nitems = g1 - g0;
   if (nitems <= 0) return 0;
   const std::vector<double>& qtab = quad_table_cached(W.N, W.S, W.nmoments);
@@ -347,9 +351,21 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   const std::vector<double>& btab =
       W.kernel_wt ? brdf_table_cached(W.N, BR_NAZ) : (W.u10m ? brdf_table_cached(W.N, BR_NAZ_CM) : no_btab);
   double* dbt = nullptr;
-  // chunk capacity from the work budget
+  // chunk capacity from the work budget, or from the fixed block: its bytes
+  // less the tables, over the bytes of an item (work record + two list
+  // entries), checked by carving below
   size_t capi = 1;
-  if (work_cap_items(W, &capi)) return AA_EHIP;
+  const size_t tab_bytes = qtab.size() * sizeof(double) + std::max<size_t>(1, ytab.size()) * sizeof(float) +
+                           std::max<size_t>(1, ytab64.size()) * sizeof(double) +
+                           std::max<size_t>(1, btab.size()) * sizeof(double) + sizeof(int) * (W.nmoments + 2) +
+                           16 * 256;
+  if (fixed_base) {
+    const size_t per = bytes_per_item(W) + 2 * sizeof(int);
+    if (fixed_bytes <= tab_bytes + per) return AA_EDIMS;
+    capi = (fixed_bytes - tab_bytes) / per;
+  } else if (work_cap_items(W, &capi)) {
+    return AA_EHIP;
+  }
   // chunks of equal size: a small remainder chunk would run every Fourier
   // term on a nearly idle GPU
   const size_t nchunk = ((size_t)nitems + capi - 1) / capi;
@@ -372,8 +388,12 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   double* dq; int *l0, *l1, *cnt;
   RTWork tmp = W;
   size_t total = carve_all(nullptr, tmp, &dq, &l0, &l1, &cnt);
-  char* base = nullptr;
-  if (arena_reserve(st, AR_WORK, total, &base)) return AA_EHIP;
+  char* base = fixed_base;
+  if (fixed_base) {
+    if (total > fixed_bytes) return AA_EDIMS;
+  } else if (arena_reserve(st, AR_WORK, total, &base)) {
+    return AA_EHIP;
+  }
   carve_all(base, W, &dq, &l0, &l1, &cnt);
   AA_CHECK(hipMemcpyAsync(dq, qtab.data(), qtab.size() * sizeof(double), hipMemcpyHostToDevice, st));
   W.quad = dq;
@@ -841,6 +861,101 @@ int column_radiance(aa_mie* t, int km, int nobs, int nch, int nq, const int* tra
   return rcode;
 }
 
+// ------------------------------------------------------------ fixed scratch
+// Work memory of aa_column_radiance_dev_noalloc: a file-scope device array,
+// one per GPU (each GPU the library runs on holds it until the process ends).
+// Host code must not use the array's name as an address: its device address
+// is looked up once per device (hipGetSymbolAddress with that device
+// current) and only that pointer is passed to kernels and copies.
+constexpr size_t SCRATCH_BYTES = (size_t)512 << 20;
+constexpr int SCRATCH_MAX_DEVICES = 64;
+__device__ __align__(256) char scratch[SCRATCH_BYTES];
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+int scratch_ptr(char** out) {
+  int dev = 0;
+  AA_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return AA_EARG;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return AA_EHIP;
+    g_scratch_ptr[dev] = (char*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+// aa_column_radiance_dev with one stream count for all columns, its work
+// memory carved from the fixed scratch: [call block | optics block | work].
+// Columns go through in optics blocks of ob columns whose items fit the
+// work part in one chunk; blocks run one after another on the stream.
+int column_radiance_noalloc(aa_mie* t, int km, int nobs, int nch, int nq, const int* tracer_idx,
+                            const int* chan_idx, const float* d_qm, const float* d_rh, int nmom, int npol,
+                            int nstreams, int vector, int precision, const int* d_nlayers, const double* d_rot,
+                            const double* d_depol, const double* d_he, const double* d_albedo, const double* d_sza,
+                            const double* d_raa, const double* d_vza, const double* d_flux_factor, double missing,
+                            double* d_radiance, double* d_reflectance, double* d_q, double* d_u, int* d_nfourier,
+                            void* stream) {
+  if (!t || km <= 0 || km > MAX_LAYERS || nobs < 0 || nch <= 0 || nq <= 0 || nmom <= 0 || npol <= 0) return AA_EARG;
+  if (!tracer_idx || !chan_idx || !d_qm || !d_rh || !d_rot || !d_depol || !d_he || !d_albedo || !d_sza || !d_raa ||
+      !d_vza || !d_radiance || !d_reflectance || !d_nfourier)
+    return AA_EARG;
+  if (precision != 0 && precision != 1) return AA_EARG;
+  const int S = vector ? 3 : 1;
+  if (vector && (!d_q || !d_u || npol < 6)) return AA_EARG;
+  if (nstreams < 1 || nstreams > max_streams(S)) return AA_EDIMS;
+  // uploading the tables would allocate: they must already be resident
+  if (!mie_resident(t)) return AA_ENOTRES;
+  if (nobs == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  char* base = nullptr;
+  if (int e = scratch_ptr(&base)) return e;
+  RTWork W;
+  W.km = km; W.nch = nch; W.nobs = nobs; W.nmom = nmom; W.npol = npol;
+  W.N = nstreams; W.S = S; W.nmoments = std::min(2 * nstreams - 1, nmom);
+  W.missing = missing;
+  W.rot = d_rot; W.depol = d_depol; W.alpha = nullptr;
+  W.he = d_he; W.albedo = d_albedo; W.sza = d_sza; W.raa = d_raa; W.vza = d_vza;
+  W.flux = vector ? d_flux_factor : nullptr;
+  W.nlayers = d_nlayers;
+  W.radiance = d_radiance; W.reflectance = d_reflectance; W.qout = d_q; W.uout = d_u; W.nfourier = d_nfourier;
+  W.optics_f32 = 1;
+  W.precision = precision;
+  // block size: ob columns' float optics + their items' work in one chunk
+  const size_t call_bytes = 256;
+  const size_t per_obs = (size_t)km * nch * (3 + (size_t)nmom * npol) * sizeof(float);
+  const size_t per_item = bytes_per_item(W) + 2 * sizeof(int);
+  const size_t tables = (size_t)8 << 20;   // quadrature / contraction tables and counters (<= 4.2 MB)
+  if (SCRATCH_BYTES <= call_bytes + tables + per_obs + (size_t)nch * per_item + 1024) return AA_EDIMS;
+  size_t ob = (SCRATCH_BYTES - call_bytes - tables - 1024) / (per_obs + (size_t)nch * per_item);
+  ob = std::min<size_t>(ob, (size_t)nobs);
+  const int nblk = (int)((nobs + ob - 1) / ob);
+  ob = ((size_t)nobs + nblk - 1) / nblk;
+  int* lp = (int*)base;
+  const size_t n3 = ob * km * nch;
+  float* dt = (float*)(base + call_bytes);
+  float* ds = dt + n3;
+  float* dg = ds + n3;
+  float* dp = dg + n3;
+  const size_t obytes = ((3 * n3 + n3 * (size_t)nmom * npol) * sizeof(float) + 255) & ~(size_t)255;
+  char* wbase = base + call_bytes + obytes;
+  const size_t wbytes = SCRATCH_BYTES - call_bytes - obytes;
+  W.tau = dt; W.ssa = ds; W.pmom = dp;
+  int rcode = compute_latch(W, lp, st);
+  for (int b = 0; b < nblk && !rcode; ++b) {
+    const int o0 = (int)(b * ob), no = std::min((int)ob, nobs - o0);
+    rcode = launch_get_aop(t, km, no, nch, nq, tracer_idx, chan_idx, d_qm + (size_t)o0 * km * nq,
+                           d_rh + (size_t)o0 * km, nmom, npol, nq, 1, dt, ds, dg, dp, st);
+    if (rcode) break;
+    RTWork Wc = W;
+    Wc.opt_obs0 = o0; Wc.opt_nobs = no;
+    rcode = run_items(Wc, o0 * nch, (o0 + no) * nch, st, nullptr, wbase, wbytes);
+  }
+  return rcode;
+}
+
 }  // namespace
 }  // namespace aa
 
@@ -896,6 +1011,21 @@ int aa_column_radiance_modis_dev(aa_mie* t, int km, int nobs, int nch, int nq, c
                              d_nlayers, d_nstreams_col, d_rot, d_depol, d_he, nullptr, d_kernel_wt, d_param, d_br,
                              d_sza, d_raa, d_vza, nullptr, missing, d_radiance, d_reflectance, nullptr, nullptr,
                              d_nfourier, stream);
+}
+
+int aa_column_radiance_dev_noalloc(aa_mie* t, int km, int nobs, int nch, int nq, const int* tracer_idx,
+                                   const int* chan_idx, const float* d_qm, const float* d_rh, int nmom, int npol,
+                                   int nstreams, int vector, int precision, const int* d_nlayers,
+                                   const double* d_rot, const double* d_depol, const double* d_pe,
+                                   const double* d_he, const double* d_te, const double* d_albedo,
+                                   const double* d_sza, const double* d_raa, const double* d_vza,
+                                   const double* d_flux_factor, double missing, double* d_radiance,
+                                   double* d_reflectance, double* d_q, double* d_u, int* d_nfourier, void* stream) {
+  (void)d_pe; (void)d_te;
+  return aa::column_radiance_noalloc(t, km, nobs, nch, nq, tracer_idx, chan_idx, d_qm, d_rh, nmom, npol, nstreams,
+                                     vector, precision, d_nlayers, d_rot, d_depol, d_he, d_albedo, d_sza, d_raa,
+                                     d_vza, d_flux_factor, missing, d_radiance, d_reflectance, d_q, d_u, d_nfourier,
+                                     stream);
 }
 
 const char* aa_version(void) { return "aeroapps_amd 0.1 (gfx950)"; }

@@ -32,7 +32,8 @@ enum {
   AA_EHIP = -2,        /* HIP runtime error (allocation, launch) */
   AA_ENOTABLE = -3,    /* table handle has no species table */
   AA_EDIMS = -4,       /* dimension beyond the compiled maxima (streams, layers, moments) */
-  AA_EUNSUP = -5       /* a reference option this library does not implement (DO_BOA) */
+  AA_EUNSUP = -5,      /* a reference option this library does not implement (DO_BOA) */
+  AA_ENOTRES = -6      /* optics tables not yet resident on the device (see the _noalloc entry) */
 };
 
 /* Reference return codes reproduced in the rc out-arguments. */
@@ -325,6 +326,29 @@ int aa_column_radiance_modis_dev(aa_mie* t, int km, int nobs, int nch, int nq, c
                                  const double* d_param, const double* d_sza, const double* d_raa,
                                  const double* d_vza, double missing, double* d_radiance, double* d_reflectance,
                                  double* d_br, int* d_nfourier, void* stream);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): aa_column_radiance_dev with one stream count for every
+ * column (no d_nstreams_col), whose work memory is a fixed 512 MiB device
+ * array inside the library (one per GPU, held until the process ends) instead
+ * of the allocated arenas.  It calls no hipMalloc/hipFree: the optics tables
+ * must already be resident on the device (used by an earlier call after their
+ * last change), else AA_ENOTRES; a shape whose single column does not fit the
+ * array returns AA_EDIMS.  Any number of columns is accepted: they go through
+ * the array in blocks, one after another on the stream.  Results are
+ * bit-identical to aa_column_radiance_dev's.  All calls on one GPU share its
+ * array, so calls that overlap on one GPU (other streams or host threads)
+ * race: serialise them.  Profiling (aa_profile_start) creates events and is
+ * not for these callers. */
+int aa_column_radiance_dev_noalloc(aa_mie* t, int km, int nobs, int nch, int nq, const int* tracer_idx,
+                                   const int* chan_idx, const float* d_qm, const float* d_rh, int nmom, int npol,
+                                   int nstreams, int vector, int precision, const int* d_nlayers,
+                                   const double* d_rot, const double* d_depol, const double* d_pe,
+                                   const double* d_he, const double* d_te, const double* d_albedo,
+                                   const double* d_sza, const double* d_raa, const double* d_vza,
+                                   const double* d_flux_factor, double missing, double* d_radiance,
+                                   double* d_reflectance, double* d_q, double* d_u, int* d_nfourier, void* stream);
 
 /* ------------------------------------------------------------------ work memory
  * No counterpart in the reference (its work arrays are Fortran locals).

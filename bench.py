@@ -43,14 +43,18 @@ TRAFFIC_PER_ITEM_TERM = {("bvp", 6, 1, False): (287.6e3, 138.9e3), ("bvp", 6, 1,
 # ----------------------------------------------------------------- algorithmic work
 def flops_per_term(N, S_, L, nmoments, M):
     """Algorithmic FP64 flops of one (column, channel) at Fourier term M:
-    eigensystem (F_b), boundary-value solve (F_c) and source (F_d).  F_c counts
-    the structurally non-zero work of the pivoted factorisation (one panel of
-    3n rows x (4n columns + RHS) per layer, 2n x (2n + RHS) for the bottom
-    layer) and of the back-substitution; an FMA counts as 2 flops."""
+    eigensystem (F_b), boundary-value solve (F_c) and source (F_d).  F_b is
+    returned without its SAB/DAB contraction term 4 s n^2 Lm per layer (that
+    contraction runs in k_sabdab, counted by sabdab_per_term), i.e. the work
+    k_layer itself does: omega-Greek x PI products, EIGENMAT = DAB.SAB and the
+    nonsymmetric eigensolve with vectors.  F_c counts the structurally non-zero
+    work of the pivoted factorisation (one panel of 3n rows x (4n columns +
+    RHS) per layer, 2n x (2n + RHS) for the bottom layer) and of the
+    back-substitution; an FMA counts as 2 flops."""
     n = S_ * N
     n2 = 2 * n
     Lm = nmoments - M + 1
-    Fb = L * (4 * S_ * n * n * Lm + 4 * N * S_ ** 3 * Lm + 2 * n ** 3 + 25 * n ** 3)
+    Fb = L * (4 * N * S_ ** 3 * Lm + 2 * n ** 3 + 25 * n ** 3)
     mid = sum((3 * n - 1 - i) * (1 + 2 * (4 * n - i)) for i in range(n2))
     last = sum((2 * n - 1 - i) * (1 + 2 * (2 * n - i)) for i in range(n2))
     bs_mid = 2 * n2 * n2 + n2 * (n2 - 1) + n2
@@ -105,6 +109,24 @@ def _cpu_worker(args):
     return time.perf_counter() - t0, gen
 
 
+def host_cpu():
+    """(model name, logical CPUs of the host, CPUs this process may run on)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, avail
+
+
 def cpu_baseline(wl, ncols, procs, modis=False):
     import oracle
     kind = "reference" if os.path.exists(oracle.REF_SO) else "port"
@@ -119,16 +141,25 @@ def cpu_baseline(wl, ncols, procs, modis=False):
     wall = time.perf_counter() - t0
     busy = max(r[0] for r in res)
     ncol = per * procs
+    model, total, avail = host_cpu()
     return {
         "value": ncol / busy,
         "unit": "columns/s",
         "cores": procs,
         "kind": kind,
+        "cpu_model": model,
+        "host_logical_cpus": total,
+        "cpus_available_to_process": avail,
         "sample": (f"{ncol} columns ({per} per process, {procs} processes, one core each) of the same workload: "
-                   f"float32 Chem_Mie restatement + {'reference VLIDORT 2.8.3 (flang -O3)' if kind == 'reference' else 'C++ restatement'} "
-                   f"scalar {'MODIS land-surface' if modis else 'Lambertian'} driver, {len(wl['channels'])} channels, "
-                   f"N={wl['nstreams']}, nmom={wl['nmom']}; "
-                   f"slowest process {busy:.1f} s, pool wall {wall:.1f} s"),
+                   f"float32 Chem_Mie restatement + "
+                   + ("reference VLIDORT 2.8.3 core (flang -O3) under the oracle's statement-by-statement "
+                      "restatement of the AeroApps driver set-up, bit-identical to the reference's compiled "
+                      "drivers on the test fixtures"
+                      if kind == "reference" else "C++ restatement")
+                   + f", scalar {'MODIS land-surface' if modis else 'Lambertian'}, {len(wl['channels'])} channels, "
+                   f"N={wl['nstreams']}, nmom={wl['nmom']}; slowest process {busy:.1f} s, pool wall {wall:.1f} s; "
+                   f"processes = the CPUs this process may run on, at most 16 (the host CPU share of one GPU "
+                   f"on the measuring box)"),
     }
 
 
@@ -141,7 +172,7 @@ def main():
     ap.add_argument("--workload", default="scalar_500k_4ch")
     ap.add_argument("--ncol", type=int, default=0, help="override the column count")
     ap.add_argument("--fp64-only", action="store_true", help="skip the mixed-precision run")
-    ap.add_argument("--cpu-cols", type=int, default=512)
+    ap.add_argument("--cpu-cols", type=int, default=1024)
     ap.add_argument("--cpu-procs", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving-granules", type=int, default=40,
@@ -170,7 +201,7 @@ def main():
     # host-CPU baseline first, before this process touches the GPU (fork-safe)
     cpu = None
     if not args.no_cpu:
-        procs = args.cpu_procs or min(16, os.cpu_count() or 1)
+        procs = args.cpu_procs or min(16, host_cpu()[2])
         cpu = cpu_baseline(wl, args.cpu_cols, procs, modis=args.surface == "modis")
 
     import torch
@@ -319,17 +350,37 @@ def main():
                 "achieved": round(tf, 3), "peak": peak, "unit": "TFLOP/s", "frac": round(tf / peak, 4),
                 "algorithmic_GBs": round(gbs, 1), "hbm_frac": round(gbs / PEAK_HBM_GBS, 4)}
     kms, kln = main_run["kms"], main_run["kln"]
-    dom = max(("layer", "bvp"), key=lambda k: kms[k])
-    dom_flops = fc + fd if dom == "bvp" else fb
-    dom_ms = kms[dom]
-    dom_launches = max(1, kln[dom])
-    achieved = dom_flops / (dom_ms / 1000.0) / 1e12
     item_terms = float(nfh[ok].sum()) * args.steps
-    tr_key = (dom, N, Sx, rmx is not None)
-    traffic = None
-    if tr_key in TRAFFIC_PER_ITEM_TERM and wl["nstreams"] is not None and not uneven and not modis:
-        rd, wr = TRAFFIC_PER_ITEM_TERM[tr_key]
-        traffic = round((rd + wr) * item_terms / dom_launches, 0)
+    # compulsory HBM bytes of the path per (item, Fourier term): a column's
+    # inputs and outputs spread over its channels' executed terms
+    compulsory_item_term = bytes_per_column(km, nq, nch, vector) * ncol * args.steps / max(item_terms, 1.0)
+
+    def roofline(kname):
+        """FP64-VALU roofline of one radiance kernel over the timed region:
+        algorithmic flops (k_layer: F_b without the SAB/DAB contraction, which
+        k_sabdab performs; k_bvp: structural F_c + F_d) over the kernel's total
+        HIP-event time; HBM traffic per launch from the counter constants."""
+        ms = kms[kname]
+        launches = max(1, kln[kname])
+        flops = fb if kname == "layer" else fc + fd
+        ach = flops / (ms / 1000.0) / 1e12
+        r = {"bound": "fp64_valu", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+             "frac": round(ach / PEAK_FP64_TFLOPS, 4), "traffic": None,
+             "kernel": (f"k_{kname}<{N},{Sx}>" if wl["nstreams"] is not None
+                        else f"k_{kname}<{min(wl['stream_choices'])}..{N},{Sx}>"),
+             "kernel_avg_ms": round(ms / launches, 4), "kernel_launches": int(launches),
+             "flops": ("F_b without the SAB/DAB term" if kname == "layer" else "F_c + F_d (structural)"),
+             "note": "FP64 vector ALU work (no MFMA); peak is the MI355X FP64 vector rate"}
+        key = (kname, N, Sx, rmx is not None)
+        if key in TRAFFIC_PER_ITEM_TERM and wl["nstreams"] is not None and not uneven and not modis:
+            rd, wr = TRAFFIC_PER_ITEM_TERM[key]
+            r["traffic"] = round((rd + wr) * item_terms / launches, 0)
+            r["traffic_GBs"] = round((rd + wr) * item_terms / (ms / 1000.0) / 1e9, 1)
+            r["hbm_frac"] = round(r["traffic_GBs"] / PEAK_HBM_GBS, 4)
+            r["traffic_vs_compulsory"] = round((rd + wr) / compulsory_item_term, 1)
+        return r
+
+    dom = max(("layer", "bvp"), key=lambda k: kms[k])
     el = main_run["el"]
     value = main_run["value"]
     path_bytes = bytes_per_column(km, nq, nch, vector) * ncol * args.steps
@@ -359,14 +410,8 @@ def main():
                    "path": ("aa_column_radiance_modis_dev" if modis else "aa_column_radiance_dev") +
                            " (Chem_Mie optics + VLIDORT radiance, device-resident)",
                    "work_arena_gib": args.work_gib if args.work_gib > 0 else 24.0},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
-                     "traffic_GBs": (round(traffic * dom_launches / (dom_ms / 1000.0) / 1e9, 1) if traffic else None),
-                     "kernel": (f"k_{dom}<{N},{Sx}>" if wl["nstreams"] is not None
-                                else f"k_{dom}<{min(wl['stream_choices'])}..{N},{Sx}>"),
-                     "kernel_avg_ms": round(dom_ms / dom_launches, 4),
-                     "kernel_launches": int(dom_launches),
-                     "note": "FP64 VALU work (no MFMA); peak is MI355X FP64 (vector = matrix rate)"},
+        "roofline": roofline(dom),
+        "roofline_other": roofline("bvp" if dom == "layer" else "layer"),
         "hbm_path": {"bytes_per_column": bytes_per_column(km, nq, nch, vector),
                      "achieved_GBs": round(path_bytes / el / 1e9, 3),
                      "frac": round(path_bytes / el / 1e9 / PEAK_HBM_GBS, 6)},

@@ -30,6 +30,10 @@ SIGNATURES = {
     "aa_mie_query_idx": (_ci, [_vp, _cp]),
     "aa_mie_query": (_ci, [_vp, _ci, _cf, _ci, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _ci, _ci, _fp]),
     "aa_mie_dims": (_ci, [_vp, _ip, _ip, _ip, _ip]),
+    "aa_mie_channels": (_ci, [_vp, _fp]),
+    "aa_mie_tracer_table": (_ci, [_vp, _ci, _ip, _ip, _ip, _ip, _ip, _ip]),
+    "aa_mie_registry": (_ci, [_cp, _ci, _ip, _cp, _ip, _ip]),
+    "aa_mie_create_rc": (_ci, [ctypes.POINTER(_vp), _cp, _ci, _cp, _ip, _vp, _vp, _ip]),
     "aa_mie_destroy": (_ci, [_vp]),
     "aa_get_aop_vector": (_ci, [_vp, _ci, _ci, _ci, _ci, _fp, _cp, _ci, _fp, _fp, _ci, _ci, _fp, _fp, _fp, _fp, _ip]),
     "aa_get_aop_scalar": (_ci, [_vp, _ci, _ci, _ci, _ci, _fp, _cp, _ci, _fp, _fp, _fp, _fp, _fp, _ip]),

@@ -82,6 +82,79 @@ class MieTables:
         return outs
 
     @classmethod
+    def from_rc(cls, rcfile: str, reader, chem_reg=None):
+        """Chem_MieCreate(rcfile, rc[, chemReg]) (src/Shared/GMAO_Shared/Chem_Base/
+        Chem_MieMod.F90:124-411) in the library: the resource file and the
+        registry are parsed there, the seven optics-table files are read by
+        ``reader(path)``, a callable returning the file's variables as a dict
+        of float64 arrays in Fortran shapes (lambda_table, rh, reff, bext, bsca,
+        bbck, g and the optional pback, pmom, gf, rhop, refr, refi, as for
+        ``add_table_native``).  ``chem_reg`` is (names, species) as from
+        ``registry()``; by default Chem_MieRegistry.rc, else Chem_Registry.rc,
+        in the working directory.  Returns (MieTables or None, rc)."""
+        import os
+        from ._lib import _vp
+        if chem_reg is None:
+            path = "Chem_MieRegistry.rc" if os.path.exists("Chem_MieRegistry.rc") else "Chem_Registry.rc"
+            names, species, rc = registry(path)
+            if rc:
+                return None, 3
+        else:
+            names, species = chem_reg
+        nq = len(names)
+        cnames = b"".join(n.encode().ljust(16)[:16] for n in names) or b" " * 16
+        csp = np.asarray(list(species) or [-1], dtype=np.int32)
+        keep = []
+
+        class TableFile(ctypes.Structure):
+            _fields_ = [(k, ctypes.c_int) for k in ("nch", "nrh", "nbin", "nmom", "npol")] + \
+                       [(k, ctypes.c_void_p) for k in ("lambda_", "rh", "reff", "bext", "bsca", "bbck", "g", "pback",
+                                                       "pmom", "gf", "rhop", "refr", "refi")]
+
+        proto = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(TableFile))
+
+        def cb(user, path, tf):
+            try:
+                d = reader(path.decode())
+            except Exception:
+                return 1
+            f64 = lambda a: None if a is None else np.asfortranarray(np.asarray(a, dtype=np.float64))
+            arrs = {k: f64(d.get(k)) for k in ("lambda_table", "rh", "reff", "bext", "bsca", "bbck", "g", "pback",
+                                               "pmom", "gf", "rhop", "refr", "refi")}
+            keep.append(arrs)
+            t = tf.contents
+            t.nch, t.nrh, t.nbin = arrs["bext"].shape
+            t.nmom = arrs["pmom"].shape[3] if arrs["pmom"] is not None else 0
+            t.npol = arrs["pmom"].shape[4] if arrs["pmom"] is not None else (
+                arrs["pback"].shape[3] if arrs["pback"] is not None else 0)
+            ptr = lambda a: None if a is None else a.ctypes.data
+            t.lambda_ = ptr(arrs["lambda_table"])
+            for k in ("rh", "reff", "bext", "bsca", "bbck", "g", "pback", "pmom", "gf", "rhop", "refr", "refi"):
+                setattr(t, k, ptr(arrs[k]))
+            return 0
+
+        cfun = proto(cb)
+        h = ctypes.c_void_p()
+        rc = np.zeros(1, dtype=np.int32)
+        check(lib().aa_mie_create_rc(ctypes.byref(h), rcfile.encode(), nq, cnames, _p(csp, _ip),
+                                     ctypes.cast(cfun, _vp), None, _p(rc, _ip)), "aa_mie_create_rc")
+        if rc[0] != 0:
+            return None, int(rc[0])
+        t = cls.__new__(cls)
+        t.handle = h
+        nch = t.dims()[0]
+        t.channels_m = np.zeros(nch, dtype=np.float32)
+        check(lib().aa_mie_channels(h, _p(t.channels_m)), "aa_mie_channels")
+        return t, 0
+
+    def tracer_table(self, idx: int):
+        """(table, bin, nbin, nrh, nmom, npol) of tracer idx (Chem_MieQueryIdx's
+        vindex / vtableUse)."""
+        v = [ctypes.c_int() for _ in range(6)]
+        check(lib().aa_mie_tracer_table(self.handle, idx, *[ctypes.byref(x) for x in v]), "aa_mie_tracer_table")
+        return tuple(x.value for x in v)
+
+    @classmethod
     def from_synthetic(cls, tables) -> "MieTables":
         """Tables built by aeroapps_amd.synthetic.make_mie_tables."""
         t = cls(tables[0].lambda_m)
@@ -233,6 +306,27 @@ def parse_mie_rc(text: str) -> dict:
             k, v = line.split(":", 1)
             out[k.strip()] = v.strip()
     return out
+
+
+def registry(path: str):
+    """Chem_RegistryCreate (src/Shared/GMAO_Shared/Chem_Base/Chem_RegistryMod.F90:
+    200-545) in the library: (tracer names, species codes AA_SP_* or -1, rc) of
+    the registry file ``path``."""
+    nq = ctypes.c_int()
+    rc = ctypes.c_int()
+    check(lib().aa_mie_registry(path.encode(), 0, ctypes.byref(nq), None, None, ctypes.byref(rc)), "aa_mie_registry")
+    if rc.value:
+        return [], [], rc.value
+    n = nq.value
+    buf = ctypes.create_string_buffer(16 * max(n, 1))
+    sp = np.zeros(max(n, 1), dtype=np.int32)
+    check(lib().aa_mie_registry(path.encode(), n, ctypes.byref(nq), buf, _p(sp, _ip), ctypes.byref(rc)),
+          "aa_mie_registry")
+    names = [buf.raw[16 * i:16 * i + 16].decode().rstrip() for i in range(n)]
+    return names, [int(x) for x in sp[:n]], rc.value
+
+
+SPECIES_CODES = {"DU": 0, "SS": 1, "SU": 2, "OC": 3, "BC": 4, "NI": 5, "BRC": 6}
 
 
 def mie_rc_settings(text: str) -> dict:

@@ -260,6 +260,9 @@ struct aa_mie {
   // reff/gf/rhop/rhod/vol/area [rh], pback [ch][rh][pol]; empty for pre-interpolated tables
   std::vector<std::vector<float>> reff_t, gf_t, rhop_t, rhod_t, vol_t, area_t, pback_t;
   std::vector<int> npback_t;
+  // table number and bin of every tracer (tables in the order they were added)
+  std::vector<int> table_t, bin_t;
+  int ntables = 0;
   // device copies, uploaded on first use after a change; `mu` serialises the
   // upload (and the table edits that mark it dirty) between host threads
   // that share one table set, e.g. the slots of a serving loop
@@ -434,6 +437,7 @@ static int add_table_locked(aa_mie* t, int nrh, int nbin, int nmom, int npol, co
   for (int b = 0; b < nbin; ++b) {
     t->names.push_back(norm_name(bin_names + 16 * b, 16));
     t->nrh_t.push_back(nrh); t->nmom_t.push_back(nmom); t->npol_t.push_back(npol);
+    t->table_t.push_back(t->ntables); t->bin_t.push_back(b);
     for (int c = 0; c < nch; ++c) for (int r = 0; r < new_nrh; ++r) {
       size_t src = ((size_t)b * nch + c) * nrh + r;   // Fortran (nrh,nch,nbin)
       bool in = r < nrh;
@@ -460,6 +464,7 @@ static int add_table_locked(aa_mie* t, int nrh, int nbin, int nmom, int npol, co
       v->emplace_back();
     t->npback_t.push_back(0);
   }
+  ++t->ntables;
   t->dirty = true;
   return 0;
 }
@@ -621,8 +626,30 @@ int aa_mie_query_extra(const aa_mie* t, int idx, float channel, int n, const flo
 int aa_mie_query_idx(const aa_mie* t, const char* qname) {
   if (!t || !qname) return -1;
   std::string n = norm_name(qname, std::strlen(qname));
+  if (n.empty()) return -1;   // bins no tracer maps to have empty names
   for (size_t i = 0; i < t->names.size(); ++i) if (t->names[i] == n) return (int)i;
   return -1;
+}
+
+int aa_mie_channels(const aa_mie* t, float* channels_m) {
+  if (!t || !channels_m) return -1;
+  for (int c = 0; c < t->nch; ++c) channels_m[c] = t->channels_m[c];
+  return 0;
+}
+
+int aa_mie_tracer_table(const aa_mie* t, int idx, int* table, int* bin, int* nbin, int* nrh, int* nmom,
+                        int* npol) {
+  if (!t || idx < 0 || idx >= (int)t->names.size()) return -1;
+  const int tb = t->table_t[idx];
+  int nb = 0;
+  for (int v : t->table_t) nb += (v == tb);
+  if (table) *table = tb;
+  if (bin) *bin = t->bin_t[idx];
+  if (nbin) *nbin = nb;
+  if (nrh) *nrh = t->nrh_t[idx];
+  if (nmom) *nmom = t->nmom_t[idx];
+  if (npol) *npol = t->npol_t[idx];
+  return 0;
 }
 
 int aa_mie_dims(const aa_mie* t, int* nch, int* nmom_min, int* npol_max, int* ntracer) {

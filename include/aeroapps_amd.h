@@ -105,6 +105,56 @@ int aa_mie_query(const aa_mie* t, int idx, float channel, int n, const float* q_
 /* Number of channels / moments / polarizations of the table set. */
 int aa_mie_dims(const aa_mie* t, int* nch, int* nmom_min, int* npol_max, int* ntracer);
 
+/* The nch channels [m] of the table set (Chem_Mie%channels). */
+int aa_mie_channels(const aa_mie* t, float* channels_m);
+
+/* Table of tracer idx (0-based, tables numbered in the order they were
+ * added), its 0-based bin in that table and the table's bins, RH nodes,
+ * moments and polarizations: what Chem_MieQueryIdx's vindex and vtableUse
+ * (CB/Chem_MieMod.F90:720-777) hold for a tracer.  Any output may be NULL. */
+int aa_mie_tracer_table(const aa_mie* t, int idx, int* table, int* bin, int* nbin, int* nrh, int* nmom,
+                        int* npol);
+
+/* ---- Construction from resource files (Chem_MieCreateFromRC,
+ * CB/Chem_MieMod.F90:124-411).  Species codes of the seven optics tables. */
+enum { AA_SP_DU = 0, AA_SP_SS = 1, AA_SP_SU = 2, AA_SP_OC = 3, AA_SP_BC = 4, AA_SP_NI = 5, AA_SP_BRC = 6 };
+
+/* The arrays of one optics-table file, in the layouts of
+ * aa_mie_add_table_native (float64, Fortran order); a reader fills them and
+ * keeps them valid until it is called again or aa_mie_create_rc returns. */
+typedef struct aa_mie_table_file {
+  int nch, nrh, nbin, nmom, npol;     /* file dimensions (nmom = npol = 0 without moments) */
+  const double *lambda, *rh, *reff, *bext, *bsca, *bbck, *g, *pback, *pmom, *gf, *rhop, *refr, *refi;
+} aa_mie_table_file;
+/* Reads the optics table `path` (the file named in the resource file); 0 on
+ * success.  The reference reads these netCDF files with Chem_MieTableRead
+ * (CB/Chem_MieTableMod.F90:238-719); here the host application supplies the
+ * reader. */
+typedef int (*aa_mie_table_reader)(void* user, const char* path, aa_mie_table_file* table);
+
+/* The chemistry registry file (Chem_RegistryCreate,
+ * CB/Chem_RegistryMod.F90:200-545, i90 format): the tracers of every species
+ * with doing_XX: yes, in the registry's order, named from variable_table_XX::
+ * or by prefix and bin number (du001 ...).  nq receives their number; names
+ * (16 characters each, blank padded) and species (AA_SP_* code, -1 for a
+ * species without an optics table) are filled when nq <= nq_max.  rc: 0; 1
+ * the file cannot be read; 2 a species is on without nbins or units; 3 its name table
+ * has too few rows (the reference stops in each case). */
+int aa_mie_registry(const char* path, int nq_max, int* nq, char* names, int* species, int* rc);
+
+/* Chem_MieCreate(rcfile, rc, chemReg): reads n_channels, n_moments,
+ * r_channels [m] and filename_optical_properties_{DU,SS,BC,OC,SU,NI,BRC} from
+ * rcfile, reads the seven tables with `reader` in the reference's order (DU,
+ * SS, SU, OC, BC, NI, BRC) and adds each with aa_mie_add_table_native (nmom =
+ * n_moments), its bins served to the registry tracers (names/species as from
+ * aa_mie_registry) of that species in order (the sulfate table to
+ * so4/SO4/sul/SUL names only, the nitrate table to no3/NO3 only,
+ * CB/Chem_MieMod.F90:338-400).  rc: 0; 1 unreadable rc file or a missing
+ * label; 2 the reader failed; otherwise aa_mie_add_table_native's rc.  *out
+ * is NULL unless rc = 0 (the reference stops with die() instead). */
+int aa_mie_create_rc(aa_mie** out, const char* rcfile, int nq, const char* names, const int* species,
+                     aa_mie_table_reader reader, void* user, int* rc);
+
 int aa_mie_destroy(aa_mie* t);
 
 /* Replaces VLIDORT_getAOPvector (GEO/VLIDORT_MieMod.F90:124-336).  Fortran

@@ -102,8 +102,13 @@ template <int N, int S, bool MIX>
 __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __restrict__ active,
                                               const int* __restrict__ nactive) {
   constexpr int n = N * S, LM = 2 * N;
-  // SAB/DAB come from the MFMA contraction (FP32 when MIX, else FP64); all
-  // other arithmetic of the layer is FP64.
+  // MIX (the mixed path): SAB/DAB come from the FP32 MFMA contraction and the
+  // source assembly -- the user-angle homogeneous solutions (Legendre sums of
+  // X+- with the quadrature and user PI matrices) and the phase-matrix sums of
+  // the beam source -- runs in FP32 (type TR); the eigensystem, the
+  // normalisation and the particular solution's coefficients stay FP64.
+  // Without MIX everything is FP64.
+  using TR = typename std::conditional<MIX, float, double>::type;
   constexpr LOff O(N, S);
   constexpr QOff QO(N, S);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -258,23 +263,23 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
 #pragma unroll
       for (int o = 0; o < S; ++o) { updn[k][o] = 0.0; upup[k][o] = 0.0; }
     } else {
-      double gaux[LM][S];
+      TR gaux[LM][S];
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
-        double hs[S];
+        TR hs[S];
 #pragma unroll
         for (int o1 = 0; o1 < S; ++o1) {
-          double sps = 0.0;
+          TR sps = 0.0;
           if (inl(l)) {
 #pragma unroll
             for (int j = 0; j < N; ++j) {
-              const double a5 = qhw[j];
-              double sp = 0.0, sm = 0.0;
+              const TR a5 = (TR)qhw[j];
+              TR sp = 0.0, sm = 0.0;
 #pragma unroll
               for (int o = 0; o < S; ++o) {
-                const double xqv = XQ(l, j, o1, o);
-                sp += xqv * a5 * xp[j * S + o][k];
-                sm += (DMS[o1] * xqv * pisign(l)) * a5 * xn[j * S + o][k];
+                const TR xqv = (TR)XQ(l, j, o1, o);
+                sp += xqv * a5 * (TR)xp[j * S + o][k];
+                sm += ((TR)DMS[o1] * xqv * (TR)pisign(l)) * a5 * (TR)xn[j * S + o][k];
               }
               sps = sps + sp + sm;
             }
@@ -283,26 +288,26 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
         }
 #pragma unroll
         for (int o1 = 0; o1 < S; ++o1) {
-          double s = 0.0;
+          TR s = 0.0;
 #pragma unroll
-          for (int o = 0; o < S; ++o) s += og[l][o1][o] * hs[o];
+          for (int o = 0; o < S; ++o) s += (TR)og[l][o1][o] * hs[o];
           gaux[l][o1] = s;
         }
       }
 #pragma unroll
       for (int o1 = 0; o1 < S; ++o1) {
-        double sdn = 0.0, sup = 0.0;
+        TR sdn = 0.0, sup = 0.0;
 #pragma unroll
         for (int o = 0; o < S; ++o)
 #pragma unroll
           for (int l = 0; l < LM; ++l) {
             if (!inl(l)) continue;
-            const double xum = (DMS[o1] * xup[l][o1][o] * DMS[o] * pisign(l));
-            const double xupre = (DMS[o1] * xup[l][o1][o]);
+            const TR xum = (TR)(DMS[o1] * xup[l][o1][o] * DMS[o] * pisign(l));
+            const TR xupre = (TR)(DMS[o1] * xup[l][o1][o]);
             sdn += xum * gaux[l][o];
             sup += xupre * gaux[l][o];
           }
-        updn[k][o1] = sdn; upup[k][o1] = sup;
+        updn[k][o1] = (double)sdn; upup[k][o1] = (double)sup;
       }
     }
     const double udel = tduser;
@@ -352,14 +357,14 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
       else cfunc[k] = cst * (zdel - wdel) / gamm[k];
       dfunc[k] = cst * (1.0 - zwdel) / gamp[k];
     }
-    double hq[LM][S];
+    TR hq[LM][S];
 #pragma unroll
     for (int o1 = 0; o1 < S; ++o1)
 #pragma unroll
       for (int l = 0; l < LM; ++l) {
-        double gsum = 0.0;
+        TR gsum = 0.0;
 #pragma unroll
-        for (int o2 = 0; o2 < S; ++o2) gsum = gsum + og[l][o1][o2] * x0p[l][o2][0];
+        for (int o2 = 0; o2 < S; ++o2) gsum = gsum + (TR)og[l][o1][o2] * (TR)x0p[l][o2][0];
         hq[l][o1] = gsum;
       }
     double dpi[N][S], dmi[N][S];
@@ -367,17 +372,17 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
     for (int o1 = 0; o1 < S; ++o1)
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        double sp = 0.0, sm = 0.0;
+        TR sp = 0.0, sm = 0.0;
 #pragma unroll
         for (int o = 0; o < S; ++o)
 #pragma unroll
           for (int l = 0; l < LM; ++l) {
             if (!inl(l)) continue;
-            const double xqv = XQ(l, i, o1, o);
+            const TR xqv = (TR)XQ(l, i, o1, o);
             sp += xqv * hq[l][o];
-            sm += (xqv * DMS[o] * pisign(l)) * hq[l][o];
+            sm += (xqv * (TR)DMS[o] * (TR)pisign(l)) * hq[l][o];
           }
-        dpi[i][o1] = F1 * sp; dmi[i][o1] = F1 * sm;
+        dpi[i][o1] = F1 * (double)sp; dmi[i][o1] = F1 * (double)sm;
       }
     double gup[n], gdn[n];
 #pragma unroll
@@ -581,20 +586,16 @@ __device__ __forceinline__ unsigned grp_max_u(unsigned x) {
 
 // Lanes per item in k_bvp (G) and panel rows per lane (SL): the 3n rows of a
 // panel fill SL row slots per lane (8 lanes and up to 3 slots for n <= 8, so
-// a wavefront carries 8 items), and the back-substitution puts ceil(2n / G)
-// unknowns of a layer on each lane.
+// a wavefront carries 8 items), and the adjoint accumulation of the TOA
+// radiance puts ceil(2n / G) unknowns of a layer on each lane.
 __host__ __device__ constexpr int bvp_group(int n) {
   return n <= 8 ? 8 : (2 * n <= 16 ? 16 : (2 * n <= 32 ? 32 : 64));
 }
 __host__ __device__ constexpr int bvp_slots(int n) { return (3 * n + bvp_group(n) - 1) / bvp_group(n); }
 
-// U rows of a layer panel are kept in compact form: row i from column
-// (i + 1) & ~1 (the columns right of its pivot, from an even column so the
-// pairs stay 16-byte aligned) through the right-hand side and 1/pivot.
+// A pivot row goes through LDS from column (i + 1) & ~1 (the columns right of
+// its pivot, from an even column so the pairs stay 16-byte aligned).
 __host__ __device__ constexpr int urow_start(int i) { return (i + 1) & ~1; }
-// sum over r < i of (uw - urow_start(r)), in closed form
-__host__ __device__ constexpr int urow_off(int uw, int i) { return i * uw - 2 * (i / 2) * ((i + 1) / 2); }
-static_assert(urow_off(26, 12) == 240 && urow_off(26, 1) == 26 && urow_off(26, 3) == 74, "compact U row offsets");
 
 // Pivot of each group from per-slot candidate flags: the candidate of the
 // group is the one in the lowest slot, and in it the lowest lane (a group
@@ -622,18 +623,21 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
                                             int* __restrict__ nnext) {
   constexpr int n = N * S, n2 = 2 * n, NC = 4 * n, RH = NC, UW = NC + 2;
   constexpr int G = bvp_group(n), IPW = 64 / G, SL = bvp_slots(n);
-  constexpr int UC = urow_off(UW, n2);   // doubles per layer panel of compact U rows
-  constexpr int BR = (n2 + G - 1) / G;   // U rows per lane in the back-substitution
+  constexpr int NAC = (NC + G - 1) / G;   // accumulator columns per lane in the adjoint of the TOA recursion
   static_assert(2 * n <= 64 && SL <= 3 && n <= G, "panel rows exceed the wavefront");
   constexpr LOff O(N, S);
   constexpr BOff BO(N, n);
   constexpr int BIG = 0x3fffffff;
-  // compact U rows of the current panel (columns right of the pivot, RHS,
-  // 1/pivot); column c of row i is at us[g][urow_off(UW, i) - urow_start(i) + c].
-  // The groups' blocks are padded by two doubles so that the groups' broadcast
-  // reads of the same row fall in different LDS banks.
-  __shared__ __align__(16) double us[IPW][UC + 2];
-  __shared__ double xs[IPW][n2];       // integration constants of the current/next layer
+  // The current pivot row of each group (columns right of the pivot, RHS,
+  // 1/pivot), two rows in turn; column c of row i is at us[g][(i & 1) * UW + c].
+  // The groups' blocks are padded by two doubles so that the groups'
+  // broadcast reads of the same column fall in different LDS banks.
+  constexpr int SR = 2 * UW + 2;
+  __shared__ __align__(16) double us[IPW][SR];
+  // adjoint state of each group: acc[o][h][c] (h = layer parity, c < 2n) and
+  // the current layer's c_o; padded like us
+  constexpr int AS = 3 * S * n2 + 2;
+  __shared__ double adj[IPW][AS];
   const int lane = threadIdx.x, g = lane / G, t = lane - g * G;
   const unsigned long long gm = (G == 64) ? ~0ull : (((1ull << (G & 63)) - 1ull) << (g * G));
   const unsigned long long below = (1ull << lane) - 1ull;
@@ -661,7 +665,6 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
   const int LS = W.lstride;
   const double* __restrict__ lb = W.lbuf + (size_t)it * km * LS;
   const double* __restrict__ bb = W.bbuf + (size_t)it * W.bstride;
-  double* __restrict__ ub = W.ubuf + (size_t)it * (size_t)n2 * km * UW;
   constexpr BROff RO(N);
   const bool brdf = (S == 1) && W.brdf != nullptr;
   const double* __restrict__ brec = brdf ? W.brdf + (size_t)it * W.brdf_stride : nullptr;
@@ -693,6 +696,33 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
     posl[sl] = BIG;
   }
   bool sing = false;
+
+  // ---------------- adjoint of the TOA source recursion
+  // The Fourier component of the TOA radiance at the user angle
+  // (REG/vlidort_intensity.f90:145-643) is linear in the integration
+  // constants x of the boundary-value problem:  I_o = c_o . x + d_o, with
+  //   c_o(layer k, unknown r) = P_k coef_k(o, r)   (+ P_L kmult bcoef(r) on
+  //                                                  the bottom layer, o = 0)
+  //   d_o = sum_k P_k csrc_k(o)                     (+ P_L kmult bc, o = 0)
+  // and P_k the product of the user-angle transmittances of the layers above
+  // layer k (the source recursion cum = (sv + cs) + tdu cum unrolled).  With
+  // the band LU of the elimination, U x = b' (b' the transformed right-hand
+  // side), so c.x = w.b' where U^T w = c, and w_j = (c_j - sum_{p<j} U_pj
+  // w_p) / U_jj is known as soon as row j of U is.  The rows of U are used
+  // once, in the elimination, and never stored.  The sums acc_c of the
+  // current and the next layer's columns and c of the current layer are kept
+  // per group in LDS; lane t updates columns t + G b.  Every lane of the
+  // group carries the same w_j and the same running w . b'.
+  double ia[S], dsum[S];
+#pragma unroll
+  for (int o = 0; o < S; ++o) {
+    ia[o] = 0.0;
+    dsum[o] = 0.0;
+  }
+  for (int e = t; e < AS; e += G) adj[g][e] = 0.0;
+  double ptr = 1.0;   // P_k
+  const double kmult = brdf ? ((M == 0) ? 2.0 : 1.0) : 2.0 * albedo;
+  const double* __restrict__ trr = W.tr + (size_t)it * km * TR_NFIELDS;
   if (valid && t < n) {   // TOA rows (BVP_MATRIX_SETUP, upper boundary), layer 0
     const int i = t / S, o1 = t - (t / S) * S;
 #pragma unroll
@@ -708,7 +738,7 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
   auto lu_panel = [&](auto lastc, int k) __attribute__((always_inline)) {
     constexpr bool LAST = decltype(lastc)::value;
     constexpr int NCOL = LAST ? n2 : NC;
-    const int j0 = n2 * k;
+    const int j0 = n2 * k, cur = k & 1;
 #pragma unroll
     for (int i = 0; i < n2; ++i) {
       const int j = j0 + i;
@@ -766,7 +796,7 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
                                          __builtin_amdgcn_ds_bpermute(src, __double2loint(myv)));
       const int ppos = __builtin_amdgcn_ds_bpermute(src, myp);
       const double rp = 1.0 / pv;
-      double* __restrict__ urow = &us[g][urow_off(UW, i) - urow_start(i)];   // column-indexed
+      double* __restrict__ urow = &us[g][(i & 1) * UW];   // column-indexed
 #pragma unroll
       for (int sl = 0; sl < SL; ++sl) {
         if (ispiv[sl]) {   // aligned pairs from the row's first stored column
@@ -796,12 +826,24 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
         }
       }
       sing = sing || (pv == 0.0);
-    }
-    __syncthreads();
-    {
-      double2* dst = reinterpret_cast<double2*>(ub + (size_t)k * UC);
-      const double2* srcu = reinterpret_cast<const double2*>(&us[g][0]);
-      for (int e = t; e < UC / 2; e += G) dst[e] = srcu[e];
+      // adjoint: every lane of the group forms w_j = (c_j - acc_j) / U_jj
+      // and adds w_j b'_j (b'_j = u[RH], the transformed right-hand side of
+      // the pivot row); each lane adds U_jc w_j to the accumulators of its
+      // columns c > j (the rest of this layer, then the next)
+#pragma unroll
+      for (int o = 0; o < S; ++o) {
+        double* __restrict__ acc = &adj[g][o * 2 * n2];
+        const double w = (adj[g][2 * S * n2 + o * n2 + i] - acc[cur * n2 + i]) * rp;
+        ia[o] = fma(w, u[RH], ia[o]);
+#pragma unroll
+        for (int b = 0; b < NAC; ++b) {
+          const int c = t + G * b;
+          if (c > i && c < NCOL) {
+            const int e = (c < n2) ? cur * n2 + c : (cur ^ 1) * n2 + (c - n2);
+            acc[e] = fma(urow[c], w, acc[e]);
+          }
+        }
+      }
     }
     if (!LAST) {
 #pragma unroll
@@ -873,6 +915,24 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
       }
       base += __popcll(fb);
     }
+    if (here) {   // c and d of layer k; the next layer's accumulators start at 0
+      const double* lr = lb + (size_t)k * LS;
+      const double tdu = trr[(size_t)k * TR_NFIELDS + TR_TDUSER];
+      const double pl = ptr * tdu;   // P_L on the bottom layer
+      const int nxt = (k & 1) ^ 1;
+#pragma unroll
+      for (int o = 0; o < S; ++o) {
+        for (int c = t; c < n2; c += G) {
+          double cv = ptr * lr[O.coef(o, c)];
+          if (o == 0 && lastp && surf) cv = fma(pl * kmult, bb[BO.bcoef(c)], cv);
+          adj[g][2 * S * n2 + o * n2 + c] = cv;
+          if (k > 0) adj[g][o * 2 * n2 + nxt * n2 + c] = 0.0;
+        }
+        dsum[o] = fma(ptr, lr[O.csrc(o)], dsum[o]);
+      }
+      if (lastp && surf) dsum[0] = fma(pl * kmult, bb[BO.bc()], dsum[0]);
+      ptr = pl;
+    }
     if (__ballot(here && !lastp) != 0ull) {
       if (here && !lastp) lu_panel(std::integral_constant<bool, false>(), k);
     }
@@ -880,140 +940,10 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
       if (lastp) lu_panel(std::integral_constant<bool, true>(), k);
     }
   }
-
-  // ---------------- back-substitution (DTBSV column order) fused with the
-  // TOA source recursion (REG/vlidort_intensity.f90:145-643): lane t holds U
-  // rows t + G*b of the panel (b < BR) and ends with their x, the integration
-  // constants of the layer (LCON for rows < n, MCON after).  With one row per
-  // lane the U rows and source coefficients of the next panel up are loaded
-  // one panel ahead; x_i is broadcast within the group by lane reads.
-  const double kmult = brdf ? ((M == 0) ? 2.0 : 1.0) : 2.0 * albedo;
-  constexpr bool PF = (BR == 1);
   double cum[S];
 #pragma unroll
-  for (int o = 0; o < S; ++o) cum[o] = 0.0;
-  const double* trr = W.tr + (size_t)it * km * TR_NFIELDS;
-  // the lane's compact U rows: offset in the panel and first stored column
-  int ut0[BR], uoff[BR];
-#pragma unroll
-  for (int b = 0; b < BR; ++b) {
-    const int r = min(t + G * b, n2);
-    ut0[b] = urow_start(r);
-    uoff[b] = urow_off(UW, r);
-  }
-  // U rows, source coefficients and user transmittance of layer K into U, CF,
-  // CS, TDU (zeros outside the item's layers)
-#define AA_BS_LOAD(K, U, CF, CS, TDU)                                                             \
-  {                                                                                               \
-    const int k_ = (K);                                                                           \
-    const bool ok_ = valid && k_ >= 0 && k_ < L;                                                  \
-    const double* lr_ = lb + (size_t)(ok_ ? k_ : 0) * LS;                                         \
-    _Pragma("unroll") for (int b = 0; b < BR; ++b) {                                              \
-      const int r_ = t + G * b;                                                                   \
-      if (ok_ && r_ < n2) {                                                                       \
-        const double2* src_ = reinterpret_cast<const double2*>(ub + (size_t)k_ * UC + uoff[b]) - ut0[b] / 2; \
-        _Pragma("unroll") for (int e = 0; e < UW / 2; ++e) {                                      \
-          const double2 v_ = (2 * e >= ut0[b]) ? src_[e] : make_double2(0.0, 0.0);                \
-          U[b][2 * e] = v_.x;                                                                     \
-          U[b][2 * e + 1] = v_.y;                                                                 \
-        }                                                                                         \
-        _Pragma("unroll") for (int o = 0; o < S; ++o) CF[b][o] = lr_[O.coef(o, r_)];              \
-      } else {                                                                                    \
-        _Pragma("unroll") for (int e = 0; e < UW; ++e) U[b][e] = 0.0;                             \
-        _Pragma("unroll") for (int o = 0; o < S; ++o) CF[b][o] = 0.0;                             \
-      }                                                                                           \
-    }                                                                                             \
-    if (ok_ && t < n2) {                                                                          \
-      _Pragma("unroll") for (int o = 0; o < S; ++o) CS[o] = lr_[O.csrc(o)];                       \
-      TDU = trr[(size_t)k_ * TR_NFIELDS + TR_TDUSER];                                             \
-    } else {                                                                                      \
-      _Pragma("unroll") for (int o = 0; o < S; ++o) CS[o] = 0.0;                                  \
-      TDU = 0.0;                                                                                  \
-    }                                                                                             \
-  }
-  double u[BR][UW], cf[BR][S], cs[S], tdu = 0.0;
-  if (PF) AA_BS_LOAD(Lmax - 1, u, cf, cs, tdu);
-  for (int k = Lmax - 1; k >= 0; --k) {
-    double uc[BR][UW], cfc[BR][S], csc[S], tduc;
-    if (PF) {
-#pragma unroll
-      for (int b = 0; b < BR; ++b) {
-#pragma unroll
-        for (int e = 0; e < UW; ++e) uc[b][e] = u[b][e];
-#pragma unroll
-        for (int o = 0; o < S; ++o) cfc[b][o] = cf[b][o];
-      }
-#pragma unroll
-      for (int o = 0; o < S; ++o) csc[o] = cs[o];
-      tduc = tdu;
-      AA_BS_LOAD(k - 1, u, cf, cs, tdu);
-    } else {
-      AA_BS_LOAD(k, uc, cfc, csc, tduc);
-    }
-    const bool here = valid && k < L;
-    if (__ballot(here) == 0ull) continue;
-    const bool lastp = k == L - 1;
-    bool own[BR];
-    double tv[BR];
-#pragma unroll
-    for (int b = 0; b < BR; ++b) {
-      own[b] = here && t + G * b < n2;
-      tv[b] = uc[b][RH];
-    }
-    if (here && !lastp) {
-#pragma unroll
-      for (int b = 0; b < BR; ++b)
-#pragma unroll
-        for (int c = n2 - 1; c >= 0; --c) tv[b] = fma(-uc[b][n2 + c], xs[g][c], tv[b]);
-    }
-    double xt[BR];
-#pragma unroll
-    for (int b = 0; b < BR; ++b) xt[b] = 0.0;
-#pragma unroll
-    for (int i = n2 - 1; i >= 0; --i) {
-      const int bi = i / G, ti = i - bi * G;   // lane and slot of row i
-      const double cand = tv[bi] * uc[bi][RH + 1];
-      double xv = 0.0;
-#pragma unroll
-      for (int q = 0; q < IPW; ++q) {
-        const double v = readlane_d(cand, q * G + ti);
-        xv = (g == q) ? v : xv;
-      }
-      if (t == ti) xt[bi] = xv;
-#pragma unroll
-      for (int b = 0; b < BR; ++b)
-        if (t + G * b < i) tv[b] = fma(-uc[b][i], xv, tv[b]);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int b = 0; b < BR; ++b)
-      if (own[b]) xs[g][t + G * b] = xt[b];
-    __syncthreads();
-    if (here) {
-      double sv[S];
-#pragma unroll
-      for (int o = 0; o < S; ++o) {
-        double p = own[0] ? xt[0] * cfc[0][o] : 0.0;
-#pragma unroll
-        for (int b = 1; b < BR; ++b) p = own[b] ? fma(xt[b], cfc[b][o], p) : p;
-        sv[o] = grp_sum<G>(p);
-      }
-      if (lastp) {
-#pragma unroll
-        for (int o = 0; o < S; ++o) cum[o] = 0.0;
-        if (surf) {
-          double p = own[0] ? xt[0] * bb[BO.bcoef(t)] : 0.0;
-#pragma unroll
-          for (int b = 1; b < BR; ++b) p = own[b] ? fma(xt[b], bb[BO.bcoef(t + G * b)], p) : p;
-          const double bv = grp_sum<G>(p);
-          cum[0] = kmult * (bv + bb[BO.bc()]);
-        }
-      }
-#pragma unroll
-      for (int o = 0; o < S; ++o) cum[o] = (sv[o] + csc[o]) + tduc * cum[o];
-    }
-  }
-#undef AA_BS_LOAD
+  for (int o = 0; o < S; ++o) cum[o] = ia[o] + dsum[o];
+
 
   // ---------------- Fourier convergence (VLIDORT_CONVERGE_OBSGEO)
   if (t != 0 || !valid) return;

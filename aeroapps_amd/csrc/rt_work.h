@@ -91,8 +91,7 @@ struct RTWork {
   double* lostrans = nullptr;      // [it][km+1]
   double* lbuf = nullptr;          // [it][km][lstride] homogeneous + particular solutions
   double* bbuf = nullptr;          // [it][bstride] surface terms of the bottom layer
-  double* ubuf = nullptr;          // [it][2*n*km][ustride] band-LU U rows + transformed RHS
-  int lstride = 0, bstride = 0, ustride = 0;
+  int lstride = 0, bstride = 0;
   // ---- quadrature and PI tables of the quadrature streams (device)
   const double* quad = nullptr;    // [qs N][qw N][xq: M][l][i][S][S]
   // ---- mixed precision: FP32 phase-matrix contraction (SAB/DAB) by MFMA

@@ -268,7 +268,6 @@ size_t carve_work(Carver& C, RTWork& W, int cap) {
   W.lstride = LOff(W.N, W.S).size();
   W.bstride = BOff(W.N, n).size();
   W.brdf_stride = (W.kernel_wt || W.u10m) ? BROff(W.N).size() : 0;
-  W.ustride = 4 * n + 2;   // U row of a layer panel: 4n columns, transformed RHS, 1/pivot
   W.cap = cap;
   W.status = C.take<int>(cap);
   W.L_it = C.take<int>(cap);
@@ -295,7 +294,6 @@ size_t carve_work(Carver& C, RTWork& W, int cap) {
   W.bbuf = C.take<double>((size_t)cap * W.bstride);
   W.brdf = (W.kernel_wt || W.u10m) ? C.take<double>((size_t)cap * W.brdf_stride) : nullptr;
   W.wsa_cache = W.kernel_wt ? C.take<double>(4) : nullptr;
-  W.ubuf = C.take<double>((size_t)cap * (2 * n * km) * W.ustride);
   W.sab32 = (W.precision == 1) ? C.take<float>((size_t)cap * km * 2 * n * n) : nullptr;
   W.sab64 = (W.precision == 0) ? C.take<double>((size_t)cap * km * 2 * n * n) : nullptr;
   return C.off;

@@ -36,7 +36,7 @@ PROF_NAMES = ["mie", "check", "optics", "geom", "item", "layer", "bvp", "brdf", 
 # scalar, keyed (kernel, N, S, mixed), from FETCH_SIZE/WRITE_SIZE counter runs
 # scaled by a copy-kernel calibration
 # (profiles/hbm_traffic_40k_columns_4ch_12streams.md)
-TRAFFIC_PER_ITEM_TERM = {("bvp", 6, 1, False): (287.6e3, 138.9e3), ("bvp", 6, 1, True): (287.6e3, 138.9e3),
+TRAFFIC_PER_ITEM_TERM = {("bvp", 6, 1, False): (159.9e3, 10.7e3), ("bvp", 6, 1, True): (159.9e3, 10.7e3),
                          ("layer", 6, 1, False): (56.1e3, 70.8e3), ("layer", 6, 1, True): (35.7e3, 74.0e3)}
 
 
@@ -49,18 +49,21 @@ def flops_per_term(N, S_, L, nmoments, M):
     k_layer itself does: omega-Greek x PI products, EIGENMAT = DAB.SAB and the
     nonsymmetric eigensolve with vectors.  F_c counts the structurally non-zero
     work of the pivoted factorisation (one panel of 3n rows x (4n columns +
-    RHS) per layer, 2n x (2n + RHS) for the bottom layer) and of the
-    back-substitution; an FMA counts as 2 flops."""
+    RHS) per layer, 2n x (2n + RHS) for the bottom layer) and of the adjoint
+    solve U^T w = c with the running w . b' (per eliminated column: w_j, its
+    product with b'_j, and the update of the accumulators of the row's
+    remaining columns, per Stokes output); F_d the source functional c, d
+    (per layer S (2n + 2) + 1).  An FMA counts as 2 flops."""
     n = S_ * N
     n2 = 2 * n
     Lm = nmoments - M + 1
     Fb = L * (4 * N * S_ ** 3 * Lm + 2 * n ** 3 + 25 * n ** 3)
     mid = sum((3 * n - 1 - i) * (1 + 2 * (4 * n - i)) for i in range(n2))
     last = sum((2 * n - 1 - i) * (1 + 2 * (2 * n - i)) for i in range(n2))
-    bs_mid = 2 * n2 * n2 + n2 * (n2 - 1) + n2
-    bs_last = n2 * (n2 - 1) + n2
-    Fc = (L - 1) * (mid + bs_mid) + last + bs_last
-    Fd = L * S_ * (2 * n2 + 3)
+    adj_mid = S_ * sum(4 + 2 * (2 * n2 - 1 - i) for i in range(n2))
+    adj_last = S_ * sum(4 + 2 * (n2 - 1 - i) for i in range(n2))
+    Fc = (L - 1) * (mid + adj_mid) + last + adj_last
+    Fd = L * (S_ * (n2 + 2) + 1)
     return Fb, Fc, Fd
 
 
@@ -304,10 +307,14 @@ def main():
         ms_per_step = el * 1000.0 / args.steps
         nfh = nf.cpu().numpy().ravel().copy()
         radh = rad.cpu().numpy().ravel().copy()
+        prods = {"reflectance": refl.cpu().numpy().ravel().copy()}
+        if vector:
+            prods["q"] = qo.cpu().numpy().ravel().copy()
+            prods["u"] = uo.cpu().numpy().ravel().copy()
         print(f"precision {precision}: {args.steps} timed steps done, {ms_per_step:.1f} ms/step", file=sys.stderr,
               flush=True)
         return dict(el=el, kms=kms, kln=kln, ms_per_step=ms_per_step, value=ncol / (ms_per_step / 1000.0), nf=nfh,
-                    rad=radh)
+                    rad=radh, prods=prods)
 
     if args.work_gib > 0:
         _lib.check(L.aa_set_arena_bytes(int(args.work_gib * (1 << 30)), 0), "aa_set_arena_bytes")
@@ -404,8 +411,9 @@ def main():
                                f" streams, {nmom} moments, {nq} tracers, "
                                f"{'vector' if vector else 'scalar'} "
                                f"{'MODIS land BRDF (Lambertian + Ross-thick + Li-sparse)' if modis else 'Lambertian'}",
-                   "precision": "mixed (FP32 Chem_Mie optics and FP32 phase-matrix contraction on MFMA; FP64 "
-                                "everything else, the FP64 line with an FP64 MFMA contraction)"
+                   "precision": "mixed (FP32 Chem_Mie optics, FP32 phase-matrix contraction on MFMA, FP32 source "
+                                "assembly; FP64 eigensystem and boundary-value elimination; the FP64 line is FP64 "
+                                "throughout with an FP64 MFMA contraction)"
                                 if rmx is not None else "fp64",
                    "path": ("aa_column_radiance_modis_dev" if modis else "aa_column_radiance_dev") +
                            " (Chem_Mie optics + VLIDORT radiance, device-resident)",
@@ -430,6 +438,17 @@ def main():
                                 "max_rel_err_radiance": float(rel.max()) if rel.size else None,
                                 "p99_rel_err_radiance": float(np.percentile(rel, 99)) if rel.size else None,
                                 "fourier_count_mismatch_frac": float(1 - same.sum() / ok.sum())}
+        # error of each product against the FP64 path (elements with the same
+        # Fourier-term count): relative for radiance and reflectance, Q and U
+        # relative to |I|
+        per = {"radiance": rel}
+        i64 = np.abs(r64["rad"][same])
+        for k, v in rmx["prods"].items():
+            d = np.abs(v[same] - r64["prods"][k][same])
+            per[k] = d / (np.abs(r64["prods"][k][same]) if k == "reflectance" else i64)
+        out["mixed_vs_fp64"]["per_product"] = {
+            k: {"max": float(v.max()) if v.size else None, "p99": float(np.percentile(v, 99)) if v.size else None,
+                "relative_to": "itself" if k in ("radiance", "reflectance") else "|I|"} for k, v in per.items()}
     if cpu:
         out["gpu_vs_cpu"] = round(value / cpu["value"], 1)
         out["fp64_vs_cpu"] = round(r64["value"] / cpu["value"], 1)

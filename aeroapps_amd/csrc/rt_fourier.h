@@ -252,64 +252,80 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
   pimatrix_x<S, LM>(M, NMOM, W.csza[it], x0p);
 
   // ---- UHOM + HMULT (:1167-1505, :1651-1790)
+  // The Legendre sums of the user-angle solutions, with the quadrature
+  // weights folded into the PI products and the sums over X+ and X- taken
+  // apart:  hs(l, o1, k) = P(l, o1, k) + DMS(o1) pisign(l) Q(l, o1, k),
+  //   P = sum_{j,o} (XQ(l, j, o1, o) a5_j) X+(j, o, k),  Q likewise with X-
+  // (the reference forms each XQ X+- product inside one sum; the terms are
+  // the same, the summation order differs at rounding level).
   double updn[n][S], upup[n][S], hm1[n], hm2[n];
+  {
+    TR ud[n][S], uu[n][S];
+#pragma unroll
+    for (int k = 0; k < n; ++k)
+#pragma unroll
+      for (int o = 0; o < S; ++o) { ud[k][o] = 0.0; uu[k][o] = 0.0; }
+    if (lsc) {
+#pragma unroll
+      for (int l = 0; l < LM; ++l) {
+        if (!inl(l)) continue;
+        TR pa[n][S], qa[n][S];
+#pragma unroll
+        for (int k = 0; k < n; ++k)
+#pragma unroll
+          for (int o1 = 0; o1 < S; ++o1) { pa[k][o1] = 0.0; qa[k][o1] = 0.0; }
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+#pragma unroll
+          for (int o1 = 0; o1 < S; ++o1)
+#pragma unroll
+            for (int o = 0; o < S; ++o) {
+              const TR aq = (TR)XQ(l, j, o1, o) * (TR)qhw[j];
+#pragma unroll
+              for (int k = 0; k < n; ++k) {
+                pa[k][o1] += aq * (TR)xp[j * S + o][k];
+                qa[k][o1] += aq * (TR)xn[j * S + o][k];
+              }
+            }
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+          TR hs[S], ga[S];
+#pragma unroll
+          for (int o1 = 0; o1 < S; ++o1) hs[o1] = pa[k][o1] + (TR)(DMS[o1] * pisign(l)) * qa[k][o1];
+#pragma unroll
+          for (int o1 = 0; o1 < S; ++o1) {
+            TR g = 0.0;
+#pragma unroll
+            for (int o = 0; o < S; ++o) g += (TR)og[l][o1][o] * hs[o];
+            ga[o1] = g;
+          }
+#pragma unroll
+          for (int o1 = 0; o1 < S; ++o1) {
+            TR sdn = 0.0, sup = 0.0;
+#pragma unroll
+            for (int o = 0; o < S; ++o) {
+              const TR xum = (TR)(DMS[o1] * xup[l][o1][o] * DMS[o] * pisign(l));
+              const TR xupre = (TR)(DMS[o1] * xup[l][o1][o]);
+              sdn += xum * ga[o];
+              sup += xupre * ga[o];
+            }
+            ud[k][o1] += sdn;
+            uu[k][o1] += sup;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < n; ++k)
+#pragma unroll
+      for (int o = 0; o < S; ++o) { updn[k][o] = (double)ud[k][o]; upup[k][o] = (double)uu[k][o]; }
+  }
 #pragma unroll
   for (int k = 0; k < n; ++k) {
     const double rho_p = usec + keig[k], rho_m = usec - keig[k];
     const double zeta_p = 1.0 / rho_p;
     const bool hsingo = fabs(rho_m) < TAYLOR_SMALL;
     const double zeta_m = hsingo ? rho_m : 1.0 / rho_m;
-    if (!lsc) {
-#pragma unroll
-      for (int o = 0; o < S; ++o) { updn[k][o] = 0.0; upup[k][o] = 0.0; }
-    } else {
-      TR gaux[LM][S];
-#pragma unroll
-      for (int l = 0; l < LM; ++l) {
-        TR hs[S];
-#pragma unroll
-        for (int o1 = 0; o1 < S; ++o1) {
-          TR sps = 0.0;
-          if (inl(l)) {
-#pragma unroll
-            for (int j = 0; j < N; ++j) {
-              const TR a5 = (TR)qhw[j];
-              TR sp = 0.0, sm = 0.0;
-#pragma unroll
-              for (int o = 0; o < S; ++o) {
-                const TR xqv = (TR)XQ(l, j, o1, o);
-                sp += xqv * a5 * (TR)xp[j * S + o][k];
-                sm += ((TR)DMS[o1] * xqv * (TR)pisign(l)) * a5 * (TR)xn[j * S + o][k];
-              }
-              sps = sps + sp + sm;
-            }
-          }
-          hs[o1] = sps;
-        }
-#pragma unroll
-        for (int o1 = 0; o1 < S; ++o1) {
-          TR s = 0.0;
-#pragma unroll
-          for (int o = 0; o < S; ++o) s += (TR)og[l][o1][o] * hs[o];
-          gaux[l][o1] = s;
-        }
-      }
-#pragma unroll
-      for (int o1 = 0; o1 < S; ++o1) {
-        TR sdn = 0.0, sup = 0.0;
-#pragma unroll
-        for (int o = 0; o < S; ++o)
-#pragma unroll
-          for (int l = 0; l < LM; ++l) {
-            if (!inl(l)) continue;
-            const TR xum = (TR)(DMS[o1] * xup[l][o1][o] * DMS[o] * pisign(l));
-            const TR xupre = (TR)(DMS[o1] * xup[l][o1][o]);
-            sdn += xum * gaux[l][o];
-            sup += xupre * gaux[l][o];
-          }
-        updn[k][o1] = (double)sdn; upup[k][o1] = (double)sup;
-      }
-    }
     const double udel = tduser;
     const double zdel = tdel[k], zudel = zdel * udel;
     const double th2 = 1.0 - zudel, th1 = zdel - udel;

@@ -1,9 +1,10 @@
 // Device building blocks of the discrete-ordinate radiance kernels.
 //
-// Everything here is FP64 and follows the reference routines' operation order
-// (no contraction: the library is compiled with -ffp-contract=off) so results
-// agree with the reference to rounding level.  Array indices are compile-time
-// wherever possible so that the per-thread matrices live in VGPRs.
+// Everything here is FP64 and follows the reference routines' steps, so
+// results agree with the reference to rounding level (the Fourier-term
+// kernels that include this header are compiled with FMA contraction, the
+// column set-up without).  Array indices are compile-time wherever possible
+// so that the per-thread matrices live in VGPRs.
 #pragma once
 #include "aa_common.h"
 
@@ -69,7 +70,10 @@ __device__ __forceinline__ double taylor2(double eps, double y, double delta, do
 // ---------------------------------------------------------------------------
 // ASYMTX (REG/vlidort_aux.f90:386-959) for an M x M matrix held in registers.
 //
-// Same arithmetic as the reference routine; the control flow is restated with
+// Same steps as the reference routine (a Householder reflection's f / h is
+// formed once per row or column and the balancing scale's exact reciprocal
+// once per row, where the reference divides per element: rounding level);
+// the control flow is restated with
 // compile-time array indices: loops whose bounds depend on the deflation state
 // run over all static positions under a predicate.  The row/column isolation
 // prepass of the reference only triggers for matrices with an all-zero
@@ -142,11 +146,13 @@ __device__ int asymtx_dev(double tol, double (&a)[M][M], double (&evec)[M][M], d
         // (the caps only matter for non-finite input, which the reference would not finish either)
         for (int it = 0; it < 256 && col < g; ++it) { f = f * C5; col = col * C6; }
         g = row * C5;
-        for (int it = 0; it < 256 && col >= g; ++it) { f = f / C5; col = col / C6; }
-        if ((col + row) / f < C4 * h) {
+        // f is a power of 16: dividing by it is multiplying by its exact reciprocal
+        for (int it = 0; it < 256 && col >= g; ++it) { f = f * (1.0 / C5); col = col * (1.0 / C6); }
+        const double rf = 1.0 / f;
+        if ((col + row) * rf < C4 * h) {
           wk[i] = wk[i] * f; noconv = true;
 #pragma unroll
-          for (int j = 0; j < M; ++j) a[i][j] = a[i][j] / f;
+          for (int j = 0; j < M; ++j) a[i][j] = a[i][j] * rf;
 #pragma unroll
           for (int j = 0; j < M; ++j) a[j][i] = a[j][i] * f;
         }
@@ -169,21 +175,25 @@ __device__ int asymtx_dev(double tol, double (&a)[M][M], double (&evec)[M][M], d
         double g = -fsign(sqrt(h), hv[n]);
         h = h - hv[n] * g;
         hv[n] = hv[n] - g;
+        // (hv f / h as hv (f / h): one division per row or column of the
+        // reflection instead of one per element; rounding level)
 #pragma unroll
         for (int j = n; j < M; ++j) {
           double f = 0.0;
 #pragma unroll
           for (int i = M - 1; i >= n; --i) f += hv[i] * a[i][j];
+          const double fh = f / h;
 #pragma unroll
-          for (int i = n; i < M; ++i) a[i][j] = a[i][j] - hv[i] * f / h;
+          for (int i = n; i < M; ++i) a[i][j] = a[i][j] - hv[i] * fh;
         }
 #pragma unroll
         for (int i = 0; i < M; ++i) {
           double f = 0.0;
 #pragma unroll
           for (int j = M - 1; j >= n; --j) f += hv[j] * a[i][j];
+          const double fh = f / h;
 #pragma unroll
-          for (int j = n; j < M; ++j) a[i][j] = a[i][j] - hv[j] * f / h;
+          for (int j = n; j < M; ++j) a[i][j] = a[i][j] - hv[j] * fh;
         }
         hv[n] = scale * hv[n];
         a[n][n - 1] = scale * g;

@@ -211,9 +211,9 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
     double s = 0.0;
 #pragma unroll
     for (int r = 0; r < n; ++r) s += evec[r][c] * evec[r][c];
-    const double nr = sqrt(s);
+    const double rn = 1.0 / sqrt(s);
 #pragma unroll
-    for (int r = 0; r < n; ++r) evec[r][c] = evec[r][c] / nr;
+    for (int r = 0; r < n; ++r) evec[r][c] = evec[r][c] * rn;
   }
   double keig[n], tdel[n];
   double xp[n][n], xn[n][n];   // X(i,o,k) for i < N at row i*S+o, column k
@@ -225,10 +225,10 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
     double s = 0.0;
 #pragma unroll
     for (int r = 0; r < n; ++r) s += evec[r][k] * evec[r][k];
-    const double norm_r = sqrt(s);
+    const double rnorm_r = 1.0 / sqrt(s);
     double sum[n];
 #pragma unroll
-    for (int r = 0; r < n; ++r) sum[r] = evec[r][k] / norm_r;
+    for (int r = 0; r < n; ++r) sum[r] = evec[r][k] * rnorm_r;
 #pragma unroll
     for (int r = 0; r < n; ++r) {
       double h = 0.0;

@@ -221,7 +221,7 @@ __global__ void k_check(RTWork W, int nitems) {
                       is_missing(W.mr[I.ch], W.missing)))
     st = 1;
   int L = W.nlayers ? W.nlayers[I.obs] : W.km;
-  if (L < 2 || L > W.km) { if (!st) st = 4; L = 2; }
+  if (L < 1 || L > W.km) { if (!st) st = 4; L = 1; }
   if (!st) {
     if (sza < 0.0 || sza >= 90.0) st = 3;
     if (raa > 360.0 || raa < 0.0) st = 3;

@@ -34,7 +34,8 @@
 //                    FOC/FO_VectorSS_RTCalcs_I.f90:116-785
 //
 // Scope: NSTOKES 1 or 3 (real eigensolver branch only; NSTOKES=4 is not on the
-// AeroApps path), NLAYERS >= 2, one observation geometry per call,
+// AeroApps path), NLAYERS >= 1 (one layer: the band LU covers the dense
+// 2n x 2n system of the reference's DGETRF branch), one observation geometry per call,
 // refraction off, plane-parallel optional.
 #include <cmath>
 #include <cstdio>
@@ -1378,7 +1379,8 @@ Result run_column(const Column& c) {
     }
     // ---- BVP (BVP_MATRIXSETUP_MASTER + BVP_SOLUTION_MASTER), band storage
     std::vector<double> lcon((size_t)n * L), mcon((size_t)n * L);
-    if (L >= 2) {
+    {   // NLAYERS = 1: the band covers the whole 2n x 2n matrix (KL = KU = 2n-1), and
+        // DGBTF2 picks the pivots of DGETRF's partial pivoting
       std::fill(ab.begin(), ab.end(), 0.0);
       const int KV = KU + KL;
       auto AB = [&](int r, int cidx) -> double& {  // 1-based matrix (r,c) -> band storage
@@ -1522,8 +1524,6 @@ Result run_column(const Column& c) {
         int cc0 = (lay - 1) * n2;
         for (int k = 0; k < n; ++k) { lcon[(size_t)(lay - 1) * n + k] = colv[cc0 + k]; mcon[(size_t)(lay - 1) * n + k] = colv[cc0 + k + n]; }
       }
-    } else {
-      res.status = 4; return res;   // NLAYERS == 1 (dense DGETRF path) is not on the AeroApps path
     }
     // ---- VLIDORT_UPUSER_INTENSITY -> TOA (REG/vlidort_intensity.f90:145-643)
     double cum[3] = {0, 0, 0};

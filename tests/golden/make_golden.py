@@ -43,10 +43,14 @@ CASES = {
     # GEOS optics-table moment counts (nmom up to 300, npol 6)
     "scalar_n6_nmom300": (3, (550.0, 870.0), 6, 300, False, False, False),
     "vector_n4_nmom300": (3, (550.0,), 4, 300, True, False, False),
+    # one-layer atmospheres (the bottom layer of each column): NLAYERS = 1,
+    # the reference's dense DGETRF branch of the boundary-value problem
+    "scalar_n4_km1": (6, (550.0, 870.0), 4, 32, False, False, False, True),
+    "vector_n2_km1": (4, (550.0,), 2, 32, True, False, False, True),
 }
 
 
-def make_inputs(ncol, channels, nmom, vector, edge, seed=7, col0=0):
+def make_inputs(ncol, channels, nmom, vector, edge, seed=7, col0=0, one_layer=False):
     npol = 6 if vector else 1
     cols = S.make_columns(ncol, 72, channels, seed=seed, col0=col0)
     tabs = S.make_mie_tables(channels, nmom=nmom, npol=npol)
@@ -60,9 +64,17 @@ def make_inputs(ncol, channels, nmom, vector, edge, seed=7, col0=0):
         albedo[:, 9] = 0.0      # no surface term
         albedo[:, 10] = S.MISSING
         raa[11] = 0.0
-    return dict(km=72, channels=np.asarray(channels, float), nmom=nmom, npol=npol,
-                tau=tau, ssa=ssa, g=g, pmom=pm, rot=cols.rot, depol=cols.depol, pe=cols.pe, he=cols.he,
-                te=cols.te, albedo=albedo, sza=sza, vza=vza, raa=raa, flux_factor=cols.flux_factor)
+    d = dict(km=72, channels=np.asarray(channels, float), nmom=nmom, npol=npol,
+             tau=tau, ssa=ssa, g=g, pmom=pm, rot=cols.rot, depol=cols.depol, pe=cols.pe, he=cols.he,
+             te=cols.te, albedo=albedo, sza=sza, vza=vza, raa=raa, flux_factor=cols.flux_factor)
+    if one_layer:
+        k = 71
+        d["km"] = 1
+        for key in ("tau", "ssa", "g", "pmom", "rot"):
+            d[key] = np.ascontiguousarray(d[key][..., k:k + 1])
+        for key in ("pe", "he", "te"):
+            d[key] = np.ascontiguousarray(d[key][:, k:k + 2])
+    return d
 
 
 def run_driver(drv, d, nstreams, vector, plane_parallel):
@@ -102,10 +114,11 @@ def main():
     if ref is None:
         raise SystemExit("reference library not built: make -C oracle -f Makefile.ref")
     only = set(sys.argv[1:])
-    for name, (ncol, ch, N, nmom, vector, pp, edge) in CASES.items():
+    for name, spec in CASES.items():
         if only and name not in only:
             continue
-        d = make_inputs(ncol, ch, nmom, vector, edge)
+        ncol, ch, N, nmom, vector, pp, edge = spec[:7]
+        d = make_inputs(ncol, ch, nmom, vector, edge, one_layer=len(spec) > 7 and spec[7])
         out = run_driver(ref, d, N, vector, pp)
         want = driver_outputs(D, d, N, vector, pp)
         for k, v in want.items():

@@ -32,6 +32,7 @@ const std::vector<double>& brdf_table_cached(int N, int A);
 #define AA_DECL(N, S) \
   int rt_fourier_##N##_##S(const RTWork& W, int nitems, int* const lists[2], int* counts, hipStream_t st);
 AA_DECL(1, 1) AA_DECL(2, 1) AA_DECL(3, 1) AA_DECL(4, 1) AA_DECL(5, 1) AA_DECL(6, 1) AA_DECL(7, 1) AA_DECL(8, 1)
+AA_DECL(9, 1) AA_DECL(10, 1) AA_DECL(11, 1) AA_DECL(12, 1)
 AA_DECL(1, 3) AA_DECL(2, 3) AA_DECL(3, 3) AA_DECL(4, 3)
 #undef AA_DECL
 
@@ -48,6 +49,10 @@ int rt_fourier(const RTWork& W, int nitems, int* const lists[2], int* counts, hi
       case 6: return rt_fourier_6_1(W, nitems, lists, counts, st);
       case 7: return rt_fourier_7_1(W, nitems, lists, counts, st);
       case 8: return rt_fourier_8_1(W, nitems, lists, counts, st);
+      case 9: return rt_fourier_9_1(W, nitems, lists, counts, st);
+      case 10: return rt_fourier_10_1(W, nitems, lists, counts, st);
+      case 11: return rt_fourier_11_1(W, nitems, lists, counts, st);
+      case 12: return rt_fourier_12_1(W, nitems, lists, counts, st);
       default: return -4;
     }
   }
@@ -164,8 +169,13 @@ void quad_table(int N, int S, int nmoments, std::vector<double>& q) {
       case 5: quad_table_t<1, 10>(N, nmoments, q); break;
       case 6: quad_table_t<1, 12>(N, nmoments, q); break;
       case 7: quad_table_t<1, 14>(N, nmoments, q); break;
-      default: quad_table_t<1, 16>(N, nmoments, q); break;
+      case 8: quad_table_t<1, 16>(N, nmoments, q); break;
+      case 9: quad_table_t<1, 18>(N, nmoments, q); break;
+      case 10: quad_table_t<1, 20>(N, nmoments, q); break;
+      case 11: quad_table_t<1, 22>(N, nmoments, q); break;
+      default: quad_table_t<1, 24>(N, nmoments, q); break;
     }
+    static_assert(MAX_STREAMS == 12, "one quadrature-table case per scalar stream count");
   } else {
     switch (N) {
       case 1: quad_table_t<3, 2>(N, nmoments, q); break;

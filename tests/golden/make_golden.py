@@ -46,6 +46,9 @@ CASES = {
     # one-layer atmospheres (the bottom layer of each column): NLAYERS = 1,
     # the reference's dense DGETRF branch of the boundary-value problem
     "scalar_n4_km1": (6, (550.0, 870.0), 4, 32, False, False, False, True),
+    # the stream counts of the reference's own callers (nstreams 10, 12)
+    "scalar_n10": (3, (550.0,), 10, 32, False, False, False),
+    "scalar_n12": (3, (470.0, 870.0), 12, 64, False, False, False),
     "vector_n2_km1": (4, (550.0,), 2, 32, True, False, False, True),
 }
 

@@ -103,6 +103,7 @@ RAD_CASES = {
     "modis_n4_edge": (10, (550.0,), 4, 32, True),
     "modis_n2": (5, (550.0,), 2, 32, False),
     "modis_n8": (3, (550.0,), 8, 32, False),
+    "modis_n12": (2, (550.0,), 12, 32, False),
 }
 
 
@@ -180,6 +181,7 @@ CX_CASES = {
     "cx_n6_2ch": (6, (470.0, 660.0), 6, 64, False),
     "cx_n4_edge": (9, (550.0,), 4, 32, True),
     "cx_n8": (3, (550.0,), 8, 32, False),
+    "cx_n12": (2, (550.0,), 12, 32, False),
 }
 
 

@@ -56,6 +56,9 @@ def test_fp64_matches_reference_golden(gpu, name):
     # back-substitution rows per lane with two (N=5) or three (N=7) row slots
     (12, (470.0,), 5, 32),
     (12, (660.0,), 7, 32),
+    # n = 9..12 scalar: 32-lane groups with one (N = 9) or two (N = 11) row slots
+    (6, (550.0,), 9, 32),
+    (4, (870.0,), 11, 32),
 ])
 def test_fp64_matches_oracle_seeded(gpu, ncol, channels, nstreams, nmom):
     print(f"case seeded ncol={ncol} N={nstreams} nmom={nmom}", flush=True)

@@ -99,8 +99,7 @@ __device__ __forceinline__ double dpp_d(double x) {
 
 // ------------------------------------------------------------------ k_layer
 template <int N, int S, bool MIX>
-__global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __restrict__ active,
-                                              const int* __restrict__ nactive) {
+__global__ __launch_bounds__(64) void k_layer(RTWork W, int M) {
   constexpr int n = N * S, LM = 2 * N;
   // MIX (the mixed path): SAB/DAB come from the FP32 MFMA contraction and the
   // source assembly -- the user-angle homogeneous solutions (Legendre sums of
@@ -113,11 +112,10 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M, const int* __rest
   constexpr QOff QO(N, S);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int km = W.km;
-  const int a = t / km, layer = t - a * km;
-  if (a >= *nactive) return;
-  const int it = active[a];
+  if (t >= W.lcnt[M]) return;   // dense (item, layer) slots of this term
+  const int it = W.lown[t];
+  const int layer = t - W.loff[it];
   const int L = W.L_it[it];
-  if (layer >= L) return;
   const int NMOM = W.nmoments;
   const double* __restrict__ Qt = W.quad;
   auto XQ = [&](int l, int i, int s1, int s2) -> double { return Qt[QO.xq(M, l, i, s1, s2)]; };
@@ -1010,6 +1008,9 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
   } else {
     const int slot = atomicAdd(nnext, 1);
     next[slot] = it;
+    const int off = atomicAdd(W.lcnt + M + 1, L);
+    W.loff[it] = off;
+    for (int l = 0; l < L; ++l) W.lown[off + l] = it;
   }
 }
 
@@ -1034,8 +1035,7 @@ __host__ __device__ constexpr int sab_groups(int N, int S) { return sab_tiles(N,
 // the accumulator rows of lane l are 4*(l/16) + v for f32 but l/16 + 4*v for
 // f64 (v = 0..3; checked on the device with a probe of known products).
 template <int N, int S, typename T>
-__global__ __launch_bounds__(64) void k_sabdab(RTWork W, int M, const int* __restrict__ active,
-                                               const int* __restrict__ nactive) {
+__global__ __launch_bounds__(64) void k_sabdab(RTWork W, int M) {
   constexpr bool F64 = std::is_same<T, double>::value;
   using V4 = typename std::conditional<F64, d4v, f4v>::type;
   constexpr int n = N * S, LM = 2 * N;
@@ -1046,7 +1046,7 @@ __global__ __launch_bounds__(64) void k_sabdab(RTWork W, int M, const int* __res
   constexpr bool PRE = sab_tiles(N, S) <= 32;   // A tiles held in registers
   const int lane = threadIdx.x;
   const int km = W.km;
-  const int na = *nactive;
+  const int nl = W.lcnt[M];
   const int kq = lane >> 4;
   bool valid[SAB_PG];
   size_t il[SAB_PG];
@@ -1055,10 +1055,9 @@ __global__ __launch_bounds__(64) void k_sabdab(RTWork W, int M, const int* __res
   for (int q = 0; q < SAB_PG; ++q) {
     // pair of this lane's B column / C column in group q
     const long p = ((long)blockIdx.x * SAB_PG + q) * 16 + (lane & 15);
-    const int a = (int)(p / km), layer = (int)(p - (long)a * km);
-    bool v = a < na;
-    int it = 0;
-    if (v) { it = active[a]; v = layer < W.L_it[it]; }
+    const bool v = p < nl;
+    const int it = v ? W.lown[p] : 0;
+    const int layer = v ? (int)(p - W.loff[it]) : 0;
     valid[q] = v;
     il[q] = (size_t)it * km + layer;
     any = any || v;
@@ -1139,12 +1138,12 @@ int fourier_loop(const RTWork& W, int nitems, int* const lists[2], int* counts, 
     const int* in = lists[M & 1];
     int* out = lists[(M + 1) & 1];
     prof_begin(PROF_SABDAB, st);
-    if (W.precision == 1) hipLaunchKernelGGL((k_sabdab<N, S, float>), dim3(ns), dim3(64), 0, st, W, M, in, counts + M);
-    else hipLaunchKernelGGL((k_sabdab<N, S, double>), dim3(ns), dim3(64), 0, st, W, M, in, counts + M);
+    if (W.precision == 1) hipLaunchKernelGGL((k_sabdab<N, S, float>), dim3(ns), dim3(64), 0, st, W, M);
+    else hipLaunchKernelGGL((k_sabdab<N, S, double>), dim3(ns), dim3(64), 0, st, W, M);
     prof_end(PROF_SABDAB, st);
     prof_begin(PROF_LAYER, st);
-    if (W.precision == 1) hipLaunchKernelGGL((k_layer<N, S, true>), dim3(nl), dim3(64), 0, st, W, M, in, counts + M);
-    else hipLaunchKernelGGL((k_layer<N, S, false>), dim3(nl), dim3(64), 0, st, W, M, in, counts + M);
+    if (W.precision == 1) hipLaunchKernelGGL((k_layer<N, S, true>), dim3(nl), dim3(64), 0, st, W, M);
+    else hipLaunchKernelGGL((k_layer<N, S, false>), dim3(nl), dim3(64), 0, st, W, M);
     prof_end(PROF_LAYER, st);
     prof_begin(PROF_BVP, st);
     hipLaunchKernelGGL((k_bvp<N, S>), dim3(nb), dim3(64), 0, st, W, M, in, counts + M, out, counts + M + 1);

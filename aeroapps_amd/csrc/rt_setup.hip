@@ -783,6 +783,9 @@ __global__ void k_item(RTWork W, int nitems, int* active, int* nactive) {
   W.nfour[it] = 0;
   const int slot = atomicAdd(nactive, 1);
   active[slot] = it;
+  const int off = atomicAdd(W.lcnt, L);
+  W.loff[it] = off;
+  for (int l = 0; l < L; ++l) W.lown[off + l] = it;
 }
 
 }  // namespace

@@ -73,6 +73,13 @@ struct RTWork {
   int* testconv = nullptr;
   int* fsaved = nullptr;
   int* nfour = nullptr;            // Fourier terms done
+  // the active (item, layer) pairs of the current Fourier term, numbered
+  // densely: item it owns slots loff[it] .. loff[it] + L - 1, lown[slot] = it;
+  // lcnt[M] is the number of slots of term M (filled where the term's item
+  // list is built, so columns of different layer counts leave no idle lanes)
+  int* lown = nullptr;
+  int* loff = nullptr;
+  int* lcnt = nullptr;
   double* csza = nullptr;
   double* usec = nullptr;
   double* tsb = nullptr;           // transmittance of the whole slant solar path (0 if cut off)

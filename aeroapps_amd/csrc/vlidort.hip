@@ -285,6 +285,8 @@ size_t carve_work(Carver& C, RTWork& W, int cap) {
   W.testconv = C.take<int>(cap);
   W.fsaved = C.take<int>(cap);
   W.nfour = C.take<int>(cap);
+  W.loff = C.take<int>(cap);
+  W.lown = C.take<int>((size_t)cap * km);
   W.csza = C.take<double>(cap);
   W.usec = C.take<double>(cap);
   W.tsb = C.take<double>(cap);
@@ -390,7 +392,7 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
     dbt = C.take<double>(std::max<size_t>(1, btab.size()));
     *l0 = C.take<int>(cap);
     *l1 = C.take<int>(cap);
-    *cnt = C.take<int>(W.nmoments + 2);
+    *cnt = C.take<int>(2 * (W.nmoments + 2));   // items, then (item, layer) slots, per Fourier term
     return C.off;
   };
   double* dq; int *l0, *l1, *cnt;
@@ -419,10 +421,11 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   }
   int* lists[2] = {l0, l1};
   W.item_list = d_items;
+  W.lcnt = cnt + (W.nmoments + 2);
   for (int item0 = g0; item0 < g1; item0 += cap) {
     const int ni = std::min(cap, g1 - item0);
     W.item0 = item0;
-    AA_CHECK(hipMemsetAsync(cnt, 0, sizeof(int) * (W.nmoments + 2), st));
+    AA_CHECK(hipMemsetAsync(cnt, 0, sizeof(int) * 2 * (W.nmoments + 2), st));
     if (rt_setup(W, ni, lists[0], cnt, st)) return AA_EHIP;
     int e = rt_fourier(W, ni, lists, cnt, st);
     if (e) return e == -4 ? AA_EDIMS : AA_EHIP;

@@ -655,10 +655,27 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
   const int lane = threadIdx.x, g = lane / G, t = lane - g * G;
   const unsigned long long gm = (G == 64) ? ~0ull : (((1ull << (G & 63)) - 1ull) << (g * G));
   const unsigned long long below = (1ull << lane) - 1ull;
-  const int ai = blockIdx.x * IPW + g;
-  bool valid = ai < *nactive;
-  if (__ballot(valid) == 0ull) return;
-  const int it = valid ? active[ai] : active[blockIdx.x * IPW];
+  // this wavefront's IPW items: the term's items are in LBINS lists by layer
+  // count, each list's wavefronts in turn
+  int first = -1, cntb = 0;
+  const int* lst = active;
+  {
+    int acc = 0;
+#pragma unroll
+    for (int b = 0; b < LBINS; ++b) {
+      const int c = nactive[b], nbk = (c + IPW - 1) / IPW;
+      if (first < 0 && (int)blockIdx.x < acc + nbk) {
+        first = ((int)blockIdx.x - acc) * IPW;
+        cntb = c;
+        lst = active + (size_t)b * W.cap;
+      }
+      acc += nbk;
+    }
+  }
+  if (first < 0) return;   // uniform over the wavefront
+  const int ai = first + g;
+  bool valid = ai < cntb;
+  const int it = valid ? lst[ai] : lst[first];
   const int km = W.km;
   const int gi = global_item(W, it), obs = gi / W.nch, ch = gi - obs * W.nch;
   const size_t oi = (size_t)ch * W.nobs + obs;
@@ -959,59 +976,63 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
   for (int o = 0; o < S; ++o) cum[o] = ia[o] + dsum[o];
 
 
-  // ---------------- Fourier convergence (VLIDORT_CONVERGE_OBSGEO)
-  if (t != 0 || !valid) return;
-  const double flux_multiplier = (M == 0) ? 1.0 : 2.0;
-  double* stk = W.stokes + (size_t)it * 3;
-  bool done = false;
-  if (sing) {
-    W.status[it] = 4;
-    W.radiance[oi] = W.missing; W.reflectance[oi] = W.missing;
-    if (S > 1) { W.qout[oi] = W.missing; W.uout[oi] = W.missing; }
-    W.nfourier[oi] = 0;
-    return;
+  // ---------------- Fourier convergence (VLIDORT_CONVERGE_OBSGEO), on the
+  // group's lane 0; an item that goes on gets its place in the next term's
+  // list and its (item, layer) slots, which the group's lanes then fill
+  bool goes_on = false;
+  if (t == 0 && valid) {
+    const double flux_multiplier = (M == 0) ? 1.0 : 2.0;
+    double* stk = W.stokes + (size_t)it * 3;
+    bool done = false;
+    if (sing) {
+      W.status[it] = 4;
+      W.radiance[oi] = W.missing; W.reflectance[oi] = W.missing;
+      if (S > 1) { W.qout[oi] = W.missing; W.uout[oi] = W.missing; }
+      W.nfourier[oi] = 0;
+    } else {
+      double sf[S];
+#pragma unroll
+      for (int o = 0; o < S; ++o) sf[o] = flux_multiplier * cum[o];
+      int testconv = W.testconv[it], fsaved = W.fsaved[it];
+      if (M == 0) {
+#pragma unroll
+        for (int o = 0; o < S; ++o) stk[o] = sf[o] + W.ss[(size_t)it * 3 + o] + W.db[(size_t)it * 3 + o];
+      } else {
+        const double azm = W.raa[obs] * (double)M * DEG_TO_RAD;
+        const double azmfac[3] = {cos(azm), cos(azm), sin(azm)};
+        double tazm[S], tnew[S];
+#pragma unroll
+        for (int o = 0; o < S; ++o) { tazm[o] = azmfac[o] * sf[o]; tnew[o] = stk[o] + tazm[o]; }
+        int count = 0;
+        if (tazm[0] != 0.0) { const double accur = fabs(tazm[0] / tnew[0]); if (accur < VLIDORT_ACCURACY_D) count++; }
+        else count++;
+#pragma unroll
+        for (int o = 0; o < S; ++o) stk[o] = tnew[o];
+        if (count == 1) {
+          testconv++;
+          if (testconv == 2) { done = true; fsaved = M; }
+        } else { testconv = 0; fsaved = 2 * N - 1; }
+      }
+      W.testconv[it] = testconv;
+      W.fsaved[it] = fsaved;
+      if (M >= W.nmoments) done = true;
+      if (done) {
+        const float pif = 4.0f * atanf(1.0f);
+        const double pi = (double)pif;
+        const double I = stk[0];
+        W.radiance[oi] = I;
+        W.reflectance[oi] = (pi * I) / (cos(W.sza[obs] * pi / (double)180.0f) * ff);
+        if (S > 1) { W.qout[oi] = stk[1]; W.uout[oi] = stk[2]; }
+        W.nfourier[oi] = fsaved + 1;
+      } else {
+        goes_on = true;
+      }
+    }
   }
-  double sf[S];
-#pragma unroll
-  for (int o = 0; o < S; ++o) sf[o] = flux_multiplier * cum[o];
-  int testconv = W.testconv[it], fsaved = W.fsaved[it];
-  if (M == 0) {
-#pragma unroll
-    for (int o = 0; o < S; ++o) stk[o] = sf[o] + W.ss[(size_t)it * 3 + o] + W.db[(size_t)it * 3 + o];
-  } else {
-    const double azm = W.raa[obs] * (double)M * DEG_TO_RAD;
-    const double azmfac[3] = {cos(azm), cos(azm), sin(azm)};
-    double tazm[S], tnew[S];
-#pragma unroll
-    for (int o = 0; o < S; ++o) { tazm[o] = azmfac[o] * sf[o]; tnew[o] = stk[o] + tazm[o]; }
-    int count = 0;
-    if (tazm[0] != 0.0) { const double accur = fabs(tazm[0] / tnew[0]); if (accur < VLIDORT_ACCURACY_D) count++; }
-    else count++;
-#pragma unroll
-    for (int o = 0; o < S; ++o) stk[o] = tnew[o];
-    if (count == 1) {
-      testconv++;
-      if (testconv == 2) { done = true; fsaved = M; }
-    } else { testconv = 0; fsaved = 2 * N - 1; }
-  }
-  W.testconv[it] = testconv;
-  W.fsaved[it] = fsaved;
-  if (M >= W.nmoments) done = true;
-  if (done) {
-    const float pif = 4.0f * atanf(1.0f);
-    const double pi = (double)pif;
-    const double I = stk[0];
-    W.radiance[oi] = I;
-    W.reflectance[oi] = (pi * I) / (cos(W.sza[obs] * pi / (double)180.0f) * ff);
-    if (S > 1) { W.qout[oi] = stk[1]; W.uout[oi] = stk[2]; }
-    W.nfourier[oi] = fsaved + 1;
-  } else {
-    const int slot = atomicAdd(nnext, 1);
-    next[slot] = it;
-    const int off = atomicAdd(W.lcnt + M + 1, L);
-    W.loff[it] = off;
-    for (int l = 0; l < L; ++l) W.lown[off + l] = it;
-  }
+  const int noff = append_items(W, it, L, goes_on, nnext, next, W.lcnt + M + 1);
+  const int goff = __builtin_amdgcn_ds_bpermute(4 * (lane - t), noff);
+  if (goff >= 0)
+    for (int l = t; l < L; l += G) W.lown[goff + l] = it;
 }
 
 // ------------------------------------------------------------------ k_sabdab
@@ -1133,7 +1154,7 @@ template <int N, int S>
 int fourier_loop(const RTWork& W, int nitems, int* const lists[2], int* counts, hipStream_t st) {
   const int nl = (int)(((size_t)nitems * W.km + 63) / 64);
   const int ns = (int)(((size_t)nitems * W.km + 16 * sab_groups(N, S) - 1) / (16 * sab_groups(N, S)));
-  const int nb = (nitems + 64 / bvp_group(N * S) - 1) / (64 / bvp_group(N * S));
+  const int nb = (nitems + 64 / bvp_group(N * S) - 1) / (64 / bvp_group(N * S)) + LBINS;
   for (int M = 0; M <= W.nmoments; ++M) {
     const int* in = lists[M & 1];
     int* out = lists[(M + 1) & 1];
@@ -1146,7 +1167,8 @@ int fourier_loop(const RTWork& W, int nitems, int* const lists[2], int* counts, 
     else hipLaunchKernelGGL((k_layer<N, S, false>), dim3(nl), dim3(64), 0, st, W, M);
     prof_end(PROF_LAYER, st);
     prof_begin(PROF_BVP, st);
-    hipLaunchKernelGGL((k_bvp<N, S>), dim3(nb), dim3(64), 0, st, W, M, in, counts + M, out, counts + M + 1);
+    hipLaunchKernelGGL((k_bvp<N, S>), dim3(nb), dim3(64), 0, st, W, M, in, counts + M * LBINS, out,
+                       counts + (M + 1) * LBINS);
     prof_end(PROF_BVP, st);
   }
   AA_CHECK(hipGetLastError());

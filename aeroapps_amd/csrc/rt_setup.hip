@@ -781,11 +781,23 @@ __global__ void k_item(RTWork W, int nitems, int* active, int* nactive) {
   W.testconv[it] = 0;
   W.fsaved[it] = 0;
   W.nfour[it] = 0;
-  const int slot = atomicAdd(nactive, 1);
-  active[slot] = it;
-  const int off = atomicAdd(W.lcnt, L);
-  W.loff[it] = off;
-  for (int l = 0; l < L; ++l) W.lown[off + l] = it;
+}
+
+// The first Fourier term's item lists and (item, layer) slots: one thread per
+// item, wavefront-aggregated appends.
+__global__ __launch_bounds__(64) void k_enlist(RTWork W, int nitems, int* active, int* nactive) {
+  const int it = blockIdx.x * 64 + threadIdx.x;
+  const bool take = it < nitems && W.status[it] == 0;
+  append_items(W, it, take ? W.L_it[it] : 0, take, nactive, active, W.lcnt);
+}
+
+// Owners of the first Fourier term's (item, layer) slots: one thread per
+// (item, layer), so a wavefront writes consecutive slots.
+__global__ void k_owner(RTWork W, int nitems) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int it = t / W.km, l = t - it * W.km;
+  if (it >= nitems || W.status[it] != 0 || l >= W.L_it[it]) return;
+  W.lown[W.loff[it] + l] = it;
 }
 
 }  // namespace
@@ -830,6 +842,11 @@ int rt_setup(const RTWork& W, int nitems, int* d_active, int* d_nactive, hipStre
   prof_begin(PROF_ITEM, st);
   if (W.S == 1) hipLaunchKernelGGL((k_item<1>), dim3(gi), dim3(B), 0, st, W, nitems, d_active, d_nactive);
   else hipLaunchKernelGGL((k_item<3>), dim3(gi), dim3(B), 0, st, W, nitems, d_active, d_nactive);
+  hipLaunchKernelGGL(k_enlist, dim3((nitems + 63) / 64), dim3(64), 0, st, W, nitems, d_active, d_nactive);
+  {
+    const size_t go = ((size_t)nitems * W.km + 255) / 256;
+    hipLaunchKernelGGL(k_owner, dim3((unsigned)go), dim3(256), 0, st, W, nitems);
+  }
   prof_end(PROF_ITEM, st);
   AA_CHECK(hipGetLastError());
   return 0;

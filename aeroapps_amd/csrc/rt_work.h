@@ -23,6 +23,16 @@ enum TrField {
   TR_NFIELDS
 };
 
+// The active items of a Fourier term are kept in LBINS lists by layer count
+// (list k at list + k * cap, count counts[k]), so that the boundary-value
+// solve, which sweeps a wavefront's items over the largest of their layer
+// counts, gets wavefronts of items of similar height.
+constexpr int LBINS = 8;
+__host__ __device__ inline int lbin(int L, int km) {
+  const int b = (L - 1) * LBINS / km;
+  return b < 0 ? 0 : (b >= LBINS ? LBINS - 1 : b);
+}
+
 struct RTWork {
   // ---- call description
   int km = 0;          // layers allocated per column (Fortran km)
@@ -218,5 +228,42 @@ struct QOff {
   }
   __host__ __device__ constexpr int size() const { return 2 * N + 2 * N * 2 * N * N * S * S; }
 };
+
+
+// Appends the wavefront's lanes with `take` set to a Fourier term's item
+// lists (list + b * cap for layer-count bin b, counts[b]) and numbers their
+// (item, layer) slots from *lcount, with one atomic per wavefront and bin
+// instead of one per item.  Every lane of the wavefront must call it.
+// Returns the item's first slot (taking lanes) or -1.
+__device__ inline int append_items(const RTWork& W, int it, int L, bool take, int* counts, int* list, int* lcount) {
+  if (__ballot(take) == 0ull) return -1;
+  const int lane = (int)(threadIdx.x & 63u);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const int mine = take ? L : 0;
+  int v = mine;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  const int tot = __shfl(v, 63, 64);
+  int base = 0;
+  if (lane == 0) base = atomicAdd(lcount, tot);
+  base = __shfl(base, 0, 64);
+  const int off = base + v - mine;
+  const int b = lbin(L, W.km);
+#pragma unroll
+  for (int k = 0; k < LBINS; ++k) {
+    const unsigned long long m = __ballot(take && b == k);
+    if (m == 0ull) continue;
+    const int leader = __ffsll((long long)m) - 1;
+    int pos = 0;
+    if (lane == leader) pos = atomicAdd(counts + k, __popcll(m));
+    pos = __shfl(pos, leader, 64);
+    if (take && b == k) list[(size_t)k * W.cap + pos + __popcll(m & below)] = it;
+  }
+  if (take) W.loff[it] = off;
+  return take ? off : -1;
+}
 
 }  // namespace aa

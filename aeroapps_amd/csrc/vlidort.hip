@@ -390,9 +390,10 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
     dy = C.take<float>(std::max<size_t>(1, ytab.size()));
     dy64 = C.take<double>(std::max<size_t>(1, ytab64.size()));
     dbt = C.take<double>(std::max<size_t>(1, btab.size()));
-    *l0 = C.take<int>(cap);
-    *l1 = C.take<int>(cap);
-    *cnt = C.take<int>(2 * (W.nmoments + 2));   // items, then (item, layer) slots, per Fourier term
+    *l0 = C.take<int>((size_t)LBINS * cap);
+    *l1 = C.take<int>((size_t)LBINS * cap);
+    // per Fourier term: items of each layer-count list, then (item, layer) slots
+    *cnt = C.take<int>((LBINS + 1) * (W.nmoments + 2));
     return C.off;
   };
   double* dq; int *l0, *l1, *cnt;
@@ -421,11 +422,11 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   }
   int* lists[2] = {l0, l1};
   W.item_list = d_items;
-  W.lcnt = cnt + (W.nmoments + 2);
+  W.lcnt = cnt + LBINS * (W.nmoments + 2);
   for (int item0 = g0; item0 < g1; item0 += cap) {
     const int ni = std::min(cap, g1 - item0);
     W.item0 = item0;
-    AA_CHECK(hipMemsetAsync(cnt, 0, sizeof(int) * 2 * (W.nmoments + 2), st));
+    AA_CHECK(hipMemsetAsync(cnt, 0, sizeof(int) * (LBINS + 1) * (W.nmoments + 2), st));
     if (rt_setup(W, ni, lists[0], cnt, st)) return AA_EHIP;
     int e = rt_fourier(W, ni, lists, cnt, st);
     if (e) return e == -4 ? AA_EDIMS : AA_EHIP;

@@ -33,6 +33,9 @@ SIGNATURES = {
     "aa_mie_channels": (_ci, [_vp, _fp]),
     "aa_mie_tracer_table": (_ci, [_vp, _ci, _ip, _ip, _ip, _ip, _ip, _ip]),
     "aa_mie_registry": (_ci, [_cp, _ci, _ip, _cp, _ip, _ip]),
+    "aa_set_tuning": (_ci, [_ci, _ci, _ci, _vp]),
+    "aa_get_tuning": (_ci, [_ci, _ci, _ci, _vp]),
+    "aa_load_tuning": (_ci, [_cp]),
     "aa_mie_create_rc": (_ci, [ctypes.POINTER(_vp), _cp, _ci, _cp, _ip, _vp, _vp, _ip]),
     "aa_mie_destroy": (_ci, [_vp]),
     "aa_get_aop_vector": (_ci, [_vp, _ci, _ci, _ci, _ci, _fp, _cp, _ci, _fp, _fp, _ci, _ci, _fp, _fp, _fp, _fp, _ip]),
@@ -80,7 +83,7 @@ SIGNATURES = {
 
 ERRORS = {-1: "AA_EARG (bad argument)", -2: "AA_EHIP (HIP runtime error)", -3: "AA_ENOTABLE (no table)",
           -4: "AA_EDIMS (beyond compiled maxima)", -5: "AA_EUNSUP (option not implemented)",
-          -6: "AA_ENOTRES (tables not resident)"}
+          -7: "AA_ENOFILE (a file cannot be read)", -6: "AA_ENOTRES (tables not resident)"}
 
 _LIB = None
 

@@ -546,8 +546,8 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M) {
 }
 
 // ------------------------------------------------------------------ k_bvp
-// Group reductions over the G lanes that hold one item's rows (G = 8, 16, 32
-// or 64; every lane of the group ends with the same value).
+// Group reductions over the G lanes that hold one item's rows (G = 4, 8, 16,
+// 32 or 64; every lane of the group ends with the same value).
 // Values of the two 16-lane rows of each 32-lane half, in both rows
 // (v_permlane16_swap on x against itself): for every lane, {r.x, r.y} holds
 // the value of its own row and of the partner row at the same position.
@@ -561,7 +561,7 @@ template <int G>
 __device__ __forceinline__ double grp_sum(double x) {
   x += dpp_d<0xB1>(x);
   x += dpp_d<0x4E>(x);
-  x += dpp_d<0x141>(x);
+  if (G >= 8) x += dpp_d<0x141>(x);
   if (G >= 16) x += dpp_d<0x140>(x);
   if (G >= 32) { double p, q; xrow16_d(x, p, q); x = p + q; }
   if (G == 64) x = readlane_d(x, 0) + readlane_d(x, 32);
@@ -573,7 +573,7 @@ template <int G>
 __device__ __forceinline__ int grp_min_i(int x) {
   x = min(x, __builtin_amdgcn_update_dpp(0x7fffffff, x, 0xB1, 0xF, 0xF, false));
   x = min(x, __builtin_amdgcn_update_dpp(0x7fffffff, x, 0x4E, 0xF, 0xF, false));
-  x = min(x, __builtin_amdgcn_update_dpp(0x7fffffff, x, 0x141, 0xF, 0xF, false));
+  if (G >= 8) x = min(x, __builtin_amdgcn_update_dpp(0x7fffffff, x, 0x141, 0xF, 0xF, false));
   if (G >= 16) x = min(x, __builtin_amdgcn_update_dpp(0x7fffffff, x, 0x140, 0xF, 0xF, false));
   if (G >= 32) {
     const auto r = __builtin_amdgcn_permlane16_swap((unsigned)x, (unsigned)x, false, false);
@@ -588,7 +588,7 @@ template <int G>
 __device__ __forceinline__ unsigned grp_max_u(unsigned x) {
   x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false));
   x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false));
-  x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false));
+  if (G >= 8) x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false));
   if (G >= 16) x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false));
   if (G >= 32) {
     const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
@@ -598,14 +598,10 @@ __device__ __forceinline__ unsigned grp_max_u(unsigned x) {
   return x;
 }
 
-// Lanes per item in k_bvp (G) and panel rows per lane (SL): the 3n rows of a
-// panel fill SL row slots per lane (8 lanes and up to 3 slots for n <= 8, so
-// a wavefront carries 8 items), and the adjoint accumulation of the TOA
-// radiance puts ceil(2n / G) unknowns of a layer on each lane.
-__host__ __device__ constexpr int bvp_group(int n) {
-  return n <= 8 ? 8 : (2 * n <= 16 ? 16 : (2 * n <= 32 ? 32 : 64));
-}
-__host__ __device__ constexpr int bvp_slots(int n) { return (3 * n + bvp_group(n) - 1) / bvp_group(n); }
+// Lanes per item in k_bvp (G, bvp_group / bvp_group_alt) and panel rows per
+// lane (SL): the 3n rows of a panel fill SL row slots per lane (8 lanes and
+// up to 3 slots for n <= 8, so a wavefront carries 8 items); the adjoint
+// accumulation of the TOA radiance updates columns t + G b on lane t.
 
 // A pivot row goes through LDS from column (i + 1) & ~1 (the columns right of
 // its pivot, from an even column so the pairs stay 16-byte aligned).
@@ -631,14 +627,14 @@ __device__ __forceinline__ int grp_pivot(const bool (&eq)[SL], int g, int t, boo
   return (cmin == NONE) ? -1 : g * G + (cmin & 63);
 }
 
-template <int N, int S>
+template <int N, int S, int G>
 __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int M, const int* __restrict__ active,
                                             const int* __restrict__ nactive, int* __restrict__ next,
                                             int* __restrict__ nnext) {
   constexpr int n = N * S, n2 = 2 * n, NC = 4 * n, RH = NC, UW = NC + 2;
-  constexpr int G = bvp_group(n), IPW = 64 / G, SL = bvp_slots(n);
+  constexpr int IPW = 64 / G, SL = bvp_slots(n, G);
   constexpr int NAC = (NC + G - 1) / G;   // accumulator columns per lane in the adjoint of the TOA recursion
-  static_assert(2 * n <= 64 && SL <= 3 && n <= G, "panel rows exceed the wavefront");
+  static_assert(2 * n <= 64 && SL <= 5 && G >= 4, "panel rows exceed the wavefront");
   constexpr LOff O(N, S);
   constexpr BOff BO(N, n);
   constexpr int BIG = 0x3fffffff;
@@ -754,15 +750,21 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
   double ptr = 1.0;   // P_k
   const double kmult = brdf ? ((M == 0) ? 2.0 : 1.0) : 2.0 * albedo;
   const double* __restrict__ trr = W.tr + (size_t)it * km * TR_NFIELDS;
-  if (valid && t < n) {   // TOA rows (BVP_MATRIX_SETUP, upper boundary), layer 0
-    const int i = t / S, o1 = t - (t / S) * S;
+  // TOA rows (BVP_MATRIX_SETUP, upper boundary), layer 0: row r to lane
+  // r % G, slot r / G
 #pragma unroll
-    for (int q = 0; q < n; ++q) a[0][q] = lb[O.xp(i, o1, q)];
+  for (int sl = 0; sl < SL; ++sl) {
+    const int r = t + G * sl;
+    if (valid && r < n) {
+      const int i = r / S, o1 = r - (r / S) * S;
 #pragma unroll
-    for (int q = 0; q < n; ++q) a[0][n + q] = lb[O.xn(i, o1, q)] * lb[O.tdel(q)];
-    a[0][RH] = -lb[O.wu(i, o1)];
-    act[0] = true;
-    posl[0] = t;
+      for (int q = 0; q < n; ++q) a[sl][q] = lb[O.xp(i, o1, q)];
+#pragma unroll
+      for (int q = 0; q < n; ++q) a[sl][n + q] = lb[O.xn(i, o1, q)] * lb[O.tdel(q)];
+      a[sl][RH] = -lb[O.wu(i, o1)];
+      act[sl] = true;
+      posl[sl] = r;
+    }
   }
 
   // one panel: eliminate its 2n columns; LAST: the bottom layer (no layer k+1)
@@ -1047,15 +1049,12 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
 // vector path) keeps one group and loads the A tiles as it goes.
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef double d4v __attribute__((ext_vector_type(4)));
-__host__ __device__ constexpr int sab_tiles(int N, int S) {
-  return ((2 * N * S * N * S + 15) / 16) * ((2 * N * S * S + 3) / 4);
-}
-__host__ __device__ constexpr int sab_groups(int N, int S) { return sab_tiles(N, S) <= 32 ? 4 : 1; }
+
 // T = float: the mixed path (v_mfma_f32_16x16x4f32); T = double: the FP64
 // path (v_mfma_f64_16x16x4f64).  Both take A and B in the same lane layout;
 // the accumulator rows of lane l are 4*(l/16) + v for f32 but l/16 + 4*v for
 // f64 (v = 0..3; checked on the device with a probe of known products).
-template <int N, int S, typename T>
+template <int N, int S, typename T, int PG>
 __global__ __launch_bounds__(64) void k_sabdab(RTWork W, int M) {
   constexpr bool F64 = std::is_same<T, double>::value;
   using V4 = typename std::conditional<F64, d4v, f4v>::type;
@@ -1063,7 +1062,7 @@ __global__ __launch_bounds__(64) void k_sabdab(RTWork W, int M) {
   constexpr YOff YO(N, S);
   constexpr int ROWS = YO.rows(), KK = YO.K;
   constexpr int RT = (ROWS + 15) / 16, KT = (KK + 3) / 4;
-  constexpr int SAB_PG = sab_groups(N, S);
+  constexpr int SAB_PG = PG;
   constexpr bool PRE = sab_tiles(N, S) <= 32;   // A tiles held in registers
   const int lane = threadIdx.x;
   const int km = W.km;
@@ -1150,29 +1149,45 @@ __global__ __launch_bounds__(64) void k_sabdab(RTWork W, int M) {
   }
 }
 
-template <int N, int S>
-int fourier_loop(const RTWork& W, int nitems, int* const lists[2], int* counts, hipStream_t st) {
+// One Fourier-term sweep with launch shapes (G lanes per item in k_bvp, PG
+// 16-pair groups per wavefront in k_sabdab); every shape gives the same bits.
+template <int N, int S, int G, int PG>
+int fourier_loop_shape(const RTWork& W, int nitems, int* const lists[2], int* counts, hipStream_t st) {
   const int nl = (int)(((size_t)nitems * W.km + 63) / 64);
-  const int ns = (int)(((size_t)nitems * W.km + 16 * sab_groups(N, S) - 1) / (16 * sab_groups(N, S)));
-  const int nb = (nitems + 64 / bvp_group(N * S) - 1) / (64 / bvp_group(N * S)) + LBINS;
+  const int ns = (int)(((size_t)nitems * W.km + 16 * PG - 1) / (16 * PG));
+  const int nb = (nitems + 64 / G - 1) / (64 / G) + LBINS;
   for (int M = 0; M <= W.nmoments; ++M) {
     const int* in = lists[M & 1];
     int* out = lists[(M + 1) & 1];
     prof_begin(PROF_SABDAB, st);
-    if (W.precision == 1) hipLaunchKernelGGL((k_sabdab<N, S, float>), dim3(ns), dim3(64), 0, st, W, M);
-    else hipLaunchKernelGGL((k_sabdab<N, S, double>), dim3(ns), dim3(64), 0, st, W, M);
+    if (W.precision == 1) hipLaunchKernelGGL((k_sabdab<N, S, float, PG>), dim3(ns), dim3(64), 0, st, W, M);
+    else hipLaunchKernelGGL((k_sabdab<N, S, double, PG>), dim3(ns), dim3(64), 0, st, W, M);
     prof_end(PROF_SABDAB, st);
     prof_begin(PROF_LAYER, st);
     if (W.precision == 1) hipLaunchKernelGGL((k_layer<N, S, true>), dim3(nl), dim3(64), 0, st, W, M);
     else hipLaunchKernelGGL((k_layer<N, S, false>), dim3(nl), dim3(64), 0, st, W, M);
     prof_end(PROF_LAYER, st);
     prof_begin(PROF_BVP, st);
-    hipLaunchKernelGGL((k_bvp<N, S>), dim3(nb), dim3(64), 0, st, W, M, in, counts + M * LBINS, out,
+    hipLaunchKernelGGL((k_bvp<N, S, G>), dim3(nb), dim3(64), 0, st, W, M, in, counts + M * LBINS, out,
                        counts + (M + 1) * LBINS);
     prof_end(PROF_BVP, st);
   }
   AA_CHECK(hipGetLastError());
   return 0;
+}
+
+// The launch shapes of W.shape_bvp / W.shape_sab (0: default, 1: the
+// alternative the tuner may pick).
+template <int N, int S>
+int fourier_loop(const RTWork& W, int nitems, int* const lists[2], int* counts, hipStream_t st) {
+  constexpr int n = N * S;
+  constexpr int G0 = bvp_group(n), G1 = bvp_group_alt(n);
+  constexpr int P0 = sab_groups(N, S), P1 = sab_groups_alt(N, S);
+  const bool gb = W.shape_bvp == 1, gs = W.shape_sab == 1;
+  if (!gb && !gs) return fourier_loop_shape<N, S, G0, P0>(W, nitems, lists, counts, st);
+  if (gb && !gs) return fourier_loop_shape<N, S, G1, P0>(W, nitems, lists, counts, st);
+  if (!gb && gs) return fourier_loop_shape<N, S, G0, P1>(W, nitems, lists, counts, st);
+  return fourier_loop_shape<N, S, G1, P1>(W, nitems, lists, counts, st);
 }
 
 }  // namespace

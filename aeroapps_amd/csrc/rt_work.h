@@ -88,6 +88,9 @@ struct RTWork {
   // lcnt[M] is the number of slots of term M (filled where the term's item
   // list is built, so columns of different layer counts leave no idle lanes)
   int* lown = nullptr;
+  // launch shapes of the Fourier-term kernels (0 default, 1 alternative):
+  // lanes per item of the band solver, pair groups of the SAB/DAB kernel
+  int shape_bvp = 0, shape_sab = 0;
   int* loff = nullptr;
   int* lcnt = nullptr;
   double* csza = nullptr;
@@ -229,6 +232,28 @@ struct QOff {
   __host__ __device__ constexpr int size() const { return 2 * N + 2 * N * 2 * N * N * S * S; }
 };
 
+
+// Launch shapes of the Fourier-term kernels.  Lanes per item of the band
+// solver: 8 for n = N*S <= 8 (8 items per wavefront), the smallest power of
+// two holding 2n rows above; the alternative the tuner may pick: 4 lanes (16
+// items per wavefront) for n <= 4, 16 lanes (4 items) for 5 <= n <= 8.
+__host__ __device__ constexpr int bvp_group(int n) {
+  return n <= 8 ? 8 : (2 * n <= 16 ? 16 : (2 * n <= 32 ? 32 : 64));
+}
+__host__ __device__ constexpr int bvp_group_alt(int n) { return n <= 4 ? 4 : (n <= 8 ? 16 : bvp_group(n)); }
+__host__ __device__ constexpr int bvp_slots(int n, int G) { return (3 * n + G - 1) / G; }
+// SAB/DAB kernel: 16-pair groups per wavefront (4 when the A tiles fit in
+// registers, else 1; the alternative: 1).
+__host__ __device__ constexpr int sab_tiles(int N, int S) {
+  return ((2 * N * S * N * S + 15) / 16) * ((2 * N * S * S + 3) / 4);
+}
+__host__ __device__ constexpr int sab_groups(int N, int S) { return sab_tiles(N, S) <= 32 ? 4 : 1; }
+__host__ __device__ constexpr int sab_groups_alt(int, int) { return 1; }
+
+// Launch shapes (0 default, 1 alternative) and work-chunk cap of one
+// (streams, Stokes, precision) class, from the library's tuning table.
+struct TuneEntry { int shape_bvp = 0, shape_sab = 0; long chunk_items = 0; };
+TuneEntry tuning_for(int N, int S, int precision);
 
 // Appends the wavefront's lanes with `take` set to a Fourier term's item
 // lists (list + b * cap for layer-count bin b, counts[b]) and numbers their

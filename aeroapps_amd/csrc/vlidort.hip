@@ -327,6 +327,8 @@ int work_cap_items(const RTWork& W, size_t* cap) {
   *cap = std::max<size_t>(1, budget / bytes_per_item(W));
   const size_t envcap = env_bytes("AA_CHUNK_ITEMS", 0);
   if (envcap) *cap = std::min(*cap, envcap);
+  const TuneEntry tu = tuning_for(W.N, W.S, W.precision);
+  if (tu.chunk_items > 0) *cap = std::min(*cap, (size_t)tu.chunk_items);
   return 0;
 }
 
@@ -352,6 +354,11 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
               size_t fixed_bytes = 0) {
   const int nitems = g1 - g0;
   if (nitems <= 0) return 0;
+  {
+    const TuneEntry tu = tuning_for(W.N, W.S, W.precision);
+    W.shape_bvp = tu.shape_bvp;
+    W.shape_sab = tu.shape_sab;
+  }
   const std::vector<double>& qtab = quad_table_cached(W.N, W.S, W.nmoments);
   static const std::vector<float> no_ytab;
   const std::vector<float>& ytab = (W.precision == 1) ? ytab_cached(W.N, W.S, W.nmoments) : no_ytab;
@@ -370,7 +377,7 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
                            std::max<size_t>(1, btab.size()) * sizeof(double) + sizeof(int) * (W.nmoments + 2) +
                            16 * 256;
   if (fixed_base) {
-    const size_t per = bytes_per_item(W) + 2 * sizeof(int);
+    const size_t per = bytes_per_item(W) + 2 * LBINS * sizeof(int);
     if (fixed_bytes <= tab_bytes + per) return AA_EDIMS;
     capi = (fixed_bytes - tab_bytes) / per;
   } else if (work_cap_items(W, &capi)) {

@@ -33,7 +33,8 @@ enum {
   AA_ENOTABLE = -3,    /* table handle has no species table */
   AA_EDIMS = -4,       /* dimension beyond the compiled maxima (streams, layers, moments) */
   AA_EUNSUP = -5,      /* a reference option this library does not implement (DO_BOA) */
-  AA_ENOTRES = -6      /* optics tables not yet resident on the device (see the _noalloc entry) */
+  AA_ENOTRES = -6,     /* optics tables not yet resident on the device (see the _noalloc entry) */
+  AA_ENOFILE = -7      /* a file cannot be read */
 };
 
 /* Reference return codes reproduced in the rc out-arguments. */
@@ -417,6 +418,29 @@ int aa_column_radiance_dev_noalloc(aa_mie* t, int km, int nobs, int nch, int nq,
  * memory. */
 int aa_set_arena_bytes(size_t work_bytes, size_t optics_bytes);
 int aa_release_arenas(void);
+
+/* ------------------------------------------------------------------ launch shapes
+ * No counterpart in the reference.  Per (nstreams, nstokes, precision) class
+ * of the radiance kernels: bvp_lanes, the lanes per item of the band solver
+ * (8 for n = nstreams*nstokes <= 8, 16/32/64 above; the alternative 4 for
+ * n <= 4 or 16 for 5 <= n <= 8); sab_pairs, the (item, layer) pairs per
+ * wavefront of the SAB/DAB contraction (64 when its tiles fit in registers,
+ * else 16; the alternative 16); chunk_items, a cap on the items of one work
+ * chunk (0: the memory budget alone).  0 in a field keeps the default.  Every
+ * setting gives bit-identical results.  The library reads the table from
+ * tuning_gfx950.txt beside the shared library on first use (lines
+ * "nstreams nstokes precision bvp_lanes sab_pairs chunk_items", '#' comments;
+ * written by aeroapps_amd/tune.py from measurements); aa_load_tuning(path)
+ * replaces it from another file (NULL: that default file; AA_ENOFILE when
+ * unreadable, AA_EARG when malformed), aa_set_tuning() sets one class (t NULL:
+ * back to the defaults), aa_get_tuning() reports the shapes a class runs with.
+ * The table is process-wide; a call uses the entry current when it starts. */
+typedef struct aa_tuning {
+  int bvp_lanes, sab_pairs, chunk_items;
+} aa_tuning;
+int aa_set_tuning(int nstreams, int nstokes, int precision, const aa_tuning* t);
+int aa_get_tuning(int nstreams, int nstokes, int precision, aa_tuning* t);
+int aa_load_tuning(const char* path);
 
 /* Library version string. */
 const char* aa_version(void);

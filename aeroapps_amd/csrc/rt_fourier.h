@@ -346,16 +346,44 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M) {
     normv[k] = nv;
   }
 
-  // ---- Green's-function beam solution (:2649-2932)
-  double wu[2 * N][S], wl[2 * N][S];
-  double aterm[n], bterm[n], gamp[n], gamm[n];
+  // ---- layer record: X+-, transmittances and the homogeneous source
+  // coefficients now; the beam solution's terms are written as they are
+  // formed below (so that they hold no registers past their use)
+  double* lrec = W.lbuf + il * (size_t)W.lstride;
 #pragma unroll
-  for (int k = 0; k < n; ++k) { aterm[k] = 0.0; bterm[k] = 0.0; gamp[k] = 0.0; gamm[k] = 0.0; }
+  for (int i = 0; i < N; ++i)
 #pragma unroll
-  for (int i = 0; i < 2 * N; ++i)
+    for (int o = 0; o < S; ++o)
 #pragma unroll
-    for (int o = 0; o < S; ++o) { wu[i][o] = 0.0; wl[i][o] = 0.0; }
-  if (beam) {
+      for (int k = 0; k < n; ++k) lrec[O.xp(i, o, k)] = xp[i * S + o][k];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int o = 0; o < S; ++o)
+#pragma unroll
+      for (int k = 0; k < n; ++k) lrec[O.xn(i, o, k)] = xn[i * S + o][k];
+#pragma unroll
+  for (int k = 0; k < n; ++k) lrec[O.tdel(k)] = tdel[k];
+  // upwelling source: src(o) = sum_k lcon_k A(o,k) + mcon_k B(o,k) + C(o)
+#pragma unroll
+  for (int o = 0; o < S; ++o)
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+      lrec[O.coef(o, k)] = lsc ? updn[k][o] * hm2[k] : 0.0;
+      lrec[O.coef(o, n + k)] = lsc ? upup[k][o] * hm1[k] : 0.0;
+    }
+
+  // ---- Green's-function beam solution (:2649-2932); its particular source
+  // C(o) (spar) accumulates per eigenvector as the A, B terms are formed
+  double spar[S];
+#pragma unroll
+  for (int o = 0; o < S; ++o) spar[o] = 0.0;
+  if (!beam) {
+#pragma unroll
+    for (int i = 0; i < 2 * N; ++i)
+#pragma unroll
+      for (int o = 0; o < S; ++o) { lrec[O.wu(i, o)] = 0.0; lrec[O.wl(i, o)] = 0.0; }
+  } else {
     const int g = global_item(W, it);
     const int ch = g - obs * W.nch;
     const double F1 = (W.flux ? W.flux[(size_t)obs * W.nch + ch] : 1.0) / PI4;
@@ -364,12 +392,12 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M) {
 #pragma unroll
     for (int k = 0; k < n; ++k) {
       const double zdel = tdel[k], zwdel = zdel * wdel;
-      gamp[k] = secbar + keig[k];
-      gamm[k] = secbar - keig[k];
-      const double eps = gamm[k];
+      const double gamp = secbar + keig[k];
+      const double gamm = secbar - keig[k];
+      const double eps = gamm;
       if (fabs(eps) < TAYLOR_SMALL) cfunc[k] = taylor1(eps, dtau, wdel, cst);
-      else cfunc[k] = cst * (zdel - wdel) / gamm[k];
-      dfunc[k] = cst * (1.0 - zwdel) / gamp[k];
+      else cfunc[k] = cst * (zdel - wdel) / gamm;
+      dfunc[k] = cst * (1.0 - zwdel) / gamp;
     }
     TR hq[LM][S];
 #pragma unroll
@@ -399,8 +427,10 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M) {
         dpi[i][o1] = F1 * (double)sp; dmi[i][o1] = F1 * (double)sm;
       }
     double gup[n], gdn[n];
+    const double itw = -itrans * wdel;
 #pragma unroll
     for (int k = 0; k < n; ++k) {
+      const double gamp = secbar + keig[k], gamm = secbar - keig[k];
       double av = 0.0, bv = 0.0;
 #pragma unroll
       for (int i = 0; i < N; ++i) {
@@ -417,10 +447,16 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M) {
         h2 = h2 + h2b;
         av += qw[i] * h1; bv += qw[i] * h2;
       }
-      aterm[k] = (double)av / normv[k];
-      bterm[k] = (double)bv / normv[k];
-      gdn[k] = cfunc[k] * aterm[k];
-      gup[k] = dfunc[k] * bterm[k];
+      const double aterm = (double)av / normv[k];
+      const double bterm = (double)bv / normv[k];
+      gdn[k] = cfunc[k] * aterm;
+      gup[k] = dfunc[k] * bterm;
+      double sd;
+      if (fabs(gamm) < TAYLOR_SMALL) sd = itransu * taylor2(gamm, sigp, dtau, 1.0, wdel * tduser, 1.0);
+      else sd = (itrans * hm2[k] - emultup) / gamm;
+      const double su = (itw * hm1[k] + emultup) / gamp;
+#pragma unroll
+      for (int o = 0; o < S; ++o) spar[o] = spar[o] + updn[k][o] * sd * aterm + upup[k][o] * su * bterm;
     }
 #pragma unroll
     for (int o1 = 0; o1 < S; ++o1)
@@ -432,50 +468,13 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M) {
           su += gup[k] * xn[i * S + o1][k]; su1 += gup[k] * xp[i * S + o1][k];
           sl1 += gdn[k] * xn[i * S + o1][k]; sl += gdn[k] * xp[i * S + o1][k];
         }
-        wu[i][o1] = su; wu[i + N][o1] = (o1 > 1) ? -su1 : su1;
-        wl[i + N][o1] = (o1 > 1) ? -sl1 : sl1; wl[i][o1] = sl;
+        lrec[O.wu(i, o1)] = su; lrec[O.wu(i + N, o1)] = (o1 > 1) ? -su1 : su1;
+        lrec[O.wl(i + N, o1)] = (o1 > 1) ? -sl1 : sl1; lrec[O.wl(i, o1)] = sl;
       }
   }
+#pragma unroll
+  for (int o = 0; o < S; ++o) lrec[O.csrc(o)] = beam ? spar[o] : 0.0;
 
-  // ---- write the layer record
-  double* rec = W.lbuf + il * (size_t)W.lstride;
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int o = 0; o < S; ++o)
-#pragma unroll
-      for (int k = 0; k < n; ++k) rec[O.xp(i, o, k)] = xp[i * S + o][k];
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-#pragma unroll
-    for (int o = 0; o < S; ++o)
-#pragma unroll
-      for (int k = 0; k < n; ++k) rec[O.xn(i, o, k)] = xn[i * S + o][k];
-#pragma unroll
-  for (int k = 0; k < n; ++k) rec[O.tdel(k)] = tdel[k];
-#pragma unroll
-  for (int i = 0; i < 2 * N; ++i)
-#pragma unroll
-    for (int o = 0; o < S; ++o) { rec[O.wu(i, o)] = wu[i][o]; rec[O.wl(i, o)] = wl[i][o]; }
-  // upwelling source: src(o) = sum_k lcon_k A(o,k) + mcon_k B(o,k) + C(o)
-#pragma unroll
-  for (int o = 0; o < S; ++o) {
-    double spar = 0.0;
-#pragma unroll
-    for (int k = 0; k < n; ++k) {
-      rec[O.coef(o, k)] = lsc ? updn[k][o] * hm2[k] : 0.0;
-      rec[O.coef(o, n + k)] = lsc ? upup[k][o] * hm1[k] : 0.0;
-      if (beam) {
-        const double wdel = tdmubar, itw = -itrans * wdel;
-        double sd;
-        if (fabs(gamm[k]) < TAYLOR_SMALL) sd = itransu * taylor2(gamm[k], sigp, dtau, 1.0, wdel * tduser, 1.0);
-        else sd = (itrans * hm2[k] - emultup) / gamm[k];
-        const double su = (itw * hm1[k] + emultup) / gamp[k];
-        spar = spar + updn[k][o] * sd * aterm[k] + upup[k][o] * su * bterm[k];
-      }
-    }
-    rec[O.csrc(o)] = beam ? spar : 0.0;
-  }
   // ---- surface terms of the bottom layer (BVP_SURFACE_SETUP_HOM/BEAM,
   //      REG/vlidort_bvproblem.f90:440-626, 1284+; GET_BOA_SOURCE,
   //      REG/vlidort_intensity.f90:1096-1496).  Lambertian: Fourier 0 only,
@@ -506,12 +505,12 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M) {
       }
       double rb = 0.0;
 #pragma unroll
-      for (int j = 0; j < N; ++j) rb += cw[j] * wl[j][0];
+      for (int j = 0; j < N; ++j) rb += cw[j] * lrec[O.wl(j, 0)];
       bb[BO.bc()] = rb;
       if (!brdf) {
         double rq = 0.0;
 #pragma unroll
-        for (int j = 0; j < N; ++j) rq += qsw[j] * wl[j][0];
+        for (int j = 0; j < N; ++j) rq += qsw[j] * lrec[O.wl(j, 0)];
 #pragma unroll
         for (int k = 0; k < n; ++k) {
           double sp = 0.0, sm = 0.0;
@@ -537,7 +536,7 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M) {
           }
           double rbi = 0.0;
 #pragma unroll
-          for (int j = 0; j < N; ++j) rbi += ax[j] * wl[j][0];
+          for (int j = 0; j < N; ++j) rbi += ax[j] * lrec[O.wl(j, 0)];
           bb[BO.r2beam(i)] = sfac * rbi;
         }
       }

@@ -408,6 +408,11 @@ int rt_brdf(const RTWork& W, int nitems, hipStream_t st) {
   return 0;
 }
 
+// Device memory of the synchronous host-pointer entry points (vlidort.hip):
+// one grow-only block per device, used under host_call_mutex.
+std::mutex& host_call_mutex();
+int host_call_block(size_t bytes, char** out);
+
 }  // namespace aa
 
 namespace {
@@ -429,18 +434,20 @@ int vbrdf_alone(bool cm, int nstreams, int ngeom, const double* sza, const doubl
   const double* kernel_wt = p1;
   const double* param = p2;
   const size_t ng = (size_t)ngeom;
-  double *d_tab = nullptr, *d_g = nullptr, *d_kw = nullptr, *d_kp = nullptr, *d_rec = nullptr, *d_db = nullptr;
-  int* d_ok = nullptr;
+  const size_t n1 = cm ? ng : 3 * ng, n2 = 2 * ng;   // Cox-Munk: d_kw = u10m, d_kp = (v10m, mr)
+  std::lock_guard<std::mutex> hl(host_call_mutex());
+  // device copies carved from the host-call block, 256-byte aligned
+  const size_t sz[7] = {tab.size() * sizeof(double), 3 * ng * sizeof(double), n1 * sizeof(double),
+                        n2 * sizeof(double), ng * RO.size() * sizeof(double), ng * sizeof(double), ng * sizeof(int)};
+  size_t off[7], tot = 0;
+  for (int i = 0; i < 7; ++i) { off[i] = tot; tot += (sz[i] + 255) & ~(size_t)255; }
+  char* base = nullptr;
+  if (host_call_block(tot, &base)) return AA_EHIP;
+  double *d_tab = (double*)(base + off[0]), *d_g = (double*)(base + off[1]), *d_kw = (double*)(base + off[2]);
+  double *d_kp = (double*)(base + off[3]), *d_rec = (double*)(base + off[4]), *d_db = (double*)(base + off[5]);
+  int* d_ok = (int*)(base + off[6]);
   int rc = 0;
   auto fail = [&]() { rc = AA_EHIP; };
-  if (hipMalloc(&d_tab, tab.size() * sizeof(double)) != hipSuccess) fail();
-  if (!rc && hipMalloc(&d_g, 3 * ng * sizeof(double)) != hipSuccess) fail();
-  const size_t n1 = cm ? ng : 3 * ng, n2 = cm ? 2 * ng : 2 * ng;   // Cox-Munk: d_kw = u10m, d_kp = (v10m, mr)
-  if (!rc && hipMalloc(&d_kw, n1 * sizeof(double)) != hipSuccess) fail();
-  if (!rc && hipMalloc(&d_kp, n2 * sizeof(double)) != hipSuccess) fail();
-  if (!rc && hipMalloc(&d_rec, ng * RO.size() * sizeof(double)) != hipSuccess) fail();
-  if (!rc && hipMalloc(&d_db, ng * sizeof(double)) != hipSuccess) fail();
-  if (!rc && hipMalloc(&d_ok, ng * sizeof(int)) != hipSuccess) fail();
   if (!rc) {
     bool ok = hipMemcpy(d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(d_g, sza, ng * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
@@ -492,8 +499,6 @@ int vbrdf_alone(bool cm, int nstreams, int ngeom, const double* sza, const doubl
       }
     }
   }
-  (void)hipFree(d_tab); (void)hipFree(d_g); (void)hipFree(d_kw); (void)hipFree(d_kp);
-  (void)hipFree(d_rec); (void)hipFree(d_db); (void)hipFree(d_ok);
   return rc;
 }
 }  // namespace

@@ -354,6 +354,11 @@ int upload(aa_mie* t) {
 }  // namespace
 
 namespace aa {
+// Device memory of the synchronous host-pointer entry points (vlidort.hip):
+// one grow-only block per device, used under host_call_mutex.
+std::mutex& host_call_mutex();
+int host_call_block(size_t bytes, char** out);
+
 // Accessors used by the fused radiance path.
 int mie_upload(aa_mie* t) { return upload(t); }
 // True when the device copies are current (no upload, hence no allocation,
@@ -677,12 +682,21 @@ int aa_mie_query(const aa_mie* tc, int idx, float channel, int n, const float* q
   size_t bytes[9];
   for (int i = 0; i < 8; ++i) bytes[i] = (size_t)n * 4;
   bytes[8] = (size_t)n * (pmom ? nmom * npol : 0) * 4;
-  void* dq = nullptr; void* drh = nullptr; void* dout[9] = {nullptr};
-  AA_CHECK(hipMalloc(&dq, (size_t)n * 4));
-  AA_CHECK(hipMalloc(&drh, (size_t)n * 4));
+  std::lock_guard<std::mutex> hl(aa::host_call_mutex());
+  // device copies carved from the host-call block, 256-byte aligned
+  size_t off[12], tot = 0;
+  auto carve = [&](int k, size_t b) { off[k] = tot; tot += (b + 255) & ~(size_t)255; };
+  carve(0, (size_t)n * 4);
+  carve(1, (size_t)n * 4);
+  for (int i = 0; i < 9; ++i) carve(2 + i, (outs[i] && bytes[i]) ? bytes[i] : 0);
+  char* base = nullptr;
+  if (aa::host_call_block(tot, &base)) return -2;
+  void* dq = base + off[0];
+  void* drh = base + off[1];
+  void* dout[9] = {nullptr};
+  for (int i = 0; i < 9; ++i) if (outs[i] && bytes[i]) dout[i] = base + off[2 + i];
   AA_CHECK(hipMemcpy(dq, q_mass, (size_t)n * 4, hipMemcpyHostToDevice));
   AA_CHECK(hipMemcpy(drh, rh, (size_t)n * 4, hipMemcpyHostToDevice));
-  for (int i = 0; i < 9; ++i) if (outs[i] && bytes[i]) AA_CHECK(hipMalloc(&dout[i], bytes[i]));
   int threads = 256;
   hipLaunchKernelGGL(aa::mie_query_kernel, dim3((n + threads - 1) / threads), dim3(threads), 0, 0, t->dev, idx, ich,
                      n, (const float*)dq, (const float*)drh, (float*)dout[0], (float*)dout[1], (float*)dout[2],
@@ -691,8 +705,6 @@ int aa_mie_query(const aa_mie* tc, int idx, float channel, int n, const float* q
   AA_CHECK(hipGetLastError());
   AA_CHECK(hipDeviceSynchronize());
   for (int i = 0; i < 9; ++i) if (dout[i]) AA_CHECK(hipMemcpy(outs[i], dout[i], bytes[i], hipMemcpyDeviceToHost));
-  for (int i = 0; i < 9; ++i) if (dout[i]) (void)hipFree(dout[i]);
-  (void)hipFree(dq); (void)hipFree(drh);
   return 0;
 }
 
@@ -741,12 +753,16 @@ static int get_aop_host(aa_mie* t, int km, int nobs, int nch, int nq, const floa
   }
   if (nobs == 0) { *rc = rcv; return 0; }
   if (upload(t)) return -2;
-  void *dq, *drh, *dt, *ds, *dg, *dp = nullptr, *dti, *dci;
-  AA_CHECK(hipMalloc(&dq, (size_t)km * nq * nobs * 4));
-  AA_CHECK(hipMalloc(&drh, (size_t)km * nobs * 4));
-  AA_CHECK(hipMalloc(&dt, ne * 4)); AA_CHECK(hipMalloc(&ds, ne * 4)); AA_CHECK(hipMalloc(&dg, ne * 4));
-  if (pmom && nmom > 0) AA_CHECK(hipMalloc(&dp, ne * nmom * npol * 4));
-  AA_CHECK(hipMalloc(&dti, nq * 4)); AA_CHECK(hipMalloc(&dci, nch * 4));
+  std::lock_guard<std::mutex> hl(aa::host_call_mutex());
+  // device copies carved from the host-call block, 256-byte aligned
+  const size_t sz[8] = {(size_t)km * nq * nobs * 4, (size_t)km * nobs * 4, ne * 4, ne * 4, ne * 4,
+                        (pmom && nmom > 0) ? ne * nmom * npol * 4 : 0, (size_t)nq * 4, (size_t)nch * 4};
+  size_t off[8], tot = 0;
+  for (int i = 0; i < 8; ++i) { off[i] = tot; tot += (sz[i] + 255) & ~(size_t)255; }
+  char* base = nullptr;
+  if (aa::host_call_block(tot, &base)) return -2;
+  void *dq = base + off[0], *drh = base + off[1], *dt = base + off[2], *ds = base + off[3], *dg = base + off[4];
+  void *dp = sz[5] ? base + off[5] : nullptr, *dti = base + off[6], *dci = base + off[7];
   AA_CHECK(hipMemcpy(dq, qm, (size_t)km * nq * nobs * 4, hipMemcpyHostToDevice));
   AA_CHECK(hipMemcpy(drh, rh, (size_t)km * nobs * 4, hipMemcpyHostToDevice));
   AA_CHECK(hipMemcpy(dti, tidx.data(), nq * 4, hipMemcpyHostToDevice));
@@ -760,7 +776,6 @@ static int get_aop_host(aa_mie* t, int km, int nobs, int nch, int nq, const floa
   AA_CHECK(hipMemcpy(ssa, ds, ne * 4, hipMemcpyDeviceToHost));
   AA_CHECK(hipMemcpy(g, dg, ne * 4, hipMemcpyDeviceToHost));
   if (dp) AA_CHECK(hipMemcpy(pmom, dp, ne * nmom * npol * 4, hipMemcpyDeviceToHost));
-  for (void* p : {dq, drh, dt, ds, dg, dp, dti, dci}) if (p) (void)hipFree(p);
   *rc = rcv;
   return 0;
 }

@@ -22,6 +22,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #define F32(x) ((double)(x##f))
@@ -157,14 +158,24 @@ int rot_launch(int km, int nch, int nobs, const double* channels_nm, const doubl
   return 0;
 }
 
-template <class T>
-struct DevBuf {
-  T* p = nullptr;
-  explicit DevBuf(size_t n) { if (hipMalloc(&p, sizeof(T) * (n ? n : 1)) != hipSuccess) p = nullptr; }
-  ~DevBuf() { if (p) (void)hipFree(p); }
-};
-
 }  // namespace
+
+// Device memory of the synchronous host-pointer entry points (vlidort.hip):
+// one grow-only block per device, used under host_call_mutex.
+std::mutex& host_call_mutex();
+int host_call_block(size_t bytes, char** out);
+
+// The host-call block split into 256-byte-aligned float arrays of the given
+// lengths; false when the block cannot be had.  Caller holds host_call_mutex.
+template <int K>
+bool host_floats(const size_t (&len)[K], float* (&out)[K]) {
+  size_t off[K], tot = 0;
+  for (int i = 0; i < K; ++i) { off[i] = tot; tot += (len[i] * sizeof(float) + 255) & ~(size_t)255; }
+  char* base = nullptr;
+  if (host_call_block(tot, &base)) return false;
+  for (int i = 0; i < K; ++i) out[i] = (float*)(base + off[i]);
+  return true;
+}
 }  // namespace aa
 
 extern "C" {
@@ -177,14 +188,18 @@ int aa_rot_calc(int km, int nch, int nobs, const double* channels, const double*
   if (km <= 0 || nch <= 0 || nobs < 0 || !channels || !pe || !he || !te || !rot || !depol) return AA_EARG;
   if (km > aa::MAX_LAYERS) { *rc = 3; return AA_EDIMS; }   // VLIDORT_Init layer-dimension check, as the drivers
   const size_t ne = (size_t)(km + 1) * nobs, nr = (size_t)km * nobs * nch;
-  aa::DevBuf<double> dpe(ne), dhe(ne), dte(ne), drot(nr);
-  if (!dpe.p || !dhe.p || !dte.p || !drot.p) return AA_EHIP;
-  AA_CHECK(hipMemcpy(dpe.p, pe, sizeof(double) * ne, hipMemcpyHostToDevice));
-  AA_CHECK(hipMemcpy(dhe.p, he, sizeof(double) * ne, hipMemcpyHostToDevice));
-  AA_CHECK(hipMemcpy(dte.p, te, sizeof(double) * ne, hipMemcpyHostToDevice));
-  int e = aa::rot_launch(km, nch, nobs, channels, dpe.p, dhe.p, dte.p, drot.p, depol, 0);
+  std::lock_guard<std::mutex> hl(aa::host_call_mutex());
+  const size_t se = (ne * sizeof(double) + 255) & ~(size_t)255;
+  char* base = nullptr;
+  if (aa::host_call_block(3 * se + nr * sizeof(double), &base)) return AA_EHIP;
+  double *dpe = (double*)base, *dhe = (double*)(base + se), *dte = (double*)(base + 2 * se);
+  double* drot = (double*)(base + 3 * se);
+  AA_CHECK(hipMemcpy(dpe, pe, sizeof(double) * ne, hipMemcpyHostToDevice));
+  AA_CHECK(hipMemcpy(dhe, he, sizeof(double) * ne, hipMemcpyHostToDevice));
+  AA_CHECK(hipMemcpy(dte, te, sizeof(double) * ne, hipMemcpyHostToDevice));
+  int e = aa::rot_launch(km, nch, nobs, channels, dpe, dhe, dte, drot, depol, 0);
   if (e) return e;
-  AA_CHECK(hipMemcpy(rot, drot.p, sizeof(double) * nr, hipMemcpyDeviceToHost));
+  AA_CHECK(hipMemcpy(rot, drot, sizeof(double) * nr, hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -210,15 +225,16 @@ int aa_edge_vars(int km, int nobs, const float* airdens, const float* delp, floa
                  float* te) {
   if (km <= 0 || nobs < 0 || !airdens || !delp || !pe || !ze || !te) return AA_EARG;
   const size_t nl = (size_t)km * nobs, ne = (size_t)(km + 1) * nobs;
-  aa::DevBuf<float> dad(nl), ddp(nl), dpe(ne), dze(ne), dte(ne);
-  if (!dad.p || !ddp.p || !dpe.p || !dze.p || !dte.p) return AA_EHIP;
-  AA_CHECK(hipMemcpy(dad.p, airdens, sizeof(float) * nl, hipMemcpyHostToDevice));
-  AA_CHECK(hipMemcpy(ddp.p, delp, sizeof(float) * nl, hipMemcpyHostToDevice));
-  int e = aa_edge_vars_dev(km, nobs, dad.p, ddp.p, ptop, dpe.p, dze.p, dte.p, nullptr);
+  std::lock_guard<std::mutex> hl(aa::host_call_mutex());
+  float* d[5];
+  if (!aa::host_floats({nl, nl, ne, ne, ne}, d)) return AA_EHIP;
+  AA_CHECK(hipMemcpy(d[0], airdens, sizeof(float) * nl, hipMemcpyHostToDevice));
+  AA_CHECK(hipMemcpy(d[1], delp, sizeof(float) * nl, hipMemcpyHostToDevice));
+  int e = aa_edge_vars_dev(km, nobs, d[0], d[1], ptop, d[2], d[3], d[4], nullptr);
   if (e) return e;
-  AA_CHECK(hipMemcpy(pe, dpe.p, sizeof(float) * ne, hipMemcpyDeviceToHost));
-  AA_CHECK(hipMemcpy(ze, dze.p, sizeof(float) * ne, hipMemcpyDeviceToHost));
-  AA_CHECK(hipMemcpy(te, dte.p, sizeof(float) * ne, hipMemcpyDeviceToHost));
+  AA_CHECK(hipMemcpy(pe, d[2], sizeof(float) * ne, hipMemcpyDeviceToHost));
+  AA_CHECK(hipMemcpy(ze, d[3], sizeof(float) * ne, hipMemcpyDeviceToHost));
+  AA_CHECK(hipMemcpy(te, d[4], sizeof(float) * ne, hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -235,13 +251,14 @@ int aa_calc_qm_dev(int km, int nq, int nobs, const float* d_q, const float* d_de
 int aa_calc_qm(int km, int nq, int nobs, const float* q, const float* delp, float* qm) {
   if (km <= 0 || nq <= 0 || nobs < 0 || !q || !delp || !qm) return AA_EARG;
   const size_t nqe = (size_t)km * nq * nobs, nl = (size_t)km * nobs;
-  aa::DevBuf<float> dq(nqe), ddp(nl), dqm(nqe);
-  if (!dq.p || !ddp.p || !dqm.p) return AA_EHIP;
-  AA_CHECK(hipMemcpy(dq.p, q, sizeof(float) * nqe, hipMemcpyHostToDevice));
-  AA_CHECK(hipMemcpy(ddp.p, delp, sizeof(float) * nl, hipMemcpyHostToDevice));
-  int e = aa_calc_qm_dev(km, nq, nobs, dq.p, ddp.p, dqm.p, nullptr);
+  std::lock_guard<std::mutex> hl(aa::host_call_mutex());
+  float* d[3];
+  if (!aa::host_floats({nqe, nl, nqe}, d)) return AA_EHIP;
+  AA_CHECK(hipMemcpy(d[0], q, sizeof(float) * nqe, hipMemcpyHostToDevice));
+  AA_CHECK(hipMemcpy(d[1], delp, sizeof(float) * nl, hipMemcpyHostToDevice));
+  int e = aa_calc_qm_dev(km, nq, nobs, d[0], d[1], d[2], nullptr);
   if (e) return e;
-  AA_CHECK(hipMemcpy(qm, dqm.p, sizeof(float) * nqe, hipMemcpyDeviceToHost));
+  AA_CHECK(hipMemcpy(qm, d[2], sizeof(float) * nqe, hipMemcpyDeviceToHost));
   return 0;
 }
 

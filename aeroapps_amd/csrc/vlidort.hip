@@ -100,7 +100,7 @@ int arena_reserve(hipStream_t st, int purpose, size_t bytes, char** out) {
   *out = a.base;
   return 0;
 }
-enum ArenaPurpose { AR_WORK = 0, AR_OPTICS = 1, AR_CALL = 2 };
+enum ArenaPurpose { AR_WORK = 0, AR_OPTICS = 1, AR_CALL = 2, AR_HOST = 3 };
 
 // Budgets of the work and optics arenas of later calls (0: AA_WORK_BYTES /
 // AA_OPTICS_BYTES, else 24 GiB / 4 GiB, which leaves room for several
@@ -441,21 +441,18 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   return 0;
 }
 
-// Device copy helper for the host-pointer drivers.
-struct DevBuf {
-  std::vector<void*> ptrs;
-  ~DevBuf() { for (void* p : ptrs) (void)hipFree(p); }
-  template <class T>
-  T* up(const T* h, size_t n) {
-    void* p = nullptr;
-    if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
-    ptrs.push_back(p);
-    if (h && n && hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
-    return (T*)p;
-  }
-  template <class T>
-  T* alloc(size_t n) { return up<T>(nullptr, n); }
-};
+}  // namespace
+
+// Device memory of the host-pointer entry points: one grow-only block per
+// device, which the synchronous host-pointer calls use one at a time
+// (host_call_mutex) instead of allocating and freeing per call.
+std::mutex& host_call_mutex() {
+  static std::mutex m;
+  return m;
+}
+int host_call_block(size_t bytes, char** out) { return arena_reserve(nullptr, AR_HOST, bytes, out); }
+
+namespace {
 
 int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane_parallel, int nmom, int npol,
                  const double* rot, const double* depol, const double* alpha, const double* tau, const double* ssa,
@@ -494,37 +491,67 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
   W.missing = missing;
   W.precision = precision;
   const size_t nl = (size_t)km * nobs * nch;
-  DevBuf B;
-  W.rot = B.up(rot, nl);
-  W.depol = B.up(depol, (size_t)nch);
-  W.alpha = (vector && alpha) ? B.up(alpha, nl) : nullptr;
-  W.tau = B.up(tau, nl);
-  W.ssa = B.up(ssa, nl);
-  W.pmom = B.up(pmom, nl * nmom * npol);
-  W.optics_f32 = 0; W.opt_obs0 = 0; W.opt_nobs = nobs;
-  W.he = B.up(he, (size_t)(km + 1) * nobs);
-  W.albedo = modis ? nullptr : B.up(albedo, (size_t)nobs * nch);
-  if (kernel_wt) {
-    W.kernel_wt = B.up(kernel_wt, (size_t)nobs * nch * 3);
-    W.kparam = B.up(kparam, (size_t)nobs * nch * 2);
-  }
-  if (cx) {
-    W.u10m = B.up(u10m, (size_t)nobs);
-    W.v10m = B.up(v10m, (size_t)nobs);
-    W.mr = B.up(mr, (size_t)nch);
-  }
-  if (modis) W.br_out = B.alloc<double>((size_t)nobs * nch);
-  W.sza = B.up(sza, (size_t)nobs);
-  W.raa = B.up(raa, (size_t)nobs);
-  W.vza = B.up(vza, (size_t)nobs);
-  W.flux = vector ? B.up(flux_factor, (size_t)nch * nobs) : nullptr;
   const size_t no = (size_t)nobs * nch;
-  W.radiance = B.alloc<double>(no);
-  W.reflectance = B.alloc<double>(no);
-  W.qout = S == 3 ? B.alloc<double>(no) : nullptr;
-  W.uout = S == 3 ? B.alloc<double>(no) : nullptr;
-  W.nfourier = B.alloc<int>(no);
-  for (void* p : B.ptrs) if (!p) return AA_EHIP;
+  std::lock_guard<std::mutex> hl(host_call_mutex());
+  // device copies of the inputs and the outputs, carved from the host-call block
+  struct Dev {
+    double *rot, *depol, *alpha, *tau, *ssa, *pmom, *he, *albedo, *kw, *kp, *u10, *v10, *mr, *br, *sza, *raa, *vza,
+        *flux, *rad, *refl, *q, *u;
+    int* nf;
+  } d{};
+  auto plan = [&](Carver& C) {
+    d.rot = C.take<double>(nl);
+    d.depol = C.take<double>(nch);
+    d.alpha = (vector && alpha) ? C.take<double>(nl) : nullptr;
+    d.tau = C.take<double>(nl);
+    d.ssa = C.take<double>(nl);
+    d.pmom = C.take<double>(nl * nmom * npol);
+    d.he = C.take<double>((size_t)(km + 1) * nobs);
+    d.albedo = modis ? nullptr : C.take<double>(no);
+    d.kw = kernel_wt ? C.take<double>(no * 3) : nullptr;
+    d.kp = kernel_wt ? C.take<double>(no * 2) : nullptr;
+    d.u10 = cx ? C.take<double>(nobs) : nullptr;
+    d.v10 = cx ? C.take<double>(nobs) : nullptr;
+    d.mr = cx ? C.take<double>(nch) : nullptr;
+    d.br = modis ? C.take<double>(no) : nullptr;
+    d.sza = C.take<double>(nobs);
+    d.raa = C.take<double>(nobs);
+    d.vza = C.take<double>(nobs);
+    d.flux = vector ? C.take<double>(no) : nullptr;
+    d.rad = C.take<double>(no);
+    d.refl = C.take<double>(no);
+    d.q = S == 3 ? C.take<double>(no) : nullptr;
+    d.u = S == 3 ? C.take<double>(no) : nullptr;
+    d.nf = C.take<int>(no);
+  };
+  {
+    Carver c0(nullptr);
+    plan(c0);
+    char* base = nullptr;
+    if (host_call_block(c0.off, &base)) return AA_EHIP;
+    Carver c1(base);
+    plan(c1);
+  }
+  auto up = [](double* dst, const double* src, size_t n) {
+    return (n && src) ? hipMemcpy(dst, src, n * sizeof(double), hipMemcpyHostToDevice) : hipSuccess;
+  };
+  if (up(d.rot, rot, nl) || up(d.depol, depol, nch) || (d.alpha && up(d.alpha, alpha, nl)) || up(d.tau, tau, nl) ||
+      up(d.ssa, ssa, nl) || up(d.pmom, pmom, nl * nmom * npol) || up(d.he, he, (size_t)(km + 1) * nobs) ||
+      (d.albedo && up(d.albedo, albedo, no)) || (d.kw && up(d.kw, kernel_wt, no * 3)) ||
+      (d.kp && up(d.kp, kparam, no * 2)) || (d.u10 && up(d.u10, u10m, nobs)) || (d.v10 && up(d.v10, v10m, nobs)) ||
+      (d.mr && up(d.mr, mr, nch)) || up(d.sza, sza, nobs) || up(d.raa, raa, nobs) || up(d.vza, vza, nobs) ||
+      (d.flux && up(d.flux, flux_factor, no)))
+    return AA_EHIP;
+  W.rot = d.rot; W.depol = d.depol; W.alpha = d.alpha;
+  W.tau = d.tau; W.ssa = d.ssa; W.pmom = d.pmom;
+  W.optics_f32 = 0; W.opt_obs0 = 0; W.opt_nobs = nobs;
+  W.he = d.he; W.albedo = d.albedo;
+  W.kernel_wt = d.kw; W.kparam = d.kp;
+  W.u10m = d.u10; W.v10m = d.v10; W.mr = d.mr;
+  W.br_out = d.br;
+  W.sza = d.sza; W.raa = d.raa; W.vza = d.vza;
+  W.flux = d.flux;
+  W.radiance = d.rad; W.reflectance = d.refl; W.qout = d.q; W.uout = d.u; W.nfourier = d.nf;
   int* lp = nullptr;
   {
     char* cb = nullptr;

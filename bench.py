@@ -479,15 +479,22 @@ def main():
         cols_h = S.make_columns(hc, km, wl["channels"], seed=7)
         chs = np.asarray(wl["channels"], dtype=np.float32)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        tau_h, ssa_h, g_h, pm_h, rc_a = AM.get_aop_vector(mie, chs, S.TRACERS, 0, cols_h.qm.T, cols_h.rh.T, nmom, npol)
-        t1 = time.perf_counter()
-        rad_h, refl_h, rc_l = AV.scalar_lambert(cols_h.channels, N, 0, cols_h.rot.transpose(2, 1, 0), cols_h.depol,
-                                                tau_h, ssa_h, g_h, pm_h, cols_h.pe.T, cols_h.he.T, cols_h.te.T,
-                                                cols_h.albedo.T, cols_h.sza, cols_h.raa, cols_h.vza, S.MISSING, 0)
-        t2 = time.perf_counter()
+        times = []
+        for _ in range(2):   # the first pass also grows the library's host-call device block
+            t0 = time.perf_counter()
+            tau_h, ssa_h, g_h, pm_h, rc_a = AM.get_aop_vector(mie, chs, S.TRACERS, 0, cols_h.qm.T, cols_h.rh.T,
+                                                              nmom, npol)
+            t1 = time.perf_counter()
+            rad_h, refl_h, rc_l = AV.scalar_lambert(cols_h.channels, N, 0, cols_h.rot.transpose(2, 1, 0),
+                                                    cols_h.depol, tau_h, ssa_h, g_h, pm_h, cols_h.pe.T, cols_h.he.T,
+                                                    cols_h.te.T, cols_h.albedo.T, cols_h.sza, cols_h.raa, cols_h.vza,
+                                                    S.MISSING, 0)
+            t2 = time.perf_counter()
+            times.append((t0, t1, t2))
+        (f0, _, f2), (t0, t1, t2) = times
         out["host_path"] = {"columns": hc, "getaop_s": round(t1 - t0, 3), "lambert_s": round(t2 - t1, 3),
-                            "columns_per_s": round(hc / (t2 - t0), 1), "rc": [int(rc_a), int(rc_l)],
+                            "columns_per_s": round(hc / (t2 - t0), 1),
+                            "first_call_columns_per_s": round(hc / (f2 - f0), 1), "rc": [int(rc_a), int(rc_l)],
                             "note": "host numpy arrays in, host arrays out (FP64 optics widened on the host, "
                                     "PCIe copies inside the calls); not the headline value"}
     print(json.dumps(out))

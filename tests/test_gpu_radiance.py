@@ -115,3 +115,43 @@ def test_chunked_equals_single_launch(gpu, monkeypatch):
     b = run_product(d)
     assert np.array_equal(a["radiance"], b["radiance"])
     assert np.array_equal(a["nfourier"], b["nfourier"])
+
+
+def test_host_calls_from_threads_match_serial(gpu):
+    """Host-pointer calls share one device block per GPU under a lock: calls
+    made from several threads at once (ctypes releases the GIL) must give the
+    results of the same calls made one after the other, with inputs of
+    different sizes so that the block grows between calls."""
+    import threading
+    print("case threaded host calls", flush=True)
+    f = lambda a: np.asarray(a, dtype=np.float64)
+    jobs = []
+    for k, (ncol, N) in enumerate([(24, 6), (8, 4), (40, 6), (16, 8)]):
+        ch = (550.0, 870.0)
+        cols = S.make_columns(ncol, 72, ch, seed=50 + k, col0=100 * k)
+        tabs = S.make_mie_tables(ch, nmom=32, npol=1)
+        tau, ssa, g, pm, rc = CM.get_aop_vector(tabs, ch, tabs[0].lambda_m, S.TRACERS, cols.qm, cols.rh, 32, 1)
+        args = (cols.channels, N, False, cols.rot.T, cols.depol, f(tau).T, f(ssa).T, f(g).T, f(pm).T, cols.pe.T,
+                cols.he.T, cols.te.T, cols.albedo.T, cols.sza, cols.raa, cols.vza, S.MISSING)
+        jobs.append(args)
+    serial = [V.scalar_lambert(*a, return_nfourier=True) for a in jobs]
+    got = [None] * (2 * len(jobs))
+    errs = []
+
+    def work(i):
+        try:
+            got[i] = V.scalar_lambert(*jobs[i % len(jobs)], return_nfourier=True)
+        except Exception as e:   # noqa: BLE001 - reported below
+            errs.append(e)
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(len(got))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errs, errs
+    for i, out in enumerate(got):
+        ref = serial[i % len(jobs)]
+        assert out[2] == ref[2]
+        assert np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1])
+        assert np.array_equal(out[3], ref[3])

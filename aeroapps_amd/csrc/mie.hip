@@ -724,8 +724,13 @@ static int get_aop_host(aa_mie* t, int km, int nobs, int nch, int nq, const floa
     return -1;
   *rc = 0;
   const size_t ne = (size_t)km * nch * nobs;
-  std::fill(tau, tau + ne, 0.0f); std::fill(ssa, ssa + ne, 0.0f); std::fill(g, g + ne, 0.0f);
-  if (pmom) std::fill(pmom, pmom + ne * nmom * npol, 0.0f);
+  // outputs start at zero (GEO/VLIDORT_MieMod.F90:234-239); the kernel writes
+  // every element, so only the channel-mismatch return clears them here (the
+  // reference returns rc 99 before its zeroing and leaves them undefined)
+  auto clear = [&]() {
+    std::fill(tau, tau + ne, 0.0f); std::fill(ssa, ssa + ne, 0.0f); std::fill(g, g + ne, 0.0f);
+    if (pmom) std::fill(pmom, pmom + ne * nmom * npol, 0.0f);
+  };
   // channel matching in float32 (GEO/VLIDORT_MieMod.F90:210-225)
   std::vector<int> chan(nch, -1);
   for (int n = 0; n < nch; ++n)
@@ -734,7 +739,7 @@ static int get_aop_host(aa_mie* t, int km, int nobs, int nch, int nq, const floa
       volatile float diff = channels[n] - scaled;
       if (std::fabs(diff) < 1.0f) { chan[n] = m; break; }
     }
-  for (int n = 0; n < nch; ++n) if (chan[n] < 0) { *rc = 99; return 0; }
+  for (int n = 0; n < nch; ++n) if (chan[n] < 0) { clear(); *rc = 99; return 0; }
   // tracer resolution and the reference's early returns
   std::vector<int> tidx(nq, -1);
   int ntr_eff = nq, normalize = 1, rcv = 0;

@@ -101,13 +101,15 @@ __device__ __forceinline__ double dpp_d(double x) {
 template <int N, int S, bool MIX>
 __global__ __launch_bounds__(64) void k_layer(RTWork W, int M) {
   constexpr int n = N * S, LM = 2 * N;
-  // MIX (the mixed path): SAB/DAB come from the FP32 MFMA contraction and the
-  // source assembly -- the user-angle homogeneous solutions (Legendre sums of
-  // X+- with the quadrature and user PI matrices) and the phase-matrix sums of
-  // the beam source -- runs in FP32 (type TR); the eigensystem, the
-  // normalisation and the particular solution's coefficients stay FP64.
-  // Without MIX everything is FP64.
-  using TR = typename std::conditional<MIX, float, double>::type;
+  // MIX (the mixed path): SAB/DAB come from the FP32 MFMA contraction; from
+  // N = 8 streams on, the source assembly -- the user-angle homogeneous
+  // solutions (Legendre sums of X+- with the quadrature and user PI matrices)
+  // and the phase-matrix sums of the beam source -- also runs in FP32 (type
+  // TR), where its smaller registers pay.  Below N = 8 it stays FP64: without
+  // packed FP32 the FP32 VALU rate is the FP64 one, so the conversions only
+  // cost there.  The eigensystem, the normalisation and the particular
+  // solution's coefficients are FP64 throughout; without MIX everything is.
+  using TR = typename std::conditional<MIX && (N >= 8), float, double>::type;
   constexpr LOff O(N, S);
   constexpr QOff QO(N, S);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;

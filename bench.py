@@ -36,8 +36,8 @@ PROF_NAMES = ["mie", "check", "optics", "geom", "item", "layer", "bvp", "brdf", 
 # scalar, keyed (kernel, N, S, mixed), from FETCH_SIZE/WRITE_SIZE counter runs
 # scaled by a copy-kernel calibration
 # (profiles/hbm_traffic_40k_columns_4ch_12streams.md)
-TRAFFIC_PER_ITEM_TERM = {("bvp", 6, 1, False): (159.9e3, 10.7e3), ("bvp", 6, 1, True): (159.9e3, 10.7e3),
-                         ("layer", 6, 1, False): (56.1e3, 70.8e3), ("layer", 6, 1, True): (35.7e3, 74.0e3)}
+TRAFFIC_PER_ITEM_TERM = {("bvp", 6, 1, False): (153.2e3, 10.9e3), ("bvp", 6, 1, True): (153.2e3, 10.9e3),
+                         ("layer", 6, 1, False): (55.2e3, 84.1e3), ("layer", 6, 1, True): (34.6e3, 86.7e3)}
 
 
 # ----------------------------------------------------------------- algorithmic work
@@ -412,8 +412,8 @@ def main():
                                f"{'vector' if vector else 'scalar'} "
                                f"{'MODIS land BRDF (Lambertian + Ross-thick + Li-sparse)' if modis else 'Lambertian'}",
                    "precision": "mixed (FP32 Chem_Mie optics, FP32 phase-matrix contraction on MFMA, FP32 source "
-                                "assembly; FP64 eigensystem and boundary-value elimination; the FP64 line is FP64 "
-                                "throughout with an FP64 MFMA contraction)"
+                                "assembly from 8 streams on; FP64 eigensystem and boundary-value elimination; the "
+                                "FP64 line is FP64 throughout with an FP64 MFMA contraction)"
                                 if rmx is not None else "fp64",
                    "path": ("aa_column_radiance_modis_dev" if modis else "aa_column_radiance_dev") +
                            " (Chem_Mie optics + VLIDORT radiance, device-resident)",

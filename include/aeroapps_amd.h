@@ -192,10 +192,10 @@ int aa_get_aop_vector_dev(aa_mie* t, int km, int nobs, int nch, int nq, const in
  * nfourier (may be NULL) receives the number of Fourier terms used per
  * (obs, channel), 0 where MISSING.  precision: 0 = FP64 throughout (the
  * drop-in), 1 = mixed (FP32 Chem_Mie optics, FP32 phase-matrix (SAB/DAB)
- * contraction on the matrix cores, FP32 source assembly: the user-angle
- * homogeneous solutions and the phase-matrix sums of the beam source; FP64
- * eigensystem, normalisation, particular-solution coefficients and
- * boundary-value solve).  Host pointers, synchronous. */
+ * contraction on the matrix cores and, from nstreams = 8 on, FP32 source
+ * assembly: the user-angle homogeneous solutions and the phase-matrix sums of
+ * the beam source; FP64 eigensystem, normalisation, particular-solution
+ * coefficients and boundary-value solve).  Host pointers, synchronous. */
 int aa_vlidort_scalar_lambert(int km, int nch, int nobs, const double* channels, int nstreams,
                               int plane_parallel, int nmom, int npol, const double* rot, const double* depol,
                               const double* tau, const double* ssa, const double* g, const double* pmom,

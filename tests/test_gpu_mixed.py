@@ -1,9 +1,9 @@
 """Mixed precision (precision=1: FP32 phase-matrix contraction on the matrix
-cores and FP32 source assembly, FP64 eigensystem and band elimination) against
-the FP64 oracle.  Bar: TOA radiance within 1e-5 relative for every element
-whose Fourier-term count matches; the fraction of elements whose count differs
-is reported (a different stop changes I by up to the 1e-4 convergence
-criterion) and must stay small."""
+cores and, from 8 streams on, FP32 source assembly; FP64 eigensystem and band
+elimination) against the FP64 oracle.  Bar: TOA radiance within 1e-5 relative
+for every element whose Fourier-term count matches; the fraction of elements
+whose count differs is reported (a different stop changes I by up to the 1e-4
+convergence criterion) and must stay small."""
 import numpy as np
 import pytest
 

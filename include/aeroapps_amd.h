@@ -411,8 +411,12 @@ int aa_column_radiance_dev_noalloc(aa_mie* t, int km, int nobs, int nch, int nq,
  * one-stream bulk caller on a 288 GB GPU gains from 64 GiB / 4 GiB: fewer,
  * larger chunks keep the Fourier terms' shrinking active lists filling the
  * GPU.  The work budget is further capped at 60% of the free device memory.
+ * The synchronous host-pointer entry points copy their arguments through one
+ * further grow-only block per GPU and run one at a time in a process (a lock
+ * held for the whole call), so several host threads may call them.
  * aa_release_arenas() synchronises the current GPU and frees every arena the
- * library holds on it (they are grown again on the next call).  It must not
+ * library holds on it, that block included (they are grown again on the next
+ * call).  It must not
  * be called while another host thread is inside a library call on that GPU:
  * that call keeps the arena pointers it carved and would launch into freed
  * memory. */

@@ -840,7 +840,12 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
           *reinterpret_cast<double2*>(&urow[RH]) = make_double2(a[sl][RH], rp);
         }
       }
-      __syncthreads();
+      // the workgroup is one wavefront, whose LDS operations complete in
+      // order: a wavefront-scope fence (no wait on outstanding memory) keeps
+      // the compiler from moving the reads above the pivot row's writes
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       double u[NC + 1];
 #pragma unroll
       for (int c = 0; c < NCOL; ++c)

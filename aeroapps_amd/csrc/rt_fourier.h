@@ -34,6 +34,7 @@
 // (with 1/pivot) go, in compact form, to a per-item buffer that the
 // back-substitution reads layer by layer.
 #pragma once
+#include <algorithm>
 #include <type_traits>
 #include "rt_work.h"
 #include "rt_device.h"
@@ -1177,6 +1178,7 @@ int fourier_loop_shape(const RTWork& W, int nitems, int* const lists[2], int* co
     hipLaunchKernelGGL((k_bvp<N, S, G>), dim3(nb), dim3(64), 0, st, W, M, in, counts + M * LBINS, out,
                        counts + (M + 1) * LBINS);
     prof_end(PROF_BVP, st);
+    if (W.mid_event && M == std::min(W.mid_term, W.nmoments)) AA_CHECK(hipEventRecord(W.mid_event, st));
   }
   AA_CHECK(hipGetLastError());
   return 0;

@@ -91,6 +91,11 @@ struct RTWork {
   // launch shapes of the Fourier-term kernels (0 default, 1 alternative):
   // lanes per item of the band solver, pair groups of the SAB/DAB kernel
   int shape_bvp = 0, shape_sab = 0;
+  // host side: an event recorded on the launch stream once the Fourier loop
+  // has enqueued term mid_term (nullptr: none); the next optics block of a
+  // call starts on another stream when it completes
+  hipEvent_t mid_event = nullptr;
+  int mid_term = 0;
   int* loff = nullptr;
   int* lcnt = nullptr;
   double* csza = nullptr;

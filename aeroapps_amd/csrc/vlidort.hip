@@ -781,6 +781,58 @@ namespace {
 
 // Fused device path shared by the Lambertian and the MODIS land-surface entry
 // points (d_kernel_wt set: BRDF surface, d_albedo unused).
+// Optics blocks of a one-stream-count call on two library streams (see the
+// caller).  Each stream has its own optics block and work arena; block b + 1
+// waits for block b's mid-loop event, block b + 2 follows block b on its
+// stream.  The caller stream waits for both streams before this returns.
+constexpr int BLOCK_MID_TERM = 4;
+int blocks_two_streams(aa_mie* t, const RTWork& W, hipStream_t st, int nblk, size_t ob, int nq,
+                       const int* tracer_idx, const int* chan_idx, const float* d_qm, const float* d_rh,
+                       size_t obytes) {
+  const int km = W.km, nch = W.nch, nobs = W.nobs;
+  hipStream_t ss[2] = {bucket_stream(st, 0), bucket_stream(st, 1)};
+  if (!ss[0] || !ss[1]) return AA_EHIP;
+  float* opt[2];
+  for (int k = 0; k < 2; ++k) {
+    char* base = nullptr;
+    if (arena_reserve(ss[k], AR_OPTICS, obytes, &base)) return AA_EHIP;
+    opt[k] = (float*)base;
+  }
+  const size_t n3 = ob * km * nch;
+  hipEvent_t ready = nullptr, mid[2] = {nullptr, nullptr}, done[2] = {nullptr, nullptr};
+  int rcode = 0;
+  auto mk = [&](hipEvent_t* e) {
+    if (!rcode && hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) rcode = AA_EHIP;
+  };
+  mk(&ready); mk(&mid[0]); mk(&mid[1]); mk(&done[0]); mk(&done[1]);
+  if (!rcode && hipEventRecord(ready, st) != hipSuccess) rcode = AA_EHIP;
+  for (int k = 0; k < 2 && !rcode; ++k)
+    if (hipStreamWaitEvent(ss[k], ready, 0) != hipSuccess) rcode = AA_EHIP;
+  for (int b = 0; b < nblk && !rcode; ++b) {
+    const int k = b & 1;
+    hipStream_t s = ss[k];
+    if (b > 0 && hipStreamWaitEvent(s, mid[k ^ 1], 0) != hipSuccess) { rcode = AA_EHIP; break; }
+    const int o0 = (int)(b * ob), no = std::min((int)ob, nobs - o0);
+    float* dt = opt[k];
+    float* ds = dt + n3;
+    float* dg = ds + n3;
+    float* dp = dg + n3;
+    rcode = launch_get_aop(t, km, no, nch, nq, tracer_idx, chan_idx, d_qm + (size_t)o0 * km * nq,
+                           d_rh + (size_t)o0 * km, W.nmom, W.npol, nq, 1, dt, ds, dg, dp, s);
+    if (rcode) break;
+    RTWork Wc = W;
+    Wc.tau = dt; Wc.ssa = ds; Wc.pmom = dp;
+    Wc.opt_obs0 = o0; Wc.opt_nobs = no;
+    Wc.mid_event = mid[k]; Wc.mid_term = BLOCK_MID_TERM;
+    rcode = run_items(Wc, o0 * nch, (o0 + no) * nch, s);
+  }
+  // join both streams to the caller stream, also after an error
+  for (int k = 0; k < 2; ++k)
+    if (done[k] && hipEventRecord(done[k], ss[k]) == hipSuccess) (void)hipStreamWaitEvent(st, done[k], 0);
+  for (hipEvent_t e : {ready, mid[0], mid[1], done[0], done[1]}) if (e) (void)hipEventDestroy(e);
+  return rcode;
+}
+
 int column_radiance(aa_mie* t, int km, int nobs, int nch, int nq, const int* tracer_idx, const int* chan_idx,
                     const float* d_qm, const float* d_rh, int nmom, int npol, int nstreams, int vector, int precision,
                     const int* d_nlayers, const int* d_nstreams_col, const double* d_rot, const double* d_depol,
@@ -825,9 +877,14 @@ int column_radiance(aa_mie* t, int km, int nobs, int nch, int nq, const int* tra
   // counts split a block into buckets, each chunked on its own)
   const size_t per_obs = (size_t)km * nch * (3 + (size_t)nmom * npol) * sizeof(float);
   size_t ob = std::max<size_t>(1, optics_budget() / per_obs);
+  // one stream count and AA_BLOCK_STREAMS=2: blocks run on two library
+  // streams (below), each with half the optics and work budgets
+  const bool two = !d_nstreams_col && env_bytes("AA_BLOCK_STREAMS", 1) >= 2;
+  if (two) ob = std::max<size_t>(1, ob / 2);
   if (!d_nstreams_col) {
     size_t wcap = 1;
     if (work_cap_items(W, &wcap)) return AA_EHIP;
+    if (two) wcap = std::max<size_t>(1, wcap / 2);
     ob = std::min<size_t>(ob, std::max<size_t>(1, wcap / (size_t)nch));
   }
   ob = std::min<size_t>(ob, (size_t)nobs);
@@ -859,16 +916,30 @@ int column_radiance(aa_mie* t, int km, int nobs, int nch, int nq, const int* tra
   int* d_items = (int*)(cb + 256);
   if (!hitems.empty())
     AA_CHECK(hipMemcpyAsync(d_items, hitems.data(), sizeof(int) * hitems.size(), hipMemcpyHostToDevice, st));
-  char* obase = nullptr;
   const size_t n3 = ob * km * nch;
   const size_t obytes = (3 * n3 + n3 * (size_t)nmom * npol) * sizeof(float) + 1024;
-  if (arena_reserve(st, AR_OPTICS, obytes, &obase)) return AA_EHIP;
+  int rcode = compute_latch(W, lp, st);
+  // With AA_BLOCK_STREAMS=2 (one stream count), consecutive optics blocks
+  // alternate between two library streams, each with its own optics block and
+  // work arena (half the budgets each), and block b + 1 starts once block b
+  // has enqueued its Fourier term BLOCK_MID_TERM, so that its first, full
+  // terms fill the GPU while block b's last terms, with few items left,
+  // finish: 2% more columns/s on 500,000 columns x 4 channels, but the
+  // kernels of the two streams overlap, so per-kernel times no longer add up
+  // to the step.  By default the blocks run on the caller stream one after
+  // the other.
+  if (!rcode && two && nblk > 1) {
+    rcode = blocks_two_streams(t, W, st, nblk, ob, nq, tracer_idx, chan_idx, d_qm, d_rh, obytes);
+    const hipError_t se = hipStreamSynchronize(st);
+    return rcode ? rcode : (se != hipSuccess ? AA_EHIP : 0);
+  }
+  char* obase = nullptr;
+  if (!rcode && arena_reserve(st, AR_OPTICS, obytes, &obase)) rcode = AA_EHIP;
   float* dt = (float*)obase;
   float* ds = dt + n3;
   float* dg = ds + n3;
   float* dp = dg + n3;
   W.tau = dt; W.ssa = ds; W.pmom = dp;
-  int rcode = compute_latch(W, lp, st);
   for (int b = 0; b < nblk && !rcode; ++b) {
     const int o0 = (int)(b * ob), no = std::min((int)ob, nobs - o0);
     rcode = launch_get_aop(t, km, no, nch, nq, tracer_idx, chan_idx, d_qm + (size_t)o0 * km * nq,

@@ -69,6 +69,12 @@ def test_fused_equals_host_path(gpu, monkeypatch, ncol, channels, N, nmom):
     monkeypatch.setenv("AA_CHUNK_ITEMS", "7")
     rad2, refl2, nf2 = fused(torch, mie, cols, channels, N, nmom, 1)
     assert np.array_equal(rad, rad2) and np.array_equal(nf, nf2)
+    # the blocks on two library streams instead of one after the other on the
+    # caller stream
+    monkeypatch.setenv("AA_BLOCK_STREAMS", "2")
+    rad4, refl4, nf4 = fused(torch, mie, cols, channels, N, nmom, 1)
+    monkeypatch.delenv("AA_BLOCK_STREAMS")
+    assert np.array_equal(rad, rad4) and np.array_equal(refl, refl4) and np.array_equal(nf, nf4)
     # the same through budgets set by the API (they override the environment),
     # after the arenas were released
     monkeypatch.delenv("AA_OPTICS_BYTES")

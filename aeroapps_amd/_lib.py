@@ -64,6 +64,9 @@ SIGNATURES = {
     "aa_vbrdf_modis": (_ci, [_ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip]),
     "aa_vlidort_scalar_cx": (_ci, [_ci, _ci, _ci, _dp, _ci, _ci, _ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                    _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _cd, _ci, _ci, _dp, _dp, _dp, _ip, _ip]),
+    "aa_vlidort_vector_cx": (_ci, [_ci, _ci, _ci, _dp, _ci, _ci, _ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                   _dp, _dp, _dp, _dp, _dp, _dp, _dp, _cd, _ci, _ci, _dp, _dp, _dp, _dp, _dp,
+                                   _dp, _dp, _ip, _ip]),
     "aa_vbrdf_coxmunk": (_ci, [_ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip]),
     "aa_column_radiance_modis_dev": (_ci, [_vp, _ci, _ci, _ci, _ci, _vp, _vp, _vp, _vp, _ci, _ci, _ci, _vp, _vp,
                                            _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _cd, _vp, _vp, _vp,

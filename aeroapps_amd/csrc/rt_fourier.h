@@ -481,10 +481,13 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M) {
   // ---- surface terms of the bottom layer (BVP_SURFACE_SETUP_HOM/BEAM,
   //      REG/vlidort_bvproblem.f90:440-626, 1284+; GET_BOA_SOURCE,
   //      REG/vlidort_intensity.f90:1096-1496).  Lambertian: Fourier 0 only,
-  //      the same reflected term in every quadrature row.  MODIS BRDF (scalar):
-  //      every Fourier term, AXBID(i,j) = w_j mu_j BRDF_F(M,i,j) per row, and
-  //      the BOA source weights w_j mu_j USER_BRDF_F(M,j).
-  const bool brdf = (S == 1) && W.brdf != nullptr;
+  //      the same reflected term in every quadrature row.  BRDF: every Fourier
+  //      term, AXBID(i,j) = w_j mu_j BRDF_F(M,i,j) per row, and the BOA source
+  //      weights w_j mu_j USER_BRDF_F(M,j).  With three Stokes components both
+  //      surfaces reflect I into I only (the Lambertian reflection matrix and
+  //      the Cox-Munk kernel, vsup/vbrdf/vbrdf_sup_kernels.f90:1018-1141, have
+  //      only the (1,1) element), so the terms act on Stokes component 0.
+  const bool brdf = W.brdf != nullptr;
   if (layer == L - 1 && (brdf || M == 0)) {
     const int g = global_item(W, it);
     const int ch = g - obs * W.nch;
@@ -695,7 +698,7 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
   const double* __restrict__ lb = W.lbuf + (size_t)it * km * LS;
   const double* __restrict__ bb = W.bbuf + (size_t)it * W.bstride;
   constexpr BROff RO(N);
-  const bool brdf = (S == 1) && W.brdf != nullptr;
+  const bool brdf = W.brdf != nullptr;
   const double* __restrict__ brec = brdf ? W.brdf + (size_t)it * W.brdf_stride : nullptr;
   const double albedo = brdf ? 0.0 : W.albedo[oi];
   const bool surf = brdf || ((M == 0) && (albedo != 0.0));

@@ -465,7 +465,7 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
   if (!rc) return AA_EARG;
   *rc = 0;
   const bool cx = u10m != nullptr;
-  if (cx && (vector || !v10m || !mr || !br || kernel_wt)) return AA_EARG;
+  if (cx && (!v10m || !mr || !br || kernel_wt || stokes1)) return AA_EARG;
   const bool modis = kernel_wt != nullptr || cx;   // a BRDF surface instead of the albedo
   if (km <= 0 || nch <= 0 || nobs < 0 || nmom <= 0 || npol <= 0) return AA_EARG;
   if (!rot || !depol || !tau || !ssa || !pmom || !he || (!albedo && !modis) || !sza || !raa || !vza || !radiance ||
@@ -475,7 +475,8 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
   // VLIDORT_Init of the MODIS driver (V283/VLIDORT_Mod.F90:369) rejects nMom
   // beyond the input maximum for the whole call
   if (modis && nmom > MAX_MOMENTS_IN) { *rc = 5; return 0; }
-  if (vector && (!q || !u || !flux_factor || npol < 6)) return AA_EARG;
+  // (the vector Cox-Munk driver sets no flux factor: VLIDORT_Init's 1.0)
+  if (vector && (!q || !u || (!flux_factor && !cx) || npol < 6)) return AA_EARG;
   if (precision != 0 && precision != 1) return AA_EARG;
   // stokes1: the vector driver run with NSTOKES = 1 (its DO_2OS_CORRECTION
   // option): Stokes-1 arithmetic on the vector driver's inputs (gas
@@ -517,7 +518,7 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
     d.sza = C.take<double>(nobs);
     d.raa = C.take<double>(nobs);
     d.vza = C.take<double>(nobs);
-    d.flux = vector ? C.take<double>(no) : nullptr;
+    d.flux = (vector && flux_factor) ? C.take<double>(no) : nullptr;
     d.rad = C.take<double>(no);
     d.refl = C.take<double>(no);
     d.q = S == 3 ? C.take<double>(no) : nullptr;
@@ -758,6 +759,29 @@ int aa_vlidort_scalar_cx(int km, int nch, int nobs, const double* channels, int 
                           pmom, he, nullptr, solar_zenith, relat_azymuth, sensor_zenith, nullptr, missing, precision,
                           radiance, reflectance, nullptr, nullptr, nfourier, rc, nullptr, nullptr, brdf, u10m, v10m,
                           mr);
+}
+
+int aa_vlidort_vector_cx(int km, int nch, int nobs, const double* channels, int nstreams, int plane_parallel,
+                         int nmom, int npol, const double* rot, const double* depol, const double* tau,
+                         const double* ssa, const double* pmom, const double* pe, const double* he, const double* te,
+                         const double* u10m, const double* v10m, const double* mr, const double* solar_zenith,
+                         const double* relat_azymuth, const double* sensor_zenith, double missing, int verbose,
+                         int precision, double* radiance, double* reflectance, double* q, double* u, double* brdf,
+                         double* brdf_q, double* brdf_u, int* nfourier, int* rc) {
+  (void)channels; (void)pe; (void)te; (void)verbose;
+  if (!u10m || !brdf_q || !brdf_u) return AA_EARG;
+  const int e = aa::lambert_host(true, km, nch, nobs, nstreams, plane_parallel, nmom, npol, rot, depol, nullptr,
+                                 tau, ssa, pmom, he, nullptr, solar_zenith, relat_azymuth, sensor_zenith, nullptr,
+                                 missing, precision, radiance, reflectance, q, u, nfourier, rc, nullptr, nullptr,
+                                 brdf, u10m, v10m, mr);
+  if (e || !rc || *rc) return e;
+  // BRDF_Q = BRDF_U = 0 where the driver ran the surface (GEO/VLIDORT_BRDF_CX.F90:724-725)
+  const size_t no = (size_t)nobs * nch;
+  for (size_t i = 0; i < no; ++i) {
+    const bool miss = std::fabs(brdf[i] / missing - 1.0) < 0.001;
+    brdf_q[i] = brdf_u[i] = miss ? missing : 0.0;
+  }
+  return 0;
 }
 
 int aa_vlidort_scalar_landmodis(int km, int nch, int nobs, const double* channels, int nstreams, int plane_parallel,

@@ -189,6 +189,21 @@ module aeroapps_amd_capi
       integer(c_int), intent(out) :: rc
     end function
 
+    integer(c_int) function aa_vlidort_vector_cx(km, nch, nobs, channels, nstreams, plane_parallel, nmom, &
+        npol, rot, depol, tau, ssa, pmom, pe, he, te, u10m, v10m, mr, sza, raa, vza, missing, verbose, &
+        precision, radiance, reflectance, q, u, brdf, brdf_q, brdf_u, nfourier, rc) &
+        bind(C, name="aa_vlidort_vector_cx")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: km, nch, nobs, nstreams, plane_parallel, nmom, npol
+      real(c_double), intent(in) :: channels(*), rot(*), depol(*), tau(*), ssa(*), pmom(*)
+      real(c_double), intent(in) :: pe(*), he(*), te(*), u10m(*), v10m(*), mr(*), sza(*), raa(*), vza(*)
+      real(c_double), value :: missing
+      integer(c_int), value :: verbose, precision
+      real(c_double), intent(out) :: radiance(*), reflectance(*), q(*), u(*), brdf(*), brdf_q(*), brdf_u(*)
+      type(c_ptr), value :: nfourier
+      integer(c_int), intent(out) :: rc
+    end function
+
     integer(c_int) function aa_vlidort_vector_lambert(km, nch, nobs, channels, nstreams, plane_parallel, nmom, &
         npol, rot, depol, alpha, tau, ssa, pmom, pe, he, te, albedo, sza, raa, vza, flux_factor, missing, verbose, &
         precision, radiance, reflectance, q, u, nfourier, rc) bind(C, name="aa_vlidort_vector_lambert")

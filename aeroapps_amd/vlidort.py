@@ -148,6 +148,32 @@ def scalar_cx(channels, nstreams, plane_parallel, ROT, depol, tau, ssa, g, pmom,
     return rad, refl, brdf, int(rc[0])
 
 
+def vector_cx(channels, nstreams, plane_parallel, ROT, depol, tau, ssa, pmom, pe, he, te, U10m, V10m, mr,
+              solar_zenith, relat_azymuth, sensor_zenith, MISSING, verbose=0, precision=0, return_nfourier=False):
+    """VLIDORT_Vector_CX_SingleGeom (src/Components/rtms/geo_vlidort/VLIDORT_BRDF_CX.F90:587-762),
+    the contract of the f2py ``VLIDORT_POLAR_.vector_cx``
+    (src/Components/rtms/leo_vlidort/VLIDORT_POLAR_py.F90:830-900): the Cox-Munk
+    ocean with NSTOKES = 3.  Returns (radiance, reflectance, BR, Q, U, BR_Q,
+    BR_U, rc), each array (nobs, nch)."""
+    channels, ROT, depol, tau, ssa, pmom, pe, he, te, u, v, m, sza, raa, vza = map(
+        _f64, (channels, ROT, depol, tau, ssa, pmom, pe, he, te, U10m, V10m, mr, solar_zenith, relat_azymuth,
+               sensor_zenith))
+    km, nch, nobs = tau.shape
+    nmom, npol = pmom.shape[3], pmom.shape[4]
+    rad, refl, br, q, uu, brq, bru = (np.zeros((nobs, nch), order="F") for _ in range(7))
+    nf = np.zeros((nobs, nch), dtype=np.int32, order="F")
+    rc = np.zeros(1, dtype=np.int32)
+    e = lib().aa_vlidort_vector_cx(km, nch, nobs, _ptr(channels), int(nstreams), int(bool(plane_parallel)), nmom,
+                                   npol, _ptr(ROT), _ptr(depol), _ptr(tau), _ptr(ssa), _ptr(pmom), _ptr(pe),
+                                   _ptr(he), _ptr(te), _ptr(u), _ptr(v), _ptr(m), _ptr(sza), _ptr(raa), _ptr(vza),
+                                   float(MISSING), int(verbose), int(precision), _ptr(rad), _ptr(refl), _ptr(q),
+                                   _ptr(uu), _ptr(br), _ptr(brq), _ptr(bru), _ptr(nf, _ip), _ptr(rc, _ip))
+    check(e, "aa_vlidort_vector_cx")
+    if return_nfourier:
+        return rad, refl, br, q, uu, brq, bru, int(rc[0]), nf
+    return rad, refl, br, q, uu, brq, bru, int(rc[0])
+
+
 def vbrdf_coxmunk(nstreams, sza, vza, raa, u10m, v10m, mr):
     """The Cox-Munk supplement alone for ngeom geometries (see vbrdf_modis)."""
     sza, vza, raa, u, v, m = (np.ascontiguousarray(np.atleast_1d(a), dtype=np.float64)

@@ -303,6 +303,24 @@ int aa_vlidort_scalar_cx(int km, int nch, int nobs, const double* channels, int 
                          const double* sensor_zenith, double missing, int verbose, int precision,
                          double* radiance, double* reflectance, double* brdf, int* nfourier, int* rc);
 
+/* Replaces VLIDORT_Vector_CX_SingleGeom (GEO/VLIDORT_BRDF_CX.F90:587-762; f2py
+ * VLIDORT_POLAR_.vector_cx, src/Components/rtms/leo_vlidort/
+ * VLIDORT_POLAR_py.F90:830-900): the same surface with NSTOKES = 3 and
+ * VLIDORT_Run_Vector, the flux factor VLIDORT_Init's 1.  Arrays as
+ * aa_vlidort_scalar_cx without g, pmom(km,nch,nobs,nMom,nPol) with nPol >= 6;
+ * q, u (nobs,nch) receive the Stokes components, brdf the exact direct-bounce
+ * BRDF, brdf_q and brdf_u 0 (the Cox-Munk kernel reflects I only).  Outputs of
+ * elements the reference skips (missing geometry, wind or refractive index)
+ * are MISSING. */
+int aa_vlidort_vector_cx(int km, int nch, int nobs, const double* channels, int nstreams, int plane_parallel,
+                         int nmom, int npol, const double* rot, const double* depol, const double* tau,
+                         const double* ssa, const double* pmom, const double* pe, const double* he,
+                         const double* te, const double* u10m, const double* v10m, const double* mr,
+                         const double* solar_zenith, const double* relat_azymuth, const double* sensor_zenith,
+                         double missing, int verbose, int precision, double* radiance, double* reflectance,
+                         double* q, double* u, double* brdf, double* brdf_q, double* brdf_u, int* nfourier,
+                         int* rc);
+
 /* The Cox-Munk supplement alone, as aa_vbrdf_modis with u10m, v10m, mr per
  * geometry (status 4: white-sky albedo outside (0,1], the kernel's Fourier
  * terms are then zero). */

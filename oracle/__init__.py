@@ -145,6 +145,26 @@ class ModisRef:
         self.fc.restype = None
         self.fc.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                             _dp, _dp, _dp, _dp, _dp, _dp, cd, _dp, _dp, _dp, _ip, _ip]
+        self.fvc = lib.ref_vector_cx
+        self.fvc.restype = None
+        self.fvc.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci] + [_dp] * 14 + [cd] + [_dp] * 7 + [_ip, _ip]
+
+    def vector_cx(self, km, channels, nstreams, nmom, rot, depol, tau, ssa, pmom, pe, he, te, u10m, v10m, mr,
+                  sza, raa, vza, missing=1e15, plane_parallel=False):
+        """VLIDORT_Vector_CX_SingleGeom: arrays as for scalar_cx without g (pmom with 6
+        polarisations); returns rad, refl, Q, U, BR, BR_Q, BR_U [nch, nobs], nf, rc."""
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        channels, rot, depol, tau, ssa, pmom, pe, he, te, u10m, v10m, mr, sza, raa, vza = map(
+            f, (channels, rot, depol, tau, ssa, pmom, pe, he, te, u10m, v10m, mr, sza, raa, vza))
+        nch, nobs, npol = len(channels), len(sza), pmom.shape[0]
+        out = [np.zeros((nch, nobs)) for _ in range(7)]
+        nf = np.zeros((nch, nobs), dtype=np.int32)
+        rc = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fvc, km, nch, nobs, _p(channels), nstreams, int(plane_parallel), nmom, npol,
+                        _p(rot), _p(depol), _p(tau), _p(ssa), _p(pmom), _p(pe), _p(he), _p(te), _p(u10m),
+                        _p(v10m), _p(mr), _p(sza), _p(raa), _p(vza), missing, *[_p(o) for o in out],
+                        _p(nf, np.int32), _p(rc, np.int32))
+        return (*out, nf, int(rc[0]))
 
     def vbrdf_coxmunk(self, nstreams, sza, vza, raa, u10m, v10m, m):
         n, nm = nstreams, 2 * nstreams
@@ -235,6 +255,9 @@ class RefDrivers:
         self.fc = lib.drv_scalar_cx
         self.fc.restype = None
         self.fc.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci] + [_dp] * 15 + [cd, _dp, _dp, _dp, _ip]
+        self.fvc = lib.drv_vector_cx
+        self.fvc.restype = None
+        self.fvc.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci] + [_dp] * 14 + [cd] + [_dp] * 7 + [_ip]
 
     @staticmethod
     def _f(*arrs):
@@ -323,6 +346,21 @@ class RefDrivers:
                         _p(u10m), _p(v10m), _p(mr), _p(sza), _p(raa), _p(vza), missing, _p(rad), _p(refl),
                         _p(br), _p(rc, np.int32))
         return rad, refl, br, int(rc[0])
+
+
+    def vector_cx(self, km, channels, nstreams, nmom, rot, depol, tau, ssa, pmom, pe, he, te, u10m, v10m, mr,
+                  sza, raa, vza, missing=1e15, plane_parallel=False):
+        """Returns rad, refl, Q, U, BR, BR_Q, BR_U [nch, nobs], rc."""
+        channels, rot, depol, tau, ssa, pmom, pe, he, te, u10m, v10m, mr, sza, raa, vza = self._f(
+            channels, rot, depol, tau, ssa, pmom, pe, he, te, u10m, v10m, mr, sza, raa, vza)
+        nch, nobs, npol = len(channels), len(sza), pmom.shape[0]
+        out = [np.zeros((nch, nobs)) for _ in range(7)]
+        rc = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fvc, km, nch, nobs, _p(channels), nstreams, int(plane_parallel), nmom, npol,
+                        _p(rot), _p(depol), _p(tau), _p(ssa), _p(pmom), _p(pe), _p(he), _p(te), _p(u10m),
+                        _p(v10m), _p(mr), _p(sza), _p(raa), _p(vza), missing, *[_p(o) for o in out],
+                        _p(rc, np.int32))
+        return (*out, int(rc[0]))
 
 
 _DRV_LIB = None

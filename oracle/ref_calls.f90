@@ -10,6 +10,7 @@
 !   drv_coef_depol        -> coef_depol                        (V283/VLIDORT_ScatMod.F90:245-260)
 !   drv_scalar_landmodis  -> VLIDORT_Scalar_LandMODIS          (GEO/VLIDORT_BRDF_MODIS.F90:35-202)
 !   drv_scalar_cx         -> VLIDORT_Scalar_CX                 (GEO/VLIDORT_BRDF_CX.F90:420-586)
+!   drv_vector_cx         -> VLIDORT_Vector_CX_SingleGeom      (GEO/VLIDORT_BRDF_CX.F90:587-762)
 !
 ! These drivers run VLIDORT_Run_Scalar / VLIDORT_Run_Vector
 ! (V283/VLIDORT_ScatMod.F90) and VLIDORT_Init (V283/VLIDORT_Mod.F90) over the
@@ -22,7 +23,9 @@
 !     the wrapper passes Q/U buffers padded by one channel so that write stays
 !     inside the buffer, and refuses (rc = -99) an observation list whose
 !     first observation has missing geometry (there i is undefined);
-!   * BR of an observation with missing geometry (MODIS and Cox-Munk drivers).
+!   * BR of an observation with missing geometry (MODIS and Cox-Munk drivers),
+!     and Q, U, BR, BR_Q, BR_U of the elements the vector Cox-Munk driver
+!     skips (missing geometry, winds or refractive index).
 ! The reference drivers do not return the Fourier-term count; the oracle's
 ! statement-by-statement harness (ref_driver.f90) does, and the tests require
 ! the two to agree bit for bit, which pins that count.
@@ -221,5 +224,37 @@ contains
          solar_zenith, relat_azymuth, sensor_zenith, miss, 0, radiance_VL, reflectance_VL, BR, irc)
     rc = irc
   end subroutine drv_scalar_cx
+
+  subroutine drv_vector_cx(km, nch, nobs, channels, nstreams, plane_parallel, nMom, &
+       nPol, ROT, depol, tau, ssa, pmom, pe, he, te, u10m, v10m, mr, &
+       solar_zenith, relat_azymuth, sensor_zenith, MISSING, &
+       radiance_VL, reflectance_VL, Q, U, BR, BR_Q, BR_U, rc) bind(C, name="drv_vector_cx")
+    integer(c_int), value, intent(in) :: km, nch, nobs, nstreams, plane_parallel, nMom, nPol
+    real(c_double), intent(in) :: channels(nch), ROT(km,nobs,nch), depol(nch)
+    real(c_double), intent(in) :: tau(km,nch,nobs), ssa(km,nch,nobs)
+    real(c_double), intent(in) :: pmom(km,nch,nobs,nMom,nPol)
+    real(c_double), intent(in) :: pe(km+1,nobs), he(km+1,nobs), te(km+1,nobs)
+    real(c_double), intent(in) :: u10m(nobs), v10m(nobs), mr(nch)
+    real(c_double), intent(in) :: solar_zenith(nobs), relat_azymuth(nobs), sensor_zenith(nobs)
+    real(c_double), value, intent(in) :: MISSING
+    real(c_double), intent(out) :: radiance_VL(nobs,nch), reflectance_VL(nobs,nch)
+    real(c_double), intent(out) :: Q(nobs,nch), U(nobs,nch), BR(nobs,nch), BR_Q(nobs,nch), BR_U(nobs,nch)
+    integer(c_int), intent(out) :: rc
+    integer :: irc
+    real*8 :: miss
+    miss = MISSING
+    radiance_VL = MISSING
+    reflectance_VL = MISSING
+    Q = MISSING
+    U = MISSING
+    BR = MISSING
+    BR_Q = MISSING
+    BR_U = MISSING
+    call VLIDORT_Vector_CX_SingleGeom(km, nch, nobs, channels, nstreams, plane_parallel /= 0, nMom, &
+         nPol, ROT, depol, tau, ssa, pmom, pe, he, te, u10m, v10m, mr, &
+         solar_zenith, relat_azymuth, sensor_zenith, miss, 0, radiance_VL, reflectance_VL, &
+         Q, U, BR, BR_Q, BR_U, irc)
+    rc = irc
+  end subroutine drv_vector_cx
 
 end module ref_calls_m

@@ -3,7 +3,7 @@
 ! procedures (VLIDORT_Scalar_Lambert / VLIDORT_Vector_Lambert from module
 ! VLIDORT_LAMBERT, single- and multi-geometry, with DO_2OS_CORRECTION,
 ! VLIDORT_Scalar_LandMODIS from module VLIDORT_BRDF_MODIS,
-! VLIDORT_Scalar_CX from module VLIDORT_BRDF_CX,
+! VLIDORT_Scalar_CX and VLIDORT_Vector_CX from module VLIDORT_BRDF_CX,
 ! VLIDORT_getAOPvector with a Chem_Mie object, Chem_MieCreate from a
 ! resource file and the Chem_MieQuery family) and writes their outputs to a
 ! raw file.
@@ -83,7 +83,7 @@ program shim_driver
   real*8, allocatable :: channels(:), rot(:,:,:), depol(:), alpha(:,:,:), tau(:,:,:), ssa(:,:,:), g(:,:,:)
   real*8, allocatable :: pmom(:,:,:,:,:), pe(:,:), he(:,:), te(:,:), albedo(:,:), sza(:), raa(:), vza(:)
   real*8, allocatable :: ff(:,:), rad(:,:), refl(:,:), q(:,:), uu(:,:), kw(:,:,:), kp(:,:,:), br(:,:)
-  real*8, allocatable :: u10(:), v10(:), mr(:), brx(:,:,:)
+  real*8, allocatable :: u10(:), v10(:), mr(:), brx(:,:,:), brq(:,:), bru(:,:)
   ! Mie mode
   integer :: nq, ntab, nrh, nbin, t, b
   type(Chem_Mie) :: mie
@@ -171,6 +171,22 @@ program shim_driver
     end if
     open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
     write(u) rc, rad3, refl3, q3, u3
+    close(u)
+  else if (mode == 9) then
+    ! VLIDORT_Vector_CX (generic -> SingleGeom)
+    read(u) km, nch, nobs, nstreams, nmom, npol, ipp
+    read(u) missing
+    allocate(channels(nch), rot(km,nobs,nch), depol(nch), tau(km,nch,nobs), ssa(km,nch,nobs))
+    allocate(pmom(km,nch,nobs,nmom,npol), pe(km+1,nobs), he(km+1,nobs), te(km+1,nobs))
+    allocate(u10(nobs), v10(nobs), mr(nch), sza(nobs), raa(nobs), vza(nobs))
+    allocate(rad(nobs,nch), refl(nobs,nch), q(nobs,nch), uu(nobs,nch), br(nobs,nch), brq(nobs,nch), bru(nobs,nch))
+    read(u) channels, rot, depol, tau, ssa, pmom, pe, he, te, u10, v10, mr, sza, raa, vza
+    close(u)
+    call VLIDORT_Vector_CX(km, nch, nobs, channels, nstreams, ipp /= 0, nmom, npol, rot, depol, tau, ssa, &
+                           pmom, pe, he, te, u10, v10, mr, sza, raa, vza, missing, 0, rad, refl, q, uu, &
+                           br, brq, bru, rc)
+    open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+    write(u) rc, rad, refl, q, uu, br, brq, bru
     close(u)
   else if (mode == 5) then
     ! VLIDORT_Scalar_CX

@@ -18,9 +18,12 @@
   Fourier-term counts are stored with them.
 * supplement_cx.npz, cx_*.npz: the same for the Cox-Munk ocean surface
   (VLIDORT_CoxMunk_SingleGeom supplement, VLIDORT_Scalar_CX radiances).
+* vcx_*.npz: the vector Cox-Munk driver (VLIDORT_Vector_CX_SingleGeom:
+  NSTOKES = 3, I, Q, U and the direct-bounce BRDF outputs).
 
 Run here (the reference is only available in this container):
-    python tests/golden/make_golden_brdf.py
+    python tests/golden/make_golden_brdf.py          # everything
+    python tests/golden/make_golden_brdf.py vcx      # the vector Cox-Munk cases only
 """
 import json
 import os
@@ -212,7 +215,51 @@ def cx_case(R, ncol, channels, nstreams, nmom, edge, seed=13):
                 out_radiance=rad, out_reflectance=refl, out_br=br, out_nfourier=nf, out_rc=rcr)
 
 
+VCX_CASES = {
+    # name: (ncol, channels, nstreams, nmom, edge)
+    "vcx_n4_2ch": (6, (470.0, 660.0), 4, 32, False),
+    "vcx_n2_edge": (9, (550.0,), 2, 32, True),
+    "vcx_n3": (3, (870.0,), 3, 32, False),
+}
+
+
+def vcx_case(R, ncol, channels, nstreams, nmom, edge, seed=17):
+    cols = S.make_columns(ncol, 72, channels, seed=seed)
+    tabs = S.make_mie_tables(channels, nmom=nmom, npol=6)
+    tau, ssa, g, pm, rc = CM.get_aop_vector(tabs, channels, tabs[0].lambda_m, S.TRACERS, cols.qm, cols.rh, nmom, 6)
+    assert rc == 0
+    u, v, mr = S.make_ocean(ncol, channels, seed=seed)
+    sza, vza, raa = cols.sza.copy(), cols.vza.copy(), cols.raa.copy()
+    if edge:
+        vza[1] = 0.0                 # nadir view
+        sza[2] = S.MISSING           # missing geometry
+        u[3] = S.MISSING             # missing wind: element MISSING
+        sza[4], vza[4], raa[4] = 35.0, 35.0, 180.0   # specular glint
+        sza[5] = 0.0                 # overhead sun latches nadir FO for later columns
+        u[7], v[7] = 60.0, 10.0      # wind capped at 50 m/s
+    f = lambda a: np.asarray(a, dtype=np.float64)
+    args = (72, cols.channels, nstreams, nmom, cols.rot, cols.depol, f(tau), f(ssa), f(pm), cols.pe, cols.he,
+            cols.te, u, v, mr, sza, raa, vza)
+    rad, refl, q, uu, br, brq, bru, nf, rcr = R.vector_cx(*args)
+    d = oracle.drivers().vector_cx(*args)
+    assert d[-1] == rcr
+    _same_as_driver((rad, refl, q, uu, br, brq, bru), d[:7])
+    return dict(km=72, channels=np.asarray(channels, float), nmom=nmom, npol=6, meta_nstreams=nstreams,
+                tau=tau, ssa=ssa, pmom=pm, rot=cols.rot, depol=cols.depol, pe=cols.pe, he=cols.he,
+                te=cols.te, u10m=u, v10m=v, mr=mr, sza=sza, vza=vza, raa=raa,
+                out_radiance=rad, out_reflectance=refl, out_q=q, out_u=uu, out_br=br, out_br_q=brq,
+                out_br_u=bru, out_nfourier=nf, out_rc=rcr)
+
+
 def main():
+    if sys.argv[1:] == ["vcx"]:
+        R = oracle.drv_harness_modis()
+        assert R is not None, "build oracle/_ref first (make -C oracle -f Makefile.ref)"
+        for name, c in VCX_CASES.items():
+            d = vcx_case(R, *c)
+            np.savez_compressed(os.path.join(OUT, name + ".npz"), **d)
+            print(name, d["out_rc"], d["out_radiance"].ravel()[:4], d["out_nfourier"].ravel()[:6])
+        return
     os.makedirs(OUT, exist_ok=True)
     with open(os.path.join(OUT, "selftest.json"), "w") as fh:
         json.dump(selftest_vectors(), fh, indent=1)
@@ -226,6 +273,10 @@ def main():
     np.savez_compressed(os.path.join(OUT, "supplement_cx.npz"), **supplement_cx(R))
     for name, c in CX_CASES.items():
         d = cx_case(R, *c)
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), **d)
+        print(name, d["out_rc"], d["out_radiance"].ravel()[:4], d["out_nfourier"].ravel()[:6])
+    for name, c in VCX_CASES.items():
+        d = vcx_case(R, *c)
         np.savez_compressed(os.path.join(OUT, name + ".npz"), **d)
         print(name, d["out_rc"], d["out_radiance"].ravel()[:4], d["out_nfourier"].ravel()[:6])
 

@@ -170,3 +170,48 @@ def test_scalar_cx_mixed_vs_reference(gpu):
     same = nf == nf0
     assert same.mean() >= 0.9
     assert np.max(np.abs(rad[same] - want[same]) / np.abs(want[same])) < 1e-5
+
+
+# ---------------------------------------------------------------- vector Cox-Munk
+VCX_CASES = ["vcx_n4_2ch", "vcx_n2_edge", "vcx_n3"]
+
+
+def _run_vcx(d, precision):
+    f = lambda a: np.asarray(a, dtype=np.float64)
+    return V.vector_cx(d["channels"], int(d["meta_nstreams"]), False, d["rot"].T, d["depol"], f(d["tau"]).T,
+                       f(d["ssa"]).T, f(d["pmom"]).T, d["pe"].T, d["he"].T, d["te"].T, d["u10m"], d["v10m"],
+                       d["mr"], d["sza"], d["raa"], d["vza"], S.MISSING, precision=precision, return_nfourier=True)
+
+
+@pytest.mark.parametrize("name", VCX_CASES)
+def test_vector_cx_fp64_vs_reference(gpu, name):
+    """VLIDORT_Vector_CX_SingleGeom: I and reflectance within 1e-9, Q and U
+    within 1e-9 of |I|, the direct-bounce BRDF to rounding, BRDF_Q = BRDF_U =
+    0, identical Fourier counts, MISSING where the reference skips."""
+    print(f"case vector cx fp64 {name}", flush=True)
+    d = _load(name)
+    rad, refl, br, q, u, brq, bru, rc, nf = _run_vcx(d, 0)
+    assert rc == int(d["out_rc"]) == 0
+    assert max_rel(rad, d["out_radiance"].T) < RTOL_FP64
+    assert max_rel(refl, d["out_reflectance"].T) < RTOL_FP64
+    I = np.abs(d["out_radiance"].T)
+    ok = np.abs(I / S.MISSING - 1) > 1e-3
+    for got, key in ((q, "out_q"), (u, "out_u")):
+        want = d[key].T
+        assert (np.abs(got[~ok] / S.MISSING - 1) < 1e-3).all()
+        assert (np.abs(got - want)[ok] / I[ok]).max() < RTOL_FP64, key
+    assert max_rel(br, d["out_br"].T) < 1e-13
+    assert np.array_equal(brq, d["out_br_q"].T) and np.array_equal(bru, d["out_br_u"].T)
+    assert np.array_equal(nf, d["out_nfourier"].T)
+
+
+def test_vector_cx_mixed_vs_reference(gpu):
+    print("case vector cx mixed", flush=True)
+    d = _load("vcx_n4_2ch")
+    rad, refl, br, q, u, brq, bru, rc, nf = _run_vcx(d, 1)
+    want, nf0 = d["out_radiance"].T, d["out_nfourier"].T
+    same = nf == nf0
+    assert same.mean() >= 0.9
+    assert np.max(np.abs(rad[same] - want[same]) / np.abs(want[same])) < 1e-5
+    for got, key in ((q, "out_q"), (u, "out_u")):
+        assert np.max(np.abs(got - d[key].T)[same] / np.abs(want[same])) < 1e-5, key

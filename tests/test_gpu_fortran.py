@@ -99,6 +99,30 @@ def test_scalar_cx_through_fortran_shim(gpu, tmp_path):
     assert max_rel(arr[2], d["out_br"]) < 1e-13
 
 
+def test_vector_cx_through_fortran_shim(gpu, tmp_path):
+    print("case fortran vcx_n2_edge", flush=True)
+    z = np.load(os.path.join(ROOT, "tests", "golden", "brdf", "vcx_n2_edge.npz"), allow_pickle=False)
+    d = {k: z[k] for k in z.files}
+    km, nch, nobs = int(d["km"]), len(d["channels"]), len(d["sza"])
+    f8 = lambda a: np.ascontiguousarray(a, dtype=np.float64).tobytes()
+    hdr = np.array([9, km, nch, nobs, int(d["meta_nstreams"]), int(d["nmom"]), int(d["npol"]), 0],
+                   dtype=np.int32).tobytes() + np.float64(S.MISSING).tobytes()
+    body = b"".join(f8(d[k]) for k in ("channels", "rot", "depol", "tau", "ssa", "pmom", "pe", "he", "te",
+                                       "u10m", "v10m", "mr", "sza", "raa", "vza"))
+    out = _run(tmp_path, hdr + body)
+    rc = int(np.frombuffer(out[:4], dtype=np.int32)[0])
+    arr = np.frombuffer(out[4:], dtype=np.float64).reshape(7, nch, nobs)
+    assert rc == int(d["out_rc"]) == 0
+    assert max_rel(arr[0], d["out_radiance"]) < RTOL_FP64
+    assert max_rel(arr[1], d["out_reflectance"]) < RTOL_FP64
+    I = np.abs(d["out_radiance"])
+    ok = np.abs(I / S.MISSING - 1) > 1e-3
+    for k, key in ((2, "out_q"), (3, "out_u")):
+        assert (np.abs(arr[k] - d[key])[ok] / I[ok]).max() < RTOL_FP64, key
+    assert max_rel(arr[4], d["out_br"]) < 1e-13
+    assert np.array_equal(arr[5], d["out_br_q"]) and np.array_equal(arr[6], d["out_br_u"])
+
+
 def test_get_aop_vector_through_fortran_shim(gpu, tmp_path):
     channels = (550.0, 870.0)
     nmom, npol = 16, 6

@@ -3,7 +3,7 @@
 * The statement-by-statement harness (oracle/ref_driver.f90), which the golden
   generators use for the Fourier-term counts, reproduces the reference's
   compiled drivers (VLIDORT_Scalar_Lambert, VLIDORT_Vector_Lambert,
-  VLIDORT_Scalar_LandMODIS, VLIDORT_Scalar_CX over VLIDORT_ScatMod /
+  VLIDORT_Scalar_LandMODIS, VLIDORT_Scalar_CX, VLIDORT_Vector_CX over VLIDORT_ScatMod /
   VLIDORT_SurfaceMod, oracle/_ref/libvlidort_drv.so) bit for bit, edge cases
   included.  These run where the reference was built (skipped elsewhere).
 * The Rayleigh restatement oracle/prep.py equals the reference's
@@ -100,6 +100,21 @@ def test_harness_equals_reference_brdf_drivers(surface):
         assert np.array_equal(a[k], b[k]), k
     # and the stored golden is that output
     assert np.array_equal(a[0], d["out_radiance"])
+
+
+@needs_drivers
+def test_harness_equals_reference_vector_cx_driver():
+    d = np.load(os.path.join(GOLDEN, "brdf", "vcx_n2_edge.npz"), allow_pickle=False)
+    D, H = oracle.drivers(), oracle.drv_harness_modis()
+    args = (72, d["channels"], int(d["meta_nstreams"]), int(d["nmom"]), d["rot"], d["depol"], f64(d["tau"]),
+            f64(d["ssa"]), f64(d["pmom"]), d["pe"], d["he"], d["te"], d["u10m"], d["v10m"], d["mr"], d["sza"],
+            d["raa"], d["vza"])
+    a, b = H.vector_cx(*args), D.vector_cx(*args)
+    for k in range(7):   # I, reflectance, Q, U, BRDF, BRDF_Q, BRDF_U
+        assert np.array_equal(a[k], b[k]), k
+    assert a[-1] == b[-1] == 0
+    assert np.array_equal(a[0], d["out_radiance"]) and np.array_equal(a[2], d["out_q"])
+    assert np.array_equal(a[7], d["out_nfourier"])
 
 
 def test_rot_calc_matches_reference_bit_for_bit():

@@ -31,7 +31,7 @@ COMMON = [
 # the compiler default.
 NO_CONTRACT = {"mie.hip", "prep.hip", "brdf.hip", "rt_setup.hip"}
 
-FOURIER_PARTS = [(n, 1) for n in range(1, 13)] + [(n, 3) for n in range(1, 5)]
+FOURIER_PARTS = [(n, 1) for n in range(1, 21)] + [(n, 3) for n in range(1, 5)]
 
 
 def _jobs():
@@ -43,10 +43,14 @@ def _jobs():
         if f in NO_CONTRACT:
             flags.append("-ffp-contract=off")
         jobs.append((os.path.join(CSRC, f), os.path.join(BUILD, f[:-4] + ".o"), flags))
-    for n, s in FOURIER_PARTS:
+    # the Fourier parts first, largest n = N*S first: their compile time grows
+    # steeply with n (the per-thread eigensolver is unrolled), so starting them
+    # early keeps the parallel build short
+    parts = []
+    for n, s in sorted(FOURIER_PARTS, key=lambda p: -p[0] * p[1]):
         flags = list(COMMON) + [f"-DAA_PART_N={n}", f"-DAA_PART_S={s}"]
-        jobs.append((os.path.join(CSRC, "rt_fourier_inst.hip"), os.path.join(BUILD, f"rt_fourier_{n}_{s}.o"), flags))
-    return jobs
+        parts.append((os.path.join(CSRC, "rt_fourier_inst.hip"), os.path.join(BUILD, f"rt_fourier_{n}_{s}.o"), flags))
+    return parts + jobs
 
 
 def _compile(job):

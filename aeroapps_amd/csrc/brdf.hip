@@ -400,7 +400,7 @@ int rt_brdf(const RTWork& W, int nitems, hipStream_t st) {
     if (W.kernel_wt) hipLaunchKernelGGL((k_brdf<NN, false>), dim3(nitems), dim3(64), 0, st, W, nitems);  \
     else hipLaunchKernelGGL((k_brdf<NN, true>), dim3(nitems), dim3(64), 0, st, W, nitems);             \
     break;
-    AA_BR(1) AA_BR(2) AA_BR(3) AA_BR(4) AA_BR(5) AA_BR(6) AA_BR(7) AA_BR(8) AA_BR(9) AA_BR(10) AA_BR(11) AA_BR(12)
+    AA_BR(1) AA_BR(2) AA_BR(3) AA_BR(4) AA_BR(5) AA_BR(6) AA_BR(7) AA_BR(8) AA_BR(9) AA_BR(10) AA_BR(11) AA_BR(12) AA_BR(13) AA_BR(14) AA_BR(15) AA_BR(16) AA_BR(17) AA_BR(18) AA_BR(19) AA_BR(20)
 #undef AA_BR
     default: return -4;
   }
@@ -470,7 +470,7 @@ int vbrdf_alone(bool cm, int nstreams, int ngeom, const double* sza, const doubl
       hipLaunchKernelGGL(k_brdf_only<NN>, dim3(ngeom), dim3(64), 0, 0, ngeom, d_tab, d_kw, d_kp, d_g, d_g + ng, \
                          d_g + 2 * ng, d_rec, RO.size(), d_db, d_ok);                                           \
     break;
-      AA_BR(1) AA_BR(2) AA_BR(3) AA_BR(4) AA_BR(5) AA_BR(6) AA_BR(7) AA_BR(8) AA_BR(9) AA_BR(10) AA_BR(11) AA_BR(12)
+      AA_BR(1) AA_BR(2) AA_BR(3) AA_BR(4) AA_BR(5) AA_BR(6) AA_BR(7) AA_BR(8) AA_BR(9) AA_BR(10) AA_BR(11) AA_BR(12) AA_BR(13) AA_BR(14) AA_BR(15) AA_BR(16) AA_BR(17) AA_BR(18) AA_BR(19) AA_BR(20)
 #undef AA_BR
     }
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) fail();

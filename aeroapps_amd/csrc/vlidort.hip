@@ -32,7 +32,8 @@ const std::vector<double>& brdf_table_cached(int N, int A);
 #define AA_DECL(N, S) \
   int rt_fourier_##N##_##S(const RTWork& W, int nitems, int* const lists[2], int* counts, hipStream_t st);
 AA_DECL(1, 1) AA_DECL(2, 1) AA_DECL(3, 1) AA_DECL(4, 1) AA_DECL(5, 1) AA_DECL(6, 1) AA_DECL(7, 1) AA_DECL(8, 1)
-AA_DECL(9, 1) AA_DECL(10, 1) AA_DECL(11, 1) AA_DECL(12, 1)
+AA_DECL(9, 1) AA_DECL(10, 1) AA_DECL(11, 1) AA_DECL(12, 1) AA_DECL(13, 1) AA_DECL(14, 1) AA_DECL(15, 1)
+AA_DECL(16, 1) AA_DECL(17, 1) AA_DECL(18, 1) AA_DECL(19, 1) AA_DECL(20, 1)
 AA_DECL(1, 3) AA_DECL(2, 3) AA_DECL(3, 3) AA_DECL(4, 3)
 #undef AA_DECL
 
@@ -53,6 +54,14 @@ int rt_fourier(const RTWork& W, int nitems, int* const lists[2], int* counts, hi
       case 10: return rt_fourier_10_1(W, nitems, lists, counts, st);
       case 11: return rt_fourier_11_1(W, nitems, lists, counts, st);
       case 12: return rt_fourier_12_1(W, nitems, lists, counts, st);
+      case 13: return rt_fourier_13_1(W, nitems, lists, counts, st);
+      case 14: return rt_fourier_14_1(W, nitems, lists, counts, st);
+      case 15: return rt_fourier_15_1(W, nitems, lists, counts, st);
+      case 16: return rt_fourier_16_1(W, nitems, lists, counts, st);
+      case 17: return rt_fourier_17_1(W, nitems, lists, counts, st);
+      case 18: return rt_fourier_18_1(W, nitems, lists, counts, st);
+      case 19: return rt_fourier_19_1(W, nitems, lists, counts, st);
+      case 20: return rt_fourier_20_1(W, nitems, lists, counts, st);
       default: return -4;
     }
   }
@@ -173,9 +182,17 @@ void quad_table(int N, int S, int nmoments, std::vector<double>& q) {
       case 9: quad_table_t<1, 18>(N, nmoments, q); break;
       case 10: quad_table_t<1, 20>(N, nmoments, q); break;
       case 11: quad_table_t<1, 22>(N, nmoments, q); break;
-      default: quad_table_t<1, 24>(N, nmoments, q); break;
+      case 12: quad_table_t<1, 24>(N, nmoments, q); break;
+      case 13: quad_table_t<1, 26>(N, nmoments, q); break;
+      case 14: quad_table_t<1, 28>(N, nmoments, q); break;
+      case 15: quad_table_t<1, 30>(N, nmoments, q); break;
+      case 16: quad_table_t<1, 32>(N, nmoments, q); break;
+      case 17: quad_table_t<1, 34>(N, nmoments, q); break;
+      case 18: quad_table_t<1, 36>(N, nmoments, q); break;
+      case 19: quad_table_t<1, 38>(N, nmoments, q); break;
+      default: quad_table_t<1, 40>(N, nmoments, q); break;
     }
-    static_assert(MAX_STREAMS == 12, "one quadrature-table case per scalar stream count");
+    static_assert(MAX_STREAMS == 20, "one quadrature-table case per scalar stream count");
   } else {
     switch (N) {
       case 1: quad_table_t<3, 2>(N, nmoments, q); break;

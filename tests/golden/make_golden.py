@@ -49,6 +49,8 @@ CASES = {
     # the stream counts of the reference's own callers (nstreams 10, 12)
     "scalar_n10": (3, (550.0,), 10, 32, False, False, False),
     "scalar_n12": (3, (470.0, 870.0), 12, 64, False, False, False),
+    "scalar_n16": (2, (550.0,), 16, 64, False, False, False),
+    "scalar_n20": (2, (660.0,), 20, 64, False, False, False),
     "vector_n2_km1": (4, (550.0,), 2, 32, True, False, False, True),
 }
 

@@ -24,6 +24,7 @@
 Run here (the reference is only available in this container):
     python tests/golden/make_golden_brdf.py          # everything
     python tests/golden/make_golden_brdf.py vcx      # the vector Cox-Munk cases only
+    python tests/golden/make_golden_brdf.py cx_n20   # named cases only
 """
 import json
 import os
@@ -107,6 +108,7 @@ RAD_CASES = {
     "modis_n2": (5, (550.0,), 2, 32, False),
     "modis_n8": (3, (550.0,), 8, 32, False),
     "modis_n12": (2, (550.0,), 12, 32, False),
+    "modis_n16": (2, (550.0,), 16, 64, False),
 }
 
 
@@ -185,6 +187,7 @@ CX_CASES = {
     "cx_n4_edge": (9, (550.0,), 4, 32, True),
     "cx_n8": (3, (550.0,), 8, 32, False),
     "cx_n12": (2, (550.0,), 12, 32, False),
+    "cx_n20": (2, (550.0,), 20, 64, False),
 }
 
 
@@ -252,11 +255,19 @@ def vcx_case(R, ncol, channels, nstreams, nmom, edge, seed=17):
 
 
 def main():
-    if sys.argv[1:] == ["vcx"]:
+    names = sys.argv[1:]
+    if names:   # only the named radiance cases ("vcx": every vector Cox-Munk case)
         R = oracle.drv_harness_modis()
         assert R is not None, "build oracle/_ref first (make -C oracle -f Makefile.ref)"
-        for name, c in VCX_CASES.items():
-            d = vcx_case(R, *c)
+        if names == ["vcx"]:
+            names = list(VCX_CASES)
+        for name in names:
+            if name in RAD_CASES:
+                d = radiance_case(R, *RAD_CASES[name])
+            elif name in CX_CASES:
+                d = cx_case(R, *CX_CASES[name])
+            else:
+                d = vcx_case(R, *VCX_CASES[name])
             np.savez_compressed(os.path.join(OUT, name + ".npz"), **d)
             print(name, d["out_rc"], d["out_radiance"].ravel()[:4], d["out_nfourier"].ravel()[:6])
         return

@@ -22,7 +22,7 @@ from _fixtures import RTOL_FP64, max_rel
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 BDIR = os.path.join(HERE, "golden", "brdf")
-RAD_CASES = ["modis_n6_4ch", "modis_n4_edge", "modis_n2", "modis_n8", "modis_n12"]
+RAD_CASES = ["modis_n6_4ch", "modis_n4_edge", "modis_n2", "modis_n8", "modis_n12", "modis_n16"]
 
 
 def _load(name):
@@ -124,7 +124,7 @@ def test_fused_modis_equals_host_path(gpu):
 
 
 # ---------------------------------------------------------------- Cox-Munk ocean
-CX_CASES = ["cx_n6_2ch", "cx_n4_edge", "cx_n8", "cx_n12"]
+CX_CASES = ["cx_n6_2ch", "cx_n4_edge", "cx_n8", "cx_n12", "cx_n20"]
 
 
 def test_coxmunk_supplement_vs_reference(gpu):

@@ -59,6 +59,9 @@ def test_fp64_matches_reference_golden(gpu, name):
     # n = 9..12 scalar: 32-lane groups with one (N = 9) or two (N = 11) row slots
     (6, (550.0,), 9, 32),
     (4, (870.0,), 11, 32),
+    # n = 13..20: 32-lane groups (n <= 16) and 64-lane groups with one row slot
+    (2, (550.0,), 14, 32),
+    (2, (470.0,), 17, 40),
 ])
 def test_fp64_matches_oracle_seeded(gpu, ncol, channels, nstreams, nmom):
     print(f"case seeded ncol={ncol} N={nstreams} nmom={nmom}", flush=True)

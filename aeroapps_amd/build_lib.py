@@ -31,7 +31,7 @@ COMMON = [
 # the compiler default.
 NO_CONTRACT = {"mie.hip", "prep.hip", "brdf.hip", "rt_setup.hip"}
 
-FOURIER_PARTS = [(n, 1) for n in range(1, 21)] + [(n, 3) for n in range(1, 5)]
+FOURIER_PARTS = [(n, 1) for n in range(1, 21)] + [(n, 3) for n in range(1, 7)]
 
 
 def _jobs():

@@ -19,7 +19,8 @@ namespace aa {
 
 // Compiled maxima of the radiance kernels.
 constexpr int MAX_LAYERS = 72;       // V283/vlidort_def/vlidort_pars.f90:140
-constexpr int MAX_STREAMS = 20;      // per hemisphere, scalar (the vector path: 4); the workloads use 2..8
+constexpr int MAX_STREAMS = 20;        // streams per hemisphere, scalar; the workloads use 2..8
+constexpr int MAX_STREAMS_VECTOR = 6;  // streams per hemisphere with three Stokes components (n = 18)
 constexpr int MAX_MOMENTS_IN = 1000; // V283/vlidort_def/vlidort_pars.f90:149
 constexpr int MAX_OBSGEOMS = 2;      // MAX_SZANGLES / MAX_USER_OBSGEOMS, V283/vlidort_def/vlidort_pars.f90:162,177
 constexpr int NRH_MAP = 991;         // CB/Chem_MieTableMod.F90:81-82

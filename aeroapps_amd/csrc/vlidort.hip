@@ -34,7 +34,7 @@ const std::vector<double>& brdf_table_cached(int N, int A);
 AA_DECL(1, 1) AA_DECL(2, 1) AA_DECL(3, 1) AA_DECL(4, 1) AA_DECL(5, 1) AA_DECL(6, 1) AA_DECL(7, 1) AA_DECL(8, 1)
 AA_DECL(9, 1) AA_DECL(10, 1) AA_DECL(11, 1) AA_DECL(12, 1) AA_DECL(13, 1) AA_DECL(14, 1) AA_DECL(15, 1)
 AA_DECL(16, 1) AA_DECL(17, 1) AA_DECL(18, 1) AA_DECL(19, 1) AA_DECL(20, 1)
-AA_DECL(1, 3) AA_DECL(2, 3) AA_DECL(3, 3) AA_DECL(4, 3)
+AA_DECL(1, 3) AA_DECL(2, 3) AA_DECL(3, 3) AA_DECL(4, 3) AA_DECL(5, 3) AA_DECL(6, 3)
 #undef AA_DECL
 
 // Fourier-term kernels for the call's (streams, Stokes) pair; -4 when the
@@ -70,6 +70,8 @@ int rt_fourier(const RTWork& W, int nitems, int* const lists[2], int* counts, hi
     case 2: return rt_fourier_2_3(W, nitems, lists, counts, st);
     case 3: return rt_fourier_3_3(W, nitems, lists, counts, st);
     case 4: return rt_fourier_4_3(W, nitems, lists, counts, st);
+    case 5: return rt_fourier_5_3(W, nitems, lists, counts, st);
+    case 6: return rt_fourier_6_3(W, nitems, lists, counts, st);
     default: return -4;
   }
 }
@@ -198,12 +200,15 @@ void quad_table(int N, int S, int nmoments, std::vector<double>& q) {
       case 1: quad_table_t<3, 2>(N, nmoments, q); break;
       case 2: quad_table_t<3, 4>(N, nmoments, q); break;
       case 3: quad_table_t<3, 6>(N, nmoments, q); break;
-      default: quad_table_t<3, 8>(N, nmoments, q); break;
+      case 4: quad_table_t<3, 8>(N, nmoments, q); break;
+      case 5: quad_table_t<3, 10>(N, nmoments, q); break;
+      default: quad_table_t<3, 12>(N, nmoments, q); break;
     }
+    static_assert(MAX_STREAMS_VECTOR == 6, "one quadrature-table case per vector stream count");
   }
 }
 
-int max_streams(int S) { return S == 1 ? MAX_STREAMS : 4; }
+int max_streams(int S) { return S == 1 ? MAX_STREAMS : MAX_STREAMS_VECTOR; }
 
 // Quadrature tables are built once per (N, S, nmoments) and kept for the
 // process lifetime (the host copy source of every asynchronous upload).

@@ -31,7 +31,8 @@ enum {
   AA_EARG = -1,        /* bad size / null pointer / unsupported option */
   AA_EHIP = -2,        /* HIP runtime error (allocation, launch) */
   AA_ENOTABLE = -3,    /* table handle has no species table */
-  AA_EDIMS = -4,       /* dimension beyond the compiled maxima (streams, layers, moments) */
+  AA_EDIMS = -4,       /* dimension beyond the compiled maxima (streams: 20 per hemisphere scalar,
+                          6 with three Stokes components; layers; moments) */
   AA_EUNSUP = -5,      /* a reference option this library does not implement (DO_BOA) */
   AA_ENOTRES = -6,     /* optics tables not yet resident on the device (see the _noalloc entry) */
   AA_ENOFILE = -7      /* a file cannot be read */

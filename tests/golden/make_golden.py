@@ -52,6 +52,9 @@ CASES = {
     "scalar_n16": (2, (550.0,), 16, 64, False, False, False),
     "scalar_n20": (2, (660.0,), 20, 64, False, False, False),
     "vector_n2_km1": (4, (550.0,), 2, 32, True, False, False, True),
+    # vector stream counts above 4 per hemisphere (n = 15, 18 Stokes-streams)
+    "vector_n5": (2, (550.0,), 5, 32, True, False, False),
+    "vector_n6": (3, (470.0, 870.0), 6, 64, True, False, False),
 }
 
 

@@ -40,7 +40,7 @@ def test_mixed_vs_oracle(gpu, ncol, channels, N, nmom):
     assert rel.max() < 2e-4
 
 
-@pytest.mark.parametrize("name", ["scalar_n4_edge", "scalar_n6_4ch", "vector_n4"])
+@pytest.mark.parametrize("name", ["scalar_n4_edge", "scalar_n6_4ch", "vector_n4", "vector_n6"])
 def test_mixed_vs_reference_golden(gpu, name):
     d = load(name)
     f = lambda a: np.asarray(a, dtype=np.float64)

@@ -84,6 +84,8 @@ def test_fp64_matches_oracle_seeded(gpu, ncol, channels, nstreams, nmom):
 
 @pytest.mark.parametrize("ncol,channels,nstreams,nmom", [
     (8, (550.0,), 3, 16),   # n = 9: one row slot per lane in 32-lane groups
+    (4, (550.0,), 5, 32),   # n = 15: two row slots per lane in 32-lane groups
+    (3, (870.0,), 6, 24),   # n = 18: one row slot per lane in 64-lane groups
 ])
 def test_vector_fp64_matches_oracle_seeded(gpu, ncol, channels, nstreams, nmom):
     print(f"case vector seeded ncol={ncol} N={nstreams} nmom={nmom}", flush=True)

@@ -68,6 +68,13 @@ SIGNATURES = {
                                    _dp, _dp, _dp, _dp, _dp, _dp, _dp, _cd, _ci, _ci, _dp, _dp, _dp, _dp, _dp,
                                    _dp, _dp, _ip, _ip]),
     "aa_vbrdf_coxmunk": (_ci, [_ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip]),
+    "aa_vbrdf_bpdf": (_ci, [_ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _ip]),
+    "aa_vlidort_vector_lambert_bpdf": (_ci, [_ci, _ci, _ci, _dp, _ci, _ci, _ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp,
+                                             _dp, _dp, _dp, _dp, _dp, _dp, _dp, _cd, _ci, _ci, _dp, _dp, _dp, _dp,
+                                             _dp, _dp, _dp, _ip, _ip]),
+    "aa_vlidort_vector_landmodis_bpdf": (_ci, [_ci, _ci, _ci, _dp, _ci, _ci, _ci, _ci, _dp, _dp, _dp, _dp, _dp, _dp,
+                                               _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _cd, _ci, _ci, _dp, _dp, _dp,
+                                               _dp, _dp, _dp, _dp, _ip, _ip]),
     "aa_column_radiance_modis_dev": (_ci, [_vp, _ci, _ci, _ci, _ci, _vp, _vp, _vp, _vp, _ci, _ci, _ci, _vp, _vp,
                                            _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _cd, _vp, _vp, _vp,
                                            _vp, _vp]),

@@ -171,4 +171,72 @@ __host__ __device__ inline double cm_db(double sigma2, double m2, double sza, do
   return 0.0 + 1.0 * k;
 }
 
+// ---- BPDF-NDVI polarised land kernel (BPDFNDVI_VFUNCTION,
+// VB/vbrdf_sup_kernels.f90:3017-3198) with the vector Fresnel reflection
+// amplitudes of a plane facet (FRESNEL_VECTOR, :3575-3702).  pars: the
+// refractive index of the facets, NDVI (outside [-1, 1] taken as 0) and the
+// scaling factor C.  Writes the 3 x 3 block k[3 o1 + o2] of the reflection
+// matrix (full = false: the 2 x 2 block only, as the white-sky sums ask for
+// the (1,1) element).  Direction cosines within 1e-9 of 1 are moved to
+// 0.999999999999 (their sines kept), as the reference does to its arguments.
+__host__ __device__ inline void bpdf_ndvi(double rn, double ndvi, double cfac, bool full, double xi, double sxi,
+                                          double xj, double sxj, double cphi, double skphi, double* k) {
+#pragma clang fp contract(off)
+  if (fabs(xi - 1.0) < 1e-9) xi = 0.999999999999;
+  if (fabs(xj - 1.0) < 1e-9) xj = 0.999999999999;
+  double z = xi * xj - sxi * sxj * cphi;
+  if (z > 1.0) z = 1.0;
+  const double zh = cos(acos(z) * 0.5);       // cosine of half the scattering angle
+  if (ndvi > 1.0 || ndvi < -1.0) ndvi = 0.0;
+  const double fp0 = 0.25 * exp(-ndvi) / (xi + xj);
+  const double sg = sqrt(1.0 - zh * zh);
+  const double att = exp(-sg / zh);
+  double h = cfac * fp0 * att;
+  // Fresnel amplitudes: incident direction (sxi, 0, -xi) at azimuth 0,
+  // reflected (sxj cphi, sxj skphi, xj); the facet normal bisects them
+  const double vi1 = sxi * 1.0, vi2 = sxi * 0.0, vi3 = -xi;
+  const double vr1 = sxj * cphi, vr2 = sxj * skphi, vr3 = xj;
+  const double u1 = vi1 - vr1, u2 = vi2 - vr2, u3 = vi3 - vr3;
+  const double un = sqrt(1.0 / (u1 * u1 + u2 * u2 + u3 * u3));
+  const double cinc = un * (u1 * vi1 + u2 * vi2 + u3 * vi3);   // cosine of the local incidence angle
+  const double ctr = sqrt(1.0 - (1.0 - cinc * cinc) * 1.0 * 1.0 / (rn * rn));
+  double c1 = 1.0 * cinc, c2 = rn * ctr;
+  const double rper = (c1 - c2) / (c1 + c2);
+  c1 = rn * cinc; c2 = 1.0 * ctr;
+  const double rpar = (c1 - c2) / (c1 + c2);
+  const double ti1 = -xi * 1.0, ti2 = -xi * 0.0, ti3 = -sxi;
+  const double tr1 = xj * cphi, tr2 = xj * skphi, tr3 = -sxj;
+  const double pi1 = -0.0, pi2 = 1.0, pi3 = 0.0;
+  const double pr1 = -skphi, pr2 = cphi, pr3 = 0.0;
+  const double pikr = pi1 * vr1 + pi2 * vr2 + pi3 * vr3;
+  const double prki = pr1 * vi1 + pr2 * vi2 + pr3 * vi3;
+  const double tikr = ti1 * vr1 + ti2 * vr2 + ti3 * vr3;
+  const double trki = tr1 * vi1 + tr2 * vi2 + tr3 * vi3;
+  const double e1 = pikr * prki, e2 = tikr * trki, e3 = tikr * prki, e4 = pikr * trki;
+  double f11 = e1 * rper + e2 * rpar;
+  const double f12 = -(e3 * rper) + e4 * rpar;
+  const double f21 = -(e4 * rper) + e3 * rpar;
+  double f22 = e2 * rper + e1 * rpar;
+  const double vp1 = vi2 * vr3 - vi3 * vr2, vp2 = vi3 * vr1 - vi1 * vr3, vp3 = vi1 * vr2 - vi2 * vr1;
+  double dmod = vp1 * vp1 + vp2 * vp2 + vp3 * vp3;
+  dmod = dmod * dmod;
+  if (dmod == 0.0) { f11 = rpar; f22 = rper; dmod = 1.0; }
+  h = h / dmod;
+  double a11 = fabs(f11), a12 = fabs(f12), a21 = fabs(f21), a22 = fabs(f22);
+  a11 = a11 * a11; a12 = a12 * a12; a21 = a21 * a21; a22 = a22 * a22;
+  const double hh = 0.5 * h;
+  k[0] = (a11 + a12 + a21 + a22) * hh;
+  k[1] = (a11 - a12 + a21 - a22) * hh;
+  k[3] = (a11 - a22 + a12 - a21) * hh;
+  k[4] = (a11 - a12 - a21 + a22) * hh;
+  if (!full) return;
+  const double c1112 = f11 * f12, c1121 = f11 * f21, c1122 = f11 * f22;
+  const double c1221 = f12 * f21, c1222 = f12 * f22, c2122 = f21 * f22;
+  k[2] = -((c1112 + c2122) * h);   // sine Fourier components
+  k[5] = -((c1112 - c2122) * h);   // sine
+  k[6] = -((c1121 + c1222) * h);   // sine
+  k[7] = -((c1121 - c1222) * h);   // sine
+  k[8] = (c1122 + c1221) * h;      // cosine
+}
+
 }  // namespace aa

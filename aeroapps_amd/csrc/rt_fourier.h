@@ -483,17 +483,80 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M) {
   //      REG/vlidort_intensity.f90:1096-1496).  Lambertian: Fourier 0 only,
   //      the same reflected term in every quadrature row.  BRDF: every Fourier
   //      term, AXBID(i,j) = w_j mu_j BRDF_F(M,i,j) per row, and the BOA source
-  //      weights w_j mu_j USER_BRDF_F(M,j).  With three Stokes components both
-  //      surfaces reflect I into I only (the Lambertian reflection matrix and
-  //      the Cox-Munk kernel, vsup/vbrdf/vbrdf_sup_kernels.f90:1018-1141, have
-  //      only the (1,1) element), so the terms act on Stokes component 0.
+  //      weights w_j mu_j USER_BRDF_F(M,j).  With three Stokes components the
+  //      Lambertian and Cox-Munk surfaces reflect I into I only (their
+  //      reflection matrices, vsup/vbrdf/vbrdf_sup_kernels.f90:201-233,
+  //      1018-1141, have only the (1,1) element), so their terms act on Stokes
+  //      component 0; the polarised BPDF surfaces fill every Stokes row.
   const bool brdf = W.brdf != nullptr;
+  if constexpr (S == 3) {
+    // polarised BRDF (BPDF land surfaces): every Stokes row of the BOA
+    // reflects every Stokes component of the downwelling field through the
+    // 3 x 3 Mueller blocks of BRDF_F / USER_BRDF_F (the R2 and AXBID sums of
+    // REG/vlidort_bvproblem.f90:575-619, 1700-1735 and the BOA source of
+    // REG/vlidort_intensity.f90:1375-1395 with MUELLER_INDEX(o1, o2))
+    if (W.bpdf && layer == L - 1) {
+      constexpr BOff BO(N, n, S);
+      constexpr BPOff PO(N);
+      double* bb = W.bbuf + (size_t)it * W.bstride;
+      const double* rec = W.brdf + (size_t)it * W.brdf_stride;
+      const double sfac = (M == 0) ? 2.0 : 1.0;
+#pragma unroll
+      for (int o = 0; o < S; ++o) {
+#pragma unroll
+        for (int k = 0; k < n; ++k) {
+          double ba = 0.0, bbv = 0.0;
+#pragma unroll
+          for (int j = 0; j < N; ++j)
+#pragma unroll
+            for (int o2 = 0; o2 < S; ++o2) {
+              const double cw = qsw[j] * rec[PO.uf(M, j, o * 3 + o2)];
+              ba += cw * xp[j * S + o2][k] * tdel[k];
+              bbv += cw * xn[j * S + o2][k];
+            }
+          bb[BO.bcoef(o, k)] = ba; bb[BO.bcoef(o, n + k)] = bbv;
+        }
+        double rb = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+#pragma unroll
+          for (int o2 = 0; o2 < S; ++o2) rb += qsw[j] * rec[PO.uf(M, j, o * 3 + o2)] * lrec[O.wl(j, o2)];
+        bb[BO.bc(o)] = rb;
+      }
+#pragma unroll 1
+      for (int i = 0; i < N; ++i) {
+#pragma unroll
+        for (int o1 = 0; o1 < S; ++o1) {
+#pragma unroll
+          for (int k = 0; k < n; ++k) {
+            double sp = 0.0, sm = 0.0;
+#pragma unroll
+            for (int o2 = 0; o2 < S; ++o2)
+#pragma unroll
+              for (int j = 0; j < N; ++j) {
+                const double ax = qsw[j] * rec[PO.f(M, i, j, o1 * 3 + o2)];
+                sp += ax * xp[j * S + o2][k];
+                sm += ax * xn[j * S + o2][k];
+              }
+            bb[BO.r2p(i, o1, k)] = sfac * sp; bb[BO.r2m(i, o1, k)] = sfac * sm;
+          }
+          double rbi = 0.0;
+#pragma unroll
+          for (int j = 0; j < N; ++j)
+#pragma unroll
+            for (int o2 = 0; o2 < S; ++o2) rbi += qsw[j] * rec[PO.f(M, i, j, o1 * 3 + o2)] * lrec[O.wl(j, o2)];
+          bb[BO.r2beam(i, o1)] = sfac * rbi;
+        }
+      }
+      return;
+    }
+  }
   if (layer == L - 1 && (brdf || M == 0)) {
     const int g = global_item(W, it);
     const int ch = g - obs * W.nch;
     const double albedo = brdf ? 0.0 : W.albedo[(size_t)ch * W.nobs + obs];
     if (brdf || albedo != 0.0) {
-      constexpr BOff BO(N, n);
+      constexpr BOff BO(N, n, S);
       constexpr BROff RO(N);
       double* bb = W.bbuf + (size_t)it * W.bstride;
       const double* rec = brdf ? W.brdf + (size_t)it * W.brdf_stride : nullptr;
@@ -507,12 +570,12 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M) {
         double ba = 0.0, bbv = 0.0;
 #pragma unroll
         for (int j = 0; j < N; ++j) { ba += cw[j] * xp[j * S][k] * tdel[k]; bbv += cw[j] * xn[j * S][k]; }
-        bb[BO.bcoef(k)] = ba; bb[BO.bcoef(n + k)] = bbv;
+        bb[BO.bcoef(0, k)] = ba; bb[BO.bcoef(0, n + k)] = bbv;
       }
       double rb = 0.0;
 #pragma unroll
       for (int j = 0; j < N; ++j) rb += cw[j] * lrec[O.wl(j, 0)];
-      bb[BO.bc()] = rb;
+      bb[BO.bc(0)] = rb;
       if (!brdf) {
         double rq = 0.0;
 #pragma unroll
@@ -523,10 +586,10 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M) {
 #pragma unroll
           for (int j = 0; j < N; ++j) { sp += qsw[j] * xp[j * S][k]; sm += qsw[j] * xn[j * S][k]; }
 #pragma unroll
-          for (int i = 0; i < N; ++i) { bb[BO.r2p(i, k)] = sp * kmult; bb[BO.r2m(i, k)] = sm * kmult; }
+          for (int i = 0; i < N; ++i) { bb[BO.r2p(i, 0, k)] = sp * kmult; bb[BO.r2m(i, 0, k)] = sm * kmult; }
         }
 #pragma unroll
-        for (int i = 0; i < N; ++i) bb[BO.r2beam(i)] = rq * kmult;
+        for (int i = 0; i < N; ++i) bb[BO.r2beam(i, 0)] = rq * kmult;
       } else {
 #pragma unroll
         for (int i = 0; i < N; ++i) {
@@ -538,12 +601,12 @@ __global__ __launch_bounds__(64) void k_layer(RTWork W, int M) {
             double sp = 0.0, sm = 0.0;
 #pragma unroll
             for (int j = 0; j < N; ++j) { sp += ax[j] * xp[j * S][k]; sm += ax[j] * xn[j * S][k]; }
-            bb[BO.r2p(i, k)] = sfac * sp; bb[BO.r2m(i, k)] = sfac * sm;
+            bb[BO.r2p(i, 0, k)] = sfac * sp; bb[BO.r2m(i, 0, k)] = sfac * sm;
           }
           double rbi = 0.0;
 #pragma unroll
           for (int j = 0; j < N; ++j) rbi += ax[j] * lrec[O.wl(j, 0)];
-          bb[BO.r2beam(i)] = sfac * rbi;
+          bb[BO.r2beam(i, 0)] = sfac * rbi;
         }
       }
     }
@@ -641,7 +704,7 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
   constexpr int NAC = (NC + G - 1) / G;   // accumulator columns per lane in the adjoint of the TOA recursion
   static_assert(2 * n <= 64 && SL <= 5 && G >= 4, "panel rows exceed the wavefront");
   constexpr LOff O(N, S);
-  constexpr BOff BO(N, n);
+  constexpr BOff BO(N, n, S);
   constexpr int BIG = 0x3fffffff;
   // The current pivot row of each group (columns right of the pivot, RHS,
   // 1/pivot), two rows in turn; column c of row i is at us[g][(i & 1) * UW + c].
@@ -698,7 +761,9 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
   const double* __restrict__ lb = W.lbuf + (size_t)it * km * LS;
   const double* __restrict__ bb = W.bbuf + (size_t)it * W.bstride;
   constexpr BROff RO(N);
+  constexpr BPOff PO(N);
   const bool brdf = W.brdf != nullptr;
+  const bool pol = S == 3 && W.bpdf != nullptr;   // polarised BRDF: every Stokes row reflects
   const double* __restrict__ brec = brdf ? W.brdf + (size_t)it * W.brdf_stride : nullptr;
   const double albedo = brdf ? 0.0 : W.albedo[oi];
   const bool surf = brdf || ((M == 0) && (albedo != 0.0));
@@ -733,9 +798,10 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
   // The Fourier component of the TOA radiance at the user angle
   // (REG/vlidort_intensity.f90:145-643) is linear in the integration
   // constants x of the boundary-value problem:  I_o = c_o . x + d_o, with
-  //   c_o(layer k, unknown r) = P_k coef_k(o, r)   (+ P_L kmult bcoef(r) on
-  //                                                  the bottom layer, o = 0)
-  //   d_o = sum_k P_k csrc_k(o)                     (+ P_L kmult bc, o = 0)
+  //   c_o(layer k, unknown r) = P_k coef_k(o, r)   (+ P_L kmult bcoef(o, r) on
+  //                                                  the bottom layer: o = 0, or
+  //                                                  every o for a polarised BRDF)
+  //   d_o = sum_k P_k csrc_k(o)                     (+ P_L kmult bc(o), the same)
   // and P_k the product of the user-angle transmittances of the layers above
   // layer k (the source recursion cum = (sv + cs) + tdu cum unrolled).  With
   // the band LU of the elimination, U x = b' (b' the transformed right-hand
@@ -918,24 +984,24 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
       for (int q = 0; q < n; ++q) r[3 * n + q] = -ll[O.tdel(q)] * (sm * xnl[q]);
       r[RH] = ll[O.wu(i, o1)] - lu[O.wl(i, o1)];
     } else {
-      const bool sr = surf && o1 == 0;
+      const bool sr = surf && (o1 == 0 || pol);
       const double* xpr = lu + O.xrow(true, i + N, o1, sp);
       const double* xnr = lu + O.xrow(false, i + N, o1, sm);
 #pragma unroll
       for (int q = 0; q < n; ++q) {
         double xpn = sp * xpr[q];
-        if (sr) xpn = xpn - bb[BO.r2p(i, q)];
+        if (sr) xpn = xpn - bb[BO.r2p(i, o1, q)];
         r[q] = lu[O.tdel(q)] * xpn;
       }
 #pragma unroll
       for (int q = 0; q < n; ++q) {
         double xmn = sm * xnr[q];
-        if (sr) xmn = xmn - bb[BO.r2m(i, q)];
+        if (sr) xmn = xmn - bb[BO.r2m(i, o1, q)];
         r[n + q] = xmn;
       }
       double v = -lu[O.wl(i + N, o1)];
-      if (sr) v = v + bb[BO.r2beam(i)];
-      if (sr) v = v + (brdf ? attn * brec[RO.f0(M, i)] : rf_direct);
+      if (sr) v = v + bb[BO.r2beam(i, o1)];
+      if (sr) v = v + (pol ? attn * brec[PO.f0(M, i, o1)] : brdf ? attn * brec[RO.f0(M, i)] : rf_direct);
       r[RH] = v;
     }
   };
@@ -967,13 +1033,17 @@ __global__ __launch_bounds__(64, (N * S <= 6) ? 2 : 1) void k_bvp(RTWork W, int 
       for (int o = 0; o < S; ++o) {
         for (int c = t; c < n2; c += G) {
           double cv = ptr * lr[O.coef(o, c)];
-          if (o == 0 && lastp && surf) cv = fma(pl * kmult, bb[BO.bcoef(c)], cv);
+          if ((o == 0 || pol) && lastp && surf) cv = fma(pl * kmult, bb[BO.bcoef(o, c)], cv);
           adj[g][2 * S * n2 + o * n2 + c] = cv;
           if (k > 0) adj[g][o * 2 * n2 + nxt * n2 + c] = 0.0;
         }
         dsum[o] = fma(ptr, lr[O.csrc(o)], dsum[o]);
       }
-      if (lastp && surf) dsum[0] = fma(pl * kmult, bb[BO.bc()], dsum[0]);
+      if (lastp && surf) {
+#pragma unroll
+        for (int o = 0; o < S; ++o)
+          if (o == 0 || pol) dsum[o] = fma(pl * kmult, bb[BO.bc(o)], dsum[o]);
+      }
       ptr = pl;
     }
     if (__ballot(here && !lastp) != 0ull) {

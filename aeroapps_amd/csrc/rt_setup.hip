@@ -187,7 +187,7 @@ __device__ __forceinline__ ItemIdx item_of(const RTWork& W, int it) {
 // only such items latch.
 __global__ void k_latch(int nitems, int nch, int nobs, const double* sza, const double* vza, const double* raa,
                         const double* albedo, const double* kernel_wt, const double* kparam, const double* u10m,
-                        const double* v10m, const double* mr, double missing, int* latch) {
+                        const double* v10m, const double* mr, const double* bpdf, double missing, int* latch) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nitems) return;
   int obs = g / nch, ch = g - obs * nch;
@@ -198,7 +198,26 @@ __global__ void k_latch(int nitems, int nch, int nobs, const double* sza, const 
   // (src/Components/rtms/geo_vlidort/VLIDORT_BRDF_CX.F90:506-512)
   if (u10m && (is_missing(u10m[obs], missing) || is_missing(v10m[obs], missing) || is_missing(mr[ch], missing)))
     return;
-  if (kernel_wt) {
+  if (bpdf) {
+    // polarised land surfaces: the (1,1) direct bounce of the Lambertian (or
+    // MODIS) kernels plus BPDF-NDVI
+    const size_t kq = (size_t)obs * nch + ch;
+    const double* bp = bpdf + kq * 3;
+    const double mu0 = cos(sza[obs] * DEG_TO_RAD), s0 = sqrt(1.0 - mu0 * mu0);
+    const double muv = cos(vza[obs] * DEG_TO_RAD), sv = sqrt(1.0 - muv * muv);
+    const double phi = raa[obs] * DEG_TO_RAD;
+    double kb[9];
+    bpdf_ndvi(bp[0], bp[1], bp[2], false, mu0, s0, muv, sv, cos(phi), sin(phi), kb);
+    double db;
+    if (kernel_wt) {
+      const double* kw = kernel_wt + kq * 3;
+      const double* kp = kparam + kq * 2;
+      db = modis_db(kw[0], kw[1], kw[2], kp[0], kp[1], sza[obs], vza[obs], raa[obs]);
+    } else {
+      db = albedo[(size_t)ch * nobs + obs] * 1.0;
+    }
+    if (db + 1.0 * kb[0] < 0.0) return;
+  } else if (kernel_wt) {
     const double* kw = kernel_wt + ((size_t)obs * nch + ch) * 3;
     const double* kp = kparam + ((size_t)obs * nch + ch) * 2;
     if (modis_db(kw[0], kw[1], kw[2], kp[0], kp[1], sza[obs], vza[obs], raa[obs]) < 0.0) return;
@@ -663,7 +682,9 @@ __global__ void k_item(RTWork W, int nitems, int* active, int* nactive) {
     // status 6: negative direct-bounce BRDF, reported as -500 by the MODIS driver
     const double fill = (W.status[it] == 6) ? -500.0 : W.missing;
     W.radiance[oi] = fill; W.reflectance[oi] = fill;
-    if (S > 1) { W.qout[oi] = W.missing; W.uout[oi] = W.missing; }
+    // (the polarised drivers write -500 into Q and U as well)
+    const double qfill = (W.status[it] == 6 && W.bpdf) ? -500.0 : W.missing;
+    if (S > 1) { W.qout[oi] = qfill; W.uout[oi] = qfill; }
     W.nfourier[oi] = 0;
     return;
   }
@@ -743,10 +764,17 @@ __global__ void k_item(RTWork W, int nitems, int* active, int* nactive) {
   // surface term of the direct bounce: M4*albedo*atten*fvec (Lambertian) or
   // M4*(BRDF . fvec)*atten with the exact direct-bounce BRDF
   // (FOC/FO_VectorSS_RTCalcs_I.f90:698-706)
-  double cdb;
-  if (W.brdf) cdb = 4.0 * mu0 * (W.brdf[(size_t)it * W.brdf_stride + BROff(W.N).db()] * fvec1) * atten(L);
-  else cdb = 4.0 * mu0 * W.albedo[oi] * atten(L) * fvec1;
-  double cdbv[3] = {cdb, 0.0, 0.0};
+  double cdbv[3] = {0.0, 0.0, 0.0};
+  if (S == 3 && W.bpdf) {
+    // polarised surface: the first column of the reflection matrix times the
+    // unpolarised flux vector, for each Stokes component
+    const double* rec = W.brdf + (size_t)it * W.brdf_stride;
+    for (int o = 0; o < S; ++o) cdbv[o] = 4.0 * mu0 * (rec[BPOff(W.N).db(o * 3)] * fvec1) * atten(L);
+  } else if (W.brdf) {
+    cdbv[0] = 4.0 * mu0 * (W.brdf[(size_t)it * W.brdf_stride + BROff(W.N).db()] * fvec1) * atten(L);
+  } else {
+    cdbv[0] = 4.0 * mu0 * W.albedo[oi] * atten(L) * fvec1;
+  }
   const double* lt = W.lostrans + (size_t)it * (km + 1);
   const double* ex = W.exact + (size_t)it * km * 3;
   for (int nl = L; nl >= 1; --nl) {
@@ -813,7 +841,7 @@ int rt_setup(const RTWork& W, int nitems, int* d_active, int* d_nactive, hipStre
   prof_begin(PROF_CHECK, st);
   hipLaunchKernelGGL(k_check, dim3(gi), dim3(B), 0, st, W, nitems);
   prof_end(PROF_CHECK, st);
-  if (W.kernel_wt || W.u10m) {
+  if (W.kernel_wt || W.u10m || W.bpdf) {
     prof_begin(PROF_BRDF, st);
     if (rt_brdf(W, nitems, st)) return -2;
     prof_end(PROF_BRDF, st);
@@ -854,10 +882,11 @@ int rt_setup(const RTWork& W, int nitems, int* d_active, int* d_nactive, hipStre
 
 int rt_latch(int nitems, int nch, int nobs, const double* sza, const double* vza, const double* raa,
              const double* albedo, const double* kernel_wt, const double* kparam, const double* u10m,
-             const double* v10m, const double* mr, double missing, int* d_latch, hipStream_t st) {
+             const double* v10m, const double* mr, const double* bpdf, double missing, int* d_latch,
+             hipStream_t st) {
   const int B = 256;
   hipLaunchKernelGGL(k_latch, dim3((nitems + B - 1) / B), dim3(B), 0, st, nitems, nch, nobs, sza, vza, raa, albedo,
-                     kernel_wt, kparam, u10m, v10m, mr, missing, d_latch);
+                     kernel_wt, kparam, u10m, v10m, mr, bpdf, missing, d_latch);
   AA_CHECK(hipGetLastError());
   return 0;
 }

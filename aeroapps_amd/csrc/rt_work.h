@@ -65,7 +65,13 @@ struct RTWork {
   const double* u10m = nullptr;       // (nobs) m/s
   const double* v10m = nullptr;       // (nobs) m/s
   const double* mr = nullptr;         // (nch) refractive index of water
+  // polarised land surfaces (three Stokes components): the BPDF-NDVI kernel
+  // after a Lambertian kernel of factor albedo(obs, ch) or, with kernel_wt,
+  // after the MODIS kernels
+  const double* bpdf = nullptr;       // (3,nch,nobs): refractive index, NDVI, scaling factor C
   double* br_out = nullptr;           // (nobs,nch) exact direct-bounce BRDF
+  double* brq_out = nullptr;          // (nobs,nch) its Mueller elements (1,2) and (1,3) (polarised surfaces)
+  double* bru_out = nullptr;
   const double* brdf_tab = nullptr;   // azimuth / scaling quadrature tables (BrTab)
   double* brdf = nullptr;             // [it][brdf_stride] Fourier components of the BRDF
   int brdf_stride = 0;
@@ -167,14 +173,33 @@ struct LOff {
 // layer for the current Fourier term, per quadrature row i < N (the
 // Lambertian surface has the same value in every row and only at Fourier 0).
 struct BOff {
-  int N, n;
-  __host__ __device__ constexpr BOff(int N_, int n_) : N(N_), n(n_) {}
-  __host__ __device__ constexpr int r2p(int i, int k) const { return i * n + k; }
-  __host__ __device__ constexpr int r2m(int i, int k) const { return N * n + i * n + k; }
-  __host__ __device__ constexpr int bcoef(int q) const { return 2 * N * n + q; }   // q < 2n
-  __host__ __device__ constexpr int r2beam(int i) const { return 2 * N * n + 2 * n + i; }
-  __host__ __device__ constexpr int bc() const { return 2 * N * n + 2 * n + N; }
-  __host__ __device__ constexpr int size() const { return 2 * N * n + 2 * n + N + 1; }
+  int N, n, S;
+  __host__ __device__ constexpr BOff(int N_, int n_, int S_ = 1) : N(N_), n(n_), S(S_) {}
+  // R2 terms of BOA row (i, o1) for unknown k, beam R2 of the row, the BOA
+  // source coefficients of user Stokes component o (q < 2n) and its beam term
+  __host__ __device__ constexpr int r2p(int i, int o1, int k) const { return (i * S + o1) * n + k; }
+  __host__ __device__ constexpr int r2m(int i, int o1, int k) const { return N * S * n + (i * S + o1) * n + k; }
+  __host__ __device__ constexpr int bcoef(int o, int q) const { return 2 * N * S * n + o * 2 * n + q; }
+  __host__ __device__ constexpr int r2beam(int i, int o1) const { return 2 * N * S * n + 2 * n * S + i * S + o1; }
+  __host__ __device__ constexpr int bc(int o) const { return 2 * N * S * n + 2 * n * S + N * S + o; }
+  __host__ __device__ constexpr int size() const { return 2 * N * S * n + 2 * n * S + N * S + S; }
+};
+
+// Offsets inside one polarised BRDF record (per item, three Stokes
+// components): the 3 x 3 Mueller blocks q = 3 o1 + o2 of the Fourier
+// components M < 2N (BRDF_F incident j -> reflected i, USER_BRDF_F j ->
+// user), the first column (o2 = 0) of BRDF_F_0 (solar -> i; the solar flux
+// vector is (1, 0, 0)) and the exact direct-bounce BRDF of every block.
+struct BPOff {
+  int N;
+  __host__ __device__ constexpr BPOff(int N_) : N(N_) {}
+  __host__ __device__ constexpr int f(int M, int i, int j, int q) const { return ((M * N + i) * N + j) * 9 + q; }
+  __host__ __device__ constexpr int f0(int M, int i, int o1) const { return 18 * N * N * N + (M * N + i) * 3 + o1; }
+  __host__ __device__ constexpr int uf(int M, int j, int q) const {
+    return 18 * N * N * N + 6 * N * N + (M * N + j) * 9 + q;
+  }
+  __host__ __device__ constexpr int db(int q) const { return 18 * N * N * N + 24 * N * N + q; }
+  __host__ __device__ constexpr int size() const { return (18 * N * N * N + 24 * N * N + 9 + 1) & ~1; }
 };
 
 // Offsets inside one BRDF record (per item, scalar): Fourier components
@@ -211,7 +236,10 @@ struct BrTab {
   __host__ __device__ constexpr int qs(int i) const { return (4 + 2 * N) * A + 3 * BR_NSC + i; }
   __host__ __device__ constexpr int qsin(int i) const { return (4 + 2 * N) * A + 3 * BR_NSC + N + i; }
   __host__ __device__ constexpr int wsa_ross() const { return (4 + 2 * N) * A + 3 * BR_NSC + 2 * N; }
-  __host__ __device__ constexpr int size() const { return (4 + 2 * N) * A + 3 * BR_NSC + 2 * N + 1; }
+  // sine factors a_k sin(M x_k) (the full-range table of the polarised
+  // surfaces only; zero in the half-range tables)
+  __host__ __device__ constexpr int saz(int M, int k) const { return (4 + 2 * N) * A + 3 * BR_NSC + 2 * N + 1 + M * A + k; }
+  __host__ __device__ constexpr int size() const { return (4 + 4 * N) * A + 3 * BR_NSC + 2 * N + 1; }
 };
 
 // Rows/columns of the SAB/DAB contraction table for (N, S): rows r*n+c (SAB)

@@ -27,8 +27,10 @@ namespace aa {
 int rt_setup(const RTWork& W, int nitems, int* d_active, int* d_nactive, hipStream_t st);
 int rt_latch(int nitems, int nch, int nobs, const double* sza, const double* vza, const double* raa,
              const double* albedo, const double* kernel_wt, const double* kparam, const double* u10m,
-             const double* v10m, const double* mr, double missing, int* d_latch, hipStream_t st);
+             const double* v10m, const double* mr, const double* bpdf, double missing, int* d_latch,
+             hipStream_t st);
 const std::vector<double>& brdf_table_cached(int N, int A);
+const std::vector<double>& brdf_table_full_cached(int N);
 #define AA_DECL(N, S) \
   int rt_fourier_##N##_##S(const RTWork& W, int nitems, int* const lists[2], int* counts, hipStream_t st);
 AA_DECL(1, 1) AA_DECL(2, 1) AA_DECL(3, 1) AA_DECL(4, 1) AA_DECL(5, 1) AA_DECL(6, 1) AA_DECL(7, 1) AA_DECL(8, 1)
@@ -298,8 +300,8 @@ hipStream_t bucket_stream(hipStream_t caller, int k) {
 size_t carve_work(Carver& C, RTWork& W, int cap) {
   const int n = W.N * W.S, LM = 2 * W.N, km = W.km;
   W.lstride = LOff(W.N, W.S).size();
-  W.bstride = BOff(W.N, n).size();
-  W.brdf_stride = (W.kernel_wt || W.u10m) ? BROff(W.N).size() : 0;
+  W.bstride = BOff(W.N, n, W.S).size();
+  W.brdf_stride = W.bpdf ? BPOff(W.N).size() : ((W.kernel_wt || W.u10m) ? BROff(W.N).size() : 0);
   W.cap = cap;
   W.status = C.take<int>(cap);
   W.L_it = C.take<int>(cap);
@@ -326,7 +328,7 @@ size_t carve_work(Carver& C, RTWork& W, int cap) {
   W.lostrans = C.take<double>((size_t)cap * (km + 1));
   W.lbuf = C.take<double>((size_t)cap * km * W.lstride);
   W.bbuf = C.take<double>((size_t)cap * W.bstride);
-  W.brdf = (W.kernel_wt || W.u10m) ? C.take<double>((size_t)cap * W.brdf_stride) : nullptr;
+  W.brdf = (W.kernel_wt || W.u10m || W.bpdf) ? C.take<double>((size_t)cap * W.brdf_stride) : nullptr;
   W.wsa_cache = W.kernel_wt ? C.take<double>(4) : nullptr;
   W.sab32 = (W.precision == 1) ? C.take<float>((size_t)cap * km * 2 * n * n) : nullptr;
   W.sab64 = (W.precision == 0) ? C.take<double>((size_t)cap * km * 2 * n * n) : nullptr;
@@ -361,7 +363,7 @@ int compute_latch(RTWork& W, int* lp, hipStream_t st) {
   AA_CHECK(hipMemcpyAsync(lp, &big, sizeof(int), hipMemcpyHostToDevice, st));
   const int nitems = W.nobs * W.nch;
   if (nitems > 0 && rt_latch(nitems, W.nch, W.nobs, W.sza, W.vza, W.raa, W.albedo, W.kernel_wt, W.kparam, W.u10m,
-                             W.v10m, W.mr, W.missing, lp, st))
+                             W.v10m, W.mr, W.bpdf, W.missing, lp, st))
     return AA_EHIP;
   W.latch = lp;
   return 0;
@@ -388,7 +390,8 @@ int run_items(RTWork W, int g0, int g1, hipStream_t st, const int* d_items = nul
   const std::vector<double>& ytab64 = (W.precision == 0) ? ytab64_cached(W.N, W.S, W.nmoments) : no_ytab64;
   static const std::vector<double> no_btab;
   const std::vector<double>& btab =
-      W.kernel_wt ? brdf_table_cached(W.N, BR_NAZ) : (W.u10m ? brdf_table_cached(W.N, BR_NAZ_CM) : no_btab);
+      W.bpdf ? brdf_table_full_cached(W.N)
+             : (W.kernel_wt ? brdf_table_cached(W.N, BR_NAZ) : (W.u10m ? brdf_table_cached(W.N, BR_NAZ_CM) : no_btab));
   double* dbt = nullptr;
   // chunk capacity from the work budget, or from the fixed block: its bytes
   // less the tables, over the bytes of an item (work record + two list
@@ -483,22 +486,26 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
                  double* reflectance, double* q, double* u, int* nfourier, int* rc,
                  const double* kernel_wt = nullptr, const double* kparam = nullptr, double* br = nullptr,
                  const double* u10m = nullptr, const double* v10m = nullptr, const double* mr = nullptr,
-                 bool stokes1 = false, const int* host_latch = nullptr) {
+                 bool stokes1 = false, const int* host_latch = nullptr, const double* bpdf = nullptr,
+                 double* br_q = nullptr, double* br_u = nullptr) {
   if (!rc) return AA_EARG;
   *rc = 0;
   const bool cx = u10m != nullptr;
-  if (cx && (!v10m || !mr || !br || kernel_wt || stokes1)) return AA_EARG;
+  const bool pol = bpdf != nullptr;                // polarised land surface (BPDF kernel last)
+  if (cx && (!v10m || !mr || !br || kernel_wt || stokes1 || pol)) return AA_EARG;
+  if (pol && (!vector || stokes1 || !br || !br_q || !br_u)) return AA_EARG;
   const bool modis = kernel_wt != nullptr || cx;   // a BRDF surface instead of the albedo
+  const bool surf_brdf = modis || pol;             // the radiance runs on a BRDF record
   if (km <= 0 || nch <= 0 || nobs < 0 || nmom <= 0 || npol <= 0) return AA_EARG;
   if (!rot || !depol || !tau || !ssa || !pmom || !he || (!albedo && !modis) || !sza || !raa || !vza || !radiance ||
       !reflectance)
     return AA_EARG;
-  if (kernel_wt && (vector || !kparam || !br)) return AA_EARG;
-  // VLIDORT_Init of the MODIS driver (V283/VLIDORT_Mod.F90:369) rejects nMom
+  if (kernel_wt && ((vector && !pol) || !kparam || !br)) return AA_EARG;
+  // VLIDORT_Init of the BRDF drivers (V283/VLIDORT_Mod.F90:369) rejects nMom
   // beyond the input maximum for the whole call
-  if (modis && nmom > MAX_MOMENTS_IN) { *rc = 5; return 0; }
-  // (the vector Cox-Munk driver sets no flux factor: VLIDORT_Init's 1.0)
-  if (vector && (!q || !u || (!flux_factor && !cx) || npol < 6)) return AA_EARG;
+  if (surf_brdf && nmom > MAX_MOMENTS_IN) { *rc = 5; return 0; }
+  // (the vector Cox-Munk and BPDF drivers set no flux factor: VLIDORT_Init's 1.0)
+  if (vector && (!q || !u || (!flux_factor && !cx && !pol) || npol < 6)) return AA_EARG;
   if (precision != 0 && precision != 1) return AA_EARG;
   // stokes1: the vector driver run with NSTOKES = 1 (its DO_2OS_CORRECTION
   // option): Stokes-1 arithmetic on the vector driver's inputs (gas
@@ -519,7 +526,7 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
   // device copies of the inputs and the outputs, carved from the host-call block
   struct Dev {
     double *rot, *depol, *alpha, *tau, *ssa, *pmom, *he, *albedo, *kw, *kp, *u10, *v10, *mr, *br, *sza, *raa, *vza,
-        *flux, *rad, *refl, *q, *u;
+        *flux, *rad, *refl, *q, *u, *bp, *brq, *bru;
     int* nf;
   } d{};
   auto plan = [&](Carver& C) {
@@ -536,7 +543,10 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
     d.u10 = cx ? C.take<double>(nobs) : nullptr;
     d.v10 = cx ? C.take<double>(nobs) : nullptr;
     d.mr = cx ? C.take<double>(nch) : nullptr;
-    d.br = modis ? C.take<double>(no) : nullptr;
+    d.br = surf_brdf ? C.take<double>(no) : nullptr;
+    d.bp = pol ? C.take<double>(no * 3) : nullptr;
+    d.brq = pol ? C.take<double>(no) : nullptr;
+    d.bru = pol ? C.take<double>(no) : nullptr;
     d.sza = C.take<double>(nobs);
     d.raa = C.take<double>(nobs);
     d.vza = C.take<double>(nobs);
@@ -563,7 +573,7 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
       (d.albedo && up(d.albedo, albedo, no)) || (d.kw && up(d.kw, kernel_wt, no * 3)) ||
       (d.kp && up(d.kp, kparam, no * 2)) || (d.u10 && up(d.u10, u10m, nobs)) || (d.v10 && up(d.v10, v10m, nobs)) ||
       (d.mr && up(d.mr, mr, nch)) || up(d.sza, sza, nobs) || up(d.raa, raa, nobs) || up(d.vza, vza, nobs) ||
-      (d.flux && up(d.flux, flux_factor, no)))
+      (d.flux && up(d.flux, flux_factor, no)) || (d.bp && up(d.bp, bpdf, no * 3)))
     return AA_EHIP;
   W.rot = d.rot; W.depol = d.depol; W.alpha = d.alpha;
   W.tau = d.tau; W.ssa = d.ssa; W.pmom = d.pmom;
@@ -572,6 +582,7 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
   W.kernel_wt = d.kw; W.kparam = d.kp;
   W.u10m = d.u10; W.v10m = d.v10; W.mr = d.mr;
   W.br_out = d.br;
+  W.bpdf = d.bp; W.brq_out = d.brq; W.bru_out = d.bru;
   W.sza = d.sza; W.raa = d.raa; W.vza = d.vza;
   W.flux = d.flux;
   W.radiance = d.rad; W.reflectance = d.refl; W.qout = d.q; W.uout = d.u; W.nfourier = d.nf;
@@ -605,7 +616,11 @@ int lambert_host(bool vector, int km, int nch, int nobs, int nstreams, int plane
     }
   }
   if (nfourier) AA_CHECK(hipMemcpy(nfourier, W.nfourier, no * sizeof(int), hipMemcpyDeviceToHost));
-  if (modis) AA_CHECK(hipMemcpy(br, W.br_out, no * sizeof(double), hipMemcpyDeviceToHost));
+  if (surf_brdf) AA_CHECK(hipMemcpy(br, W.br_out, no * sizeof(double), hipMemcpyDeviceToHost));
+  if (pol) {
+    AA_CHECK(hipMemcpy(br_q, W.brq_out, no * sizeof(double), hipMemcpyDeviceToHost));
+    AA_CHECK(hipMemcpy(br_u, W.bru_out, no * sizeof(double), hipMemcpyDeviceToHost));
+  }
   return 0;
 }
 
@@ -804,6 +819,38 @@ int aa_vlidort_vector_cx(int km, int nch, int nobs, const double* channels, int 
     brdf_q[i] = brdf_u[i] = miss ? missing : 0.0;
   }
   return 0;
+}
+
+int aa_vlidort_vector_lambert_bpdf(int km, int nch, int nobs, const double* channels, int nstreams,
+                                   int plane_parallel, int nmom, int npol, const double* rot, const double* depol,
+                                   const double* tau, const double* ssa, const double* pmom, const double* pe,
+                                   const double* he, const double* te, const double* albedo, const double* bpdf_param,
+                                   const double* solar_zenith, const double* relat_azymuth,
+                                   const double* sensor_zenith, double missing, int verbose, int precision,
+                                   double* radiance, double* reflectance, double* br, double* q, double* u,
+                                   double* br_q, double* br_u, int* nfourier, int* rc) {
+  (void)channels; (void)pe; (void)te; (void)verbose;
+  if (!albedo || !bpdf_param) return AA_EARG;
+  return aa::lambert_host(true, km, nch, nobs, nstreams, plane_parallel, nmom, npol, rot, depol, nullptr, tau, ssa,
+                          pmom, he, albedo, solar_zenith, relat_azymuth, sensor_zenith, nullptr, missing, precision,
+                          radiance, reflectance, q, u, nfourier, rc, nullptr, nullptr, br, nullptr, nullptr, nullptr,
+                          false, nullptr, bpdf_param, br_q, br_u);
+}
+
+int aa_vlidort_vector_landmodis_bpdf(int km, int nch, int nobs, const double* channels, int nstreams,
+                                     int plane_parallel, int nmom, int npol, const double* rot, const double* depol,
+                                     const double* tau, const double* ssa, const double* pmom, const double* pe,
+                                     const double* he, const double* te, const double* kernel_wt,
+                                     const double* rtls_param, const double* bpdf_param, const double* solar_zenith,
+                                     const double* relat_azymuth, const double* sensor_zenith, double missing,
+                                     int verbose, int precision, double* radiance, double* reflectance, double* br,
+                                     double* q, double* u, double* br_q, double* br_u, int* nfourier, int* rc) {
+  (void)channels; (void)pe; (void)te; (void)verbose;
+  if (!kernel_wt || !rtls_param || !bpdf_param) return AA_EARG;
+  return aa::lambert_host(true, km, nch, nobs, nstreams, plane_parallel, nmom, npol, rot, depol, nullptr, tau, ssa,
+                          pmom, he, nullptr, solar_zenith, relat_azymuth, sensor_zenith, nullptr, missing, precision,
+                          radiance, reflectance, q, u, nfourier, rc, kernel_wt, rtls_param, br, nullptr, nullptr,
+                          nullptr, false, nullptr, bpdf_param, br_q, br_u);
 }
 
 int aa_vlidort_scalar_landmodis(int km, int nch, int nobs, const double* channels, int nstreams, int plane_parallel,

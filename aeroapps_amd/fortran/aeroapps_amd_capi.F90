@@ -204,6 +204,36 @@ module aeroapps_amd_capi
       integer(c_int), intent(out) :: rc
     end function
 
+    integer(c_int) function aa_vlidort_vector_lambert_bpdf(km, nch, nobs, channels, nstreams, plane_parallel, &
+        nmom, npol, rot, depol, tau, ssa, pmom, pe, he, te, albedo, bpdf, sza, raa, vza, missing, verbose, &
+        precision, radiance, reflectance, br, q, u, br_q, br_u, nfourier, rc) &
+        bind(C, name="aa_vlidort_vector_lambert_bpdf")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: km, nch, nobs, nstreams, plane_parallel, nmom, npol
+      real(c_double), intent(in) :: channels(*), rot(*), depol(*), tau(*), ssa(*), pmom(*)
+      real(c_double), intent(in) :: pe(*), he(*), te(*), albedo(*), bpdf(*), sza(*), raa(*), vza(*)
+      real(c_double), value :: missing
+      integer(c_int), value :: verbose, precision
+      real(c_double), intent(out) :: radiance(*), reflectance(*), br(*), q(*), u(*), br_q(*), br_u(*)
+      type(c_ptr), value :: nfourier
+      integer(c_int), intent(out) :: rc
+    end function
+
+    integer(c_int) function aa_vlidort_vector_landmodis_bpdf(km, nch, nobs, channels, nstreams, plane_parallel, &
+        nmom, npol, rot, depol, tau, ssa, pmom, pe, he, te, kernel_wt, param, bpdf, sza, raa, vza, missing, &
+        verbose, precision, radiance, reflectance, br, q, u, br_q, br_u, nfourier, rc) &
+        bind(C, name="aa_vlidort_vector_landmodis_bpdf")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: km, nch, nobs, nstreams, plane_parallel, nmom, npol
+      real(c_double), intent(in) :: channels(*), rot(*), depol(*), tau(*), ssa(*), pmom(*)
+      real(c_double), intent(in) :: pe(*), he(*), te(*), kernel_wt(*), param(*), bpdf(*), sza(*), raa(*), vza(*)
+      real(c_double), value :: missing
+      integer(c_int), value :: verbose, precision
+      real(c_double), intent(out) :: radiance(*), reflectance(*), br(*), q(*), u(*), br_q(*), br_u(*)
+      type(c_ptr), value :: nfourier
+      integer(c_int), intent(out) :: rc
+    end function
+
     integer(c_int) function aa_vlidort_vector_lambert(km, nch, nobs, channels, nstreams, plane_parallel, nmom, &
         npol, rot, depol, alpha, tau, ssa, pmom, pe, he, te, albedo, sza, raa, vza, flux_factor, missing, verbose, &
         precision, radiance, reflectance, q, u, nfourier, rc) bind(C, name="aa_vlidort_vector_lambert")

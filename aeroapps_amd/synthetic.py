@@ -67,6 +67,7 @@ NRH = 36
 # field ids for the counter-based generator
 _F_DELP, _F_Q1, _F_Q2, _F_RH, _F_SZA, _F_VZA, _F_RAA, _F_NLAY, _F_NSTR = range(1, 10)
 _F_FISO, _F_FVOL, _F_FGEO, _F_U10, _F_V10 = range(10, 15)
+_F_NDVI, _F_BPDFC = range(15, 17)
 _F_TBEXT, _F_TSSA, _F_TG, _F_TC12, _F_TC33, _F_TC34, _F_TREFR, _F_TREFI, _F_TREFF = range(20, 29)
 
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
@@ -317,6 +318,20 @@ def make_modis_kernels(ncol: int, nch: int, seed: int = 7, col0: int = 0):
     par[..., 0] = 2.0
     par[..., 1] = 1.0
     return kw, par
+
+
+def make_bpdf(ncol: int, nch: int, seed: int = 7, col0: int = 0):
+    """BPDF-NDVI parameters per (column, channel), bpdf [ncol, nch, 3] in the
+    order of the reference's BPDFparam(3,nch,nobs): refractive index 1.5, NDVI
+    uniform on [-0.2, 0.9] and the scaling factor C uniform on [4, 8] (the
+    range of Maignan et al. 2009, Table 1)."""
+    cols = np.arange(col0, col0 + ncol, dtype=np.int64)[:, None]
+    chs = np.arange(nch, dtype=np.int64)[None, :]
+    out = np.empty((ncol, nch, 3))
+    out[..., 0] = 1.5
+    out[..., 1] = -0.2 + 1.1 * uniform(seed, _F_NDVI, cols, chs)
+    out[..., 2] = 4.0 + 4.0 * uniform(seed, _F_BPDFC, cols, chs)
+    return out
 
 
 def make_ocean(ncol: int, channels_nm: Sequence[float], seed: int = 7, col0: int = 0):

@@ -10,6 +10,8 @@ Extra keyword arguments: ``precision`` (0 = FP64, the drop-in) and
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 
 from ._lib import check, lib, _dp, _ip
@@ -172,6 +174,89 @@ def vector_cx(channels, nstreams, plane_parallel, ROT, depol, tau, ssa, pmom, pe
     if return_nfourier:
         return rad, refl, br, q, uu, brq, bru, int(rc[0]), nf
     return rad, refl, br, q, uu, brq, bru, int(rc[0])
+
+
+def vector_lambert_bpdf(channels, nstreams, plane_parallel, ROT, depol, tau, ssa, pmom, pe, he, te, albedo,
+                        BPDFparam, solar_zenith, relat_azymuth, sensor_zenith, MISSING, verbose=0, precision=0,
+                        return_nfourier=False):
+    """VLIDORT_Vector_Lambert_BPDF (src/Components/rtms/geo_vlidort/VLIDORT_LAMBERT_BPDF.F90:23-222):
+    Lambertian albedo (nobs, nch) plus the polarised BPDF-NDVI kernel,
+    BPDFparam (3, nch, nobs) = (refractive index, NDVI, C).  Returns
+    (radiance, reflectance, BR, Q, U, BR_Q, BR_U, rc), each array (nobs, nch)."""
+    channels, ROT, depol, tau, ssa, pmom, pe, he, te, alb, bp, sza, raa, vza = map(
+        _f64, (channels, ROT, depol, tau, ssa, pmom, pe, he, te, albedo, BPDFparam, solar_zenith, relat_azymuth,
+               sensor_zenith))
+    km, nch, nobs = tau.shape
+    nmom, npol = pmom.shape[3], pmom.shape[4]
+    rad, refl, br, q, uu, brq, bru = (np.zeros((nobs, nch), order="F") for _ in range(7))
+    nf = np.zeros((nobs, nch), dtype=np.int32, order="F")
+    rc = np.zeros(1, dtype=np.int32)
+    e = lib().aa_vlidort_vector_lambert_bpdf(km, nch, nobs, _ptr(channels), int(nstreams), int(bool(plane_parallel)),
+                                             nmom, npol, _ptr(ROT), _ptr(depol), _ptr(tau), _ptr(ssa), _ptr(pmom),
+                                             _ptr(pe), _ptr(he), _ptr(te), _ptr(alb), _ptr(bp), _ptr(sza),
+                                             _ptr(raa), _ptr(vza), float(MISSING), int(verbose), int(precision),
+                                             _ptr(rad), _ptr(refl), _ptr(br), _ptr(q), _ptr(uu), _ptr(brq),
+                                             _ptr(bru), _ptr(nf, _ip), _ptr(rc, _ip))
+    check(e, "aa_vlidort_vector_lambert_bpdf")
+    if return_nfourier:
+        return rad, refl, br, q, uu, brq, bru, int(rc[0]), nf
+    return rad, refl, br, q, uu, brq, bru, int(rc[0])
+
+
+def vector_brdf_modis_bpdf(channels, nstreams, plane_parallel, ROT, depol, tau, ssa, pmom, pe, he, te, kernel_wt,
+                           RTLSparam, BPDFparam, solar_zenith, relat_azymuth, sensor_zenith, MISSING, verbose=0,
+                           precision=0, return_nfourier=False):
+    """VLIDORT_Vector_LandMODIS_BPDF (src/Components/rtms/geo_vlidort/VLIDORT_BRDF_MODIS_BPDF.F90:24-222),
+    the contract of the f2py ``VLIDORT_POLAR_.vector_brdf_modis_bpdf``
+    (src/Components/rtms/leo_vlidort/VLIDORT_POLAR_py.F90:214-293): MODIS
+    kernel_wt (3, nch, nobs), RTLSparam (2, nch, nobs) and the BPDF-NDVI
+    kernel, BPDFparam (3, nch, nobs).  Returns (radiance, reflectance, BR, Q,
+    U, BR_Q, BR_U, rc), each array (nobs, nch)."""
+    channels, ROT, depol, tau, ssa, pmom, pe, he, te, kw, kp, bp, sza, raa, vza = map(
+        _f64, (channels, ROT, depol, tau, ssa, pmom, pe, he, te, kernel_wt, RTLSparam, BPDFparam, solar_zenith,
+               relat_azymuth, sensor_zenith))
+    km, nch, nobs = tau.shape
+    nmom, npol = pmom.shape[3], pmom.shape[4]
+    rad, refl, br, q, uu, brq, bru = (np.zeros((nobs, nch), order="F") for _ in range(7))
+    nf = np.zeros((nobs, nch), dtype=np.int32, order="F")
+    rc = np.zeros(1, dtype=np.int32)
+    e = lib().aa_vlidort_vector_landmodis_bpdf(km, nch, nobs, _ptr(channels), int(nstreams),
+                                               int(bool(plane_parallel)), nmom, npol, _ptr(ROT), _ptr(depol),
+                                               _ptr(tau), _ptr(ssa), _ptr(pmom), _ptr(pe), _ptr(he), _ptr(te),
+                                               _ptr(kw), _ptr(kp), _ptr(bp), _ptr(sza), _ptr(raa), _ptr(vza),
+                                               float(MISSING), int(verbose), int(precision), _ptr(rad), _ptr(refl),
+                                               _ptr(br), _ptr(q), _ptr(uu), _ptr(brq), _ptr(bru), _ptr(nf, _ip),
+                                               _ptr(rc, _ip))
+    check(e, "aa_vlidort_vector_landmodis_bpdf")
+    if return_nfourier:
+        return rad, refl, br, q, uu, brq, bru, int(rc[0]), nf
+    return rad, refl, br, q, uu, brq, bru, int(rc[0])
+
+
+def vbrdf_bpdf(nstreams, sza, vza, raa, bpdf_param, albedo=None, kernel_wt=None, rtls_param=None):
+    """The polarised supplement alone for ngeom geometries: VLIDORT_LAMBERTIAN_BPDF
+    (albedo [ngeom]) or VLIDORT_LANDMODIS_BPDF (kernel_wt [ngeom, 3],
+    rtls_param [ngeom, 2]) with bpdf_param [ngeom, 3].  Returns dict
+    F[g, M, q, i, j], F0[g, M, q, i], UF[g, M, q, j], DB[g, q] (Mueller index
+    q = 4 o1 + o2) and status[g]."""
+    sza, vza, raa = (np.ascontiguousarray(np.atleast_1d(a), dtype=np.float64) for a in (sza, vza, raa))
+    bp = np.ascontiguousarray(bpdf_param, dtype=np.float64).reshape(-1, 3)
+    ng, n, nm = sza.size, int(nstreams), 2 * int(nstreams)
+    lam = None if albedo is None else np.ascontiguousarray(np.atleast_1d(albedo), dtype=np.float64)
+    kw = None if kernel_wt is None else np.ascontiguousarray(kernel_wt, dtype=np.float64).reshape(-1, 3)
+    kp = None if rtls_param is None else np.ascontiguousarray(rtls_param, dtype=np.float64).reshape(-1, 2)
+    F = np.zeros((nm, 16, n, n, ng), order="F")
+    F0 = np.zeros((nm, 16, n, ng), order="F")
+    UF = np.zeros((nm, 16, n, ng), order="F")
+    db = np.zeros((16, ng), order="F")
+    st = np.zeros(ng, dtype=np.int32)
+    nul = ctypes.POINTER(ctypes.c_double)()
+    e = lib().aa_vbrdf_bpdf(n, ng, _ptr(sza), _ptr(vza), _ptr(raa), nul if lam is None else _ptr(lam),
+                            nul if kw is None else _ptr(kw), nul if kp is None else _ptr(kp), _ptr(bp), _ptr(F),
+                            _ptr(F0), _ptr(UF), _ptr(db), _ptr(st, _ip))
+    check(e, "aa_vbrdf_bpdf")
+    return dict(F=np.moveaxis(F, 4, 0), F0=np.moveaxis(F0, 3, 0), UF=np.moveaxis(UF, 3, 0), DB=db.T.copy(),
+                status=st)
 
 
 def vbrdf_coxmunk(nstreams, sza, vza, raa, u10m, v10m, mr):

@@ -322,6 +322,57 @@ int aa_vlidort_vector_cx(int km, int nch, int nobs, const double* channels, int 
                          double* q, double* u, double* brdf, double* brdf_q, double* brdf_u, int* nfourier,
                          int* rc);
 
+/* Replace the polarised land drivers VLIDORT_Vector_Lambert_BPDF
+ * (GEO/VLIDORT_LAMBERT_BPDF.F90:23-222: Lambertian kernel of factor
+ * albedo(nobs,nch), then BPDF-NDVI) and VLIDORT_Vector_LandMODIS_BPDF
+ * (GEO/VLIDORT_BRDF_MODIS_BPDF.F90:24-222: Lambertian fiso, Ross-thick fvol,
+ * Li-sparse fgeo with kernel_wt(3,nch,nobs) = (fiso, fgeo, fvol) and
+ * rtls_param(2,nch,nobs) = (h/b, b/r), then BPDF-NDVI), surface set-ups
+ * VLIDORT_LAMBERTIAN_BPDF / VLIDORT_LANDMODIS_BPDF (V283/VLIDORT_SurfaceMod.F90:
+ * 1822-2200): NSTOKES = 3, 50 full-range BRDF azimuths, the white-sky check
+ * dropping the BPDF kernel's Fourier terms; VLIDORT_Run_Vector with the flux
+ * factor of VLIDORT_Init (1).  bpdf_param(3,nch,nobs) = (refractive index,
+ * NDVI, scaling factor C).  Outputs (nobs,nch): radiance, reflectance, Q, U,
+ * br, br_q, br_u = the direct-bounce BRDF elements (1,1), (1,2), (1,3);
+ * all -500 where the (1,1) direct bounce is negative (the element is not
+ * run).  Missing geometry (and, for the Lambertian form, a missing albedo)
+ * gives MISSING; missing kernel weights or BPDF parameters are used as given,
+ * as in the reference (its checks leave only their own inner loops).
+ * nstreams <= 6.  Host pointers, synchronous. */
+int aa_vlidort_vector_lambert_bpdf(int km, int nch, int nobs, const double* channels, int nstreams,
+                                   int plane_parallel, int nmom, int npol, const double* rot, const double* depol,
+                                   const double* tau, const double* ssa, const double* pmom, const double* pe,
+                                   const double* he, const double* te, const double* albedo, const double* bpdf_param,
+                                   const double* solar_zenith, const double* relat_azymuth,
+                                   const double* sensor_zenith, double missing, int verbose, int precision,
+                                   double* radiance, double* reflectance, double* br, double* q, double* u,
+                                   double* br_q, double* br_u, int* nfourier, int* rc);
+
+int aa_vlidort_vector_landmodis_bpdf(int km, int nch, int nobs, const double* channels, int nstreams,
+                                     int plane_parallel, int nmom, int npol, const double* rot, const double* depol,
+                                     const double* tau, const double* ssa, const double* pmom, const double* pe,
+                                     const double* he, const double* te, const double* kernel_wt,
+                                     const double* rtls_param, const double* bpdf_param, const double* solar_zenith,
+                                     const double* relat_azymuth, const double* sensor_zenith, double missing,
+                                     int verbose, int precision, double* radiance, double* reflectance, double* br,
+                                     double* q, double* u, double* br_q, double* br_u, int* nfourier, int* rc);
+
+/* The polarised supplement alone for ngeom geometries (degrees): the BRDF of
+ * VLIDORT_LAMBERTIAN_BPDF (kernel_wt NULL: Lambertian factor lamb(g), BPDF)
+ * or VLIDORT_LANDMODIS_BPDF (kernel_wt(3,g), rtls_param(2,g): MODIS kernels,
+ * BPDF), bpdf_param(3,g), as VBRDF_MAINMASTER returns it
+ * (VB/vbrdf_sup_masters.f90:1559-3293) with NSTOKES = 3, in the arrays
+ * BS_BRDF_F(0:2N-1, 16, N, N), BS_BRDF_F_0(0:2N-1, 16, N) (first Mueller
+ * column), BS_USER_BRDF_F(0:2N-1, 16, N) and BS_DBOUNCE_BRDFUNC(16) per
+ * geometry, Mueller index q = 4 (o1 - 1) + o2 (entries of the fourth Stokes
+ * component 0).  status(g): 0, 4 (white-sky albedo outside (0, 1]: the BPDF
+ * kernel's Fourier terms left out), -1 (negative (1,1) direct bounce:
+ * Fourier components not computed, as the drivers skip the element).
+ * nstreams <= 6. */
+int aa_vbrdf_bpdf(int nstreams, int ngeom, const double* sza, const double* vza, const double* raa,
+                  const double* lamb, const double* kernel_wt, const double* rtls_param, const double* bpdf_param,
+                  double* brdf_f, double* brdf_f0, double* user_brdf_f, double* dbounce, int* status);
+
 /* The Cox-Munk supplement alone, as aa_vbrdf_modis with u10m, v10m, mr per
  * geometry (status 4: white-sky albedo outside (0,1], the kernel's Fourier
  * terms are then zero). */

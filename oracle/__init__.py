@@ -148,6 +148,13 @@ class ModisRef:
         self.fvc = lib.ref_vector_cx
         self.fvc.restype = None
         self.fvc.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci] + [_dp] * 14 + [cd] + [_dp] * 7 + [_ip, _ip]
+        self.fbp = lib.ref_vbrdf_bpdf
+        self.fbp.restype = None
+        self.fbp.argtypes = [ci, cd, cd, cd, ci] + [cd] * 8 + [_dp] * 5 + [_ip]
+        self.fvp = lib.ref_vector_bpdf
+        self.fvp.restype = None
+        self.fvp.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci] + [_dp] * 8 + [ci] + [_dp] * 7 + [cd] + [_dp] * 7 \
+            + [_ip, _ip]
 
     def vector_cx(self, km, channels, nstreams, nmom, rot, depol, tau, ssa, pmom, pe, he, te, u10m, v10m, mr,
                   sza, raa, vza, missing=1e15, plane_parallel=False):
@@ -164,6 +171,39 @@ class ModisRef:
                         _p(rot), _p(depol), _p(tau), _p(ssa), _p(pmom), _p(pe), _p(he), _p(te), _p(u10m),
                         _p(v10m), _p(mr), _p(sza), _p(raa), _p(vza), missing, *[_p(o) for o in out],
                         _p(nf, np.int32), _p(rc, np.int32))
+        return (*out, nf, int(rc[0]))
+
+    def vbrdf_bpdf(self, nstreams, sza, vza, raa, modis, fiso, fgeo, fvol, p1, p2, b1, b2, b3):
+        """Polarised supplement (VLIDORT_LANDMODIS_BPDF when modis, else
+        VLIDORT_LAMBERTIAN_BPDF with albedo fiso) for one geometry: dict
+        F[M,q,i,j], F0[M,q,i], UF[M,q,j] and DB[q] over the 16 Mueller indices
+        q = 4 o1 + o2 (0-based), wsa, status."""
+        n, nm = nstreams, 2 * nstreams
+        F, F0, UF = np.zeros((n, n, 16, nm)), np.zeros((n, 16, nm)), np.zeros((n, 16, nm))
+        db, wsa = np.zeros(16), np.zeros(1)
+        st = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fbp, n, sza, vza, raa, int(modis), fiso, fgeo, fvol, p1, p2, b1, b2, b3, _p(F),
+                        _p(F0), _p(UF), _p(db), _p(wsa), _p(st, np.int32))
+        return dict(F=F.transpose(3, 2, 1, 0), F0=F0.transpose(2, 1, 0), UF=UF.transpose(2, 1, 0), DB=db,
+                    wsa=float(wsa[0]), status=int(st[0]))
+
+    def vector_bpdf(self, km, channels, nstreams, nmom, rot, depol, tau, ssa, pmom, pe, he, te, modis, albedo,
+                    kernel_wt, param, bpdf, sza, raa, vza, missing=1e15, plane_parallel=False):
+        """VLIDORT_Vector_LandMODIS_BPDF (modis) or VLIDORT_Vector_Lambert_BPDF:
+        kernel_wt [nobs, nch, 3], param [nobs, nch, 2], bpdf [nobs, nch, 3]
+        (C order of the Fortran (3, nch, nobs) arrays), albedo [nch, nobs];
+        returns rad, refl, Q, U, BR, BR_Q, BR_U [nch, nobs], nf, rc."""
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        channels, rot, depol, tau, ssa, pmom, pe, he, te, albedo, kernel_wt, param, bpdf, sza, raa, vza = map(
+            f, (channels, rot, depol, tau, ssa, pmom, pe, he, te, albedo, kernel_wt, param, bpdf, sza, raa, vza))
+        nch, nobs, npol = len(channels), len(sza), pmom.shape[0]
+        out = [np.zeros((nch, nobs)) for _ in range(7)]
+        nf = np.zeros((nch, nobs), dtype=np.int32)
+        rc = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fvp, km, nch, nobs, _p(channels), nstreams, int(plane_parallel), nmom, npol,
+                        _p(rot), _p(depol), _p(tau), _p(ssa), _p(pmom), _p(pe), _p(he), _p(te), int(modis),
+                        _p(albedo), _p(kernel_wt), _p(param), _p(bpdf), _p(sza), _p(raa), _p(vza), missing,
+                        *[_p(o) for o in out], _p(nf, np.int32), _p(rc, np.int32))
         return (*out, nf, int(rc[0]))
 
     def vbrdf_coxmunk(self, nstreams, sza, vza, raa, u10m, v10m, m):
@@ -258,6 +298,12 @@ class RefDrivers:
         self.fvc = lib.drv_vector_cx
         self.fvc.restype = None
         self.fvc.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci] + [_dp] * 14 + [cd] + [_dp] * 7 + [_ip]
+        self.fvlb = lib.drv_vector_lambert_bpdf
+        self.fvlb.restype = None
+        self.fvlb.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci] + [_dp] * 13 + [cd] + [_dp] * 7 + [_ip]
+        self.fvmb = lib.drv_vector_landmodis_bpdf
+        self.fvmb.restype = None
+        self.fvmb.argtypes = [ci, ci, ci, _dp, ci, ci, ci, ci] + [_dp] * 14 + [cd] + [_dp] * 7 + [_ip]
 
     @staticmethod
     def _f(*arrs):
@@ -361,6 +407,34 @@ class RefDrivers:
                         _p(v10m), _p(mr), _p(sza), _p(raa), _p(vza), missing, *[_p(o) for o in out],
                         _p(rc, np.int32))
         return (*out, int(rc[0]))
+
+    def vector_lambert_bpdf(self, km, channels, nstreams, nmom, rot, depol, tau, ssa, pmom, pe, he, te, albedo,
+                            bpdf, sza, raa, vza, missing=1e15, plane_parallel=False):
+        """VLIDORT_Vector_Lambert_BPDF; returns rad, refl, Q, U, BR, BR_Q, BR_U [nch, nobs], rc."""
+        channels, rot, depol, tau, ssa, pmom, pe, he, te, albedo, bpdf, sza, raa, vza = self._f(
+            channels, rot, depol, tau, ssa, pmom, pe, he, te, albedo, bpdf, sza, raa, vza)
+        nch, nobs, npol = len(channels), len(sza), pmom.shape[0]
+        rad, refl, br, q, u, bq, bu = (np.zeros((nch, nobs)) for _ in range(7))
+        rc = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fvlb, km, nch, nobs, _p(channels), nstreams, int(plane_parallel), nmom, npol,
+                        _p(rot), _p(depol), _p(tau), _p(ssa), _p(pmom), _p(pe), _p(he), _p(te), _p(albedo),
+                        _p(bpdf), _p(sza), _p(raa), _p(vza), missing, _p(rad), _p(refl), _p(br), _p(q), _p(u),
+                        _p(bq), _p(bu), _p(rc, np.int32))
+        return rad, refl, q, u, br, bq, bu, int(rc[0])
+
+    def vector_landmodis_bpdf(self, km, channels, nstreams, nmom, rot, depol, tau, ssa, pmom, pe, he, te,
+                              kernel_wt, param, bpdf, sza, raa, vza, missing=1e15, plane_parallel=False):
+        """VLIDORT_Vector_LandMODIS_BPDF; returns rad, refl, Q, U, BR, BR_Q, BR_U [nch, nobs], rc."""
+        channels, rot, depol, tau, ssa, pmom, pe, he, te, kernel_wt, param, bpdf, sza, raa, vza = self._f(
+            channels, rot, depol, tau, ssa, pmom, pe, he, te, kernel_wt, param, bpdf, sza, raa, vza)
+        nch, nobs, npol = len(channels), len(sza), pmom.shape[0]
+        rad, refl, br, q, u, bq, bu = (np.zeros((nch, nobs)) for _ in range(7))
+        rc = np.zeros(1, dtype=np.int32)
+        _big_stack_call(self.fvmb, km, nch, nobs, _p(channels), nstreams, int(plane_parallel), nmom, npol,
+                        _p(rot), _p(depol), _p(tau), _p(ssa), _p(pmom), _p(pe), _p(he), _p(te), _p(kernel_wt),
+                        _p(param), _p(bpdf), _p(sza), _p(raa), _p(vza), missing, _p(rad), _p(refl), _p(br),
+                        _p(q), _p(u), _p(bq), _p(bu), _p(rc, np.int32))
+        return rad, refl, q, u, br, bq, bu, int(rc[0])
 
 
 _DRV_LIB = None

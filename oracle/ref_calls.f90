@@ -36,6 +36,8 @@ module ref_calls_m
   use VLIDORT_ROT
   use VLIDORT_BRDF_MODIS
   use VLIDORT_BRDF_CX
+  use VLIDORT_LAMBERT_BPDF
+  use VLIDORT_BRDF_MODIS_BPDF
   use VLIDORT_ScatMod, only: coef_depol
   implicit none
 
@@ -256,5 +258,69 @@ contains
          Q, U, BR, BR_Q, BR_U, irc)
     rc = irc
   end subroutine drv_vector_cx
+
+  subroutine drv_vector_lambert_bpdf(km, nch, nobs, channels, nstreams, plane_parallel, nMom, &
+       nPol, ROT, depol, tau, ssa, pmom, pe, he, te, albedo, bpdf, &
+       solar_zenith, relat_azymuth, sensor_zenith, MISSING, &
+       radiance_VL, reflectance_VL, BR, Q, U, BR_Q, BR_U, rc) bind(C, name="drv_vector_lambert_bpdf")
+    integer(c_int), value, intent(in) :: km, nch, nobs, nstreams, plane_parallel, nMom, nPol
+    real(c_double), intent(in) :: channels(nch), ROT(km,nobs,nch), depol(nch)
+    real(c_double), intent(in) :: tau(km,nch,nobs), ssa(km,nch,nobs)
+    real(c_double), intent(in) :: pmom(km,nch,nobs,nMom,nPol)
+    real(c_double), intent(in) :: pe(km+1,nobs), he(km+1,nobs), te(km+1,nobs)
+    real(c_double), intent(in) :: albedo(nobs,nch), bpdf(3,nch,nobs)
+    real(c_double), intent(in) :: solar_zenith(nobs), relat_azymuth(nobs), sensor_zenith(nobs)
+    real(c_double), value, intent(in) :: MISSING
+    real(c_double), intent(out) :: radiance_VL(nobs,nch), reflectance_VL(nobs,nch)
+    real(c_double), intent(out) :: Q(nobs,nch), U(nobs,nch), BR(nobs,nch), BR_Q(nobs,nch), BR_U(nobs,nch)
+    integer(c_int), intent(out) :: rc
+    integer :: irc
+    real*8 :: miss
+    miss = MISSING
+    radiance_VL = MISSING
+    reflectance_VL = MISSING
+    Q = MISSING
+    U = MISSING
+    BR = MISSING
+    BR_Q = MISSING
+    BR_U = MISSING
+    call VLIDORT_Vector_Lambert_BPDF(km, nch, nobs, channels, nstreams, plane_parallel /= 0, nMom, &
+         nPol, ROT, depol, tau, ssa, pmom, pe, he, te, albedo, bpdf, &
+         solar_zenith, relat_azymuth, sensor_zenith, miss, 0, radiance_VL, reflectance_VL, &
+         BR, Q, U, BR_Q, BR_U, irc)
+    rc = irc
+  end subroutine drv_vector_lambert_bpdf
+
+  subroutine drv_vector_landmodis_bpdf(km, nch, nobs, channels, nstreams, plane_parallel, nMom, &
+       nPol, ROT, depol, tau, ssa, pmom, pe, he, te, kernel_wt, param, bpdf, &
+       solar_zenith, relat_azymuth, sensor_zenith, MISSING, &
+       radiance_VL, reflectance_VL, BR, Q, U, BR_Q, BR_U, rc) bind(C, name="drv_vector_landmodis_bpdf")
+    integer(c_int), value, intent(in) :: km, nch, nobs, nstreams, plane_parallel, nMom, nPol
+    real(c_double), intent(in) :: channels(nch), ROT(km,nobs,nch), depol(nch)
+    real(c_double), intent(in) :: tau(km,nch,nobs), ssa(km,nch,nobs)
+    real(c_double), intent(in) :: pmom(km,nch,nobs,nMom,nPol)
+    real(c_double), intent(in) :: pe(km+1,nobs), he(km+1,nobs), te(km+1,nobs)
+    real(c_double), intent(in) :: kernel_wt(3,nch,nobs), param(2,nch,nobs), bpdf(3,nch,nobs)
+    real(c_double), intent(in) :: solar_zenith(nobs), relat_azymuth(nobs), sensor_zenith(nobs)
+    real(c_double), value, intent(in) :: MISSING
+    real(c_double), intent(out) :: radiance_VL(nobs,nch), reflectance_VL(nobs,nch)
+    real(c_double), intent(out) :: Q(nobs,nch), U(nobs,nch), BR(nobs,nch), BR_Q(nobs,nch), BR_U(nobs,nch)
+    integer(c_int), intent(out) :: rc
+    integer :: irc
+    real*8 :: miss
+    miss = MISSING
+    radiance_VL = MISSING
+    reflectance_VL = MISSING
+    Q = MISSING
+    U = MISSING
+    BR = MISSING
+    BR_Q = MISSING
+    BR_U = MISSING
+    call VLIDORT_Vector_LandMODIS_BPDF(km, nch, nobs, channels, nstreams, plane_parallel /= 0, nMom, &
+         nPol, ROT, depol, tau, ssa, pmom, pe, he, te, kernel_wt, param, bpdf, &
+         solar_zenith, relat_azymuth, sensor_zenith, miss, 0, radiance_VL, reflectance_VL, &
+         BR, Q, U, BR_Q, BR_U, irc)
+    rc = irc
+  end subroutine drv_vector_landmodis_bpdf
 
 end module ref_calls_m

@@ -75,6 +75,7 @@ program shim_driver
   use VLIDORT_LAMBERT
   use VLIDORT_BRDF_MODIS
   use VLIDORT_BRDF_CX
+  use VLIDORT_BRDF_MODIS_BPDF
   use raw_table_reader, only: read_raw_table, ncalls
   implicit none
   character(len=512) :: fin, fout
@@ -83,7 +84,7 @@ program shim_driver
   real*8, allocatable :: channels(:), rot(:,:,:), depol(:), alpha(:,:,:), tau(:,:,:), ssa(:,:,:), g(:,:,:)
   real*8, allocatable :: pmom(:,:,:,:,:), pe(:,:), he(:,:), te(:,:), albedo(:,:), sza(:), raa(:), vza(:)
   real*8, allocatable :: ff(:,:), rad(:,:), refl(:,:), q(:,:), uu(:,:), kw(:,:,:), kp(:,:,:), br(:,:)
-  real*8, allocatable :: u10(:), v10(:), mr(:), brx(:,:,:), brq(:,:), bru(:,:)
+  real*8, allocatable :: u10(:), v10(:), mr(:), brx(:,:,:), brq(:,:), bru(:,:), bp(:,:,:)
   ! Mie mode
   integer :: nq, ntab, nrh, nbin, t, b
   type(Chem_Mie) :: mie
@@ -171,6 +172,22 @@ program shim_driver
     end if
     open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
     write(u) rc, rad3, refl3, q3, u3
+    close(u)
+  else if (mode == 10) then
+    ! VLIDORT_Vector_LandMODIS_BPDF
+    read(u) km, nch, nobs, nstreams, nmom, npol, ipp
+    read(u) missing
+    allocate(channels(nch), rot(km,nobs,nch), depol(nch), tau(km,nch,nobs), ssa(km,nch,nobs))
+    allocate(pmom(km,nch,nobs,nmom,npol), pe(km+1,nobs), he(km+1,nobs), te(km+1,nobs))
+    allocate(kw(3,nch,nobs), kp(2,nch,nobs), bp(3,nch,nobs), sza(nobs), raa(nobs), vza(nobs))
+    allocate(rad(nobs,nch), refl(nobs,nch), q(nobs,nch), uu(nobs,nch), br(nobs,nch), brq(nobs,nch), bru(nobs,nch))
+    read(u) channels, rot, depol, tau, ssa, pmom, pe, he, te, kw, kp, bp, sza, raa, vza
+    close(u)
+    call VLIDORT_Vector_LandMODIS_BPDF(km, nch, nobs, channels, nstreams, ipp /= 0, nmom, npol, rot, depol, &
+                                       tau, ssa, pmom, pe, he, te, kw, kp, bp, sza, raa, vza, missing, 0, &
+                                       rad, refl, br, q, uu, brq, bru, rc)
+    open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+    write(u) rc, rad, refl, q, uu, br, brq, bru
     close(u)
   else if (mode == 9) then
     ! VLIDORT_Vector_CX (generic -> SingleGeom)

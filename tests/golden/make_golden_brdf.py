@@ -20,10 +20,15 @@
   (VLIDORT_CoxMunk_SingleGeom supplement, VLIDORT_Scalar_CX radiances).
 * vcx_*.npz: the vector Cox-Munk driver (VLIDORT_Vector_CX_SingleGeom:
   NSTOKES = 3, I, Q, U and the direct-bounce BRDF outputs).
+* supplement_bpdf.npz, vbpdf_*.npz: the polarised land surfaces
+  (VLIDORT_LAMBERTIAN_BPDF / VLIDORT_LANDMODIS_BPDF supplements with NSTOKES
+  = 3; VLIDORT_Vector_Lambert_BPDF and VLIDORT_Vector_LandMODIS_BPDF
+  radiances, I, Q, U and the BR, BR_Q, BR_U outputs).
 
 Run here (the reference is only available in this container):
     python tests/golden/make_golden_brdf.py          # everything
     python tests/golden/make_golden_brdf.py vcx      # the vector Cox-Munk cases only
+    python tests/golden/make_golden_brdf.py vbpdf    # the polarised land cases and their supplement
     python tests/golden/make_golden_brdf.py cx_n20   # named cases only
 """
 import json
@@ -254,6 +259,89 @@ def vcx_case(R, ncol, channels, nstreams, nmom, edge, seed=17):
                 out_br_u=bru, out_nfourier=nf, out_rc=rcr)
 
 
+BPDF_SUP_CASES = [
+    # (nstreams, sza, vza, raa, modis, fiso|albedo, fgeo, fvol, h/b, b/r, n, NDVI, C)
+    (4, 35.0, 20.0, 60.0, 1, 0.10, 0.03, 0.05, 2.0, 1.0, 1.5, 0.4, 6.0),
+    (4, 50.0, 40.0, 150.0, 0, 0.08, 0.0, 0.0, 0.0, 0.0, 1.5, 0.7, 5.0),
+    (6, 60.0, 10.0, 100.0, 1, 0.20, 0.05, 0.10, 2.0, 1.0, 1.5, -0.1, 7.5),
+    (2, 25.0, 0.0, 0.0, 0, 0.12, 0.0, 0.0, 0.0, 0.0, 1.5, 0.2, 4.0),      # nadir view
+    (3, 0.0, 30.0, 180.0, 1, 0.15, 0.02, 0.09, 2.0, 1.0, 1.5, 1.4, 6.0),  # overhead sun, NDVI > 1 taken as 0
+    (4, 30.0, 20.0, 40.0, 1, 1.00, 0.01, 0.50, 2.0, 1.0, 1.5, 0.3, 6.0),  # white-sky albedo > 1
+    (5, 45.0, 30.0, 90.0, 0, 0.05, 0.0, 0.0, 0.0, 0.0, 1.5, 0.5, 8.0),
+]
+
+
+def supplement_bpdf(R):
+    recs = {k: [] for k in ("F", "F0", "UF", "DB", "wsa", "status")}
+    nmax = max(c[0] for c in BPDF_SUP_CASES)
+    for c in BPDF_SUP_CASES:
+        r = R.vbrdf_bpdf(*c)
+        n = c[0]
+        F = np.zeros((2 * nmax, 16, nmax, nmax)); F[:2 * n, :, :n, :n] = r["F"]
+        F0 = np.zeros((2 * nmax, 16, nmax)); F0[:2 * n, :, :n] = r["F0"]
+        UF = np.zeros((2 * nmax, 16, nmax)); UF[:2 * n, :, :n] = r["UF"]
+        for k, v in (("F", F), ("F0", F0), ("UF", UF), ("DB", r["DB"]), ("wsa", r["wsa"]), ("status", r["status"])):
+            recs[k].append(v)
+    out = {k: np.asarray(v) for k, v in recs.items()}
+    out["cases"] = np.asarray(BPDF_SUP_CASES, dtype=np.float64)
+    return out
+
+
+VBPDF_CASES = {
+    # name: (ncol, channels, nstreams, nmom, modis, edge)
+    "vbpdf_modis_n4_2ch": (5, (470.0, 870.0), 4, 32, True, False),
+    "vbpdf_lamb_n4_edge": (10, (550.0,), 4, 32, False, True),
+    "vbpdf_modis_n3_edge": (10, (660.0,), 3, 32, True, True),
+    "vbpdf_lamb_n6": (3, (550.0, 870.0), 6, 64, False, False),
+    "vbpdf_modis_n2": (4, (550.0,), 2, 32, True, False),
+}
+
+
+def vbpdf_case(R, ncol, channels, nstreams, nmom, modis, edge, seed=23):
+    cols = S.make_columns(ncol, 72, channels, seed=seed)
+    tabs = S.make_mie_tables(channels, nmom=nmom, npol=6)
+    tau, ssa, g, pm, rc = CM.get_aop_vector(tabs, channels, tabs[0].lambda_m, S.TRACERS, cols.qm, cols.rh, nmom, 6)
+    assert rc == 0
+    kw, par = S.make_modis_kernels(ncol, len(channels), seed=seed)
+    bp = S.make_bpdf(ncol, len(channels), seed=seed)
+    albedo = cols.albedo.copy()
+    sza, vza, raa = cols.sza.copy(), cols.vza.copy(), cols.raa.copy()
+    if edge:
+        vza[1] = 0.0                       # nadir view
+        if modis:
+            # missing geometry (not in the Lambertian form: its driver then
+            # writes Q(j,i), U(j,i) with i left at nch + 1 by the previous
+            # column's channel loop, outside the arrays, GEO/VLIDORT_LAMBERT_BPDF.F90:128-134)
+            sza[2] = S.MISSING
+            kw[3, 0, :] = (0.05, 0.30, 0.40)   # negative direct bounce -> -500
+            kw[4, 0, :] = (1.00, 0.01, 0.50)   # white-sky albedo > 1: BPDF Fourier terms left out
+            kw[6, 0, 1] = S.MISSING            # missing kernel weight (the reference still computes)
+        else:
+            albedo[0, 3] = -0.2                # negative direct bounce -> -500
+            albedo[0, 4] = 0.97                # white-sky albedo > 1 with the BPDF kernel
+            albedo[0, 6] = S.MISSING           # missing albedo: element MISSING
+        bp[5, 0, 1] = 1.5                  # NDVI outside [-1, 1] taken as 0
+        sza[7] = 0.0                       # overhead sun latches nadir FO for later columns
+        raa[8] = 0.0
+    f = lambda a: np.asarray(a, dtype=np.float64)
+    common = (72, cols.channels, nstreams, nmom, cols.rot, cols.depol, f(tau), f(ssa), f(pm), cols.pe, cols.he,
+              cols.te)
+    out = R.vector_bpdf(*common, int(modis), albedo, kw, par, bp, sza, raa, vza)
+    rad, refl, q, uu, br, brq, bru, nf, rcr = out
+    D = oracle.drivers()
+    if modis:
+        d = D.vector_landmodis_bpdf(*common, kw, par, bp, sza, raa, vza)
+    else:
+        d = D.vector_lambert_bpdf(*common, albedo, bp, sza, raa, vza)
+    assert d[-1] == rcr
+    _same_as_driver((rad, refl, q, uu, br, brq, bru), d[:7])
+    return dict(km=72, channels=np.asarray(channels, float), nmom=nmom, npol=6, meta_nstreams=nstreams,
+                meta_modis=int(modis), tau=tau, ssa=ssa, pmom=pm, rot=cols.rot, depol=cols.depol, pe=cols.pe,
+                he=cols.he, te=cols.te, albedo=albedo, kernel_wt=kw, param=par, bpdf=bp, sza=sza, vza=vza, raa=raa,
+                out_radiance=rad, out_reflectance=refl, out_q=q, out_u=uu, out_br=br, out_br_q=brq,
+                out_br_u=bru, out_nfourier=nf, out_rc=rcr)
+
+
 def main():
     names = sys.argv[1:]
     if names:   # only the named radiance cases ("vcx": every vector Cox-Munk case)
@@ -261,8 +349,13 @@ def main():
         assert R is not None, "build oracle/_ref first (make -C oracle -f Makefile.ref)"
         if names == ["vcx"]:
             names = list(VCX_CASES)
+        if names == ["vbpdf"]:
+            np.savez_compressed(os.path.join(OUT, "supplement_bpdf.npz"), **supplement_bpdf(R))
+            names = list(VBPDF_CASES)
         for name in names:
-            if name in RAD_CASES:
+            if name in VBPDF_CASES:
+                d = vbpdf_case(R, *VBPDF_CASES[name])
+            elif name in RAD_CASES:
                 d = radiance_case(R, *RAD_CASES[name])
             elif name in CX_CASES:
                 d = cx_case(R, *CX_CASES[name])
@@ -288,6 +381,11 @@ def main():
         print(name, d["out_rc"], d["out_radiance"].ravel()[:4], d["out_nfourier"].ravel()[:6])
     for name, c in VCX_CASES.items():
         d = vcx_case(R, *c)
+        np.savez_compressed(os.path.join(OUT, name + ".npz"), **d)
+        print(name, d["out_rc"], d["out_radiance"].ravel()[:4], d["out_nfourier"].ravel()[:6])
+    np.savez_compressed(os.path.join(OUT, "supplement_bpdf.npz"), **supplement_bpdf(R))
+    for name, c in VBPDF_CASES.items():
+        d = vbpdf_case(R, *c)
         np.savez_compressed(os.path.join(OUT, name + ".npz"), **d)
         print(name, d["out_rc"], d["out_radiance"].ravel()[:4], d["out_nfourier"].ravel()[:6])
 

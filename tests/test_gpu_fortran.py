@@ -123,6 +123,33 @@ def test_vector_cx_through_fortran_shim(gpu, tmp_path):
     assert np.array_equal(arr[5], d["out_br_q"]) and np.array_equal(arr[6], d["out_br_u"])
 
 
+def test_landmodis_bpdf_through_fortran_shim(gpu, tmp_path):
+    """VLIDORT_Vector_LandMODIS_BPDF through the Fortran module against the
+    reference's own driver (edge columns: missing geometry, negative direct
+    bounce, white-sky failure, SZA = 0 latch, RAA 0)."""
+    print("case fortran vbpdf_modis_n3_edge", flush=True)
+    z = np.load(os.path.join(ROOT, "tests", "golden", "brdf", "vbpdf_modis_n3_edge.npz"), allow_pickle=False)
+    d = {k: z[k] for k in z.files}
+    km, nch, nobs = int(d["km"]), len(d["channels"]), len(d["sza"])
+    f8 = lambda a: np.ascontiguousarray(a, dtype=np.float64).tobytes()
+    hdr = np.array([10, km, nch, nobs, int(d["meta_nstreams"]), int(d["nmom"]), int(d["npol"]), 0],
+                   dtype=np.int32).tobytes() + np.float64(S.MISSING).tobytes()
+    body = b"".join(f8(d[k]) for k in ("channels", "rot", "depol", "tau", "ssa", "pmom", "pe", "he", "te",
+                                       "kernel_wt", "param", "bpdf", "sza", "raa", "vza"))
+    out = _run(tmp_path, hdr + body)
+    rc = int(np.frombuffer(out[:4], dtype=np.int32)[0])
+    arr = np.frombuffer(out[4:], dtype=np.float64).reshape(7, nch, nobs)
+    assert rc == int(d["out_rc"]) == 0
+    assert max_rel(arr[0], d["out_radiance"]) < RTOL_FP64
+    assert max_rel(arr[1], d["out_reflectance"]) < RTOL_FP64
+    I = np.abs(d["out_radiance"])
+    ok = (np.abs(I / S.MISSING - 1) > 1e-3) & (d["out_radiance"] != -500.0) & (d["raa"][None, :] != 0.0)
+    for k, key in ((2, "out_q"), (3, "out_u")):
+        assert (np.abs(arr[k] - d[key])[ok] / I[ok]).max() < RTOL_FP64, key
+    for k, key in ((4, "out_br"), (5, "out_br_q"), (6, "out_br_u")):
+        assert np.max(np.abs(arr[k] - d[key]) / np.maximum(np.abs(d[key]), 1e-300)) < 1e-12, key
+
+
 def test_get_aop_vector_through_fortran_shim(gpu, tmp_path):
     channels = (550.0, 870.0)
     nmom, npol = 16, 6

@@ -117,6 +117,30 @@ def test_harness_equals_reference_vector_cx_driver():
     assert np.array_equal(a[7], d["out_nfourier"])
 
 
+@needs_drivers
+@pytest.mark.parametrize("name", ["vbpdf_lamb_n4_edge", "vbpdf_modis_n3_edge"])
+def test_harness_equals_reference_bpdf_drivers(name):
+    """The harness's polarised land routine equals the reference's
+    VLIDORT_Vector_Lambert_BPDF / VLIDORT_Vector_LandMODIS_BPDF bit for bit,
+    and the stored golden is that output."""
+    d = np.load(os.path.join(GOLDEN, "brdf", name + ".npz"), allow_pickle=False)
+    D, H = oracle.drivers(), oracle.drv_harness_modis()
+    common = (72, d["channels"], int(d["meta_nstreams"]), int(d["nmom"]), d["rot"], d["depol"], f64(d["tau"]),
+              f64(d["ssa"]), f64(d["pmom"]), d["pe"], d["he"], d["te"])
+    modis = int(d["meta_modis"])
+    a = H.vector_bpdf(*common, modis, d["albedo"], d["kernel_wt"], d["param"], d["bpdf"], d["sza"], d["raa"],
+                      d["vza"])
+    if modis:
+        b = D.vector_landmodis_bpdf(*common, d["kernel_wt"], d["param"], d["bpdf"], d["sza"], d["raa"], d["vza"])
+    else:
+        b = D.vector_lambert_bpdf(*common, d["albedo"], d["bpdf"], d["sza"], d["raa"], d["vza"])
+    for k in range(7):   # I, reflectance, Q, U, BR, BR_Q, BR_U
+        assert np.array_equal(a[k], b[k]), k
+    assert a[-1] == b[-1] == 0
+    assert np.array_equal(a[0], d["out_radiance"]) and np.array_equal(a[2], d["out_q"])
+    assert np.array_equal(a[7], d["out_nfourier"])
+
+
 def test_rot_calc_matches_reference_bit_for_bit():
     z = np.load(os.path.join(GOLDEN, "prep", "rot_calc.npz"), allow_pickle=False)
     rot, depol = P.rot_calc(z["channels"], z["pe"].T, z["he"].T, z["te"].T)

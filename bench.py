@@ -185,6 +185,8 @@ def main():
                     help="surface model: Lambertian albedo or the MODIS land BRDF (scalar workloads)")
     ap.add_argument("--host-cols", type=int, default=20000,
                     help="columns of the host-pointer (PCIe-inclusive) drop-in phase (0 skips it)")
+    ap.add_argument("--no-other-configs", dest="other_configs", action="store_false",
+                    help="skip the vector (200k x 2 ch) and uneven (300k) configurations of the default run")
     ap.add_argument("--work-gib", type=float, default=64.0,
                     help="work-arena budget of the throughput runs in GiB (one stream; 0 keeps the library default)")
     args = ap.parse_args()
@@ -497,7 +499,35 @@ def main():
                             "first_call_columns_per_s": round(hc / (f2 - f0), 1), "rc": [int(rc_a), int(rc_l)],
                             "note": "host numpy arrays in, host arrays out (FP64 optics widened on the host, "
                                     "PCIe copies inside the calls); not the headline value"}
+    # the other device-resident configurations, each in a child process of
+    # its own (one bench line; this process first gives its device memory back)
+    if args.other_configs and args.workload == "scalar_500k_4ch" and not args.ncol:
+        _lib.check(L.aa_release_arenas(), "aa_release_arenas")
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        out["other_configs"] = {w: other_config(w) for w in ("vector_200k_2ch", "uneven_300k_1ch")}
     print(json.dumps(out))
+
+
+def other_config(workload):
+    """One warm-up and one timed step of `workload` in both precisions, from a
+    child run of this script; the rates and step times of its bench line."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", workload, "--steps", "1", "--warmup", "1",
+           "--no-cpu", "--serving-granules", "0", "--host-cols", "0", "--no-other-configs"]
+    print(f"other configuration {workload}: starting", file=sys.stderr, flush=True)
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, text=True)   # progress lines go straight to stderr
+    if p.returncode != 0:
+        return {"error": f"exit {p.returncode}"}
+    line = json.loads(p.stdout.strip().splitlines()[-1])
+    r = {"workload": line["config"]["workload"], "columns_per_s_mixed": line["value"],
+         "ms_per_step_mixed": line["ms_per_step"], "columns_per_s_fp64": line["fp64"]["value"],
+         "ms_per_step_fp64": line["fp64"]["ms_per_step"], "steps": 1, "warmup": 1}
+    if "mixed_vs_fp64" in line:
+        r["mixed_vs_fp64"] = {k: line["mixed_vs_fp64"][k] for k in ("speedup", "max_rel_err_radiance",
+                                                                    "fourier_count_mismatch_frac")}
+    r["kernel_ms_mixed"] = line.get("kernel_ms")
+    return r
 
 
 if __name__ == "__main__":

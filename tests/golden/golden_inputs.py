@@ -67,6 +67,9 @@ SAMPLED = {
                              nmom=32, vector=True, ngeom=2),
     "vector_2os": dict(kind="lambert", ncol=12, col0=70, seed=5, channels=(550.0, 870.0), nstreams=6,
                        nmom=32, vector=True, do_2os=True),
+    # 48 columns of the vector workload over the polarised MODIS land surface
+    "bpdf_modis_48x2": dict(kind="bpdf_modis", ncol=48, col0=120_000, seed=7, channels=(550.0, 870.0),
+                            nstreams=4, nmom=32, vector=True),
 }
 
 
@@ -98,6 +101,9 @@ def sampled_inputs(spec: dict) -> dict:
     d = dict(km=72, channels=np.asarray(ch, float), nmom=nmom, npol=npol, tau=tau, ssa=ssa, g=g, pmom=pm,
              rot=cols.rot, depol=cols.depol, pe=cols.pe, he=cols.he, te=cols.te, albedo=cols.albedo.copy(),
              sza=sza, vza=vza, raa=raa, flux_factor=cols.flux_factor, qm=cols.qm, rh=cols.rh)
+    if kind == "bpdf_modis":
+        d["kernel_wt"], d["param"] = S.make_modis_kernels(ncol, len(ch), seed=spec["seed"], col0=spec["col0"])
+        d["bpdf"] = S.make_bpdf(ncol, len(ch), seed=spec["seed"], col0=spec["col0"])
     if kind == "uneven":
         d["nlayers"] = cols.nlayers.astype(np.int32)
         d["nstreams_col"] = S.stream_choice(ncol, seed=spec["seed"], col0=spec["col0"]).astype(np.int32)

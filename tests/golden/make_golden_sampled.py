@@ -77,6 +77,19 @@ def run_case(D, H, spec, d):
     kind = spec["kind"]
     if kind in ("lambert", "high_sza", "grazing"):
         return lambert(D, H, d, 72, spec["nstreams"], spec["vector"], spec.get("do_2os", False))
+    if kind == "bpdf_modis":
+        M = oracle.drv_harness_modis()
+        common = (72, d["channels"], spec["nstreams"], spec["nmom"], d["rot"], d["depol"], f64(d["tau"]),
+                  f64(d["ssa"]), f64(d["pmom"]), d["pe"], d["he"], d["te"])
+        rad, refl, q, u, br, brq, bru, nf, rc = M.vector_bpdf(*common, 1, d["albedo"], d["kernel_wt"], d["param"],
+                                                              d["bpdf"], d["sza"], d["raa"], d["vza"])
+        want = D.vector_landmodis_bpdf(*common, d["kernel_wt"], d["param"], d["bpdf"], d["sza"], d["raa"],
+                                       d["vza"])
+        for a, b in zip((rad, refl, q, u, br, brq, bru), want[:7]):
+            assert np.array_equal(a, b)
+        assert rc == want[-1]
+        return dict(radiance=rad, reflectance=refl, q=q, u=u, br=br, br_q=brq, br_u=bru, nfourier=nf,
+                    rc=np.int32(rc))
     if kind == "uneven":
         nch, n = len(spec["channels"]), spec["ncol"]
         out = dict(radiance=np.zeros((nch, n)), reflectance=np.zeros((nch, n)),

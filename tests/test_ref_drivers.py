@@ -173,7 +173,7 @@ def _port_case(P_, spec, d, sl=slice(None), km=72, N=None):
     return I, nf, rc
 
 
-@pytest.mark.parametrize("name", [n for n in sampled_names() if n != "vector_multigeom"])
+@pytest.mark.parametrize("name", [n for n in sampled_names() if n not in ("vector_multigeom", "bpdf_modis_48x2")])
 def test_port_matches_sampled_reference(name):
     """The C++ restatement against the reference's outputs: 1e-9 and the same
     Fourier-term counts on every column of every sampled case."""

@@ -17,6 +17,11 @@ kernel k_brdf (aeroapps_amd/csrc/brdf.hip).  Nothing in the product imports it.
                          observation geometry (one geometry), half-range
                          azimuth quadrature, no emission, no glitter
                          corrections, white-sky albedo computed and checked.
+* bpdf_ndvi, fresnel_vector
+                      <- VB/vbrdf_sup_kernels.f90:3017-3198 (BPDFNDVI_VFUNCTION),
+                         :3575-3702 (FRESNEL_VECTOR): the polarised land kernel,
+                         3 x 3 Mueller block (checked against the compiled
+                         reference's direct-bounce elements and closed forms)
 * landmodis_setup     <- V283/VLIDORT_SurfaceMod.F90:1376-1596
                          (VLIDORT_LANDMODIS_SingleGeom: kernels Lambertian /
                          Ross-thick / Li-sparse, factors fiso / fvol / fgeo,
@@ -127,6 +132,65 @@ def lisparse(pars, xi, sxi, xj, sxj, phi, cphi, sphi):
         sint = math.sqrt(1.0 - cost * cost)
         q = 1.0 - ((t - sint * cost) / PIE)
     return 0.5 * p - q * r
+
+
+def fresnel_vector(n1, n2, xi, sxi, xj, sxj, ckphi, skphi):
+    """Fresnel amplitudes rotated to the meridian planes of the incident
+    (azimuth 0) and reflected (azimuth phi) directions: (f11, f12, f21, f22,
+    dmod), dmod the fourth power of |incident x reflected| (1 when they are
+    parallel, with f11 = r_par, f22 = r_per)."""
+    vi = (sxi * 1.0, sxi * 0.0, -xi)
+    vr = (sxj * ckphi, sxj * skphi, xj)
+    u = (vi[0] - vr[0], vi[1] - vr[1], vi[2] - vr[2])
+    fac = math.sqrt(1.0 / (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]))
+    ci = fac * (u[0] * vi[0] + u[1] * vi[1] + u[2] * vi[2])
+    ct = math.sqrt(1.0 - (1.0 - ci * ci) * n1 * n1 / (n2 * n2))
+    rper = (n1 * ci - n2 * ct) / (n1 * ci + n2 * ct)
+    rpar = (n2 * ci - n1 * ct) / (n2 * ci + n1 * ct)
+    ti = (-xi * 1.0, -xi * 0.0, -sxi)
+    tr = (xj * ckphi, xj * skphi, -sxj)
+    pi_ = (-0.0, 1.0, 0.0)
+    pr = (-skphi, ckphi, 0.0)
+    dot = lambda a, b: a[0] * b[0] + a[1] * b[1] + a[2] * b[2]
+    pikr, prki, tikr, trki = dot(pi_, vr), dot(pr, vi), dot(ti, vr), dot(tr, vi)
+    e1, e2, e3, e4 = pikr * prki, tikr * trki, tikr * prki, pikr * trki
+    f11 = e1 * rper + e2 * rpar
+    f12 = -(e3 * rper) + e4 * rpar
+    f21 = -(e4 * rper) + e3 * rpar
+    f22 = e2 * rper + e1 * rpar
+    vp = (vi[1] * vr[2] - vi[2] * vr[1], vi[2] * vr[0] - vi[0] * vr[2], vi[0] * vr[1] - vi[1] * vr[0])
+    dmod = dot(vp, vp) ** 2
+    if dmod == 0.0:
+        f11, f22, dmod = rpar, rper, 1.0
+    return f11, f12, f21, f22, dmod
+
+
+def bpdf_ndvi(pars, xi, sxi, xj, sxj, cphi, skphi):
+    """3 x 3 Mueller block of the BPDF-NDVI kernel, pars = (refractive index,
+    NDVI, scaling factor C)."""
+    if abs(xi - 1.0) < 1e-9:
+        xi = 0.999999999999
+    if abs(xj - 1.0) < 1e-9:
+        xj = 0.999999999999
+    z = min(xi * xj - sxi * sxj * cphi, 1.0)
+    cg = math.cos(math.acos(z) * 0.5)
+    ndvi = pars[1] if -1.0 <= pars[1] <= 1.0 else 0.0
+    fp0 = 0.25 * math.exp(-ndvi) / (xi + xj)
+    att = math.exp(-math.sqrt(1.0 - cg * cg) / cg)
+    f11, f12, f21, f22, dmod = fresnel_vector(1.0, pars[0], xi, sxi, xj, sxj, cphi, skphi)
+    h = pars[2] * fp0 * att / dmod
+    a11, a12, a21, a22 = f11 * f11, f12 * f12, f21 * f21, f22 * f22
+    k = [[0.0] * 3 for _ in range(3)]
+    k[0][0] = (a11 + a12 + a21 + a22) * 0.5 * h
+    k[0][1] = (a11 - a12 + a21 - a22) * 0.5 * h
+    k[1][0] = (a11 - a22 + a12 - a21) * 0.5 * h
+    k[1][1] = (a11 - a12 - a21 + a22) * 0.5 * h
+    k[0][2] = -(f11 * f12 + f21 * f22) * h
+    k[1][2] = -(f11 * f12 - f21 * f22) * h
+    k[2][0] = -(f11 * f21 + f12 * f22) * h
+    k[2][1] = -(f11 * f21 - f12 * f22) * h
+    k[2][2] = (f11 * f22 + f12 * f21) * h
+    return k
 
 
 def derfc_e(x: float) -> float:
